@@ -1,0 +1,403 @@
+/*
+ * pvhip.h -- C-ABI of libpvhip, the MI355X (gfx950) device layer and hot-path
+ * kernels of OpenPV's synaptic-delivery + LCA path.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers, sizes and POD
+ * structs, and returns an int status (PVHIP_OK == 0).  A non-zero status
+ * leaves a message retrievable with pvhip_last_error().  The reference turns
+ * every CUDA error into Fatal() (src/arch/cuda/cuda_util.hpp:14-27); the host
+ * wrapper of this library does the same with the status codes.
+ *
+ * Device pointers are plain `void*`/`float*` into HBM.  Streams are HIP
+ * streams passed as `pvhip_stream` (NULL = the null stream).  Nothing here
+ * allocates on a kernel call: buffers are owned by the caller, as in the
+ * reference where layers, publishers and weights own their CudaBuffers
+ * (src/components/ComponentBuffer.cpp:106, src/components/BasePublisherComponent.cpp:109-127,
+ * src/components/Weights.cpp:114-159).
+ *
+ * Data layouts (all as the reference's PV layout, feature index fastest,
+ * src/utils/conversions.hpp:251-253 and :401-407):
+ *   restricted buffer  [nbatch][ny][nx][nf]
+ *   extended buffer    [nbatch][ny+dn+up][nx+lt+rt][nf]
+ *   GSyn channel       [nbatch][ny][nx][nf]   (channels outermost: [nch][nbatch]...)
+ *   weights            [numKernels][nyp][nxp][nfp]   (one arbor)
+ *   active list        pvhip_sparse_entry[nbatch][numExtended], count int64[nbatch]
+ */
+#ifndef PVHIP_H_
+#define PVHIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PVHIP_ABI_VERSION 1
+
+enum {
+   PVHIP_OK              = 0,
+   PVHIP_ERR_INVALID     = 1, /* bad argument or shape */
+   PVHIP_ERR_RUNTIME     = 2, /* HIP runtime call failed */
+   PVHIP_ERR_UNSUPPORTED = 3, /* valid request this build does not implement */
+   PVHIP_ERR_NOMEM       = 4
+};
+
+/* Arithmetic form of a kernel. */
+typedef enum {
+   PVHIP_ARITH_EXACT = 0, /* FP32, the reference CPU summation order, mul and add rounded separately */
+   PVHIP_ARITH_FP32  = 1, /* FP32 inputs, FP32 FMA / MFMA accumulation, any order */
+   PVHIP_ARITH_BF16  = 2, /* BF16 inputs on MFMA, FP32 accumulate and FP32 state */
+   PVHIP_ARITH_FP8   = 3  /* FP8 e4m3 inputs with per-feature weight scales, FP32 accumulate */
+} pvhip_arith;
+
+/* ------------------------------------------------------------------------ */
+/* Types mirroring the reference's structs                                   */
+/* ------------------------------------------------------------------------ */
+
+/* PVHalo (src/include/PVLayerLoc.h:14-16). */
+typedef struct {
+   int lt, rt, dn, up;
+} pvhip_halo;
+
+/* PVLayerLoc (src/include/PVLayerLoc.h:21-26) for one rank: global == local. */
+typedef struct {
+   int nbatch, nx, ny, nf;
+   pvhip_halo halo;
+} pvhip_loc;
+
+/* Patch (src/structures/Patch.hpp:15-18): same field order and sizes. */
+typedef struct {
+   uint16_t nx, ny;
+   uint32_t offset;
+} pvhip_patch;
+
+/* SparseList<float>::Entry (src/structures/SparseList.hpp:19-22). */
+typedef struct {
+   uint32_t index;
+   float value;
+} pvhip_sparse_entry;
+
+/* ------------------------------------------------------------------------ */
+/* Device layer: replaces src/arch/cuda (CudaDevice, CudaBuffer, CudaTimer)  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct pvhip_device pvhip_device;
+typedef struct pvhip_buffer pvhip_buffer;
+typedef struct pvhip_event pvhip_event;
+typedef struct pvhip_graph pvhip_graph;
+typedef void *pvhip_stream; /* hipStream_t */
+
+typedef struct {
+   char name[256];
+   char arch[64]; /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+   int ordinal;
+   int compute_units;
+   int wavefront_size;
+   int max_threads_per_block;
+   int max_grid_x;
+   int clock_khz;
+   int memory_clock_khz;
+   int memory_bus_width;
+   int l2_cache_bytes;
+   size_t lds_per_block;
+   size_t total_global_mem;
+} pvhip_props;
+
+/* Error text of the last failing call on this thread (never NULL). */
+const char *pvhip_last_error(void);
+int pvhip_abi_version(void);
+
+/* CudaDevice(int) ctor: set device, create the one stream, query props
+ * (src/arch/cuda/CudaDevice.cpp:17-94). */
+int pvhip_device_create(int ordinal, pvhip_device **out);
+/* ~CudaDevice (src/arch/cuda/CudaDevice.cpp:96-109). */
+int pvhip_device_destroy(pvhip_device *dev);
+/* CudaDevice::syncDevice (src/arch/cuda/CudaDevice.cpp:96). */
+int pvhip_device_sync(pvhip_device *dev);
+/* CudaDevice getters (src/arch/cuda/CudaDevice.cpp:167-183). */
+int pvhip_device_props(pvhip_device *dev, pvhip_props *out);
+/* CudaDevice::getStream (src/arch/cuda/CudaDevice.hpp). */
+pvhip_stream pvhip_device_stream(pvhip_device *dev);
+/* Bytes allocated through this device (CudaDevice VRAM bookkeeping, :111-123). */
+size_t pvhip_device_allocated_bytes(pvhip_device *dev);
+
+/* CudaDevice::createBuffer / CudaBuffer ctor (src/arch/cuda/CudaBuffer.cpp:16-30). */
+int pvhip_buffer_alloc(pvhip_device *dev, size_t bytes, pvhip_buffer **out);
+/* ~CudaBuffer (src/arch/cuda/CudaBuffer.cpp:32-35). */
+int pvhip_buffer_free(pvhip_buffer *buf);
+/* CudaBuffer::copyToDevice(in, size, offset) (src/arch/cuda/CudaBuffer.cpp:37-49). */
+int pvhip_buffer_h2d(
+      pvhip_buffer *buf, const void *host, size_t bytes, size_t offset, pvhip_stream stream);
+/* CudaBuffer::copyFromDevice (src/arch/cuda/CudaBuffer.cpp:51-62). */
+int pvhip_buffer_d2h(
+      pvhip_buffer *buf, void *host, size_t bytes, size_t offset, pvhip_stream stream);
+void *pvhip_buffer_ptr(pvhip_buffer *buf);
+size_t pvhip_buffer_size(pvhip_buffer *buf);
+
+/* Zero-fill with a kernel (graph-capturable; no hipMemsetAsync node). */
+int pvhip_fill_zero(void *dptr, size_t bytes, pvhip_stream stream);
+int pvhip_stream_sync(pvhip_stream stream);
+
+/* CudaTimer (src/arch/cuda/CudaTimer.cpp:41-72): events on a stream. */
+int pvhip_event_create(pvhip_event **out);
+int pvhip_event_destroy(pvhip_event *ev);
+int pvhip_event_record(pvhip_event *ev, pvhip_stream stream);
+int pvhip_event_elapsed_ms(pvhip_event *start, pvhip_event *stop, float *ms);
+
+/* hipGraph capture of one timestep (no reference counterpart: the reference
+ * launches every kernel each step).  begin/end bracket stream capture. */
+int pvhip_graph_begin(pvhip_stream stream);
+int pvhip_graph_end(pvhip_stream stream, pvhip_graph **out);
+int pvhip_graph_launch(pvhip_graph *graph, pvhip_stream stream);
+int pvhip_graph_destroy(pvhip_graph *graph);
+
+/* ------------------------------------------------------------------------ */
+/* Post-perspective gather: replaces CudaRecvPost / cudnnConvolutionForward  */
+/* (src/cudakernels/CudaRecvPost.cpp:357-389) and is bit-identical, in EXACT */
+/* form, to PostsynapticPerspectiveConvolveDelivery::deliver                 */
+/* (src/delivery/PostsynapticPerspectiveConvolveDelivery.cpp:56-151).        */
+/* ------------------------------------------------------------------------ */
+
+/* Threshold / transfer function of ANNActivityBuffer. */
+typedef enum {
+   /* CPU semantics: A = V, applyVThresh, applyAMax (src/components/ANNActivityBuffer.cpp:432-556). */
+   PVHIP_TRANSFER_VTHRESH = 0,
+   /* GPU semantics: piecewise-linear vertices (src/components/ANNActivityBuffer.kpp:16-77). */
+   PVHIP_TRANSFER_VERTICES = 1
+} pvhip_transfer_mode;
+
+#define PVHIP_MAX_VERTICES 8
+#define PVHIP_MAX_CHANNELS 4
+
+typedef struct {
+   int mode;        /* pvhip_transfer_mode */
+   float VThresh;   /* -FLT_MAX disables thresholding (ANNActivityBuffer.cpp:507) */
+   float AMin;
+   float AMax;      /* FLT_MAX disables the clamp (ANNActivityBuffer.cpp:543) */
+   float AShift;
+   float VWidth;
+   int num_vertices; /* PVHIP_TRANSFER_VERTICES: setVertices/setSlopes (ANNActivityBuffer.cpp:227-397) */
+   float vertices_v[PVHIP_MAX_VERTICES];
+   float vertices_a[PVHIP_MAX_VERTICES];
+   float slopes[PVHIP_MAX_VERTICES + 1];
+} pvhip_transfer;
+
+/* HyPerLCA V/A update, the reference's K8+K6+K7 kernels in one pass:
+ *   G  = sum_ch coeff[ch] * GSyn[ch]              (GSynAccumulator.kpp:21-42)
+ *   V  = e*V + (1-e)*(G + selfInteract*A[kex])    (HyPerLCAInternalStateBuffer.kpp:48-59)
+ *   A[kex] = transfer(V)                          (ANNActivityBuffer)
+ * with e = exp_tau[b] = (float)exp(-dt[b]/(double)tau) computed on the host
+ * by pvhip_lca_exp_tau. */
+typedef struct {
+   pvhip_loc loc;
+   int num_channels;
+   float channel_coeffs[PVHIP_MAX_CHANNELS];
+   const float *gsyn;    /* [num_channels][nbatch][ny][nx][nf] */
+   float *V;             /* restricted */
+   float *A;             /* extended; halo untouched */
+   const float *exp_tau; /* device [nbatch] */
+   int self_interact;
+   pvhip_transfer transfer;
+} pvhip_lca_update_args;
+
+/* Host helper: exp_tau[b] = (float)exp(-dt[b]/(double)tau), the reference's
+ * double-precision exp then cast (HyPerLCAInternalStateBuffer.kpp:51). */
+void pvhip_lca_exp_tau(const double *dt, int nbatch, float tau, float *exp_tau_out);
+
+typedef struct {
+   pvhip_loc pre;  /* source layer; its extended activity is read */
+   pvhip_loc post; /* target layer; a restricted GSyn channel is written */
+   int nxp, nyp, nfp; /* post-perspective patch (PatchSize::calcPostPatchSize), nfp == pre.nf */
+   int num_kernels_x, num_kernels_y; /* shared data patches along x/y; kernels = kx*ky*post.nf */
+   /* Per-axis tables of PatchGeometry::getUnshrunkenStart (extended source
+    * coordinate of the patch start) and of Weights::calcDataIndexFromPatchIndex
+    * (data-patch x/y index), indexed by restricted target x / y.
+    * (src/components/PatchGeometry.cpp:179-279, src/components/Weights.cpp:205-231) */
+   const int32_t *start_x; /* [post.nx] */
+   const int32_t *start_y; /* [post.ny] */
+   const int32_t *kernel_x; /* [post.nx] */
+   const int32_t *kernel_y; /* [post.ny] */
+   float dt_factor;         /* HyPerDelivery::mDeltaTimeFactor; 1 for convolve */
+   const float *pre_data;   /* extended source activity */
+   const float *weights;    /* [kernels][nyp][nxp][nfp] post-perspective */
+   float *post_gsyn;        /* GSyn channel slice [nbatch][ny][nx][nf] */
+   int accumulate;          /* 1: GSyn += ..., 0: GSyn was reset (reference resetGSynBuffers) */
+   int arith;               /* pvhip_arith */
+   /* BF16 form: weights converted by pvhip_recv_post_bf16_weights, layout
+    * [kernels][Kpad] with Kpad = patch size rounded up to 32 (the shadow copy
+    * plays the role of the reference's cuDNN weight copy, Weights.cpp:333-341). */
+   const void *weights_lp;
+   /* Optional fused LCA epilogue (requires accumulate == 0, one channel,
+    * coeff 1): the gather result is the accumulated G of the update.  post_gsyn
+    * may then be NULL (GSyn is not materialised). */
+   const pvhip_lca_update_args *fused_update;
+} pvhip_recv_post_args;
+
+int pvhip_recv_post(const pvhip_recv_post_args *args, pvhip_stream stream);
+
+/* Low-precision weight shadow for the BF16 form; call after every weight change. */
+int pvhip_recv_post_bf16_weights(
+      const float *weights, void *weights_bf16, int num_kernels, int patch_size, pvhip_stream stream);
+size_t pvhip_recv_post_bf16_weights_bytes(int num_kernels, int patch_size);
+
+/* ------------------------------------------------------------------------ */
+/* Pre-perspective delivery: replaces CudaRecvPre / HyPerLayer_recv_pre      */
+/* (src/cudakernels/CudaRecvPre.cu:10-101); EXACT form is bit-identical to   */
+/* PresynapticPerspectiveConvolveDelivery::deliver with one thread           */
+/* (src/delivery/PresynapticPerspectiveConvolveDelivery.cpp:61-192).         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+   pvhip_loc pre;
+   pvhip_loc post;
+   int nxp, nyp, nfp; /* pre-perspective patch; nfp == post.nf */
+   int num_kernels_x, num_kernels_y;
+   /* Coverage tables built on the host from PatchGeometry (see
+    * pvhost_recv_pre_tables): for restricted post x, the pre extended columns
+    * whose (shrunken) patch covers x, ascending, with the weight column index;
+    * for (post y, patch row r), the pre extended rows whose patch row r lands
+    * on y, ascending, with the weight row index.  CSR layout. */
+   const int32_t *col_ptr; /* [post.nx + 1] */
+   const int32_t *col_xe;  /* pre extended x */
+   const int32_t *col_wx;  /* weight x inside the patch */
+   const int32_t *col_kx;  /* data-patch x index of that pre column */
+   const int32_t *row_ptr; /* [post.ny * nyp + 1], row (y, r) at y*nyp + r */
+   const int32_t *row_ye;
+   const int32_t *row_wy;
+   const int32_t *row_ky;
+   float dt_factor;
+   /* Sparse pre (sparseLayer = true): the active list of pvhip_active_compact
+    * and its position segments.  Dense pre: pos_start == NULL, pre_data used. */
+   const pvhip_sparse_entry *active; /* [nbatch][numPreExt] */
+   const uint32_t *pos_start;        /* [nbatch][nyExt*nxExt + 1] */
+   const float *pre_data;            /* [nbatch][numPreExt] (dense mode) */
+   const float *weights;             /* [kernels][nyp][nxp][nfp] pre-perspective */
+   float *post_gsyn;
+   int accumulate;
+   int arith; /* EXACT (ordered) or FP32 (same kernel, FMA) */
+} pvhip_recv_pre_args;
+
+int pvhip_recv_pre(const pvhip_recv_pre_args *args, pvhip_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Layer updates                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Fused K8+K6+K7 (see pvhip_lca_update_args). */
+int pvhip_lca_update(const pvhip_lca_update_args *args, pvhip_stream stream);
+
+/* GSynAccumulator (src/components/GSynAccumulator.kpp:21-42):
+ * out[i] = sum_ch coeff[ch]*gsyn[ch*n + i], starting from 0.0f. */
+int pvhip_gsyn_accumulate(
+      int num_neurons_across_batch,
+      int num_channels,
+      const float *channel_coeffs_host,
+      const float *gsyn,
+      float *out,
+      pvhip_stream stream);
+
+/* Non-LCA layer update: V = scale * accumulate(GSyn) (HyPerInternalStateBuffer +
+ * ErrScaleInternalStateBuffer.cpp:41-53; scale = errScale, 1 for ANNLayer),
+ * then A[kex] = transfer(V) (ANNActivityBuffer; HyPerActivityBuffer when the
+ * transfer is disabled). */
+typedef struct {
+   pvhip_loc loc;
+   int num_channels;
+   float channel_coeffs[PVHIP_MAX_CHANNELS];
+   const float *gsyn;
+   float *V;
+   float *A;
+   float scale;
+   int apply_scale; /* 0: V = G exactly (no multiply) */
+   pvhip_transfer transfer;
+} pvhip_layer_update_args;
+
+int pvhip_layer_update(const pvhip_layer_update_args *args, pvhip_stream stream);
+
+/* Transfer only: A[kex] = transfer(V[k]) (K7 / applyVThresh+applyAMax). */
+int pvhip_ann_activity(
+      const pvhip_loc *loc, const pvhip_transfer *transfer, const float *V, float *A,
+      pvhip_stream stream);
+
+/* IdentDelivery::deliver (src/delivery/IdentDelivery.cpp:99-165), dense form:
+ * GSyn[b][k] += A_ext[b][kex(k)]. */
+int pvhip_ident_deliver(
+      const pvhip_loc *loc, const float *pre_data, float *post_gsyn, int accumulate,
+      pvhip_stream stream);
+
+/* DataStore::updateActiveIndices (src/columns/DataStore.cpp:46-66): ascending
+ * scan of the extended buffer, entries {kex, a} for a != 0.0f, count per batch
+ * element written as int64.  pos_start[b][p] receives the first entry of each
+ * extended (x,y) position p and pos_start[b][nPos] the count, so that
+ * [pos_start[p], pos_start[p+1]) are the active features at p.  pos_start may
+ * be NULL.  chunk_scratch is caller-owned device memory of
+ * pvhip_active_compact_scratch_bytes(loc) bytes (no allocation inside). */
+int pvhip_active_compact(
+      const pvhip_loc *loc,
+      const float *A,
+      pvhip_sparse_entry *entries,
+      int64_t *num_active,
+      uint32_t *pos_start,
+      uint32_t *chunk_scratch,
+      pvhip_stream stream);
+size_t pvhip_active_compact_scratch_bytes(const pvhip_loc *loc);
+
+/* ------------------------------------------------------------------------ */
+/* Plasticity, normalisation, energy                                         */
+/* ------------------------------------------------------------------------ */
+
+/* HebbianUpdater::update_dW / updateInd_dW for shared weights
+ * (src/weightupdaters/HebbianUpdater.cpp:476-627): for every pre extended
+ * neuron with a_pre != 0, dW[kernel][patch item] += dWMax*a_pre*a_post and
+ * counts[item]++ over the shrunken patch.  Accumulation order per dW element:
+ * batch ascending, then pre position ascending (the reference's).  Per-axis
+ * geometry tables over the pre extended axes come from PatchGeometry. */
+typedef struct {
+   pvhip_loc pre, post;
+   int nxp, nyp, nfp; /* pre-perspective patch */
+   int num_kernels_x, num_kernels_y;
+   const int32_t *dim_x;       /* [nxExtPre] patch.nx */
+   const int32_t *dim_y;       /* [nyExtPre] patch.ny */
+   const int32_t *pstart_x;    /* [nxExtPre] first patch column kept (patch.offset x) */
+   const int32_t *pstart_y;    /* [nyExtPre] */
+   const int32_t *apost_x;     /* [nxExtPre] extended post x of the patch (aPostOffset) */
+   const int32_t *apost_y;     /* [nyExtPre] */
+   float dw_max;
+   const float *pre_data;  /* extended [nbatch][numPreExt] */
+   const float *post_data; /* extended [nbatch][numPostExt] */
+   float *dw;              /* [kernels][nyp][nxp][nfp], accumulated into (caller clears) */
+   int64_t *counts;        /* same shape, or NULL when normalizeDw is false */
+   int arith;              /* EXACT (reference order) or FP32/BF16 (MFMA correlation) */
+} pvhip_hebbian_args;
+
+int pvhip_hebbian_dw(const pvhip_hebbian_args *args, pvhip_stream stream);
+
+/* HebbianUpdater::normalize_dW (HebbianUpdater.cpp:773-807): dw /= count, 0
+ * where count == 0; then updateWeights (:822-833): w += dw. */
+int pvhip_hebbian_apply(
+      float *weights, float *dw, const int64_t *counts, size_t n, int normalize_dw,
+      pvhip_stream stream);
+
+/* NormalizeL2 per data patch (src/normalizers/NormalizeL2.cpp:36-120) with the
+ * sum of squares and scale in FP64: w *= strength / sqrt(sum w^2). Patches
+ * whose sum of squares is <= min_l2 are left unchanged. */
+int pvhip_normalize_l2_f64(
+      float *weights, int num_patches, int patch_size, double strength, double min_l2,
+      pvhip_stream stream);
+
+/* CostFunctionSum::calculateSumNoMask (src/probes/CostFunctionSum.hpp:87-104)
+ * in FP64 over the restricted part of a (possibly extended) buffer:
+ *   kind 0: L2  sum x^2, kind 1: L1 sum |x|, kind 2: firm threshold cost
+ *   (src/probes/CostFunctions.hpp:18-50, params p0 = VThresh, p1 = VWidth),
+ *   kind 3: L0 count of |x| > p0.
+ * Writes out[b] (device double[nbatch]). */
+int pvhip_cost_sum_f64(
+      const pvhip_loc *loc, const float *buffer, int extended, int kind, double p0, double p1,
+      double *out, pvhip_stream stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* PVHIP_H_ */

@@ -1,0 +1,66 @@
+/*
+ * pvhost.h -- C-ABI of libpvhost, the C++ host side above libpvhip: patch
+ * geometry and the device tables of the deliveries, weight transposition,
+ * transfer-function vertices, and the params-driven LCA column runner.
+ *
+ * The reference computes these tables on the host as well
+ * (PatchGeometry::setPatchGeometry, src/components/PatchGeometry.cpp:179-279;
+ * PostsynapticPerspectiveGPUDelivery::initializeRecvKernelArgs,
+ * src/delivery/PostsynapticPerspectiveGPUDelivery.cpp:111-237;
+ * PresynapticPerspectiveGPUDelivery::initializeRecvKernelArgs,
+ * src/delivery/PresynapticPerspectiveGPUDelivery.cpp:120-196).
+ * All functions return 0 on success and a negative value on invalid input.
+ */
+#ifndef PVHOST_H_
+#define PVHOST_H_
+
+#include "pvhip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* requiredConvolveMargin (src/utils/requiredConvolveMargin.hpp:14-40). */
+int pvhost_required_margin(int n_pre, int n_post, int patch_size);
+/* PatchSize::calcPostPatchSize (src/components/PatchSize.cpp:98-130). */
+int pvhost_post_patch_size(int pre_patch_size, int n_pre, int n_post);
+/* Shared data patches along x, y (PatchGeometry.cpp:53-55). */
+int pvhost_num_kernels(const pvhip_loc *pre, const pvhip_loc *post, int *nkx, int *nky);
+
+/* PatchGeometry::calcPatchData over one axis of the pre extended space
+ * (axis 0 = x, 1 = y); arrays sized nxExt / nyExt of pre; any may be NULL. */
+int pvhost_patch_axis(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int nyp, int axis,
+      int32_t *dim, int32_t *patch_start, int32_t *post_start_res, int32_t *post_start_ext,
+      int32_t *unshrunken_start);
+
+/* Tables of pvhip_recv_post_args for a post-perspective delivery from
+ * `source` (pre layer) into `target` (post layer) with post-perspective patch
+ * nxp x nyp: start_x/kernel_x [target.nx], start_y/kernel_y [target.ny].
+ * Returns a negative value if a patch would reach outside the source's
+ * extended buffer (halo smaller than the required margin). */
+int pvhost_recv_post_tables(const pvhip_loc *source, const pvhip_loc *target, int nxp, int nyp,
+      int32_t *start_x, int32_t *start_y, int32_t *kernel_x, int32_t *kernel_y, int *nkx, int *nky);
+
+/* Coverage tables of pvhip_recv_pre_args (CSR).  Call once with NULL arrays
+ * to get *ncol and *nrow, then with arrays of those sizes (col_ptr has
+ * post.nx + 1 entries, row_ptr post.ny*nyp + 1). */
+int pvhost_recv_pre_tables(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int nyp,
+      int32_t *col_ptr, int32_t *col_xe, int32_t *col_wx, int32_t *col_kx, int *ncol,
+      int32_t *row_ptr, int32_t *row_ye, int32_t *row_wy, int32_t *row_ky, int *nrow);
+
+/* ANNActivityBuffer::setVertices + setSlopes (src/components/ANNActivityBuffer.cpp:227-397)
+ * filled into a transfer descriptor; mode selects VTHRESH or VERTICES. */
+int pvhost_make_transfer(float VThresh, float AMin, float AMax, float AShift, float VWidth,
+      int mode, pvhip_transfer *out);
+
+/* TransposeWeights::transposeShared (src/utils/TransposeWeights.cpp:56-113):
+ * post-perspective weights of a shared-weight connection pre -> post with
+ * pre-perspective patch nxp x nyp x nfp. */
+int pvhost_transpose_weights(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int nyp,
+      int nfp, const float *pre_weights, float *post_weights);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PVHOST_H_ */

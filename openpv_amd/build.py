@@ -1,0 +1,72 @@
+"""Builds libpvhip.so (HIP, gfx950) and libpvhost.so (C++ host) in-tree.
+
+    python -m openpv_amd.build          # both libraries
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+INCLUDE = os.path.join(ROOT, "include")
+LIBDIR = os.path.join(HERE, "lib")
+CSRC = os.path.join(HERE, "csrc")
+HOSTSRC = os.path.join(HERE, "host")
+
+HIP_SOURCES = ["device.hip", "layer.hip", "recv_post.hip", "recv_pre.hip", "hebbian.hip"]
+HOST_SOURCES = ["geometry.cpp"]
+
+# -ffp-contract=off keeps mul and add separately rounded in the EXACT kernels
+# (FMA and MFMA paths call the builtins explicitly).  Packed FP32 instructions
+# are disabled for all device code.
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found; libpvhip cannot be built")
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("build failed:\n%s\n%s" % (" ".join(cmd), res.stdout))
+
+
+def build_hip(verbose=False):
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libpvhip.so")
+    tmp = out + ".tmp"
+    cmd = [_hipcc()] + HIPCC_FLAGS + ["-I" + INCLUDE, "-o", tmp] + \
+        [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    _run(cmd, verbose)
+    os.replace(tmp, out)
+    return out
+
+
+def build_host(verbose=False):
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libpvhost.so")
+    tmp = out + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + INCLUDE,
+           "-o", tmp] + [os.path.join(HOSTSRC, s) for s in HOST_SOURCES]
+    _run(cmd, verbose)
+    os.replace(tmp, out)
+    return out
+
+
+def build(verbose=False):
+    return build_hip(verbose), build_host(verbose)
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,522 @@
+// Layer-update kernels of libpvhip: the LCA V/A update (K6+K7+K8 fused), the
+// ANN/ErrScale updates, IdentDelivery, GSyn accumulation and the ordered
+// active-index compaction (DataStore::updateActiveIndices on the device).
+//
+// Every kernel here is elementwise or a scan and HBM-bound.  All float
+// arithmetic follows the reference's operation order with one rounding per
+// operation (the library is built with -ffp-contract=off).
+#include "pvhip_internal.hpp"
+
+namespace pvhip {
+
+int make_lca_params(const pvhip_lca_update_args &a, LCAParams *p) {
+   PVHIP_REQUIRE(loc_valid(a.loc), "lca_update: invalid layer loc");
+   PVHIP_REQUIRE(
+         a.num_channels >= 0 && a.num_channels <= PVHIP_MAX_CHANNELS,
+         "lca_update: num_channels %d outside [0,%d]", a.num_channels, PVHIP_MAX_CHANNELS);
+   PVHIP_REQUIRE(a.V && a.A && a.exp_tau, "lca_update: NULL V, A or exp_tau");
+   PVHIP_REQUIRE(a.num_channels == 0 || a.gsyn, "lca_update: NULL gsyn");
+   p->loc         = geo(a.loc);
+   p->numChannels = a.num_channels;
+   for (int i = 0; i < PVHIP_MAX_CHANNELS; i++) {
+      p->coeffs[i] = a.channel_coeffs[i];
+   }
+   p->gsyn         = a.gsyn;
+   p->V            = a.V;
+   p->A            = a.A;
+   p->expTau       = a.exp_tau;
+   p->selfInteract = a.self_interact ? 1.0f : 0.0f;
+   return make_transfer(a.transfer, &p->transfer);
+}
+
+namespace {
+
+// G = sum_ch coeff[ch]*GSyn[ch] starting from 0.0f (GSynAccumulator.kpp:36-40).
+__device__ __forceinline__ float accumulate_channels(
+      int numChannels, const float *coeffs, const float *gsyn, size_t n, size_t index) {
+   float value = 0.0f;
+   for (int ch = 0; ch < numChannels; ch++) {
+      float prod = coeffs[ch] * gsyn[(size_t)ch * n + index];
+      value      = value + prod;
+   }
+   return value;
+}
+
+__global__ void __launch_bounds__(256) lca_update_kernel(LCAParams p) {
+   size_t const n     = (size_t)p.loc.nbatch * p.loc.numRes();
+   size_t const index = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (index >= n) {
+      return;
+   }
+   int const numNeurons = p.loc.numRes();
+   int const b          = (int)(index / numNeurons);
+   int const k          = (int)(index % numNeurons);
+   float G              = accumulate_channels(p.numChannels, p.coeffs, p.gsyn, n, index);
+   lca_update_one(p, b, k, G);
+}
+
+struct LayerParams {
+   Geo loc;
+   int numChannels;
+   float coeffs[PVHIP_MAX_CHANNELS];
+   const float *gsyn;
+   float *V;
+   float *A;
+   float scale;
+   int applyScale;
+   Transfer transfer;
+};
+
+__global__ void __launch_bounds__(256) layer_update_kernel(LayerParams p) {
+   size_t const n     = (size_t)p.loc.nbatch * p.loc.numRes();
+   size_t const index = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (index >= n) {
+      return;
+   }
+   int const numNeurons = p.loc.numRes();
+   int const b          = (int)(index / numNeurons);
+   int const k          = (int)(index % numNeurons);
+   float V              = accumulate_channels(p.numChannels, p.coeffs, p.gsyn, n, index);
+   if (p.applyScale) {
+      V = V * p.scale; // ErrScaleInternalStateBuffer.cpp:51
+   }
+   p.V[index] = V;
+   if (p.A) {
+      p.A[(size_t)b * p.loc.numExt() + p.loc.kex(k)] = apply_transfer(p.transfer, V);
+   }
+}
+
+__global__ void __launch_bounds__(256)
+      ann_activity_kernel(Geo loc, Transfer t, const float *V, float *A) {
+   size_t const n     = (size_t)loc.nbatch * loc.numRes();
+   size_t const index = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (index >= n) {
+      return;
+   }
+   int const numNeurons = loc.numRes();
+   int const b          = (int)(index / numNeurons);
+   int const k          = (int)(index % numNeurons);
+   A[(size_t)b * loc.numExt() + loc.kex(k)] = apply_transfer(t, V[index]);
+}
+
+__global__ void __launch_bounds__(256) gsyn_accumulate_kernel(
+      int n, int numChannels, Transfer coeffHolder, const float *gsyn, float *out) {
+   int const index = blockIdx.x * blockDim.x + threadIdx.x;
+   if (index >= n) {
+      return;
+   }
+   out[index] = accumulate_channels(numChannels, coeffHolder.vV, gsyn, (size_t)n, (size_t)index);
+}
+
+// IdentDelivery dense branch (IdentDelivery.cpp:146-162): GSyn += A restricted.
+__global__ void __launch_bounds__(256)
+      ident_deliver_kernel(Geo loc, const float *pre, float *gsyn, int accumulate) {
+   size_t const n     = (size_t)loc.nbatch * loc.numRes();
+   size_t const index = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (index >= n) {
+      return;
+   }
+   int const numNeurons = loc.numRes();
+   int const b          = (int)(index / numNeurons);
+   int const k          = (int)(index % numNeurons);
+   float a              = pre[(size_t)b * loc.numExt() + loc.kex(k)];
+   float base           = accumulate ? gsyn[index] : 0.0f;
+   gsyn[index]          = base + a;
+}
+
+// ---------------------------------------------------------------- compaction
+//
+// Ordered stream compaction of the extended buffer of each batch element
+// (DataStore.cpp:55-62).  Chunks of kChunk consecutive elements per
+// workgroup; pass 1 counts, pass 2 scans the chunk counts of each batch
+// element, pass 3 writes entries and the per-position segment starts.
+
+constexpr int kCompactThreads = 256;
+constexpr int kCompactIters   = 16;
+constexpr int kChunk          = kCompactThreads * kCompactIters;
+
+__global__ void __launch_bounds__(kCompactThreads)
+      compact_count_kernel(const float *A, int numExt, uint32_t *chunkCount, int chunksPerBatch) {
+   int const b         = blockIdx.y;
+   int const chunk     = blockIdx.x;
+   float const *Ab     = A + (size_t)b * numExt;
+   int const base      = chunk * kChunk;
+   int count           = 0;
+   for (int it = 0; it < kCompactIters; it++) {
+      int e = base + it * kCompactThreads + threadIdx.x;
+      if (e < numExt && Ab[e] != 0.0f) {
+         count++;
+      }
+   }
+   // Workgroup reduction.
+   __shared__ int red[kCompactThreads / kWave];
+   for (int off = kWave / 2; off > 0; off >>= 1) {
+      count += __shfl_down(count, off, kWave);
+   }
+   int lane = threadIdx.x % kWave;
+   int wave = threadIdx.x / kWave;
+   if (lane == 0) {
+      red[wave] = count;
+   }
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      int total = 0;
+      for (int w = 0; w < kCompactThreads / kWave; w++) {
+         total += red[w];
+      }
+      chunkCount[(size_t)b * chunksPerBatch + chunk] = (uint32_t)total;
+   }
+}
+
+// One workgroup per batch element: exclusive scan of its chunk counts, in place.
+__global__ void __launch_bounds__(1024)
+      compact_scan_kernel(uint32_t *chunkCount, int chunksPerBatch, int64_t *numActive) {
+   int const b        = blockIdx.x;
+   uint32_t *c        = chunkCount + (size_t)b * chunksPerBatch;
+   __shared__ uint32_t partial[1024];
+   int const per      = (chunksPerBatch + blockDim.x - 1) / blockDim.x;
+   int const begin    = threadIdx.x * per;
+   int const end      = min(begin + per, chunksPerBatch);
+   uint32_t local     = 0;
+   for (int i = begin; i < end; i++) {
+      local += c[i];
+   }
+   partial[threadIdx.x] = local;
+   __syncthreads();
+   // Hillis-Steele inclusive scan over the per-thread sums.
+   for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+      uint32_t v = threadIdx.x >= (unsigned)off ? partial[threadIdx.x - off] : 0u;
+      __syncthreads();
+      partial[threadIdx.x] += v;
+      __syncthreads();
+   }
+   uint32_t run = threadIdx.x > 0 ? partial[threadIdx.x - 1] : 0u;
+   for (int i = begin; i < end; i++) {
+      uint32_t v = c[i];
+      c[i]       = run;
+      run += v;
+   }
+   if (threadIdx.x == blockDim.x - 1) {
+      numActive[b] = (int64_t)partial[blockDim.x - 1];
+   }
+}
+
+__global__ void __launch_bounds__(kCompactThreads) compact_write_kernel(
+      const float *A, int numExt, int nf, const uint32_t *chunkOffset, int chunksPerBatch,
+      pvhip_sparse_entry *entries, uint32_t *posStart, int numPos, const int64_t *numActive) {
+   int const b      = blockIdx.y;
+   int const chunk  = blockIdx.x;
+   float const *Ab  = A + (size_t)b * numExt;
+   pvhip_sparse_entry *Eb = entries + (size_t)b * numExt;
+   uint32_t *Pb     = posStart ? posStart + (size_t)b * (numPos + 1) : nullptr;
+   int const base   = chunk * kChunk;
+   uint32_t run     = chunkOffset[(size_t)b * chunksPerBatch + chunk];
+   int const lane   = threadIdx.x % kWave;
+   int const wave   = threadIdx.x / kWave;
+   __shared__ uint32_t waveCount[kCompactThreads / kWave];
+   for (int it = 0; it < kCompactIters; it++) {
+      int e     = base + it * kCompactThreads + threadIdx.x;
+      float a   = e < numExt ? Ab[e] : 0.0f;
+      bool flag = e < numExt && a != 0.0f;
+      uint64_t mask   = __ballot(flag);
+      uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+      if (lane == 0) {
+         waveCount[wave] = (uint32_t)__popcll(mask);
+      }
+      __syncthreads();
+      uint32_t waveBase = 0;
+      uint32_t total    = 0;
+      for (int w = 0; w < kCompactThreads / kWave; w++) {
+         uint32_t c = waveCount[w];
+         if (w < wave) {
+            waveBase += c;
+         }
+         total += c;
+      }
+      uint32_t pos = run + waveBase + before;
+      if (flag) {
+         Eb[pos].index = (uint32_t)e;
+         Eb[pos].value = a;
+      }
+      if (Pb && e < numExt && (e % nf) == 0) {
+         Pb[e / nf] = pos;
+      }
+      run += total;
+      __syncthreads();
+   }
+   if (Pb && chunk == 0 && threadIdx.x == 0) {
+      Pb[numPos] = (uint32_t)numActive[b];
+   }
+}
+
+// ---------------------------------------------------------------- FP64 probes
+
+__global__ void __launch_bounds__(1024) cost_sum_kernel(
+      Geo loc, const float *buf, int extended, int kind, double p0, double p1, double *out) {
+   int const b           = blockIdx.x;
+   int const numNeurons  = loc.numRes();
+   size_t const stride   = extended ? (size_t)loc.numExt() : (size_t)numNeurons;
+   float const *B        = buf + (size_t)b * stride;
+   double const tpw      = p0 + p1;
+   double const maxCost  = 0.5 * tpw;
+   double const quad     = 0.5 / tpw;
+   double sum            = 0.0;
+   for (int k = threadIdx.x; k < numNeurons; k += blockDim.x) {
+      double v = (double)B[extended ? loc.kex(k) : k];
+      double c;
+      if (kind == 0) {
+         c = v * v;
+      }
+      else if (kind == 1) {
+         c = fabs(v);
+      }
+      else if (kind == 2) {
+         double ax = fabs(v);
+         c         = ax < tpw ? ax * (1.0 - quad * ax) : maxCost;
+      }
+      else {
+         c = fabs(v) > p0 ? 1.0 : 0.0;
+      }
+      sum += c;
+   }
+   __shared__ double red[1024 / kWave];
+   for (int off = kWave / 2; off > 0; off >>= 1) {
+      sum += __shfl_down(sum, off, kWave);
+   }
+   if (threadIdx.x % kWave == 0) {
+      red[threadIdx.x / kWave] = sum;
+   }
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      double total = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); w++) {
+         total += red[w];
+      }
+      out[b] = total;
+   }
+}
+
+__global__ void __launch_bounds__(256) normalize_l2_f64_kernel(
+      float *w, int patchSize, double strength, double minL2) {
+   float *data = w + (size_t)blockIdx.x * patchSize;
+   double s    = 0.0;
+   for (int k = threadIdx.x; k < patchSize; k += blockDim.x) {
+      double x = (double)data[k];
+      s += x * x;
+   }
+   __shared__ double red[256 / kWave];
+   __shared__ double mult;
+   for (int off = kWave / 2; off > 0; off >>= 1) {
+      s += __shfl_down(s, off, kWave);
+   }
+   if (threadIdx.x % kWave == 0) {
+      red[threadIdx.x / kWave] = s;
+   }
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      double total = 0.0;
+      for (int i = 0; i < (int)(blockDim.x / kWave); i++) {
+         total += red[i];
+      }
+      mult = total <= minL2 ? 1.0 : strength / sqrt(total);
+   }
+   __syncthreads();
+   double m = mult;
+   for (int k = threadIdx.x; k < patchSize; k += blockDim.x) {
+      data[k] = (float)((double)data[k] * m);
+   }
+}
+
+__global__ void __launch_bounds__(256) hebbian_apply_kernel(
+      float *w, float *dw, const int64_t *counts, size_t n, int normalizeDw) {
+   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= n) {
+      return;
+   }
+   float d = dw[i];
+   if (normalizeDw) { // HebbianUpdater.cpp:794-800
+      long divisor = (long)counts[i];
+      if (divisor != 0) {
+         d = d / (float)divisor;
+      }
+      else {
+         d = 0.0f;
+      }
+      dw[i] = d;
+   }
+   w[i] = w[i] + d; // HebbianUpdater.cpp:828-830
+}
+
+inline unsigned blocks_for(size_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
+
+} // namespace
+} // namespace pvhip
+
+using namespace pvhip;
+
+extern "C" {
+
+int pvhip_lca_update(const pvhip_lca_update_args *args, pvhip_stream stream) {
+   PVHIP_REQUIRE(args != nullptr, "pvhip_lca_update: NULL args");
+   LCAParams p;
+   int st = make_lca_params(*args, &p);
+   if (st) {
+      return st;
+   }
+   size_t n = (size_t)p.loc.nbatch * p.loc.numRes();
+   hipLaunchKernelGGL(lca_update_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (hipStream_t)stream, p);
+   PVHIP_LAUNCHED("lca_update");
+   return PVHIP_OK;
+}
+
+int pvhip_gsyn_accumulate(
+      int n, int num_channels, const float *coeffs_host, const float *gsyn, float *out,
+      pvhip_stream stream) {
+   PVHIP_REQUIRE(n >= 0, "gsyn_accumulate: negative n");
+   PVHIP_REQUIRE(
+         num_channels >= 0 && num_channels <= PVHIP_MAX_CHANNELS, "gsyn_accumulate: bad channel count");
+   PVHIP_REQUIRE(out && (gsyn || num_channels == 0), "gsyn_accumulate: NULL buffer");
+   if (n == 0) {
+      return PVHIP_OK;
+   }
+   Transfer holder{};
+   for (int i = 0; i < num_channels; i++) {
+      holder.vV[i] = coeffs_host[i];
+   }
+   hipLaunchKernelGGL(gsyn_accumulate_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (hipStream_t)stream, n, num_channels, holder, gsyn, out);
+   PVHIP_LAUNCHED("gsyn_accumulate");
+   return PVHIP_OK;
+}
+
+int pvhip_layer_update(const pvhip_layer_update_args *a, pvhip_stream stream) {
+   PVHIP_REQUIRE(a != nullptr, "pvhip_layer_update: NULL args");
+   PVHIP_REQUIRE(loc_valid(a->loc), "layer_update: invalid loc");
+   PVHIP_REQUIRE(a->num_channels >= 0 && a->num_channels <= PVHIP_MAX_CHANNELS, "layer_update: bad channel count");
+   PVHIP_REQUIRE(a->V != nullptr, "layer_update: NULL V");
+   PVHIP_REQUIRE(a->num_channels == 0 || a->gsyn, "layer_update: NULL gsyn");
+   LayerParams p;
+   p.loc         = geo(a->loc);
+   p.numChannels = a->num_channels;
+   for (int i = 0; i < PVHIP_MAX_CHANNELS; i++) {
+      p.coeffs[i] = a->channel_coeffs[i];
+   }
+   p.gsyn       = a->gsyn;
+   p.V          = a->V;
+   p.A          = a->A;
+   p.scale      = a->scale;
+   p.applyScale = a->apply_scale;
+   int st       = make_transfer(a->transfer, &p.transfer);
+   if (st) {
+      return st;
+   }
+   size_t n = (size_t)p.loc.nbatch * p.loc.numRes();
+   hipLaunchKernelGGL(layer_update_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (hipStream_t)stream, p);
+   PVHIP_LAUNCHED("layer_update");
+   return PVHIP_OK;
+}
+
+int pvhip_ann_activity(
+      const pvhip_loc *loc, const pvhip_transfer *transfer, const float *V, float *A,
+      pvhip_stream stream) {
+   PVHIP_REQUIRE(loc && transfer && V && A, "ann_activity: NULL argument");
+   PVHIP_REQUIRE(loc_valid(*loc), "ann_activity: invalid loc");
+   Transfer t;
+   int st = make_transfer(*transfer, &t);
+   if (st) {
+      return st;
+   }
+   Geo g    = geo(*loc);
+   size_t n = (size_t)g.nbatch * g.numRes();
+   hipLaunchKernelGGL(ann_activity_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (hipStream_t)stream, g, t, V, A);
+   PVHIP_LAUNCHED("ann_activity");
+   return PVHIP_OK;
+}
+
+int pvhip_ident_deliver(
+      const pvhip_loc *loc, const float *pre_data, float *post_gsyn, int accumulate,
+      pvhip_stream stream) {
+   PVHIP_REQUIRE(loc && pre_data && post_gsyn, "ident_deliver: NULL argument");
+   PVHIP_REQUIRE(loc_valid(*loc), "ident_deliver: invalid loc");
+   Geo g    = geo(*loc);
+   size_t n = (size_t)g.nbatch * g.numRes();
+   hipLaunchKernelGGL(ident_deliver_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (hipStream_t)stream, g, pre_data, post_gsyn, accumulate);
+   PVHIP_LAUNCHED("ident_deliver");
+   return PVHIP_OK;
+}
+
+int pvhip_active_compact(
+      const pvhip_loc *loc, const float *A, pvhip_sparse_entry *entries, int64_t *num_active,
+      uint32_t *pos_start, uint32_t *chunk_scratch, pvhip_stream stream) {
+   PVHIP_REQUIRE(loc && A && entries && num_active, "active_compact: NULL argument");
+   PVHIP_REQUIRE(loc_valid(*loc), "active_compact: invalid loc");
+   Geo g              = geo(*loc);
+   int const numExt   = g.numExt();
+   int const chunks   = (numExt + kChunk - 1) / kChunk;
+   int const numPos   = g.nxExt() * g.nyExt();
+   PVHIP_REQUIRE(g.nbatch <= 65535, "active_compact: nbatch %d too large", g.nbatch);
+   PVHIP_REQUIRE(chunk_scratch != nullptr, "active_compact: NULL chunk_scratch");
+   uint32_t *scratch = chunk_scratch;
+   dim3 grid(chunks, g.nbatch);
+   hipLaunchKernelGGL(compact_count_kernel, grid, dim3(kCompactThreads), 0, (hipStream_t)stream,
+         A, numExt, scratch, chunks);
+   PVHIP_LAUNCHED("compact_count");
+   hipLaunchKernelGGL(compact_scan_kernel, dim3(g.nbatch), dim3(1024), 0, (hipStream_t)stream,
+         scratch, chunks, num_active);
+   PVHIP_LAUNCHED("compact_scan");
+   hipLaunchKernelGGL(compact_write_kernel, grid, dim3(kCompactThreads), 0, (hipStream_t)stream,
+         A, numExt, g.nf, scratch, chunks, entries, pos_start, numPos, num_active);
+   PVHIP_LAUNCHED("compact_write");
+   return PVHIP_OK;
+}
+
+int pvhip_cost_sum_f64(
+      const pvhip_loc *loc, const float *buffer, int extended, int kind, double p0, double p1,
+      double *out, pvhip_stream stream) {
+   PVHIP_REQUIRE(loc && buffer && out, "cost_sum_f64: NULL argument");
+   PVHIP_REQUIRE(loc_valid(*loc), "cost_sum_f64: invalid loc");
+   PVHIP_REQUIRE(kind >= 0 && kind <= 3, "cost_sum_f64: unknown kind %d", kind);
+   Geo g = geo(*loc);
+   hipLaunchKernelGGL(cost_sum_kernel, dim3(g.nbatch), dim3(1024), 0, (hipStream_t)stream, g,
+         buffer, extended, kind, p0, p1, out);
+   PVHIP_LAUNCHED("cost_sum_f64");
+   return PVHIP_OK;
+}
+
+int pvhip_normalize_l2_f64(
+      float *weights, int num_patches, int patch_size, double strength, double min_l2,
+      pvhip_stream stream) {
+   PVHIP_REQUIRE(weights && num_patches > 0 && patch_size > 0, "normalize_l2_f64: bad argument");
+   hipLaunchKernelGGL(normalize_l2_f64_kernel, dim3(num_patches), dim3(256), 0,
+         (hipStream_t)stream, weights, patch_size, strength, min_l2);
+   PVHIP_LAUNCHED("normalize_l2_f64");
+   return PVHIP_OK;
+}
+
+int pvhip_hebbian_apply(
+      float *weights, float *dw, const int64_t *counts, size_t n, int normalize_dw,
+      pvhip_stream stream) {
+   PVHIP_REQUIRE(weights && dw, "hebbian_apply: NULL buffer");
+   PVHIP_REQUIRE(!normalize_dw || counts, "hebbian_apply: normalize_dw needs counts");
+   if (n == 0) {
+      return PVHIP_OK;
+   }
+   hipLaunchKernelGGL(hebbian_apply_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (hipStream_t)stream, weights, dw, counts, n, normalize_dw);
+   PVHIP_LAUNCHED("hebbian_apply");
+   return PVHIP_OK;
+}
+
+size_t pvhip_active_compact_scratch_bytes(const pvhip_loc *loc) {
+   if (!loc || !loc_valid(*loc)) {
+      return 0;
+   }
+   Geo g = geo(*loc);
+   return (size_t)g.nbatch * (size_t)((g.numExt() + kChunk - 1) / kChunk) * sizeof(uint32_t);
+}
+
+} // extern "C"

@@ -1,0 +1,198 @@
+// Internal helpers shared by the libpvhip translation units (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "pvhip.h"
+
+namespace pvhip {
+
+// Records the message for pvhip_last_error() and returns `code`.
+int set_error(int code, const char *fmt, ...);
+void clear_error();
+
+#define PVHIP_TRY(call)                                                                     \
+   do {                                                                                    \
+      hipError_t e_ = (call);                                                              \
+      if (e_ != hipSuccess) {                                                              \
+         return ::pvhip::set_error(                                                        \
+               PVHIP_ERR_RUNTIME, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
+               __FILE__, __LINE__);                                                        \
+      }                                                                                    \
+   } while (0)
+
+#define PVHIP_LAUNCHED(name)                                                                \
+   do {                                                                                    \
+      hipError_t e_ = hipGetLastError();                                                   \
+      if (e_ != hipSuccess) {                                                              \
+         return ::pvhip::set_error(                                                        \
+               PVHIP_ERR_RUNTIME, "%s launch failed: %s", name, hipGetErrorString(e_));    \
+      }                                                                                    \
+   } while (0)
+
+#define PVHIP_REQUIRE(cond, ...)                                                            \
+   do {                                                                                    \
+      if (!(cond)) {                                                                       \
+         return ::pvhip::set_error(PVHIP_ERR_INVALID, __VA_ARGS__);                        \
+      }                                                                                    \
+   } while (0)
+
+// Geometry of a layer buffer, usable on host and device.
+struct Geo {
+   int nbatch, nx, ny, nf, lt, rt, dn, up;
+   __host__ __device__ int nxExt() const { return nx + lt + rt; }
+   __host__ __device__ int nyExt() const { return ny + dn + up; }
+   __host__ __device__ int numRes() const { return nx * ny * nf; }
+   __host__ __device__ int numExt() const { return nxExt() * nyExt() * nf; }
+   // kIndexExtended (src/utils/conversions.hpp:401-407): x offset lt, y offset up.
+   __host__ __device__ int kex(int k) const {
+      int kf = k % nf;
+      int kx = (k / nf) % nx;
+      int ky = k / (nx * nf);
+      return kf + ((kx + lt) + (ky + up) * nxExt()) * nf;
+   }
+};
+
+inline Geo geo(const pvhip_loc &l) {
+   return Geo{l.nbatch, l.nx, l.ny, l.nf, l.halo.lt, l.halo.rt, l.halo.dn, l.halo.up};
+}
+
+inline bool loc_valid(const pvhip_loc &l) {
+   return l.nbatch > 0 && l.nx > 0 && l.ny > 0 && l.nf > 0 && l.halo.lt >= 0 && l.halo.rt >= 0
+          && l.halo.dn >= 0 && l.halo.up >= 0
+          && (double)l.nbatch * (l.nx + l.halo.lt + l.halo.rt) * (l.ny + l.halo.dn + l.halo.up)
+                         * l.nf
+                   < 2147483647.0;
+}
+
+// Transfer function parameters passed by value to kernels.
+struct Transfer {
+   int mode;
+   float VThresh, AMin, AMax, AShift, VWidth;
+   int numVertices;
+   float vV[PVHIP_MAX_VERTICES];
+   float vA[PVHIP_MAX_VERTICES];
+   float slopes[PVHIP_MAX_VERTICES + 1];
+};
+
+inline int make_transfer(const pvhip_transfer &t, Transfer *out) {
+   out->mode    = t.mode;
+   out->VThresh = t.VThresh;
+   out->AMin    = t.AMin;
+   out->AMax    = t.AMax;
+   out->AShift  = t.AShift;
+   out->VWidth  = t.VWidth;
+   out->numVertices = t.num_vertices;
+   if (t.mode == PVHIP_TRANSFER_VERTICES) {
+      PVHIP_REQUIRE(
+            t.num_vertices >= 1 && t.num_vertices <= PVHIP_MAX_VERTICES,
+            "transfer: num_vertices %d outside [1, %d]", t.num_vertices, PVHIP_MAX_VERTICES);
+   }
+   else {
+      PVHIP_REQUIRE(t.mode == PVHIP_TRANSFER_VTHRESH, "transfer: unknown mode %d", t.mode);
+   }
+   for (int i = 0; i < PVHIP_MAX_VERTICES; i++) {
+      out->vV[i] = t.vertices_v[i];
+      out->vA[i] = t.vertices_a[i];
+   }
+   for (int i = 0; i <= PVHIP_MAX_VERTICES; i++) {
+      out->slopes[i] = t.slopes[i];
+   }
+   return PVHIP_OK;
+}
+
+// A = transfer(V) with the reference's exact float operation order.
+//  VTHRESH: HyPerActivityBuffer copy, applyVThresh, applyAMax
+//           (src/components/ANNActivityBuffer.cpp:490-556).
+//  VERTICES: applyVerticesANNActivityBufferOnGPU (src/components/ANNActivityBuffer.kpp:45-76).
+__device__ __forceinline__ float apply_transfer(const Transfer &t, float V) {
+   if (t.mode == PVHIP_TRANSFER_VTHRESH) {
+      float a = V;
+      if (t.VThresh > -FLT_MAX) {
+         if (V < t.VThresh) {
+            a = t.AMin;
+         }
+         else if (V < t.VThresh + t.VWidth) {
+            float span = t.VThresh + t.VWidth;
+            span       = span - t.AShift;
+            span       = span - t.AMin;
+            float num  = span * (V - t.VThresh);
+            a          = t.AMin + num / t.VWidth;
+         }
+         else {
+            a = a - t.AShift;
+         }
+      }
+      if (t.AMax < FLT_MAX) {
+         if (a > t.AMax) {
+            a = t.AMax;
+         }
+      }
+      return a;
+   }
+   int const last = t.numVertices - 1;
+   float activity = 0.0f;
+   if (V < t.vV[0]) {
+      activity = t.vA[0] + t.slopes[0] * (V - t.vV[0]);
+   }
+   else if (V >= t.vV[last]) {
+      activity = t.vA[last] + t.slopes[t.numVertices] * (V - t.vV[last]);
+   }
+   else {
+      for (int v = 0; v < last; v++) {
+         if (V < t.vV[v]) {
+            break;
+         }
+         if (V == t.vV[v]) {
+            activity = t.vA[v];
+         }
+         else if (V > t.vV[v] && V < t.vV[v + 1]) {
+            activity = t.vA[v] + t.slopes[v + 1] * (V - t.vV[v]);
+         }
+      }
+   }
+   return activity;
+}
+
+// Parameters of the LCA update passed by value (see pvhip_lca_update_args).
+struct LCAParams {
+   Geo loc;
+   int numChannels;
+   float coeffs[PVHIP_MAX_CHANNELS];
+   const float *gsyn;
+   float *V;
+   float *A;
+   const float *expTau;
+   float selfInteract;
+   Transfer transfer;
+};
+
+int make_lca_params(const pvhip_lca_update_args &a, LCAParams *p);
+
+// One neuron of the fused K6+K7 update given its accumulated input G
+// (HyPerLCAInternalStateBuffer.kpp:51-59, then the transfer).
+__device__ __forceinline__ void lca_update_one(const LCAParams &p, int b, int k, float G) {
+   float const e   = p.expTau[b];
+   size_t const vi = (size_t)b * p.loc.numRes() + k;
+   size_t const ai = (size_t)b * p.loc.numExt() + p.loc.kex(k);
+   float const Aold = p.A[ai];
+   float const Vold = p.V[vi];
+   float t1    = e * Vold;
+   float t2    = 1.0f - e;
+   float si    = p.selfInteract * Aold;
+   float t3    = G + si;
+   float t4    = t2 * t3;
+   float Vnew  = t1 + t4;
+   p.V[vi]     = Vnew;
+   p.A[ai]     = apply_transfer(p.transfer, Vnew);
+}
+
+constexpr int kWave = 64;
+
+} // namespace pvhip

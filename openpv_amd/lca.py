@@ -1,0 +1,159 @@
+"""The LCA column of the reference's sparse-coding graphs on the device.
+
+Graph (tests/GPULCATest/input/GPULCATest.params, tutorials/LCA-CIFAR):
+
+    phase 0  Input            (images, constant over the run)
+    phase 1  Residual         ANNErrorLayer   GSyn0 = Input (IdentConn),
+                                              GSyn1 = Recon(t-1) (IdentConn)
+    phase 2  V1               HyPerLCALayer   GSyn0 = TransposeConn gather of Residual
+                                              (post perspective, receiveGpu)
+    phase 3  Recon            ANNLayer        GSyn0 = CloneConn scatter of sparse V1
+                                              (pre perspective)
+
+One `step()` is HyPerCol::advanceTime (src/columns/HyPerCol.cpp:659-791) for
+this graph with a fixed dt.  All state stays in HBM; a step is a fixed
+sequence of libpvhip launches on one stream, so it can be captured once as a
+hipGraph and replayed.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import abi
+from .runtime import (ActiveList, HyPerLCAUpdate, PostsynapticPerspectiveGPUDelivery,
+                      PresynapticPerspectiveGPUDelivery, cost_sum_f64, fill_zero, ident_deliver,
+                      layer_update, loc, make_transfer, _stream)
+
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
+class LCAColumn:
+    """Input -> Residual <-> V1 -> Recon on one device.
+
+    weights: V1->Residual pre-perspective shared weights [nf_v1][nyp][nxp][nf_in]
+    (the HyPerConn the TransposeConn and CloneConn share).
+    arith: delivery arithmetic of the gather (EXACT, FP32, BF16); the scatter
+    runs EXACT for EXACT and FP32 (FMA) otherwise.
+    fused: apply the V1 update in the gather epilogue.
+    """
+
+    def __init__(self, images, weights, nx_v1, ny_v1, nf_v1, nxp, nyp, VThresh=0.1, AMin=0.0,
+                 AMax=FLT_MAX, AShift=None, VWidth=0.0, timeConstantTau=100.0, dt=1.0,
+                 self_interact=True, err_scale=1.0, vertices=False, arith=abi.ARITH_EXACT,
+                 fused=False, V0=None, res_halo=None, v1_halo=None, device="cuda"):
+        images = torch.as_tensor(images, dtype=torch.float32)
+        nb, ny, nx, nf = images.shape
+        self.device = torch.device(device)
+        H = abi.host()
+        if res_halo is None:
+            # TransposeConn Residual -> V1 with post-perspective patch: margin of Residual
+            nxpT = H.pvhost_post_patch_size(nxp, nx_v1, nx)
+            res_halo = H.pvhost_required_margin(nx, nx_v1, nxpT)
+        if v1_halo is None:
+            v1_halo = H.pvhost_required_margin(nx_v1, nx, nxp)
+        self.in_loc = loc(nb, nx, ny, nf, 0)
+        self.res_loc = loc(nb, nx, ny, nf, res_halo)
+        self.rec_loc = loc(nb, nx, ny, nf, 0)
+        self.v1_loc = loc(nb, nx_v1, ny_v1, nf_v1, v1_halo)
+        d = self.device
+        f = dict(dtype=torch.float32, device=d)
+        self.inputA = images.reshape(-1).to(d).contiguous()
+        self.res_gsyn = torch.zeros(2 * nb * self.res_loc.num_res, **f)
+        self.res_V = torch.zeros(nb * self.res_loc.num_res, **f)
+        self.res_A = torch.zeros(nb * self.res_loc.num_ext, **f)
+        self.v1_gsyn = torch.zeros(nb * self.v1_loc.num_res, **f)
+        self.v1_V = torch.zeros(nb * self.v1_loc.num_res, **f)
+        self.v1_A = torch.zeros(nb * self.v1_loc.num_ext, **f)
+        self.rec_gsyn = torch.zeros(nb * self.rec_loc.num_res, **f)
+        self.rec_V = torch.zeros(nb * self.rec_loc.num_res, **f)
+        self.rec_A = torch.zeros(nb * self.rec_loc.num_res, **f)
+        self.weights = torch.as_tensor(weights, dtype=torch.float32).reshape(-1).to(d).contiguous()
+        if AShift is None:
+            AShift = VThresh
+        # probe coefficient: the layer's float VThresh widened to double (L1NormLCAProbe)
+        self.VThresh = float(np.float32(VThresh))
+        self.v1_transfer = make_transfer(VThresh, AMin, AMax, AShift, VWidth, vertices)
+        self.none_transfer = make_transfer(-FLT_MAX, -FLT_MAX, FLT_MAX, 0.0, 0.0, False)
+        self.err_scale = float(err_scale)
+        self.arith = arith
+        self.fused = fused
+        # TransposeConn: post weights are the original's pre weights, post-perspective
+        # patch = nxp x nyp x nf_in (TransposeWeightsPair.cpp:141-147).
+        self.gather = PostsynapticPerspectiveGPUDelivery(self.res_loc, self.v1_loc, nxp, nyp,
+                                                         self.weights, arith=arith)
+        scatter_arith = abi.ARITH_EXACT if arith == abi.ARITH_EXACT else abi.ARITH_FP32
+        self.scatter = PresynapticPerspectiveGPUDelivery(self.v1_loc, self.rec_loc, nxp, nyp,
+                                                         self.weights, arith=scatter_arith)
+        self.active = ActiveList(self.v1_loc, d)
+        self.lca = HyPerLCAUpdate(self.v1_loc, self.v1_transfer, timeConstantTau, dt,
+                                  self_interact, device=d)
+        self.energy = torch.zeros(2, nb, dtype=torch.float64, device=d)
+        self._graph = None
+        self._fused_args = None
+        # InitializeState: V1 V from V0, A = T(V); Residual and Recon start at 0.
+        if V0 is not None:
+            self.v1_V.copy_(torch.as_tensor(V0, dtype=torch.float32).reshape(-1).to(d))
+        from .runtime import ann_activity
+        ann_activity(self.v1_loc, self.v1_transfer, self.v1_V, self.v1_A)
+        self.active.compact(self.v1_A)
+
+    # one timestep ---------------------------------------------------------
+    def step(self, stream=None):
+        s = stream
+        nb = self.in_loc.nbatch
+        # phase 1: Residual = err_scale * (Input - Recon(t-1))
+        n = nb * self.res_loc.num_res
+        g0, g1 = self.res_gsyn[:n], self.res_gsyn[n:]
+        ident_deliver(self.in_loc, self.inputA, g0, accumulate=False, stream=s)
+        ident_deliver(self.rec_loc, self.rec_A, g1, accumulate=False, stream=s)
+        layer_update(self.res_loc, self.res_gsyn, self.res_V, self.res_A, self.none_transfer,
+                     channel_coeffs=(1.0, -1.0), err_scale=self.err_scale, stream=s)
+        # phase 2: V1 gather + LCA update (+ threshold), then its active list
+        if self.fused:
+            if self._fused_args is None:
+                self._fused_args = self.lca.args(None, self.v1_V, self.v1_A, num_channels=1)
+            self.gather.deliver(self.res_A, None, accumulate=False, fused=self._fused_args, stream=s)
+        else:
+            self.gather.deliver(self.res_A, self.v1_gsyn, accumulate=False, stream=s)
+            self.lca.update(self.v1_gsyn, self.v1_V, self.v1_A, stream=s)
+        self.active.compact(self.v1_A, stream=s)
+        # phase 3: Recon = scatter of the sparse V1 activity
+        self.scatter.deliver(self.rec_gsyn, active=self.active, accumulate=False, stream=s)
+        layer_update(self.rec_loc, self.rec_gsyn, self.rec_V, self.rec_A, self.none_transfer,
+                     channel_coeffs=(1.0,), err_scale=None, stream=s)
+
+    def probe_energy(self, stream=None):
+        """E_b = 0.5*sum r^2 + VThresh*sum |a| in FP64 (L2NormProbe + L1NormLCAProbe)."""
+        cost_sum_f64(self.res_loc, self.res_A, True, 0, out=self.energy[0], stream=stream)
+        cost_sum_f64(self.v1_loc, self.v1_A, True, 1, out=self.energy[1], stream=stream)
+        return 0.5 * self.energy[0] + self.VThresh * self.energy[1]
+
+    def capture(self, stream):
+        """Capture one step as a hipGraph on `stream` (a non-default torch stream)."""
+        H = abi.hip()
+        with torch.cuda.stream(stream):
+            abi.check(H.pvhip_graph_begin(_stream(stream)), "pvhip_graph_begin")
+            try:
+                self.step(stream)
+            finally:
+                g = C.c_void_p()
+                st = H.pvhip_graph_end(_stream(stream), C.byref(g))
+            abi.check(st, "pvhip_graph_end")
+        self._graph = g
+        return g
+
+    def replay(self, stream):
+        abi.check(abi.hip().pvhip_graph_launch(self._graph, _stream(stream)), "pvhip_graph_launch")
+
+    def release_graph(self):
+        if self._graph is not None:
+            abi.hip().pvhip_graph_destroy(self._graph)
+            self._graph = None
+
+    # host views -----------------------------------------------------------
+    def v1_state(self):
+        return self.v1_V.cpu().numpy(), self.v1_A.cpu().numpy()
+
+    def num_active(self):
+        return self.active.num_active.cpu().numpy()

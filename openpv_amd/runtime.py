@@ -1,0 +1,340 @@
+"""Python face of the device path: the reference's delivery / layer-update /
+plasticity objects over libpvhip.
+
+Names follow the reference (PostsynapticPerspectiveGPUDelivery,
+PresynapticPerspectiveGPUDelivery, HyPerLCAInternalStateBuffer, ...).  Device
+memory and the stream come from torch (plumbing only); every computation is a
+libpvhip kernel.  Nothing here falls back to the CPU: a missing library or a
+non-GPU tensor raises.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import abi
+
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise abi.PVHIPError("device tensor required, got a %s tensor" % t.device)
+    if not t.is_contiguous():
+        raise abi.PVHIPError("contiguous tensor required")
+    return C.c_void_p(t.data_ptr())
+
+
+def _dev_i32(a, device):
+    return torch.as_tensor(np.ascontiguousarray(a, np.int32), device=device)
+
+
+def loc(nbatch, nx, ny, nf, halo=0):
+    return abi.Loc.make(nbatch, nx, ny, nf, halo)
+
+
+def make_transfer(VThresh=-FLT_MAX, AMin=-FLT_MAX, AMax=FLT_MAX, AShift=0.0, VWidth=0.0,
+                  vertices=False):
+    """ANNActivityBuffer transfer (VTHRESH = CPU semantics, vertices = GPU)."""
+    t = abi.Transfer()
+    mode = abi.TRANSFER_VERTICES if vertices else abi.TRANSFER_VTHRESH
+    r = abi.host().pvhost_make_transfer(VThresh, AMin, AMax, AShift, VWidth, mode, C.byref(t))
+    if r != 0:
+        raise abi.PVHIPError("pvhost_make_transfer failed (%d)" % r)
+    return t
+
+
+def exp_tau(dt, tau, nbatch):
+    """(float)exp(-dt/(double)tau) per batch element, on the host."""
+    dts = (C.c_double * nbatch)(*([float(dt)] * nbatch) if np.isscalar(dt) else list(map(float, dt)))
+    out = (C.c_float * nbatch)()
+    abi.hip().pvhip_lca_exp_tau(dts, nbatch, C.c_float(tau), out)
+    return np.array(out[:], np.float32)
+
+
+def scaled_tau(timeConstantTau, deltaTime):
+    """mScaledTimeConstantTau = (float)(tau / dt) (HyPerLCAInternalStateBuffer.cpp:104-112)."""
+    return float(np.float32(float(timeConstantTau) / float(deltaTime)))
+
+
+class PostsynapticPerspectiveGPUDelivery:
+    """Gather delivery (PostsynapticPerspectiveGPUDelivery + CudaRecvPost).
+
+    source: pre layer loc; target: post layer loc; nxp, nyp: post-perspective
+    patch; weights: device tensor [kernels][nyp][nxp][source.nf].
+    """
+
+    def __init__(self, source, target, nxp, nyp, weights, arith=abi.ARITH_EXACT, dt_factor=1.0):
+        self.source, self.target = source, target
+        self.nxp, self.nyp, self.nfp = nxp, nyp, source.nf
+        self.arith = arith
+        self.dt_factor = dt_factor
+        dev = weights.device
+        sx = np.zeros(target.nx, np.int32)
+        sy = np.zeros(target.ny, np.int32)
+        kx = np.zeros(target.nx, np.int32)
+        ky = np.zeros(target.ny, np.int32)
+        nkx, nky = C.c_int(), C.c_int()
+        r = abi.host().pvhost_recv_post_tables(
+            C.byref(source), C.byref(target), nxp, nyp, sx.ctypes.data_as(C.c_void_p),
+            sy.ctypes.data_as(C.c_void_p), kx.ctypes.data_as(C.c_void_p),
+            ky.ctypes.data_as(C.c_void_p), C.byref(nkx), C.byref(nky))
+        if r != 0:
+            raise abi.PVHIPError("recv_post tables: halo of %r too small for patch %dx%d (%d)"
+                                 % (source, nxp, nyp, r))
+        self.nkx, self.nky = nkx.value, nky.value
+        self.tables = [_dev_i32(a, dev) for a in (sx, sy, kx, ky)]
+        self.num_kernels = self.nkx * self.nky * target.nf
+        self.patch_size = nxp * nyp * self.nfp
+        if weights.numel() != self.num_kernels * self.patch_size:
+            raise abi.PVHIPError("weights hold %d values, expected %d x %d"
+                                 % (weights.numel(), self.num_kernels, self.patch_size))
+        self.weights = weights
+        self.weights_lp = None
+        if arith == abi.ARITH_BF16:
+            self.refresh_weights()
+
+    def refresh_weights(self, stream=None):
+        """Re-derive the low-precision shadow after a weight change."""
+        if self.arith != abi.ARITH_BF16:
+            return
+        nbytes = abi.hip().pvhip_recv_post_bf16_weights_bytes(self.num_kernels, self.patch_size)
+        if self.weights_lp is None:
+            self.weights_lp = torch.empty(nbytes // 2, dtype=torch.int16, device=self.weights.device)
+        abi.check(abi.hip().pvhip_recv_post_bf16_weights(
+            _ptr(self.weights), _ptr(self.weights_lp), self.num_kernels, self.patch_size,
+            _stream(stream)), "pvhip_recv_post_bf16_weights")
+
+    def args(self, pre_activity, gsyn, accumulate=False, fused=None):
+        a = abi.RecvPostArgs()
+        a.pre, a.post = self.source, self.target
+        a.nxp, a.nyp, a.nfp = self.nxp, self.nyp, self.nfp
+        a.num_kernels_x, a.num_kernels_y = self.nkx, self.nky
+        a.start_x, a.start_y, a.kernel_x, a.kernel_y = [_ptr(t) for t in self.tables]
+        a.dt_factor = self.dt_factor
+        a.pre_data = _ptr(pre_activity)
+        a.weights = _ptr(self.weights)
+        a.post_gsyn = _ptr(gsyn)
+        a.accumulate = int(accumulate)
+        a.arith = self.arith
+        a.weights_lp = _ptr(self.weights_lp)
+        if fused is not None:
+            a.fused_update = C.pointer(fused)
+        return a
+
+    def deliver(self, pre_activity, gsyn, accumulate=False, fused=None, stream=None):
+        a = self.args(pre_activity, gsyn, accumulate, fused)
+        abi.check(abi.hip().pvhip_recv_post(C.byref(a), _stream(stream)), "pvhip_recv_post")
+
+
+class ActiveList:
+    """The publisher's sparse activity (DataStore active indices) on the device."""
+
+    ENTRY = np.dtype([("index", np.uint32), ("value", np.float32)])
+
+    def __init__(self, loc_, device):
+        self.loc = loc_
+        nb = loc_.nbatch
+        self.entries = torch.zeros(nb * loc_.num_ext * 2, dtype=torch.int32, device=device)
+        self.num_active = torch.zeros(nb, dtype=torch.int64, device=device)
+        self.pos_start = torch.zeros(nb * (loc_.nx_ext * loc_.ny_ext + 1), dtype=torch.int32,
+                                     device=device)
+        nscr = abi.hip().pvhip_active_compact_scratch_bytes(C.byref(loc_))
+        self.scratch = torch.zeros(max(1, nscr // 4), dtype=torch.int32, device=device)
+
+    def compact(self, A, stream=None):
+        abi.check(abi.hip().pvhip_active_compact(
+            C.byref(self.loc), _ptr(A), _ptr(self.entries), _ptr(self.num_active),
+            _ptr(self.pos_start), _ptr(self.scratch), _stream(stream)), "pvhip_active_compact")
+
+    def host_entries(self):
+        """(entries structured array per batch [nb][numExt], counts) copied to the host."""
+        e = self.entries.cpu().numpy().view(self.ENTRY).reshape(self.loc.nbatch, self.loc.num_ext)
+        return e, self.num_active.cpu().numpy()
+
+
+class PresynapticPerspectiveGPUDelivery:
+    """Scatter delivery (PresynapticPerspectiveGPUDelivery + CudaRecvPre), evaluated
+    as an ordered gather over the active list."""
+
+    def __init__(self, pre, post, nxp, nyp, weights, arith=abi.ARITH_EXACT, dt_factor=1.0):
+        self.pre, self.post = pre, post
+        self.nxp, self.nyp, self.nfp = nxp, nyp, post.nf
+        self.arith = arith
+        self.dt_factor = dt_factor
+        dev = weights.device
+        H = abi.host()
+        ncol, nrow = C.c_int(), C.c_int()
+        r = H.pvhost_recv_pre_tables(C.byref(pre), C.byref(post), nxp, nyp, None, None, None, None,
+                                     C.byref(ncol), None, None, None, None, C.byref(nrow))
+        if r != 0:
+            raise abi.PVHIPError("recv_pre tables failed (%d)" % r)
+        cp = np.zeros(post.nx + 1, np.int32)
+        cx, cw, ck = (np.zeros(max(1, ncol.value), np.int32) for _ in range(3))
+        rp = np.zeros(post.ny * nyp + 1, np.int32)
+        ry, rw, rk = (np.zeros(max(1, nrow.value), np.int32) for _ in range(3))
+        P = lambda a: a.ctypes.data_as(C.c_void_p)
+        r = H.pvhost_recv_pre_tables(C.byref(pre), C.byref(post), nxp, nyp, P(cp), P(cx), P(cw),
+                                     P(ck), C.byref(ncol), P(rp), P(ry), P(rw), P(rk),
+                                     C.byref(nrow))
+        if r != 0:
+            raise abi.PVHIPError("recv_pre tables failed (%d)" % r)
+        self.tables = [_dev_i32(a, dev) for a in (cp, cx, cw, ck, rp, ry, rw, rk)]
+        nkx, nky = C.c_int(), C.c_int()
+        H.pvhost_num_kernels(C.byref(pre), C.byref(post), C.byref(nkx), C.byref(nky))
+        self.nkx, self.nky = nkx.value, nky.value
+        self.weights = weights
+
+    def deliver(self, gsyn, active=None, pre_activity=None, accumulate=False, stream=None):
+        a = abi.RecvPreArgs()
+        a.pre, a.post = self.pre, self.post
+        a.nxp, a.nyp, a.nfp = self.nxp, self.nyp, self.nfp
+        a.num_kernels_x, a.num_kernels_y = self.nkx, self.nky
+        (a.col_ptr, a.col_xe, a.col_wx, a.col_kx,
+         a.row_ptr, a.row_ye, a.row_wy, a.row_ky) = [_ptr(t) for t in self.tables]
+        a.dt_factor = self.dt_factor
+        if active is not None:
+            a.active = _ptr(active.entries)
+            a.pos_start = _ptr(active.pos_start)
+        else:
+            a.pre_data = _ptr(pre_activity)
+        a.weights = _ptr(self.weights)
+        a.post_gsyn = _ptr(gsyn)
+        a.accumulate = int(accumulate)
+        a.arith = self.arith
+        abi.check(abi.hip().pvhip_recv_pre(C.byref(a), _stream(stream)), "pvhip_recv_pre")
+
+
+class HyPerLCAUpdate:
+    """HyPerLCALayer update: GSynAccumulator + HyPerLCAInternalStateBuffer +
+    ANNActivityBuffer in one kernel."""
+
+    def __init__(self, loc_, transfer, timeConstantTau=1.0, dt=1.0, self_interact=True,
+                 channel_coeffs=(1.0,), device="cuda"):
+        self.loc = loc_
+        self.transfer = transfer
+        self.tau = scaled_tau(timeConstantTau, dt)
+        self.exp_tau = torch.as_tensor(exp_tau(dt, self.tau, loc_.nbatch), device=device)
+        self.self_interact = bool(self_interact)
+        self.coeffs = list(channel_coeffs)
+
+    def set_dt(self, dt):
+        """Per-batch dt (adaptive time scales): recompute exp(-dt/tau) on the host."""
+        self.exp_tau.copy_(torch.as_tensor(exp_tau(dt, self.tau, self.loc.nbatch)))
+
+    def args(self, gsyn, V, A, num_channels=None):
+        a = abi.LCAUpdateArgs()
+        a.loc = self.loc
+        nch = len(self.coeffs) if num_channels is None else num_channels
+        a.num_channels = nch
+        for i, c in enumerate(self.coeffs[:abi.MAX_CHANNELS]):
+            a.channel_coeffs[i] = c
+        a.gsyn = _ptr(gsyn)
+        a.V, a.A = _ptr(V), _ptr(A)
+        a.exp_tau = _ptr(self.exp_tau)
+        a.self_interact = int(self.self_interact)
+        a.transfer = self.transfer
+        return a
+
+    def update(self, gsyn, V, A, stream=None):
+        a = self.args(gsyn, V, A)
+        abi.check(abi.hip().pvhip_lca_update(C.byref(a), _stream(stream)), "pvhip_lca_update")
+
+
+def layer_update(loc_, gsyn, V, A, transfer, channel_coeffs=(1.0,), err_scale=None, stream=None):
+    """ANNLayer / ANNErrorLayer / HyPerLayer update: V = [errScale*] sum coeff*GSyn; A = T(V)."""
+    a = abi.LayerUpdateArgs()
+    a.loc = loc_
+    a.num_channels = len(channel_coeffs)
+    for i, c in enumerate(channel_coeffs):
+        a.channel_coeffs[i] = c
+    a.gsyn, a.V, a.A = _ptr(gsyn), _ptr(V), _ptr(A)
+    a.scale = 1.0 if err_scale is None else err_scale
+    a.apply_scale = 0 if err_scale is None else 1
+    a.transfer = transfer
+    abi.check(abi.hip().pvhip_layer_update(C.byref(a), _stream(stream)), "pvhip_layer_update")
+
+
+def ident_deliver(loc_, pre_activity, gsyn, accumulate=False, stream=None):
+    abi.check(abi.hip().pvhip_ident_deliver(C.byref(loc_), _ptr(pre_activity), _ptr(gsyn),
+                                            int(accumulate), _stream(stream)), "pvhip_ident_deliver")
+
+
+def ann_activity(loc_, transfer, V, A, stream=None):
+    abi.check(abi.hip().pvhip_ann_activity(C.byref(loc_), C.byref(transfer), _ptr(V), _ptr(A),
+                                           _stream(stream)), "pvhip_ann_activity")
+
+
+def fill_zero(t, stream=None):
+    abi.check(abi.hip().pvhip_fill_zero(_ptr(t), t.numel() * t.element_size(), _stream(stream)),
+              "pvhip_fill_zero")
+
+
+def cost_sum_f64(loc_, buf, extended, kind, p0=0.0, p1=0.0, out=None, stream=None):
+    """CostFunctionSum in FP64 per batch element (kind 0 L2, 1 L1, 2 firm, 3 L0)."""
+    if out is None:
+        out = torch.zeros(loc_.nbatch, dtype=torch.float64, device=buf.device)
+    abi.check(abi.hip().pvhip_cost_sum_f64(C.byref(loc_), _ptr(buf), int(extended), kind, p0, p1,
+                                           _ptr(out), _stream(stream)), "pvhip_cost_sum_f64")
+    return out
+
+
+def normalize_l2_f64(weights, num_patches, patch_size, strength=1.0, min_l2=0.0, stream=None):
+    abi.check(abi.hip().pvhip_normalize_l2_f64(_ptr(weights), num_patches, patch_size, strength,
+                                               min_l2, _stream(stream)), "pvhip_normalize_l2_f64")
+
+
+class HebbianUpdater:
+    """HebbianUpdater dW for a shared-weight connection pre -> post with
+    pre-perspective patch nxp x nyp x post.nf."""
+
+    def __init__(self, pre, post, nxp, nyp, dw_max, normalize_dw=True, device="cuda"):
+        self.pre, self.post = pre, post
+        self.nxp, self.nyp, self.nfp = nxp, nyp, post.nf
+        self.dw_max = dw_max
+        self.normalize_dw = normalize_dw
+        H = abi.host()
+        tabs = []
+        for axis, n in ((0, pre.nx_ext), (1, pre.ny_ext)):
+            dim, ps, pr, pe, us = (np.zeros(n, np.int32) for _ in range(5))
+            P = lambda a: a.ctypes.data_as(C.c_void_p)
+            H.pvhost_patch_axis(C.byref(pre), C.byref(post), nxp, nyp, axis, P(dim), P(ps), P(pr),
+                                P(pe), P(us))
+            tabs.append((dim, ps, pe))
+        (dx, psx, pex), (dy, psy, pey) = tabs
+        self.tables = [_dev_i32(a, device) for a in (dx, dy, psx, psy, pex, pey)]
+        nkx, nky = C.c_int(), C.c_int()
+        H.pvhost_num_kernels(C.byref(pre), C.byref(post), C.byref(nkx), C.byref(nky))
+        self.nkx, self.nky = nkx.value, nky.value
+        self.size = self.nkx * self.nky * pre.nf * nxp * nyp * self.nfp
+        self.dw = torch.zeros(self.size, dtype=torch.float32, device=device)
+        self.counts = torch.zeros(self.size, dtype=torch.int64, device=device) if normalize_dw else None
+
+    def update_dw(self, pre_activity, post_activity, clear=True, arith=abi.ARITH_EXACT, stream=None):
+        if clear:
+            fill_zero(self.dw, stream)
+            if self.counts is not None:
+                fill_zero(self.counts, stream)
+        a = abi.HebbianArgs()
+        a.pre, a.post = self.pre, self.post
+        a.nxp, a.nyp, a.nfp = self.nxp, self.nyp, self.nfp
+        a.num_kernels_x, a.num_kernels_y = self.nkx, self.nky
+        a.dim_x, a.dim_y, a.pstart_x, a.pstart_y, a.apost_x, a.apost_y = [_ptr(t) for t in self.tables]
+        a.dw_max = self.dw_max
+        a.pre_data, a.post_data = _ptr(pre_activity), _ptr(post_activity)
+        a.dw = _ptr(self.dw)
+        a.counts = _ptr(self.counts)
+        a.arith = arith
+        abi.check(abi.hip().pvhip_hebbian_dw(C.byref(a), _stream(stream)), "pvhip_hebbian_dw")
+
+    def apply(self, weights, stream=None):
+        abi.check(abi.hip().pvhip_hebbian_apply(_ptr(weights), _ptr(self.dw), _ptr(self.counts),
+                                                self.size, int(self.normalize_dw), _stream(stream)),
+                  "pvhip_hebbian_apply")

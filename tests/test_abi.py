@@ -1,0 +1,172 @@
+"""The C-ABI libraries load and export every declared symbol; host-side tables
+agree with the oracle's geometry.  CPU only (no compute calls on a device)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from openpv_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FMAX = float(np.finfo(np.float32).max)
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pv(?:hip|host)_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,loader,table", [
+    ("pvhip.h", abi.hip, abi.PVHIP_SYMBOLS), ("pvhost.h", abi.host, abi.PVHOST_SYMBOLS)])
+def test_library_exports_every_declared_symbol(header, loader, table):
+    lib = loader()
+    names = declared(header)
+    assert names, header
+    for n in names:
+        assert hasattr(lib, n), "%s declares %s but the library does not export it" % (header, n)
+        assert n in table, "%s not bound in openpv_amd.abi" % n
+
+
+def test_abi_version_and_error_string():
+    assert abi.hip().pvhip_abi_version() == 1
+    assert isinstance(abi.hip().pvhip_last_error(), bytes)
+
+
+def test_invalid_arguments_are_rejected_without_a_device():
+    H = abi.hip()
+    assert H.pvhip_recv_post(None, None) == 1
+    assert b"NULL" in H.pvhip_last_error()
+    args = abi.RecvPreArgs()
+    assert H.pvhip_recv_pre(C.byref(args), None) == 1  # invalid loc
+    bad = abi.Loc.make(0, 4, 4, 1)
+    assert H.pvhip_active_compact_scratch_bytes(C.byref(bad)) == 0
+
+
+CASES = [
+    ((1, 16, 16, 128, 2), (1, 32, 32, 3, 3), 8, 8),
+    ((1, 32, 32, 512, 2), (1, 128, 128, 1, 6), 16, 16),
+    ((1, 256, 256, 16, 6), (1, 128, 128, 256, 0), 6, 6),
+    ((1, 16, 16, 2, (3, 2, 1, 4)), (1, 8, 8, 5, (1, 2, 0, 3)), 6, 4),
+    ((1, 8, 8, 2, 3), (1, 32, 32, 4, 1), 8, 12),
+    ((1, 32, 32, 2, 9), (1, 8, 8, 4, 0), 5, 5),
+]
+
+
+def both(t):
+    return abi.Loc.make(*t[:4], halo=t[4]), O.Loc.make(*t[:4], halo=t[4])
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_host_patch_axis_matches_oracle(case):
+    (hp, op), (hq, oq), nxp, nyp = both(case[0]), both(case[1]), case[2], case[3]
+    for axis in (0, 1):
+        n = hp.nx_ext if axis == 0 else hp.ny_ext
+        outs = [np.zeros(n, np.int32) for _ in range(5)]
+        assert abi.host().pvhost_patch_axis(C.byref(hp), C.byref(hq), nxp, nyp, axis,
+                                            *[o.ctypes.data_as(C.c_void_p) for o in outs]) == 0
+        ref = O.patch_axis(op, oq, nxp, nyp, axis)
+        for o, k in zip(outs, ("dim", "pstart", "post_start_res", "post_start_ext", "unshrunken")):
+            assert (o == ref[k]).all(), (axis, k)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_recv_pre_coverage_tables_reproduce_the_scatter_order(case):
+    """Walking the CSR tables visits exactly the (pre neuron, patch row) pairs of the
+    reference loop, in its order, for every post neuron."""
+    (hp, op), (hq, oq), nxp, nyp = both(case[0]), both(case[1]), case[2], case[3]
+    H = abi.host()
+    nc, nr = C.c_int(), C.c_int()
+    assert H.pvhost_recv_pre_tables(C.byref(hp), C.byref(hq), nxp, nyp, None, None, None, None,
+                                    C.byref(nc), None, None, None, None, C.byref(nr)) == 0
+    cp = np.zeros(hq.nx + 1, np.int32)
+    cx, cw, ck = (np.zeros(max(1, nc.value), np.int32) for _ in range(3))
+    rp = np.zeros(hq.ny * nyp + 1, np.int32)
+    ry, rw, rk = (np.zeros(max(1, nr.value), np.int32) for _ in range(3))
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+    assert H.pvhost_recv_pre_tables(C.byref(hp), C.byref(hq), nxp, nyp, P(cp), P(cx), P(cw), P(ck),
+                                    C.byref(nc), P(rp), P(ry), P(rw), P(rk), C.byref(nr)) == 0
+    pa, gs, _, _ = O.patch_geometry(op, oq, nxp, nyp, oq.nf)
+    nxe, nye = op.nx_ext, op.ny_ext
+    # reference visit list per post (X, Y) with nf = 1 view: (y, ye, xe, weight index)
+    ref = {}
+    for y in range(nyp):
+        for ye in range(nye):
+            for xe in range(nxe):
+                kex = (ye * nxe + xe) * op.nf
+                pt = pa[kex]
+                if y >= pt["ny"]:
+                    continue
+                start = int(gs[kex]) // oq.nf
+                Y0, X0 = divmod(start, oq.nx)
+                oy, ox = divmod(int(pt["offset"]) // oq.nf, nxp)
+                for kx in range(pt["nx"]):
+                    ref.setdefault((Y0 + y, X0 + kx), []).append((y, ye, xe, oy + y, ox + kx))
+    got = {}
+    for Y in range(hq.ny):
+        for X in range(hq.nx):
+            seq = []
+            for r in range(nyp):
+                for ri in range(rp[Y * nyp + r], rp[Y * nyp + r + 1]):
+                    for ci in range(cp[X], cp[X + 1]):
+                        seq.append((r, int(ry[ri]), int(cx[ci]), int(rw[ri]), int(cw[ci])))
+            if seq:
+                got[(Y, X)] = seq
+    assert got == ref
+
+
+@pytest.mark.parametrize("case", CASES[:3])
+def test_recv_post_tables_match_unshrunken_start(case):
+    (hs, os_), (ht, ot), _, _ = both(case[1]), both(case[0]), case[2], case[3]
+    # gather from source=case[1] into target=case[0] with the original pre patch
+    nxp, nyp = case[2], case[3]
+    sx, sy, kx, ky = (np.zeros(n, np.int32) for n in (ht.nx, ht.ny, ht.nx, ht.ny))
+    nkx, nky = C.c_int(), C.c_int()
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+    r = abi.host().pvhost_recv_post_tables(C.byref(hs), C.byref(ht), nxp, nyp, P(sx), P(sy), P(kx),
+                                           P(ky), C.byref(nkx), C.byref(nky))
+    if r != 0:
+        pytest.skip("halo of source too small for this patch")
+    us_x = O.patch_axis(ot, os_, nxp, nyp, 0)["unshrunken"][ot.lt:ot.lt + ot.nx]
+    us_y = O.patch_axis(ot, os_, nxp, nyp, 1)["unshrunken"][ot.up:ot.up + ot.ny]
+    assert (sx == us_x).all() and (sy == us_y).all()
+
+
+@pytest.mark.parametrize("params", [
+    (0.1, 0.0, FMAX, 0.1, 0.0), (0.025, 0.0, FMAX, 0.0, 0.025), (0.1, 0.0, 0.5, 0.1, 0.0),
+    (-FMAX, -FMAX, FMAX, 0.0, 0.0), (0.2, -0.1, 1.0, 0.05, 0.3), (0.3, 0.0, 0.1, 0.0, 0.0)])
+def test_make_transfer_matches_oracle_vertices(params):
+    t = abi.Transfer()
+    assert abi.host().pvhost_make_transfer(*params, abi.TRANSFER_VERTICES, C.byref(t)) == 0
+    nv, vv, va, sl = O.set_vertices(*params)
+    assert t.num_vertices == nv
+    assert np.array_equal(np.array(t.vertices_v[:nv], np.float32), vv[:nv])
+    assert np.array_equal(np.array(t.vertices_a[:nv], np.float32), va[:nv])
+    assert np.array_equal(np.array(t.slopes[:nv + 1], np.float32), sl[:nv + 1], equal_nan=True)
+
+
+@pytest.mark.parametrize("case", [
+    ((1, 32, 32, 6, 5), (1, 16, 16, 10, 2), 6, 6),   # many-to-one
+    ((1, 16, 16, 12, 2), (1, 32, 32, 3, 3), 8, 8),   # one-to-many
+    ((1, 16, 16, 4, 2), (1, 16, 16, 5, 2), 5, 5),    # one-to-one
+    ((1, 16, 16, 4, 4), (1, 64, 64, 2, 0), 8, 8),    # one-to-many stride 4
+])
+def test_transpose_weights_matches_oracle(case):
+    (hp, op), (hq, oq), nxp, nyp = both(case[0]), both(case[1]), case[2], case[3]
+    kx, ky, kf = O.num_kernels(op, oq)
+    rng = np.random.default_rng(3)
+    W = rng.standard_normal(kx * ky * kf * nxp * nyp * oq.nf).astype(np.float32)
+    nxq = O.lib().or_calcPostPatchSize(nxp, op.nx, oq.nx)
+    nyq = O.lib().or_calcPostPatchSize(nyp, op.ny, oq.ny)
+    pkx, pky, pkf = O.num_kernels(oq, op)
+    n = pkx * pky * pkf * nxq * nyq * op.nf
+    a = np.full(n, np.nan, np.float32)
+    b = np.full(n, np.nan, np.float32)
+    O.lib().or_transpose_weights_shared(C.byref(op), C.byref(oq), nxp, nyp, oq.nf, W, a)
+    assert abi.host().pvhost_transpose_weights(C.byref(hp), C.byref(hq), nxp, nyp, oq.nf,
+                                               W.ctypes.data_as(C.c_void_p),
+                                               b.ctypes.data_as(C.c_void_p)) == 0
+    assert np.array_equal(a, b, equal_nan=True)

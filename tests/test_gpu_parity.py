@@ -1,0 +1,402 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bar: EXACT forms are bit-identical to the oracle (V, A, GSyn, dW, active
+lists, counts); FP32 (FMA) forms within 1e-5 relative to the largest
+magnitude; BF16 forms within the tolerance written at each test.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from openpv_amd import abi, runtime as R
+from openpv_amd.lca import LCAColumn
+
+pytestmark = pytest.mark.gpu
+FMAX = float(np.finfo(np.float32).max)
+DEV = "cuda"
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def same(a, b):
+    return np.array_equal(bits(a), bits(b))
+
+
+def both(t):
+    return abi.Loc.make(*t[:4], halo=t[4]), O.Loc.make(*t[:4], halo=t[4])
+
+
+def dev(a, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(DEV, dtype=dtype).contiguous()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def ext_random(loc, rng, density=1.0, scale=1.0):
+    """Extended buffer with zero halo and random restricted values."""
+    A = np.zeros((loc.nbatch, loc.ny_ext, loc.nx_ext, loc.nf), np.float32)
+    v = (rng.standard_normal((loc.nbatch, loc.ny, loc.nx, loc.nf)) * scale).astype(np.float32)
+    if density < 1.0:
+        v[rng.random(v.shape) >= density] = 0
+    A[:, loc.halo.up:loc.halo.up + loc.ny, loc.halo.lt:loc.halo.lt + loc.nx] = v
+    return A.ravel()
+
+
+def test_device_layer():
+    H = abi.hip()
+    d = C.c_void_p()
+    abi.check(H.pvhip_device_create(0, C.byref(d)), "device_create")
+    p = abi.Props()
+    abi.check(H.pvhip_device_props(d, C.byref(p)), "props")
+    print("device:", p.name.decode(), p.arch.decode(), "CUs", p.compute_units, "LDS/block",
+          p.lds_per_block, flush=True)
+    assert p.arch.decode().startswith("gfx950") and p.wavefront_size == 64
+    b = C.c_void_p()
+    abi.check(H.pvhip_buffer_alloc(d, 4099, C.byref(b)), "alloc")
+    src = np.arange(4099, dtype=np.uint8)
+    out = np.zeros(4099, np.uint8)
+    s = H.pvhip_device_stream(d)
+    abi.check(H.pvhip_buffer_h2d(b, src.ctypes.data_as(C.c_void_p), 4099, 0, s), "h2d")
+    abi.check(H.pvhip_buffer_d2h(b, out.ctypes.data_as(C.c_void_p), 4099, 0, s), "d2h")
+    abi.check(H.pvhip_device_sync(d), "sync")
+    assert (out == src).all()
+    # zero-fill an unaligned sub-range
+    base = H.pvhip_buffer_ptr(b)
+    abi.check(H.pvhip_fill_zero(C.c_void_p(base + 3), 1001, s), "fill_zero")
+    abi.check(H.pvhip_buffer_d2h(b, out.ctypes.data_as(C.c_void_p), 4099, 0, s), "d2h")
+    abi.check(H.pvhip_device_sync(d), "sync")
+    exp = src.copy()
+    exp[3:1004] = 0
+    assert (out == exp).all()
+    assert H.pvhip_buffer_h2d(b, src.ctypes.data_as(C.c_void_p), 4099, 8, s) == abi.PVHIP_OK + 1
+    abi.check(H.pvhip_buffer_free(b), "free")
+    abi.check(H.pvhip_device_destroy(d), "destroy")
+
+
+@pytest.mark.parametrize("vertices,params", [
+    (False, (0.1, 0.0, FMAX, 0.1, 0.0)),
+    (True, (0.1, 0.0, FMAX, 0.1, 0.0)),
+    (False, (0.025, 0.0, FMAX, 0.0, 0.025)),
+    (True, (0.025, 0.0, FMAX, 0.0, 0.025)),
+    (False, (0.1, 0.0, 0.4, 0.05, 0.1)),
+])
+def test_lca_update_bit_exact(vertices, params):
+    rng = np.random.default_rng(21)
+    hl, ol = both((3, 11, 7, 5, (2, 1, 1, 2)))
+    n = hl.nbatch * hl.num_res
+    G = (rng.standard_normal(2 * n) * 0.3).astype(np.float32)
+    V = (rng.standard_normal(n) * 0.2).astype(np.float32)
+    V[::5] = np.float32(params[0])
+    A = O.transfer(ol, V, *params, vertices=vertices)
+    # oracle: accumulate {1, -0.5}, update, transfer
+    acc = np.zeros(n, np.float32)
+    O.lib().or_gsyn_accumulate(n, 2, np.array([1.0, -0.5], np.float32), G, acc)
+    Vr = O.update_lca(ol, True, np.array([1.0, 0.5, 2.0]), 100.0, acc, A, V)
+    Ar = O.transfer(ol, Vr, *params, vertices=vertices, A=A)
+    upd = R.HyPerLCAUpdate(hl, R.make_transfer(*params, vertices=vertices), 100.0, 1.0, True,
+                           channel_coeffs=(1.0, -0.5), device=DEV)
+    upd.set_dt([1.0, 0.5, 2.0])
+    Vd, Ad = dev(V), dev(A)
+    upd.update(dev(G), Vd, Ad)
+    assert same(host(Vd), Vr) and same(host(Ad), Ar)
+
+
+def test_layer_updates_and_ident_bit_exact():
+    rng = np.random.default_rng(22)
+    hl, ol = both((2, 9, 6, 3, 3))
+    n = hl.nbatch * hl.num_res
+    G = rng.standard_normal(2 * n).astype(np.float32)
+    acc = np.zeros(n, np.float32)
+    O.lib().or_gsyn_accumulate(n, 2, np.array([1.0, -1.0], np.float32), G, acc)
+    Vr = acc.copy()
+    O.lib().or_errscale(n, np.float32(0.75), Vr)
+    Ar = O.transfer(ol, Vr, -FMAX, -FMAX, FMAX, 0.0, 0.0)
+    Vd = torch.zeros(n, device=DEV)
+    Ad = torch.zeros(hl.nbatch * hl.num_ext, device=DEV)
+    R.layer_update(hl, dev(G), Vd, Ad, R.make_transfer(), channel_coeffs=(1.0, -1.0), err_scale=0.75)
+    assert same(host(Vd), Vr) and same(host(Ad), Ar)
+    # IdentDelivery from an extended buffer, fresh and accumulating
+    pre = ext_random(hl, rng)
+    g = np.zeros(n, np.float32)
+    O.lib().or_ident_deliver(O.C.byref(ol), pre, g)
+    gd = torch.full((n,), 7.0, device=DEV)
+    R.ident_deliver(hl, dev(pre), gd, accumulate=False)
+    assert same(host(gd), g)
+    O.lib().or_ident_deliver(O.C.byref(ol), pre, g)
+    R.ident_deliver(hl, dev(pre), gd, accumulate=True)
+    assert same(host(gd), g)
+
+
+@pytest.mark.parametrize("shape,density", [
+    ((2, 16, 16, 128, 2), 0.05), ((3, 13, 7, 3, (1, 2, 3, 0)), 0.3), ((1, 36, 36, 512, 2), 0.02),
+    ((2, 8, 8, 5, 1), 0.0), ((1, 9, 9, 64, 0), 1.0)])
+def test_active_compact_bit_exact(shape, density):
+    rng = np.random.default_rng(23)
+    hl, ol = both(shape)
+    A = ext_random(hl, rng, density)
+    if A.size > 11:
+        A[11] = -0.0  # negative zero is inactive
+    ent, cnt = O.active_indices(ol, A)
+    al = R.ActiveList(hl, DEV)
+    al.compact(dev(A))
+    e, n = al.host_entries()
+    assert (n == cnt).all()
+    ps = host(al.pos_start).astype(np.int64).reshape(hl.nbatch, -1)
+    npos = hl.nx_ext * hl.ny_ext
+    for b in range(hl.nbatch):
+        eo = ent[b * ol.num_ext:b * ol.num_ext + cnt[b]]
+        assert (e[b, :cnt[b]]["index"] == eo["index"]).all()
+        assert same(e[b, :cnt[b]]["value"], eo["value"])
+        # pos_start[p] = number of active entries before position p
+        Ab = A[b * ol.num_ext:(b + 1) * ol.num_ext].reshape(npos, hl.nf)
+        counts = (Ab != 0).sum(1)
+        assert (ps[b, :npos] == np.concatenate([[0], np.cumsum(counts)[:-1]])).all()
+        assert ps[b, npos] == cnt[b]
+
+
+GATHER_CASES = [
+    # source, target, post-perspective patch
+    ((2, 32, 32, 3, 3), (2, 16, 16, 128, 2), 8),        # config[0] Residual -> V1
+    ((2, 64, 64, 1, 6), (2, 16, 16, 96, 2), 16),        # config[2] geometry, smaller
+    ((1, 64, 64, 16, 6), (1, 32, 32, 40, 0), 12),       # config[1] geometry, smaller
+    ((1, 16, 16, 4, 2), (1, 16, 16, 8, 1), 5),          # one-to-one
+    ((1, 16, 16, 4, 2), (1, 32, 32, 3, 0), 4),          # target larger: direct kernel
+    ((1, 20, 12, 2, (2, 3, 3, 2)), (1, 10, 6, 7, 1), 6),  # asymmetric halo, ragged tiles
+]
+
+
+def gather_setup(case, rng, density=1.0):
+    (hs, os_), (ht, ot), p = both(case[0]), both(case[1]), case[2]
+    nkx = ot.nx // os_.nx if ot.nx > os_.nx else 1
+    nky = ot.ny // os_.ny if ot.ny > os_.ny else 1
+    W = (rng.standard_normal((nkx * nky * ot.nf, p, p, os_.nf)) * 0.05).astype(np.float32)
+    A = ext_random(hs, rng, density)
+    return hs, os_, ht, ot, p, W, A
+
+
+@pytest.mark.parametrize("case", GATHER_CASES)
+def test_recv_post_exact_bit_exact(case):
+    rng = np.random.default_rng(24)
+    hs, os_, ht, ot, p, W, A = gather_setup(case, rng)
+    ref = O.recv_post(os_, ot, p, p, A, W)
+    d = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=abi.ARITH_EXACT)
+    g = torch.full((ht.nbatch * ht.num_res,), 3.0, device=DEV)
+    d.deliver(dev(A), g, accumulate=False)
+    assert same(host(g), ref)
+    # accumulate into existing GSyn: second delivery adds in the same order
+    ref2 = O.recv_post(os_, ot, p, p, A, W, gsyn=ref.copy())
+    d.deliver(dev(A), g, accumulate=True)
+    assert same(host(g), ref2)
+
+
+@pytest.mark.parametrize("case", GATHER_CASES[:4])
+def test_recv_post_fp32_and_bf16(case):
+    rng = np.random.default_rng(25)
+    hs, os_, ht, ot, p, W, A = gather_setup(case, rng)
+    ref = O.recv_post(os_, ot, p, p, A, W).astype(np.float64)
+    scale = np.abs(ref).max()
+    d32 = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=abi.ARITH_FP32)
+    g = torch.zeros(ht.nbatch * ht.num_res, device=DEV)
+    d32.deliver(dev(A), g)
+    assert np.abs(host(g) - ref).max() <= 1e-5 * scale
+    d16 = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=abi.ARITH_BF16)
+    g.zero_()
+    d16.deliver(dev(A), g)
+    err = np.abs(host(g) - ref)
+    # BF16 inputs (8-bit mantissa), FP32 accumulation over K terms: relative
+    # error of a K-term dot product stays within ~2^-8 * sqrt(K) of the scale.
+    K = p * p * os_.nf
+    assert err.max() <= 2.0 ** -8 * np.sqrt(K) * np.abs(W).max() * np.abs(A).max() * 4
+    assert np.sqrt((err ** 2).mean()) <= 1e-2 * np.sqrt((ref ** 2).mean())
+
+
+@pytest.mark.parametrize("case", GATHER_CASES[:3])
+@pytest.mark.parametrize("arith", [abi.ARITH_EXACT, abi.ARITH_BF16])
+def test_recv_post_fused_update(case, arith):
+    """Gather with the LCA update in its epilogue == gather then update."""
+    rng = np.random.default_rng(26)
+    hs, os_, ht, ot, p, W, A = gather_setup(case, rng)
+    n = ht.nbatch * ht.num_res
+    V0 = (rng.standard_normal(n) * 0.1).astype(np.float32)
+    tr = R.make_transfer(0.05, 0.0, FMAX, 0.05, 0.0)
+    A0 = O.transfer(ot, V0, 0.05, 0.0, FMAX, 0.05, 0.0)
+    d = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=arith)
+    upd = R.HyPerLCAUpdate(ht, tr, 100.0, 1.0, True, device=DEV)
+    g = torch.zeros(n, device=DEV)
+    Va, Aa = dev(V0), dev(A0)
+    d.deliver(dev(A), g)
+    upd.update(g, Va, Aa)
+    Vb, Ab = dev(V0), dev(A0)
+    d.deliver(dev(A), None, fused=upd.args(None, Vb, Ab, num_channels=1))
+    assert same(host(Va), host(Vb)) and same(host(Aa), host(Ab))
+
+
+SCATTER_CASES = [
+    # pre, post, pre-perspective patch, density
+    ((2, 16, 16, 128, 2), (2, 32, 32, 3, 0), 8, 0.05),     # config[0] V1 -> Recon
+    ((2, 16, 16, 96, 2), (2, 64, 64, 1, 0), 16, 0.03),     # config[2] geometry
+    ((1, 64, 64, 16, 5), (1, 32, 32, 64, 0), 6, 0.02),     # config[1] geometry
+    ((1, 12, 12, 3, 2), (1, 12, 12, 4, 1), 5, 0.3),        # one-to-one
+    ((1, 10, 14, 5, (2, 1, 3, 2)), (1, 20, 28, 3, 0), 6, 0.2),  # asymmetric halo
+]
+
+
+@pytest.mark.parametrize("case", SCATTER_CASES)
+def test_recv_pre_exact_bit_exact(case):
+    rng = np.random.default_rng(27)
+    (hp, op), (hq, oq), p, dens = both(case[0]), both(case[1]), case[2], case[3]
+    kx, ky, kf = O.num_kernels(op, oq)
+    W = (rng.standard_normal((kx * ky * kf, p, p, oq.nf)) * 0.1).astype(np.float32)
+    A = ext_random(hp, rng, dens)
+    ent, cnt = O.active_indices(op, A)
+    ref = O.recv_pre(op, oq, p, p, W, entries=ent, num_active=cnt, num_threads=1)
+    d = R.PresynapticPerspectiveGPUDelivery(hp, hq, p, p, dev(W), arith=abi.ARITH_EXACT)
+    al = R.ActiveList(hp, DEV)
+    al.compact(dev(A))
+    g = torch.full((hq.nbatch * hq.num_res,), -1.0, device=DEV)
+    d.deliver(g, active=al)
+    assert same(host(g), ref)
+    # dense pre layer (sparseLayer = false) reads A directly: same order, same bits
+    g2 = torch.zeros_like(g)
+    d.deliver(g2, pre_activity=dev(A))
+    assert same(host(g2), ref)
+    # FP32 (FMA) form
+    d32 = R.PresynapticPerspectiveGPUDelivery(hp, hq, p, p, dev(W), arith=abi.ARITH_FP32)
+    d32.deliver(g2, active=al)
+    assert np.abs(host(g2) - ref).max() <= 1e-5 * max(1e-30, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("case", [
+    ((2, 16, 16, 32, 2), (2, 32, 32, 3, 3), 8, True),     # V1 -> Residual (config[0] shape)
+    ((1, 16, 16, 24, 2), (1, 64, 64, 1, 6), 16, False),   # config[2] geometry
+    ((1, 8, 8, 5, 1), (1, 8, 8, 2, 1), 3, True),          # one-to-one
+    ((1, 32, 32, 4, 5), (1, 16, 16, 3, 2), 6, True),      # many-to-one (4 kernels)
+])
+def test_hebbian_dw_bit_exact(case):
+    rng = np.random.default_rng(28)
+    (hp, op), (hq, oq), p, norm = both(case[0]), both(case[1]), case[2], case[3]
+    a = ext_random(hp, rng, 0.2)
+    r = ext_random(hq, rng)
+    dw, cnt = O.hebbian_dw(op, oq, p, p, oq.nf, 0.25, a, r, counts=norm)
+    hb = R.HebbianUpdater(hp, hq, p, p, 0.25, normalize_dw=norm, device=DEV)
+    hb.update_dw(dev(a), dev(r))
+    assert same(host(hb.dw), dw)
+    if norm:
+        assert (host(hb.counts) == cnt).all()
+    # normalize_dW + updateWeights
+    w = rng.standard_normal(dw.size).astype(np.float32)
+    wr, dwr = w.copy(), dw.copy()
+    O.lib().or_hebbian_normalize_apply(wr, dwr, None if cnt is None else cnt.ctypes.data, dw.size,
+                                       int(norm))
+    wd = dev(w)
+    hb.apply(wd)
+    assert same(host(wd), wr) and same(host(hb.dw), dwr)
+
+
+def test_fp64_probes_and_normalize():
+    rng = np.random.default_rng(29)
+    hl, ol = both((3, 10, 9, 7, 2))
+    buf = ext_random(hl, rng)
+    for kind, p0, p1 in [(0, 0, 0), (1, 0, 0), (2, 0.025, 0.025), (3, 0.5, 0)]:
+        out = host(R.cost_sum_f64(hl, dev(buf), True, kind, p0, p1))
+        for b in range(3):
+            assert out[b] == pytest.approx(O.cost_sum(ol, buf, True, b, kind, p0, p1), rel=1e-12)
+    w = rng.standard_normal(40 * 256).astype(np.float32)
+    wr = w.copy()
+    O.lib().or_normalize_l2_f64(wr, 40, 256, 1.0, 0.0)
+    wd = dev(w)
+    R.normalize_l2_f64(wd, 40, 256)
+    assert np.abs(host(wd) - wr).max() <= 2 ** -23
+
+
+def c0_problem(nb=1, seed=0, n=32, nv=16, nfv=128, p=8, nf=3):
+    rng = np.random.default_rng(seed)
+    img = rng.standard_normal((nb, n, n, nf)).astype(np.float32)
+    img -= img.reshape(nb, -1).mean(1).reshape(nb, 1, 1, 1).astype(np.float32)
+    W = rng.standard_normal((nfv, p, p, nf))
+    W = (W / np.sqrt((W.reshape(nfv, -1) ** 2).sum(1)).reshape(-1, 1, 1, 1)).astype(np.float32)
+    return img, W
+
+
+def oracle_cfg(col, p, vt=0.1):
+    cfg = O.LCAConfig()
+    il, rl, vl = col.in_loc, col.res_loc, col.v1_loc
+    cfg.input = O.Loc.make(il.nbatch, il.nx, il.ny, il.nf, 0)
+    cfg.res = O.Loc.make(rl.nbatch, rl.nx, rl.ny, rl.nf, rl.halo_tuple())
+    cfg.v1 = O.Loc.make(vl.nbatch, vl.nx, vl.ny, vl.nf, vl.halo_tuple())
+    cfg.nxp = cfg.nyp = p
+    cfg.VThresh, cfg.AMin, cfg.AMax, cfg.AShift, cfg.VWidth = vt, 0.0, FMAX, vt, 0.0
+    cfg.transfer_vertices = 0
+    cfg.tau, cfg.dt = 100.0, 1.0
+    cfg.selfInteract = 1
+    cfg.errScale = 1.0
+    cfg.num_threads = 1
+    return cfg
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_lca_column_config0_bit_exact_200_steps(fused):
+    """config[0]: 32x32x3 -> 16x16x128, 8x8x3 stride 2, VThresh 0.1, tau 100, 200 steps."""
+    img, W = c0_problem()
+    col = LCAColumn(img, W, 16, 16, 128, 8, 8, VThresh=0.1, arith=abi.ARITH_EXACT, fused=fused,
+                    device=DEV)
+    nact = []
+    for _ in range(200):
+        col.step()
+        nact.append(int(col.num_active()[0]))
+    V, A = col.v1_state()
+    ref = O.lca_run(oracle_cfg(col, 8), W, img, None, 200)
+    assert nact == ref["num_active"][:, 0].tolist()
+    assert same(V, ref["V"]) and same(A, ref["A"])
+    assert same(host(col.rec_A), ref["recon"])
+    e = host(col.probe_energy())
+    assert e[0] == pytest.approx(ref["energy"][-1, 0], rel=1e-12)
+
+
+def test_lca_column_batched_graph_replay():
+    """hipGraph-captured step replays to the same bits as eager steps."""
+    img, W = c0_problem(nb=4, seed=3)
+    eager = LCAColumn(img, W, 16, 16, 128, 8, 8, VThresh=0.1, arith=abi.ARITH_EXACT, fused=True,
+                      device=DEV)
+    graph = LCAColumn(img, W, 16, 16, 128, 8, 8, VThresh=0.1, arith=abi.ARITH_EXACT, fused=True,
+                      device=DEV)
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    graph.capture(s)           # capture records, it does not execute
+    for i in range(12):
+        eager.step()
+        graph.replay(s)
+        s.synchronize()
+        torch.cuda.synchronize()
+        Ve, Ae = eager.v1_state()
+        Vg, Ag = graph.v1_state()
+        assert same(Ve, Vg) and same(Ae, Ag), "replay %d differs" % i
+        assert (eager.num_active() == graph.num_active()).all()
+    graph.release_graph()
+
+
+def test_lca_column_bf16_energy_and_sparsity():
+    """BF16 MFMA delivery with FP32 state vs the EXACT column on config[0] shapes."""
+    img, W = c0_problem(nb=2, seed=5)
+    ex = LCAColumn(img, W, 16, 16, 128, 8, 8, VThresh=0.1, arith=abi.ARITH_EXACT, fused=True,
+                   device=DEV)
+    bf = LCAColumn(img, W, 16, 16, 128, 8, 8, VThresh=0.1, arith=abi.ARITH_BF16, fused=True,
+                   device=DEV)
+    for _ in range(200):
+        ex.step()
+        bf.step()
+    e_ex = host(ex.probe_energy())
+    e_bf = host(bf.probe_energy())
+    n_ex, n_bf = ex.num_active(), bf.num_active()
+    rel = np.abs(e_bf - e_ex) / np.abs(e_ex)
+    print("bf16 energy rel err", rel, "active", n_ex, n_bf, flush=True)
+    assert rel.max() <= 1e-2
+    assert (np.abs(n_bf - n_ex) <= 0.05 * n_ex + 2).all()

@@ -1,0 +1,385 @@
+"""Pins the CPU oracle (oracle/pvoracle.c) -- CPU only.
+
+Each oracle function is checked against at least one of:
+  * the reference's own code compiled from its sources (oracle/_ref: PatchGeometry,
+    the .kpp kernels, the cost functions), bit for bit;
+  * golden data the reference's tests hold (tests/PatchGeometryTest expectations);
+  * a library routine / closed form chosen so that a dropped term, wrong sign,
+    wrong index or transposed operand fails (float64 torch conv2d and
+    conv_transpose2d, autograd weight gradients, exponential relaxation).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FMAX = float(np.finfo(np.float32).max)
+
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built")
+
+GEOM_CASES = [
+    # (pre nb,nx,ny,nf,halo), (post ...), nxp, nyp, nfp
+    ((1, 16, 16, 3, 2), (1, 16, 16, 10, 2), 5, 5, 10),
+    ((1, 16, 16, 128, 2), (1, 32, 32, 3, 3), 8, 8, 3),       # C0 V1 -> Residual
+    ((1, 32, 32, 3, 3), (1, 16, 16, 128, 2), 4, 4, 128),     # C0 transpose
+    ((1, 32, 32, 512, 2), (1, 128, 128, 1, 6), 16, 16, 1),   # C2 V1 -> Residual
+    ((1, 128, 128, 1, 6), (1, 32, 32, 512, 2), 4, 4, 512),   # C2 transpose
+    ((1, 256, 256, 16, 6), (1, 128, 128, 256, 0), 6, 6, 256),  # C1 pre perspective
+    ((1, 16, 16, 2, (3, 2, 1, 4)), (1, 8, 8, 5, (1, 2, 0, 3)), 6, 4, 5),  # asymmetric halos, even patch
+    ((1, 8, 8, 2, 3), (1, 32, 32, 4, 1), 8, 12, 4),          # one-to-many stride 4
+    ((1, 32, 32, 2, 9), (1, 8, 8, 4, 0), 5, 5, 4),           # many-to-one stride 4
+]
+
+
+def L(t):
+    return O.Loc.make(*t[:4], halo=t[4])
+
+
+@needs_ref
+@pytest.mark.parametrize("case", GEOM_CASES)
+def test_patch_geometry_matches_reference(case):
+    pre, post, nxp, nyp, nfp = L(case[0]), L(case[1]), *case[2:]
+    pa, gs, ap, us = O.patch_geometry(pre, post, nxp, nyp, nfp)
+    n = pre.num_ext
+    nx_ = np.zeros(n, np.uint16)
+    ny_ = np.zeros(n, np.uint16)
+    of = np.zeros(n, np.uint32)
+    g, a, u = (np.zeros(n, np.uint64) for _ in range(3))
+    kx, ky, kf = O.num_kernels(pre, post)
+    tii = np.zeros(kx * ky * kf * nxp * nyp * nfp, np.int32)
+    m = O.ref().ref_patch_geometry(np.array(pre.as_list(), np.int32), np.array(post.as_list(), np.int32),
+                                   nxp, nyp, nfp, nx_, ny_, of, g, a, u, tii.ctypes.data)
+    assert m == n
+    assert (pa["nx"] == nx_).all() and (pa["ny"] == ny_).all() and (pa["offset"] == of).all()
+    assert (gs == g).all() and (ap == a).all() and (us == u).all()
+    mine = np.array([O.lib().or_transpose_item_index(O.C.byref(pre), O.C.byref(post), nxp, nyp, nfp,
+                                                     k, i)
+                     for k in range(kx * ky * kf) for i in range(nxp * nyp * nfp)], np.int32)
+    assert (mine == tii).all()
+
+
+def test_patch_geometry_golden_cases():
+    cases = json.load(open(os.path.join(HERE, "golden", "patch_geometry_cases.json")))["cases"]
+    for c in cases:
+        pre, post = L(c["pre"]), L(c["post"])
+        for axis in (0, 1):
+            t = O.patch_axis(pre, post, c["nxp"], c["nxp"], axis)
+            assert t["dim"].tolist() == c["sizes"], c["name"]
+            assert t["pstart"].tolist() == c["starts"], c["name"]
+            assert t["post_start_res"].tolist() == c["post_res"], c["name"]
+            assert t["post_start_ext"].tolist() == c["post_ext"], c["name"]
+
+
+@needs_ref
+def test_margins_and_index_math_match_reference():
+    R = O.ref()
+    for npre, npost, p in [(32, 16, 4), (16, 32, 8), (128, 32, 4), (32, 128, 16), (16, 16, 5),
+                           (256, 128, 6), (64, 16, 3)]:
+        assert O.lib().or_requiredConvolveMargin(npre, npost, p) == R.ref_required_margin(npre, npost, p)
+    for k in range(0, 3 * 7 * 5, 7):
+        assert (O.lib().or_kIndexExtended(k, 5, 7, 3, 1, 2, 3, 4)
+                == R.ref_kindex_extended(k, 5, 7, 3, 1, 2, 3, 4))
+
+
+def _near_threshold_values(rng, n, vt):
+    v = rng.standard_normal(n).astype(np.float32) * 0.2
+    v[::7] = np.float32(vt)
+    v[1::11] = np.nextafter(np.float32(vt), np.float32(1))
+    v[2::13] = np.nextafter(np.float32(vt), np.float32(-1))
+    return v
+
+
+@needs_ref
+@pytest.mark.parametrize("self_interact", [True, False])
+def test_update_lca_matches_reference(self_interact):
+    rng = np.random.default_rng(1)
+    loc = O.Loc.make(3, 7, 5, 4, (1, 2, 2, 1))
+    n = loc.nbatch * loc.num_res
+    G = rng.standard_normal(n).astype(np.float32)
+    A = rng.standard_normal(loc.nbatch * loc.num_ext).astype(np.float32)
+    V = rng.standard_normal(n).astype(np.float32)
+    dt = np.array([1.0, 0.5, 0.25])
+    mine = O.update_lca(loc, self_interact, dt, 100.0, G, A, V)
+    ref = V.copy()
+    O.ref().ref_update_lca(np.array(loc.as_list(), np.int32), int(self_interact), dt, 100.0, G, A, ref)
+    assert np.array_equal(mine.view(np.uint32), ref.view(np.uint32))
+
+
+@needs_ref
+@pytest.mark.parametrize("params", [
+    (0.1, 0.0, FMAX, 0.1, 0.0),       # soft threshold (LCA)
+    (0.025, 0.0, FMAX, 0.0, 0.025),   # firm threshold (GPULCATest)
+    (0.1, 0.0, 0.5, 0.1, 0.0),        # with AMax
+    (-FMAX, -FMAX, FMAX, 0.0, 0.0),   # identity
+    (0.2, -0.1, 1.0, 0.05, 0.3),
+])
+def test_vertices_match_reference(params):
+    rng = np.random.default_rng(2)
+    loc = O.Loc.make(2, 9, 4, 3, 1)
+    V = _near_threshold_values(rng, loc.nbatch * loc.num_res, params[0] if params[0] > -1e30 else 0.0)
+    nv, vv, va, sl = O.set_vertices(*params)
+    mine = O.transfer(loc, V, *params, vertices=True)
+    ref = np.zeros_like(mine)
+    O.ref().ref_apply_vertices(np.array(loc.as_list(), np.int32), nv, vv, va, sl, V, ref)
+    assert np.array_equal(mine.view(np.uint32), ref.view(np.uint32))
+
+
+def test_vthresh_closed_form():
+    rng = np.random.default_rng(3)
+    loc = O.Loc.make(1, 10, 10, 4, 2)
+    V = _near_threshold_values(rng, loc.num_res, 0.1)
+    A = O.transfer(loc, V, 0.1, 0.0, FMAX, 0.1, 0.0).reshape(14, 14, 4)[2:12, 2:12].ravel()
+    expect = np.where(V >= np.float32(0.1), V - np.float32(0.1), np.float32(0)).astype(np.float32)
+    assert np.array_equal(A, expect)
+    # firm threshold: linear ramp from (VT, AMin) to (VT+VW, VT+VW-AShift), then V - AShift
+    A = O.transfer(loc, V, 0.1, 0.0, FMAX, 0.0, 0.2).reshape(14, 14, 4)[2:12, 2:12].ravel()
+    v = V.astype(np.float64)
+    ramp = (0.3 * (v - 0.1) / 0.2)
+    expect = np.where(v < 0.1, 0.0, np.where(v < 0.3, ramp, v))
+    assert np.allclose(A, expect, rtol=1e-6, atol=1e-7)
+    # soft threshold: CPU formula and GPU vertices are bit-identical (VWidth = 0)
+    Acpu = O.transfer(loc, V, 0.1, 0.0, FMAX, 0.1, 0.0)
+    Agpu = O.transfer(loc, V, 0.1, 0.0, FMAX, 0.1, 0.0, vertices=True)
+    assert np.array_equal(Acpu.view(np.uint32), Agpu.view(np.uint32))
+
+
+@needs_ref
+def test_accumulate_and_costs_match_reference():
+    rng = np.random.default_rng(4)
+    g = rng.standard_normal(3 * 50).astype(np.float32)
+    co = np.array([1.0, -1.0, 0.5], np.float32)
+    out = np.zeros(50, np.float32)
+    ref = np.zeros(50, np.float32)
+    O.lib().or_gsyn_accumulate(50, 3, co, g, out)
+    O.ref().ref_gsyn_accumulate(50, 3, co, g, ref)
+    assert np.array_equal(out, ref)
+    loc = O.Loc.make(2, 6, 5, 3, 1)
+    buf = rng.standard_normal(loc.nbatch * loc.num_ext).astype(np.float32)
+    for kind, p0, p1 in [(0, 0, 0), (1, 0, 0), (2, 0.025, 0.025), (3, 0.1, 0)]:
+        for b in range(2):
+            mine = O.cost_sum(loc, buf, True, b, kind, p0, p1)
+            x = buf.reshape(2, 7, 8, 3)[b, 1:6, 1:7].ravel().astype(np.float64)
+            ref = sum(O.ref().ref_cost(kind, float(v), p0, p1) for v in x)
+            assert mine == pytest.approx(ref, rel=1e-12, abs=1e-12)
+
+
+@pytest.mark.parametrize("case", [
+    # source (nb,nx,ny,nf,halo), target (...), post-perspective patch
+    ((2, 32, 32, 3, 3), (2, 16, 16, 128, 2), 8),      # C0 shape (Residual -> V1)
+    ((1, 64, 64, 1, 6), (1, 16, 16, 32, 2), 16),      # C2-like, stride 4
+    ((1, 16, 16, 4, 2), (1, 16, 16, 8, 0), 5),        # one-to-one
+    ((1, 32, 32, 16, 6), (1, 16, 16, 24, 0), 12),     # C1-like many-to-one
+])
+def test_recv_post_equals_conv2d(case):
+    S, T, p = L(case[0]), L(case[1]), case[2]
+    rng = np.random.default_rng(5)
+    A = np.zeros((S.nbatch, S.ny_ext, S.nx_ext, S.nf), np.float32)
+    A[:, S.up:S.up + S.ny, S.lt:S.lt + S.nx] = rng.standard_normal((S.nbatch, S.ny, S.nx, S.nf))
+    W = rng.standard_normal((T.nf, p, p, S.nf)).astype(np.float32)
+    g = O.recv_post(S, T, p, p, A, W).reshape(T.nbatch, T.ny, T.nx, T.nf)
+    # unshrunken start of target x in source extended coordinates
+    ux = O.patch_axis(T, S, p, p, 0)["unshrunken"][T.lt:T.lt + T.nx]
+    uy = O.patch_axis(T, S, p, p, 1)["unshrunken"][T.up:T.up + T.ny]
+    stride = ux[1] - ux[0]
+    x = torch.tensor(A, dtype=torch.float64).permute(0, 3, 1, 2)
+    x = x[:, :, uy[0]:, ux[0]:]
+    w = torch.tensor(W, dtype=torch.float64).permute(0, 3, 1, 2)
+    y = torch.nn.functional.conv2d(x, w, stride=int(stride))[:, :, :T.ny, :T.nx].permute(0, 2, 3, 1)
+    y = y.numpy()
+    assert np.abs(g - y).max() <= 1e-5 * np.abs(y).max()
+    # mistake detector: a transposed weight patch must not agree
+    g2 = O.recv_post(S, T, p, p, A, np.ascontiguousarray(W.transpose(0, 2, 1, 3)))
+    assert np.abs(g2.reshape(y.shape) - y).max() > 1e-3 * np.abs(y).max()
+
+
+@pytest.mark.parametrize("case", [
+    ((2, 16, 16, 128, 2), (2, 32, 32, 3, 0), 8),      # C0 V1 -> Recon
+    ((1, 16, 16, 32, 2), (1, 64, 64, 1, 0), 16),      # C2-like
+    ((1, 32, 32, 16, 6), (1, 16, 16, 24, 0), 6),      # C1-like many-to-one
+    ((1, 12, 12, 3, 2), (1, 12, 12, 4, 0), 5),        # one-to-one
+])
+def test_recv_pre_equals_conv_transpose_and_post(case):
+    pre, post, p = L(case[0]), L(case[1]), case[2]
+    rng = np.random.default_rng(6)
+    A = np.zeros((pre.nbatch, pre.ny_ext, pre.nx_ext, pre.nf), np.float32)
+    act = rng.standard_normal((pre.nbatch, pre.ny, pre.nx, pre.nf)).astype(np.float32)
+    act[rng.random(act.shape) < 0.9] = 0
+    A[:, pre.up:pre.up + pre.ny, pre.lt:pre.lt + pre.nx] = act
+    kx, ky, kf = O.num_kernels(pre, post)
+    W = rng.standard_normal((kx * ky * pre.nf, p, p, post.nf)).astype(np.float32)
+    ent, cnt = O.active_indices(pre, A)
+    g_sparse = O.recv_pre(pre, post, p, p, W, entries=ent, num_active=cnt)
+    g_dense = O.recv_pre(pre, post, p, p, W, pre_activity=A)
+    assert np.array_equal(g_sparse, g_dense)  # same order, same bits
+    # multithreaded form (per-thread GSyn) agrees to rounding
+    g_mt = O.recv_pre(pre, post, p, p, W, entries=ent, num_active=cnt, num_threads=4)
+    assert np.allclose(g_mt, g_sparse, rtol=1e-5, atol=1e-5)
+    # equals float64 brute force over the patch tables
+    pa, gs, _, _ = O.patch_geometry(pre, post, p, p, post.nf)
+    ref = np.zeros((pre.nbatch, post.num_res))
+    W2 = W.reshape(-1, p * p * post.nf).astype(np.float64)
+    for b in range(pre.nbatch):
+        for i in range(cnt[b]):
+            kex, a = int(ent[b * pre.num_ext + i]["index"]), float(ent[b * pre.num_ext + i]["value"])
+            d = O.lib().or_data_index(O.C.byref(pre), kx, ky, kex)
+            pt = pa[kex]
+            nk = int(pt["nx"]) * post.nf
+            for yy in range(pt["ny"]):
+                o = int(gs[kex]) + yy * post.nx * post.nf
+                w0 = int(pt["offset"]) + yy * p * post.nf
+                ref[b, o:o + nk] += a * W2[d, w0:w0 + nk]
+    assert np.abs(g_sparse - ref.ravel()).max() <= 1e-5 * max(1.0, np.abs(ref).max())
+
+
+def test_post_perspective_equals_pre_perspective():
+    """ReceiveFromPostTest invariant: gather with transposed weights == scatter."""
+    rng = np.random.default_rng(8)
+    nxp = 6                                 # many-to-one, pre-perspective patch
+    pre = O.Loc.make(1, 32, 32, 6, O.lib().or_requiredConvolveMargin(32, 16, nxp))  # 5
+    post = O.Loc.make(1, 16, 16, 10, 2)
+    pre_act = np.zeros((1, pre.ny_ext, pre.nx_ext, pre.nf), np.float32)
+    pre_act[:, 5:37, 5:37] = rng.standard_normal((1, 32, 32, 6))
+    W = rng.standard_normal((2 * 2 * 6, nxp, nxp, post.nf)).astype(np.float32)
+    g_pre = O.recv_pre(pre, post, nxp, nxp, W, pre_activity=pre_act)
+    nxpPost = O.lib().or_calcPostPatchSize(nxp, pre.nx, post.nx)
+    Wpost = np.zeros((post.nf, nxpPost, nxpPost, pre.nf), np.float32)
+    O.lib().or_transpose_weights_shared(O.C.byref(pre), O.C.byref(post), nxp, nxp, post.nf,
+                                        W.ravel(), Wpost.ravel())
+    g_post = O.recv_post(pre, post, nxpPost, nxpPost, pre_act, Wpost)
+    assert np.abs(g_post - g_pre).max() <= 3e-3  # the reference test's tolerance
+    assert np.abs(g_post - g_pre).max() <= 1e-5 * np.abs(g_pre).max()
+
+
+def test_single_impulse_scatter_is_patch_copy():
+    pre = O.Loc.make(1, 8, 8, 4, 2)
+    post = O.Loc.make(1, 16, 16, 3, 0)
+    W = np.random.default_rng(9).standard_normal((4, 8, 8, 3)).astype(np.float32)
+    A = np.zeros((1, 12, 12, 4), np.float32)
+    A[0, 2 + 0, 2 + 5, 2] = 1.5   # border neuron: shrunken patch
+    g = O.recv_pre(pre, post, 8, 8, W, pre_activity=A).reshape(16, 16, 3)
+    start = 2 * 5 - 3
+    exp = np.zeros((16 + 16, 16 + 16, 3), np.float32)
+    exp[8 - 3:8 + 5, 8 + start:8 + start + 8] = np.float32(1.5) * W[2]
+    assert np.array_equal(g, exp[8:24, 8:24])
+
+
+def test_active_indices_definition():
+    rng = np.random.default_rng(10)
+    loc = O.Loc.make(3, 9, 7, 5, 2)
+    A = rng.standard_normal(loc.nbatch * loc.num_ext).astype(np.float32)
+    A[rng.random(A.shape) < 0.8] = 0
+    A[5] = -0.0  # negative zero is not active
+    ent, cnt = O.active_indices(loc, A)
+    for b in range(3):
+        Ab = A[b * loc.num_ext:(b + 1) * loc.num_ext]
+        nz = np.nonzero(Ab != 0)[0]
+        assert cnt[b] == len(nz)
+        e = ent[b * loc.num_ext:b * loc.num_ext + cnt[b]]
+        assert (e["index"] == nz).all() and np.array_equal(e["value"], Ab[nz])
+
+
+def test_lca_relaxation_closed_form():
+    """Constant input G, no self interaction: V(t) -> G with V(t) = G + (V0-G) e^{-t dt/tau}."""
+    loc = O.Loc.make(1, 4, 4, 2, 0)
+    G = np.full(32, 0.7, np.float32)
+    V = np.full(32, -0.2, np.float32)
+    A = np.zeros(32, np.float32)
+    for t in range(1, 201):
+        V = O.update_lca(loc, False, np.array([1.0]), 10.0, G, A, V)
+    expect = 0.7 + (-0.2 - 0.7) * np.exp(-200 / 10.0)
+    assert np.allclose(V, expect, rtol=1e-5)
+    # fixed point with self interaction: V = G + A
+    V2 = O.update_lca(loc, True, np.array([1.0]), 10.0, G, np.full(32, 0.3, np.float32),
+                      np.full(32, 1.0, np.float32))
+    assert np.allclose(V2, 1.0, rtol=1e-6)
+
+
+@pytest.mark.parametrize("case", [
+    ((2, 16, 16, 12, 2), (2, 32, 32, 3, 3), 8),      # one-to-many (V1 -> Residual of C0)
+    ((1, 8, 8, 5, 1), (1, 8, 8, 2, 1), 3),           # one-to-one
+])
+def test_hebbian_equals_weight_gradient(case):
+    """dW (normalizeDw off) = dWMax * sum_b a_pre (x) a_post = conv weight gradient."""
+    pre, post, p = L(case[0]), L(case[1]), case[2]
+    rng = np.random.default_rng(11)
+    a = np.zeros((pre.nbatch, pre.ny_ext, pre.nx_ext, pre.nf), np.float32)
+    av = rng.standard_normal((pre.nbatch, pre.ny, pre.nx, pre.nf)).astype(np.float32)
+    av[rng.random(av.shape) < 0.7] = 0
+    a[:, pre.up:pre.up + pre.ny, pre.lt:pre.lt + pre.nx] = av
+    r = np.zeros((post.nbatch, post.ny_ext, post.nx_ext, post.nf), np.float32)
+    r[:, post.up:post.up + post.ny, post.lt:post.lt + post.nx] = rng.standard_normal(
+        (post.nbatch, post.ny, post.nx, post.nf))
+    dw, cnt = O.hebbian_dw(pre, post, p, p, post.nf, 0.5, a, r)
+    # float64 brute force through the geometry: d/dW of sum_post g * r with g = scatter(a, W)
+    Wt = torch.zeros((pre.nf, p, p, post.nf), dtype=torch.float64, requires_grad=True)
+    pa, _, apo, _ = O.patch_geometry(pre, post, p, p, post.nf)
+    kx, ky, kf = O.num_kernels(pre, post)
+    assert kx == ky == 1
+    tot = torch.zeros((), dtype=torch.float64)
+    rflat = torch.tensor(r.reshape(post.nbatch, -1), dtype=torch.float64)
+    sya = post.nx_ext * post.nf
+    for b in range(pre.nbatch):
+        for kex in np.nonzero(a[b].ravel())[0]:
+            pt = pa[kex]
+            f = kex % pre.nf
+            oy, ox = divmod(int(pt["offset"]) // post.nf, p)
+            for yy in range(pt["ny"]):
+                seg = rflat[b, int(apo[kex]) + yy * sya: int(apo[kex]) + yy * sya + pt["nx"] * post.nf]
+                tot = tot + float(a[b].ravel()[kex]) * (
+                    Wt[f, oy + yy, ox:ox + pt["nx"], :].reshape(-1) * seg).sum()
+    tot.backward()
+    ref = 0.5 * Wt.grad.numpy().ravel()
+    assert np.abs(dw - ref).max() <= 1e-5 * max(1.0, np.abs(ref).max())
+    assert cnt.sum() == sum(int(pa[k]["nx"]) * int(pa[k]["ny"]) * post.nf
+                            for b in range(pre.nbatch) for k in np.nonzero(a[b].ravel())[0])
+
+
+def test_normalize_l2():
+    rng = np.random.default_rng(12)
+    w = rng.standard_normal(24 * 50).astype(np.float32)
+    w32 = w.copy()
+    O.lib().or_normalize_l2(w32, 24, 50, 1.0, 0.0)
+    w64 = w.copy()
+    O.lib().or_normalize_l2_f64(w64, 24, 50, 2.0, 0.0)
+    n32 = np.sqrt((w32.astype(np.float64).reshape(24, 50) ** 2).sum(1))
+    n64 = np.sqrt((w64.astype(np.float64).reshape(24, 50) ** 2).sum(1))
+    assert np.allclose(n32, 1.0, rtol=1e-6) and np.allclose(n64, 2.0, rtol=1e-7)
+    # direction unchanged
+    assert np.allclose(w64.reshape(24, 50) / n64[:, None], w.reshape(24, 50) /
+                       np.sqrt((w.astype(np.float64).reshape(24, 50) ** 2).sum(1))[:, None], atol=1e-6)
+
+
+def test_lca_loop_descends_energy_and_is_sparse():
+    rng = np.random.default_rng(13)
+    nb, n, nf, nv, nfv, p = 1, 16, 3, 8, 24, 8
+    img = rng.standard_normal((nb, n, n, nf)).astype(np.float32)
+    img -= img.mean()
+    W = rng.standard_normal((nfv, p, p, nf)).astype(np.float32)
+    W /= np.sqrt((W.reshape(nfv, -1) ** 2).sum(1)).reshape(-1, 1, 1, 1)
+    cfg = lca_cfg(nb, n, nf, nv, nfv, p)
+    out = O.lca_run(cfg, W, img, None, 60)
+    e = out["energy"][:, 0]
+    assert e[-1] < e[5] < e[0]
+    assert 0 < out["num_active"][-1, 0] < nv * nv * nfv
+
+
+def lca_cfg(nb, n, nf, nv, nfv, p, vt=0.1, tau=100.0):
+    cfg = O.LCAConfig()
+    cfg.input = O.Loc.make(nb, n, n, nf, 0)
+    mres = O.lib().or_requiredConvolveMargin(n, nv, O.lib().or_calcPostPatchSize(p, nv, n))
+    mv1 = O.lib().or_requiredConvolveMargin(nv, n, p)
+    cfg.res = O.Loc.make(nb, n, n, nf, mres)
+    cfg.v1 = O.Loc.make(nb, nv, nv, nfv, mv1)
+    cfg.nxp = cfg.nyp = p
+    cfg.VThresh, cfg.AMin, cfg.AMax, cfg.AShift, cfg.VWidth = vt, 0.0, FMAX, vt, 0.0
+    cfg.transfer_vertices = 0
+    cfg.tau, cfg.dt = tau, 1.0
+    cfg.selfInteract = 1
+    cfg.errScale = 1.0
+    cfg.num_threads = 1
+    return cfg
