@@ -1,0 +1,263 @@
+"""LCA steps/s + synaptic ops/s on the batched-LCA workload, BF16-MFMA vs FP32.
+
+Workload (the metric's configuration): nbatch 64 grayscale 128x128x1 images
+with a 1/f^2 power spectrum (random phase, unit variance), V1 32x32x512 with
+16x16 patches at stride 4, VThresh 0.15, tau 100, dt 1, soft threshold.  One
+step is one LCA timestep of the Input -> Residual <-> V1 -> Recon column over
+the whole batch: gather (post perspective) + LCA update + ordered active list
++ sparse scatter (pre perspective) + residual, replayed as one hipGraph.
+
+Forms timed: EXACT (bit-identical to the reference CPU order), FP32 (FMA) and
+BF16 (MFMA gather, FP32 state).  `value` is the BF16 form's steps/s; the line
+also carries the FP32 forms, synaptic ops/s, the BF16 energy error against
+EXACT, the sparsity difference, a roofline record for the dominant kernel and
+a CPU baseline (the oracle on the host cores, bounded sample).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NB, NX, NY, NF = 64, 128, 128, 1
+NXV, NYV, NFV, NXP = 32, 32, 512, 16
+VTHRESH, TAU, DT = 0.15, 100.0, 1.0
+HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def one_over_f_images(nb, ny, nx, seed):
+    """1/f^2 power spectrum (amplitude 1/f), random phase, zero mean, unit variance."""
+    rng = np.random.default_rng(seed)
+    fy = np.fft.fftfreq(ny)[:, None]
+    fx = np.fft.fftfreq(nx)[None, :]
+    f = np.sqrt(fx * fx + fy * fy)
+    amp = np.zeros_like(f)
+    amp[f > 0] = 1.0 / f[f > 0]
+    out = np.empty((nb, ny, nx, 1), np.float32)
+    for b in range(nb):
+        phase = np.exp(2j * np.pi * rng.random((ny, nx)))
+        img = np.real(np.fft.ifft2(amp * phase))
+        img -= img.mean()
+        img /= img.std()
+        out[b, :, :, 0] = img
+    return out
+
+
+def dictionary(nf, nxp, nfp, seed):
+    """~N(0,1) then L2-normalised per feature in FP64."""
+    rng = np.random.default_rng(seed)
+    W = rng.standard_normal((nf, nxp, nxp, nfp))
+    W /= np.sqrt((W.reshape(nf, -1) ** 2).sum(1)).reshape(-1, 1, 1, 1)
+    return W.astype(np.float32)
+
+
+def scatter_macs(col):
+    """Sum over active V1 neurons of their (shrunken) patch size: the pre-perspective
+    scatter's synaptic operations for the current active lists."""
+    import ctypes as C
+    from openpv_amd import abi
+    lp, lq = col.v1_loc, col.rec_loc
+    dims = []
+    for axis, n in ((0, lp.nx_ext), (1, lp.ny_ext)):
+        d = np.zeros(n, np.int32)
+        abi.host().pvhost_patch_axis(C.byref(lp), C.byref(lq), NXP, NXP, axis,
+                                     d.ctypes.data_as(C.c_void_p), None, None, None, None)
+        dims.append(d)
+    e, n = col.active.host_entries()
+    total = 0
+    for b in range(lp.nbatch):
+        idx = e[b, :n[b]]["index"].astype(np.int64) // lp.nf
+        xe, ye = idx % lp.nx_ext, idx // lp.nx_ext
+        total += int((dims[0][xe] * dims[1][ye]).sum()) * lq.nf
+    return total
+
+
+def time_form(arith, images, W, steps, warmup, torch):
+    from openpv_amd.lca import LCAColumn
+    col = LCAColumn(images, W, NXV, NYV, NFV, NXP, NXP, VThresh=VTHRESH, timeConstantTau=TAU,
+                    dt=DT, arith=arith, fused=True, device="cuda")
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    col.capture(s)
+    for _ in range(warmup):
+        col.replay(s)
+    s.synchronize()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        col.replay(s)
+    e1.record(s)
+    s.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    return col, s, ms / steps, wall
+
+
+def kernel_timing(col, s, n, torch):
+    """Average duration of the fused gather + LCA-update launch, on its own
+    stream, bracketed by HIP events (eager launches of the same step)."""
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(n)]
+    with torch.cuda.stream(s):
+        for a, b in evs:
+            col.step(stream=s, gather_events=(a, b))
+    s.synchronize()
+    torch.cuda.synchronize()
+    d = [a.elapsed_time(b) for a, b in evs]
+    return float(np.mean(d)), float(np.min(d))
+
+
+def cpu_baseline(images, W, budget_s=12.0):
+    """The oracle (C restatement of the reference loops, OpenMP over the host
+    cores) on a bounded sample of the workload: a few images, a few steps."""
+    import oracle as O
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, 16)
+    nb = 2
+    cfg = O.LCAConfig()
+    mres = O.lib().or_requiredConvolveMargin(NX, NXV, O.lib().or_calcPostPatchSize(NXP, NXV, NX))
+    mv1 = O.lib().or_requiredConvolveMargin(NXV, NX, NXP)
+    cfg.input = O.Loc.make(nb, NX, NY, NF, 0)
+    cfg.res = O.Loc.make(nb, NX, NY, NF, mres)
+    cfg.v1 = O.Loc.make(nb, NXV, NYV, NFV, mv1)
+    cfg.nxp = cfg.nyp = NXP
+    fmax = float(np.finfo(np.float32).max)
+    cfg.VThresh, cfg.AMin, cfg.AMax, cfg.AShift, cfg.VWidth = VTHRESH, 0.0, fmax, VTHRESH, 0.0
+    cfg.transfer_vertices = 0
+    cfg.tau, cfg.dt = TAU, DT
+    cfg.selfInteract = 1
+    cfg.errScale = 1.0
+    cfg.num_threads = cores
+    img = np.ascontiguousarray(images[:nb])
+    O.lca_run(cfg, W, img, None, 1)  # warm (page-in, thread pool)
+    steps, elapsed = 1, 0.0
+    while True:
+        t0 = time.perf_counter()
+        O.lca_run(cfg, W, img, None, steps)
+        elapsed = time.perf_counter() - t0
+        if elapsed >= budget_s or steps >= 64:
+            break
+        steps = max(steps + 1, int(steps * min(4.0, budget_s / max(elapsed, 1e-3))))
+    per_image_step = elapsed / (steps * nb)
+    return {"value": 1.0 / (per_image_step * NB), "unit": "LCA steps/s", "cores": cores,
+            "kind": "port",
+            "sample": "oracle LCA loop, %d images x %d steps of the 64-image workload (%.1f s), "
+                      "scaled to the 64-image batch" % (nb, steps, elapsed)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--forms", default="exact,fp32,bf16")
+    ap.add_argument("--profile-copy", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (profiler counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (the batch would "
+              "shard over GPUs with one dW all-reduce per update, not built here).",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from openpv_amd import abi
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    abi.hip()
+    images = one_over_f_images(NB, NY, NX, seed=1)
+    W = dictionary(NFV, NXP, NF, seed=2)
+    forms = [f for f in args.forms.split(",") if f]
+    arith = {"exact": abi.ARITH_EXACT, "fp32": abi.ARITH_FP32, "bf16": abi.ARITH_BF16}
+    res = {}
+    cols = {}
+    for f in forms:
+        col, s, ms, wall = time_form(arith[f], images, W, args.steps, args.warmup, torch)
+        res[f] = {"ms_per_step": ms, "steps_per_s": 1000.0 / ms, "wall_s": wall}
+        cols[f] = (col, s)
+        print("# %s: %.4f ms/step (%.1f steps/s)" % (f, ms, 1000.0 / ms), file=sys.stderr,
+              flush=True)
+    head = "bf16" if "bf16" in res else forms[-1]
+    col, s = cols[head]
+    gather_macs = NB * NXV * NYV * NFV * NXP * NXP * NF
+    smacs = scatter_macs(col)
+    ms = res[head]["ms_per_step"]
+    # energy and sparsity of the BF16 form against the EXACT form (same steps)
+    extra = {}
+    if "exact" in cols and head != "exact":
+        e_ref = cols["exact"][0].probe_energy().cpu().numpy()
+        e_lp = col.probe_energy().cpu().numpy()
+        n_ref = cols["exact"][0].num_active().astype(np.float64)
+        n_lp = col.num_active().astype(np.float64)
+        extra["energy_rel_err_vs_exact"] = float(np.max(np.abs(e_lp - e_ref) / np.abs(e_ref)))
+        extra["active_rel_diff_vs_exact"] = float(np.max(np.abs(n_lp - n_ref) / np.maximum(n_ref, 1)))
+        extra["active_mean"] = float(n_lp.mean())
+    for f in res:
+        res[f]["steps_per_s"] = round(res[f]["steps_per_s"], 2)
+        res[f]["ms_per_step"] = round(res[f]["ms_per_step"], 5)
+        res[f]["wall_s"] = round(res[f]["wall_s"], 3)
+    # dominant kernel: fused gather + LCA update of the headline form
+    kavg, kmin = kernel_timing(col, s, 50, torch)
+    n_v1 = NB * NXV * NYV * NFV
+    src_bytes = NB * col.res_loc.num_ext * 4
+    w_bytes = NFV * NXP * NXP * NF * (2 if head == "bf16" else 4)
+    alg_bytes = n_v1 * 12 + src_bytes + w_bytes   # V read+write, A write, source, weights
+    achieved = alg_bytes / (kavg * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "recv_post_mfma_bf16 (fused LCA epilogue)" if head == "bf16" else "recv_post",
+            "kernel_ms": round(kavg, 5), "kernel_ms_min": round(kmin, 5),
+            "algorithmic_bytes": int(alg_bytes),
+            "mfma_tflops": round(2.0 * gather_macs / (kavg * 1e-3) / 1e12, 1)}
+    line = {
+        "metric": "LCA steps/s + synaptic ops/s, BF16-MFMA vs FP32 GPU at <=1e-3 rel energy err",
+        "value": round(1000.0 / ms, 2), "unit": "LCA steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": head if head != "exact" else "fp32",
+        "data": "synthetic (1/f^2 images, random N(0,1) L2-normalised dictionary)",
+        "config": {"workload": "batched LCA: 64 x 128x128x1 -> 32x32x512, 16x16 stride 4, "
+                               "VThresh 0.15, one hipGraph per timestep",
+                   "global_batch": NB, "parallelism": "1 GPU"},
+        "synaptic_ops_per_s": round((gather_macs + smacs) * 1000.0 / ms, 1),
+        "gather_macs_per_step": gather_macs, "scatter_macs_per_step": smacs,
+        "forms": res,
+    }
+    if "fp32" in res and head == "bf16":
+        line["bf16_vs_fp32"] = round(res["fp32"]["ms_per_step"] / res["bf16"]["ms_per_step"], 3)
+    if "exact" in res and head == "bf16":
+        line["bf16_vs_exact"] = round(res["exact"]["ms_per_step"] / res["bf16"]["ms_per_step"], 3)
+    line.update(extra)
+    line["roofline"] = roof
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(images, W)
+        line["gpu_vs_cpu_exact"] = (round(res["exact"]["steps_per_s"] / line["cpu_baseline"]["value"], 1)
+                                    if "exact" in res else None)
+    print(json.dumps(line), flush=True)
+    if args.profile_copy:
+        # a plain copy kernel of known size: FETCH_SIZE / WRITE_SIZE of this
+        # dispatch calibrate the counters of the kernels above
+        n = args.profile_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("# copy calibration: %d bytes read, %d bytes written per dispatch" % (4 * n, 4 * n),
+              file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main()

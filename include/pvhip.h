@@ -228,6 +228,14 @@ typedef struct {
     * [kernels][Kpad] with Kpad = patch size rounded up to 32 (the shadow copy
     * plays the role of the reference's cuDNN weight copy, Weights.cpp:333-341). */
    const void *weights_lp;
+   /* Targets larger than the source (data patches per feature > 1) on the MFMA
+    * path: the window starts of start_x / start_y span [window_x0,
+    * window_x0 + window_nx) (resp. y); inverse_x[kx*window_nx + (sx - window_x0)]
+    * is the target x with kernel x-index kx and window start sx, or -1
+    * (pvhost_recv_post_inverse).  May be NULL otherwise. */
+   const int32_t *inverse_x;
+   const int32_t *inverse_y;
+   int window_x0, window_nx, window_y0, window_ny;
    /* Optional fused LCA epilogue (requires accumulate == 0, one channel,
     * coeff 1): the gather result is the accumulated G of the update.  post_gsyn
     * may then be NULL (GSyn is not materialised). */

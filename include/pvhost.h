@@ -41,6 +41,14 @@ int pvhost_patch_axis(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int 
 int pvhost_recv_post_tables(const pvhip_loc *source, const pvhip_loc *target, int nxp, int nyp,
       int32_t *start_x, int32_t *start_y, int32_t *kernel_x, int32_t *kernel_y, int *nkx, int *nky);
 
+/* Inverse tables of pvhip_recv_post_args for the MFMA path (any number of
+ * data patches per feature): *window_x0/*window_nx (and y) receive the range
+ * of window starts; inverse_x (nkx*window_nx ints) and inverse_y may be NULL
+ * on a first call that only queries the ranges. */
+int pvhost_recv_post_inverse(const pvhip_loc *source, const pvhip_loc *target, int nxp, int nyp,
+      int32_t *inverse_x, int32_t *inverse_y, int *window_x0, int *window_nx, int *window_y0,
+      int *window_ny);
+
 /* Coverage tables of pvhip_recv_pre_args (CSR).  Call once with NULL arrays
  * to get *ncol and *nrow, then with arrays of those sizes (col_ptr has
  * post.nx + 1 entries, row_ptr post.ny*nyp + 1). */

@@ -85,6 +85,8 @@ class RecvPostArgs(C.Structure):
                 ("start_x", vp), ("start_y", vp), ("kernel_x", vp), ("kernel_y", vp),
                 ("dt_factor", f32), ("pre_data", vp), ("weights", vp), ("post_gsyn", vp),
                 ("accumulate", i32), ("arith", i32), ("weights_lp", vp),
+                ("inverse_x", vp), ("inverse_y", vp), ("window_x0", i32), ("window_nx", i32),
+                ("window_y0", i32), ("window_ny", i32),
                 ("fused_update", C.POINTER(LCAUpdateArgs))]
 
 
@@ -163,6 +165,9 @@ PVHOST_SYMBOLS = {
     "pvhost_patch_axis": (i32, [C.POINTER(Loc), C.POINTER(Loc), i32, i32, i32, vp, vp, vp, vp, vp]),
     "pvhost_recv_post_tables": (i32, [C.POINTER(Loc), C.POINTER(Loc), i32, i32, vp, vp, vp, vp,
                                       C.POINTER(i32), C.POINTER(i32)]),
+    "pvhost_recv_post_inverse": (i32, [C.POINTER(Loc), C.POINTER(Loc), i32, i32, vp, vp,
+                                       C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                       C.POINTER(i32)]),
     "pvhost_recv_pre_tables": (i32, [C.POINTER(Loc), C.POINTER(Loc), i32, i32, vp, vp, vp, vp,
                                      C.POINTER(i32), vp, vp, vp, vp, C.POINTER(i32)]),
     "pvhost_make_transfer": (i32, [f32, f32, f32, f32, f32, i32, C.POINTER(Transfer)]),

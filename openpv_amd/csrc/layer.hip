@@ -35,9 +35,12 @@ namespace {
 __device__ __forceinline__ float accumulate_channels(
       int numChannels, const float *coeffs, const float *gsyn, size_t n, size_t index) {
    float value = 0.0f;
-   for (int ch = 0; ch < numChannels; ch++) {
-      float prod = coeffs[ch] * gsyn[(size_t)ch * n + index];
-      value      = value + prod;
+#pragma unroll
+   for (int ch = 0; ch < PVHIP_MAX_CHANNELS; ch++) {
+      if (ch < numChannels) {
+         float prod = coeffs[ch] * gsyn[(size_t)ch * n + index];
+         value      = value + prod;
+      }
    }
    return value;
 }
@@ -133,8 +136,29 @@ __global__ void __launch_bounds__(256)
 
 constexpr int kCompactThreads = 256;
 constexpr int kCompactIters   = 16;
-constexpr int kChunk          = kCompactThreads * kCompactIters;
+constexpr int kCompactVec     = 4; // elements per thread per iteration
+constexpr int kChunk          = kCompactThreads * kCompactIters * kCompactVec;
 
+// Loads elements [e, e+VEC) of one batch element's extended buffer (zeros past
+// the end); VEC == 4 requires numExt % 4 == 0 (16-byte aligned rows).
+template <int VEC>
+__device__ __forceinline__ void load_vec(const float *Ab, int e, int numExt, float (&a)[kCompactVec]) {
+   if (VEC == 4) {
+      float4 v = e < numExt ? *reinterpret_cast<const float4 *>(Ab + e) : make_float4(0.f, 0.f, 0.f, 0.f);
+      a[0] = v.x;
+      a[1] = v.y;
+      a[2] = v.z;
+      a[3] = v.w;
+   }
+   else {
+#pragma unroll
+      for (int j = 0; j < kCompactVec; j++) {
+         a[j] = e + j < numExt ? Ab[e + j] : 0.0f;
+      }
+   }
+}
+
+template <int VEC>
 __global__ void __launch_bounds__(kCompactThreads)
       compact_count_kernel(const float *A, int numExt, uint32_t *chunkCount, int chunksPerBatch) {
    int const b         = blockIdx.y;
@@ -142,13 +166,16 @@ __global__ void __launch_bounds__(kCompactThreads)
    float const *Ab     = A + (size_t)b * numExt;
    int const base      = chunk * kChunk;
    int count           = 0;
+#pragma unroll 4
    for (int it = 0; it < kCompactIters; it++) {
-      int e = base + it * kCompactThreads + threadIdx.x;
-      if (e < numExt && Ab[e] != 0.0f) {
-         count++;
+      int e = base + (it * kCompactThreads + threadIdx.x) * kCompactVec;
+      float a[kCompactVec];
+      load_vec<VEC>(Ab, e, numExt, a);
+#pragma unroll
+      for (int j = 0; j < kCompactVec; j++) {
+         count += a[j] != 0.0f ? 1 : 0;
       }
    }
-   // Workgroup reduction.
    __shared__ int red[kCompactThreads / kWave];
    for (int off = kWave / 2; off > 0; off >>= 1) {
       count += __shfl_down(count, off, kWave);
@@ -201,6 +228,7 @@ __global__ void __launch_bounds__(1024)
    }
 }
 
+template <int VEC>
 __global__ void __launch_bounds__(kCompactThreads) compact_write_kernel(
       const float *A, int numExt, int nf, const uint32_t *chunkOffset, int chunksPerBatch,
       pvhip_sparse_entry *entries, uint32_t *posStart, int numPos, const int64_t *numActive) {
@@ -215,31 +243,48 @@ __global__ void __launch_bounds__(kCompactThreads) compact_write_kernel(
    int const wave   = threadIdx.x / kWave;
    __shared__ uint32_t waveCount[kCompactThreads / kWave];
    for (int it = 0; it < kCompactIters; it++) {
-      int e     = base + it * kCompactThreads + threadIdx.x;
-      float a   = e < numExt ? Ab[e] : 0.0f;
-      bool flag = e < numExt && a != 0.0f;
-      uint64_t mask   = __ballot(flag);
-      uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-      if (lane == 0) {
-         waveCount[wave] = (uint32_t)__popcll(mask);
+      int e = base + (it * kCompactThreads + threadIdx.x) * kCompactVec;
+      float a[kCompactVec];
+      load_vec<VEC>(Ab, e, numExt, a);
+      uint32_t mine = 0;
+#pragma unroll
+      for (int j = 0; j < kCompactVec; j++) {
+         mine += (e + j < numExt && a[j] != 0.0f) ? 1u : 0u;
+      }
+      // inclusive prefix of the per-thread counts across the wave
+      uint32_t incl = mine;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+         uint32_t v = __shfl_up(incl, off, kWave);
+         if (lane >= off) {
+            incl += v;
+         }
+      }
+      if (lane == kWave - 1) {
+         waveCount[wave] = incl;
       }
       __syncthreads();
-      uint32_t waveBase = 0;
-      uint32_t total    = 0;
+      uint32_t waveBase = 0, total = 0;
+#pragma unroll
       for (int w = 0; w < kCompactThreads / kWave; w++) {
          uint32_t c = waveCount[w];
-         if (w < wave) {
-            waveBase += c;
-         }
+         waveBase += w < wave ? c : 0u;
          total += c;
       }
-      uint32_t pos = run + waveBase + before;
-      if (flag) {
-         Eb[pos].index = (uint32_t)e;
-         Eb[pos].value = a;
-      }
-      if (Pb && e < numExt && (e % nf) == 0) {
-         Pb[e / nf] = pos;
+      uint32_t pos = run + waveBase + incl - mine; // entries before element e
+#pragma unroll
+      for (int j = 0; j < kCompactVec; j++) {
+         int ej = e + j;
+         if (ej < numExt) {
+            if (Pb && (ej % nf) == 0) {
+               Pb[ej / nf] = pos;
+            }
+            if (a[j] != 0.0f) {
+               Eb[pos].index = (uint32_t)ej;
+               Eb[pos].value = a[j];
+               pos++;
+            }
+         }
       }
       run += total;
       __syncthreads();
@@ -462,14 +507,27 @@ int pvhip_active_compact(
    PVHIP_REQUIRE(chunk_scratch != nullptr, "active_compact: NULL chunk_scratch");
    uint32_t *scratch = chunk_scratch;
    dim3 grid(chunks, g.nbatch);
-   hipLaunchKernelGGL(compact_count_kernel, grid, dim3(kCompactThreads), 0, (hipStream_t)stream,
-         A, numExt, scratch, chunks);
+   bool const vec4 = numExt % 4 == 0 && ((uintptr_t)A & 15) == 0;
+   if (vec4) {
+      hipLaunchKernelGGL(compact_count_kernel<4>, grid, dim3(kCompactThreads), 0, (hipStream_t)stream,
+            A, numExt, scratch, chunks);
+   }
+   else {
+      hipLaunchKernelGGL(compact_count_kernel<1>, grid, dim3(kCompactThreads), 0, (hipStream_t)stream,
+            A, numExt, scratch, chunks);
+   }
    PVHIP_LAUNCHED("compact_count");
    hipLaunchKernelGGL(compact_scan_kernel, dim3(g.nbatch), dim3(1024), 0, (hipStream_t)stream,
          scratch, chunks, num_active);
    PVHIP_LAUNCHED("compact_scan");
-   hipLaunchKernelGGL(compact_write_kernel, grid, dim3(kCompactThreads), 0, (hipStream_t)stream,
-         A, numExt, g.nf, scratch, chunks, entries, pos_start, numPos, num_active);
+   if (vec4) {
+      hipLaunchKernelGGL(compact_write_kernel<4>, grid, dim3(kCompactThreads), 0, (hipStream_t)stream,
+            A, numExt, g.nf, scratch, chunks, entries, pos_start, numPos, num_active);
+   }
+   else {
+      hipLaunchKernelGGL(compact_write_kernel<1>, grid, dim3(kCompactThreads), 0, (hipStream_t)stream,
+            A, numExt, g.nf, scratch, chunks, entries, pos_start, numPos, num_active);
+   }
    PVHIP_LAUNCHED("compact_write");
    return PVHIP_OK;
 }

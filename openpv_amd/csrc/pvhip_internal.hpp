@@ -107,53 +107,73 @@ inline int make_transfer(const pvhip_transfer &t, Transfer *out) {
    return PVHIP_OK;
 }
 
+// A = transfer(V), VTHRESH form: HyPerActivityBuffer copy, applyVThresh,
+// applyAMax (src/components/ANNActivityBuffer.cpp:490-556).
+__device__ __forceinline__ float apply_vthresh(const Transfer &t, float V) {
+   float a = V;
+   if (t.VThresh > -FLT_MAX) {
+      if (V < t.VThresh) {
+         a = t.AMin;
+      }
+      else if (V < t.VThresh + t.VWidth) {
+         float span = t.VThresh + t.VWidth;
+         span       = span - t.AShift;
+         span       = span - t.AMin;
+         float num  = span * (V - t.VThresh);
+         a          = t.AMin + num / t.VWidth;
+      }
+      else {
+         a = a - t.AShift;
+      }
+   }
+   if (t.AMax < FLT_MAX) {
+      if (a > t.AMax) {
+         a = t.AMax;
+      }
+   }
+   return a;
+}
+
 // A = transfer(V) with the reference's exact float operation order.
-//  VTHRESH: HyPerActivityBuffer copy, applyVThresh, applyAMax
-//           (src/components/ANNActivityBuffer.cpp:490-556).
+//  VTHRESH: apply_vthresh.
 //  VERTICES: applyVerticesANNActivityBufferOnGPU (src/components/ANNActivityBuffer.kpp:45-76).
 __device__ __forceinline__ float apply_transfer(const Transfer &t, float V) {
    if (t.mode == PVHIP_TRANSFER_VTHRESH) {
-      float a = V;
-      if (t.VThresh > -FLT_MAX) {
-         if (V < t.VThresh) {
-            a = t.AMin;
-         }
-         else if (V < t.VThresh + t.VWidth) {
-            float span = t.VThresh + t.VWidth;
-            span       = span - t.AShift;
-            span       = span - t.AMin;
-            float num  = span * (V - t.VThresh);
-            a          = t.AMin + num / t.VWidth;
-         }
-         else {
-            a = a - t.AShift;
-         }
-      }
-      if (t.AMax < FLT_MAX) {
-         if (a > t.AMax) {
-            a = t.AMax;
-         }
-      }
-      return a;
+      return apply_vthresh(t, V);
    }
+   // Vertex arrays are read with compile-time indices only (unrolled over
+   // the maximum), so the descriptor stays in registers / kernel arguments.
    int const last = t.numVertices - 1;
+   float vLast = t.vV[0], aLast = t.vA[0], sLast = t.slopes[1];
+#pragma unroll
+   for (int i = 0; i < PVHIP_MAX_VERTICES; i++) {
+      if (i == last) {
+         vLast = t.vV[i];
+         aLast = t.vA[i];
+         sLast = t.slopes[i + 1];
+      }
+   }
    float activity = 0.0f;
    if (V < t.vV[0]) {
       activity = t.vA[0] + t.slopes[0] * (V - t.vV[0]);
    }
-   else if (V >= t.vV[last]) {
-      activity = t.vA[last] + t.slopes[t.numVertices] * (V - t.vV[last]);
+   else if (V >= vLast) {
+      activity = aLast + sLast * (V - vLast);
    }
    else {
-      for (int v = 0; v < last; v++) {
-         if (V < t.vV[v]) {
-            break;
-         }
-         if (V == t.vV[v]) {
-            activity = t.vA[v];
-         }
-         else if (V > t.vV[v] && V < t.vV[v + 1]) {
-            activity = t.vA[v] + t.slopes[v + 1] * (V - t.vV[v]);
+      bool stop = false;
+#pragma unroll
+      for (int v = 0; v < PVHIP_MAX_VERTICES - 1; v++) {
+         if (!stop && v < last) {
+            if (V < t.vV[v]) {
+               stop = true; // jumps are continuous from the right
+            }
+            else if (V == t.vV[v]) {
+               activity = t.vA[v];
+            }
+            else if (V > t.vV[v] && V < t.vV[v + 1]) {
+               activity = t.vA[v] + t.slopes[v + 1] * (V - t.vV[v]);
+            }
          }
       }
    }

@@ -215,35 +215,114 @@ __global__ void __launch_bounds__(256) recv_post_tiled_kernel(PostParams p, int 
 
 // ------------------------------------------------------------------ mfma (bf16)
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Workgroup: 64 target positions (8x8) x MF features of one batch element,
-// 4 waves; wave w owns features [w*MF/4, (w+1)*MF/4) for all 64 positions.
+// Small-channel implicit GEMM (sources whose patch rows hold nxp*nfp values,
+// a multiple of 8): M = features (one data patch per feature), N = target
+// positions of a 16x8 tile, K = nyp*nxp*nfp in the weights' own order.
+// Workgroup: 4 waves; wave w owns MW features for all 128 positions.
 // Fragment layouts of v_mfma_f32_16x16x32_bf16 (lane l, r = l%16, g = l/16):
 //   A[r][8g..8g+7], B[8g..8g+7][r], D[4g..4g+3][r].
-constexpr int MF       = 128;          // features per workgroup
-constexpr int WAVE_F   = MF / 4;       // features per wave (2 blocks of 16)
-constexpr int FBLK     = WAVE_F / 16;  // 16-row blocks per wave
-constexpr int PBLK     = 4;            // 16-position blocks (64 positions)
+// The source tile is staged once in LDS as bf16; a B fragment is 8
+// consecutive k of one patch row, i.e. 8 consecutive LDS values, read with
+// the widest load the tile's alignment class ALIGN (elements) allows.
+constexpr int MTX = 16, MTY = 8;        // position tile
+constexpr int MPB = MTX * MTY / 16;     // 16-position blocks per wave
+constexpr int MF  = 128;                // features per workgroup
+constexpr int MFB = MF / 4 / 16;        // 16-feature blocks per wave
 
-__device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
+template <int ALIGN>
+__device__ __forceinline__ bf16x8 lds_frag(const __bf16 *p) {
+   bf16x8 v;
+   if (ALIGN >= 8) {
+      v = *reinterpret_cast<const bf16x8 *>(p);
+   }
+   else if (ALIGN >= 4) {
+      bf16x4 a = *reinterpret_cast<const bf16x4 *>(p);
+      bf16x4 b = *reinterpret_cast<const bf16x4 *>(p + 4);
+      v        = bf16x8{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+   }
+   else if (ALIGN >= 2) {
+      bf16x2 a = *reinterpret_cast<const bf16x2 *>(p);
+      bf16x2 b = *reinterpret_cast<const bf16x2 *>(p + 2);
+      bf16x2 c = *reinterpret_cast<const bf16x2 *>(p + 4);
+      bf16x2 d = *reinterpret_cast<const bf16x2 *>(p + 6);
+      v        = bf16x8{a[0], a[1], b[0], b[1], c[0], c[1], d[0], d[1]};
+   }
+   else {
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+         v[e] = p[e];
+      }
+   }
+   return v;
+}
 
+// Four consecutive features of one neuron position: GSyn store or the fused
+// LCA update with 16-byte loads and stores.  EPI: 0 GSyn only, 1 fused update
+// with the VTHRESH transfer, 2 fused update with either transfer.
+template <int EPI>
+__device__ __forceinline__ void post_epilogue4(const PostParams &p, int b, int k, f32x4 acc) {
+   size_t const gi = (size_t)b * p.post.numRes() + k;
+   float G[4];
+   if (p.accumulate) {
+      float4 base = *reinterpret_cast<const float4 *>(p.gsyn + gi);
+      G[0] = base.x + p.dtf * acc[0];
+      G[1] = base.y + p.dtf * acc[1];
+      G[2] = base.z + p.dtf * acc[2];
+      G[3] = base.w + p.dtf * acc[3];
+   }
+   else {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+         G[i] = 0.0f + p.dtf * acc[i];
+      }
+   }
+   if (EPI == 0) {
+      *reinterpret_cast<float4 *>(p.gsyn + gi) = make_float4(G[0], G[1], G[2], G[3]);
+      return;
+   }
+   const LCAParams &q = p.lca;
+   float const e      = q.expTau[b];
+   size_t const ai    = (size_t)b * q.loc.numExt() + q.loc.kex(k);
+   float4 Ao          = *reinterpret_cast<const float4 *>(q.A + ai);
+   float4 Vo          = *reinterpret_cast<const float4 *>(q.V + gi);
+   float const ao[4]  = {Ao.x, Ao.y, Ao.z, Ao.w};
+   float const vo[4]  = {Vo.x, Vo.y, Vo.z, Vo.w};
+   float vn[4], an[4];
+#pragma unroll
+   for (int i = 0; i < 4; i++) {
+      float t1 = e * vo[i];
+      float t2 = 1.0f - e;
+      float si = q.selfInteract * ao[i];
+      float t3 = G[i] + si;
+      float t4 = t2 * t3;
+      vn[i]    = t1 + t4;
+      an[i]    = EPI == 1 ? apply_vthresh(q.transfer, vn[i]) : apply_transfer(q.transfer, vn[i]);
+   }
+   *reinterpret_cast<float4 *>(q.V + gi) = make_float4(vn[0], vn[1], vn[2], vn[3]);
+   *reinterpret_cast<float4 *>(q.A + ai) = make_float4(an[0], an[1], an[2], an[3]);
+}
+
+template <int ALIGN, int EPI>
 __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
       PostParams p, const __bf16 *Wb, int srcRowsMax, int srcColsMax, int Kpad) {
    extern __shared__ float smem[];
-   int const K       = p.nxp * p.nfp * p.nyp; // full patch
+   int const K       = p.nxp * p.nfp * p.nyp;
    int const rowK    = p.nxp * p.nfp;
-   int const pitch   = srcColsMax * p.nfp;
-   float *src        = smem;                                        // fp32 source tile
-   int *koff         = reinterpret_cast<int *>(smem + srcRowsMax * pitch); // [Kpad]
-   int const tilesX  = (p.post.nx + TX - 1) / TX;
+   int const pitch   = (srcColsMax * p.nfp + 7) / 8 * 8; // bf16 elements per LDS row
+   __bf16 *src       = reinterpret_cast<__bf16 *>(smem);
+   int *koff         = reinterpret_cast<int *>(smem + (srcRowsMax * pitch + 1) / 2); // [Kpad]
+   int const tilesX  = (p.post.nx + MTX - 1) / MTX;
    int const tile    = blockIdx.x;
-   int const x0      = (tile % tilesX) * TX;
-   int const y0      = (tile / tilesX) * TY;
+   int const x0      = (tile % tilesX) * MTX;
+   int const y0      = (tile / tilesX) * MTY;
    int const fBase   = blockIdx.y * MF;
    int const b       = blockIdx.z;
-   int const xl      = min(x0 + TX, p.post.nx) - 1;
-   int const yl      = min(y0 + TY, p.post.ny) - 1;
+   int const xl      = min(x0 + MTX, p.post.nx) - 1;
+   int const yl      = min(y0 + MTY, p.post.ny) - 1;
    int const srcX0   = p.startX[x0];
    int const srcY0   = p.startY[y0];
    int const srcCols = p.startX[xl] - srcX0 + p.nxp;
@@ -255,15 +334,29 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
    }
    int const preRow  = p.pre.nxExt() * p.pre.nf;
    float const *preB = p.pre_data + (size_t)b * p.pre.numExt();
-
    int const rowElems = srcCols * p.nfp;
-   for (int i = tid; i < srcRows * rowElems; i += 256) {
-      int r = i / rowElems;
-      int c = i % rowElems;
-      src[r * pitch + c] = preB[(size_t)(srcY0 + r) * preRow + (size_t)srcX0 * p.nfp + c];
+   int const total    = srcRows * rowElems;
+   for (int base = tid; base < total; base += 256 * 8) {
+      // unconditional loads (clamped index) so all eight are in flight together
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+         int i = min(base + u * 256, total - 1);
+         int r = i / rowElems;
+         int c = i - r * rowElems;
+         v[u]  = preB[(size_t)(srcY0 + r) * preRow + (size_t)srcX0 * p.nfp + c];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+         int i = base + u * 256;
+         if (i < total) {
+            int r = i / rowElems;
+            src[r * pitch + (i - r * rowElems)] = (__bf16)v[u];
+         }
+      }
    }
-   for (int k = tid; k < Kpad; k += 256) {
-      koff[k] = k < K ? (k / rowK) * pitch + (k % rowK) : -1;
+   for (int kk = tid; kk < Kpad; kk += 256) {
+      koff[kk] = kk < K ? (kk / rowK) * pitch + (kk % rowK) : -1;
    }
    __syncthreads();
 
@@ -271,77 +364,268 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
    int const wave = tid / 64;
    int const r    = lane % 16;
    int const g    = lane / 16;
-
-   // Per-lane position base offsets for the 4 position blocks (column r).
-   int pbase[PBLK];
+   int pbase[MPB];
 #pragma unroll
-   for (int pb = 0; pb < PBLK; pb++) {
+   for (int pb = 0; pb < MPB; pb++) {
       int pj    = pb * 16 + r;
-      int x     = x0 + pj % TX;
-      int y     = y0 + pj / TX;
+      int x     = x0 + pj % MTX;
+      int y     = y0 + pj / MTX;
       bool ok   = x < p.post.nx && y < p.post.ny;
       pbase[pb] = ok ? (p.startY[y] - srcY0) * pitch + (p.startX[x] - srcX0) * p.nfp : 0;
    }
-
-   f32x4 acc[FBLK][PBLK];
+   if (ALIGN > 1) {
+      // vector LDS reads need every window start on the alignment class
+      bool bad = false;
 #pragma unroll
-   for (int fb = 0; fb < FBLK; fb++) {
+      for (int pb = 0; pb < MPB; pb++) {
+         bad = bad || (pbase[pb] % ALIGN) != 0;
+      }
+      if (__syncthreads_or(bad)) {
+         return;
+      }
+   }
+   f32x4 acc[MFB][MPB];
 #pragma unroll
-      for (int pb = 0; pb < PBLK; pb++) {
+   for (int fb = 0; fb < MFB; fb++) {
+#pragma unroll
+      for (int pb = 0; pb < MPB; pb++) {
          acc[fb][pb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
       }
    }
-   int const fWave = fBase + wave * WAVE_F;
-   for (int k0 = 0; k0 < Kpad; k0 += 32) {
-      int const kk = k0 + 8 * g;
-      bf16x8 afrag[FBLK];
+   int const fWave = fBase + wave * (MF / 4);
+   constexpr int KPRE = 8; // k-steps whose A fragments are fetched together
+   for (int kb = 0; kb < Kpad; kb += 32 * KPRE) {
+      uint4 apre[KPRE][MFB]; // bf16x8 fragments kept as raw 128-bit registers
 #pragma unroll
-      for (int fb = 0; fb < FBLK; fb++) {
-         int f = fWave + fb * 16 + r;
-         if (f < p.post.nf) {
-            afrag[fb] = *reinterpret_cast<const bf16x8 *>(Wb + (size_t)f * Kpad + kk);
-         }
-         else {
-            afrag[fb] = bf16x8{};
+      for (int st = 0; st < KPRE; st++) {
+         int const kk = kb + 32 * st + 8 * g;
+#pragma unroll
+         for (int fb = 0; fb < MFB; fb++) {
+            int f = fWave + fb * 16 + r;
+            apre[st][fb] = (f < p.post.nf && kk < Kpad)
+                                 ? *reinterpret_cast<const uint4 *>(Wb + (size_t)f * Kpad + kk)
+                                 : make_uint4(0u, 0u, 0u, 0u);
          }
       }
-      int ko[8];
 #pragma unroll
-      for (int e = 0; e < 8; e++) {
-         ko[e] = koff[kk + e];
-      }
+      for (int st = 0; st < KPRE; st++) {
+         int const kk = kb + 32 * st + 8 * g;
+         if (kb + 32 * st < Kpad) {
+         int const ko = koff[kk];
+         int kos[8];
+         if (ALIGN == 0) {
 #pragma unroll
-      for (int pb = 0; pb < PBLK; pb++) {
-         bf16x8 bfrag;
-#pragma unroll
-         for (int e = 0; e < 8; e++) {
-            bfrag[e] = ko[e] >= 0 ? to_bf16(src[pbase[pb] + ko[e]]) : to_bf16(0.0f);
+            for (int e = 0; e < 8; e++) {
+               kos[e] = koff[kk + e];
+            }
          }
 #pragma unroll
-         for (int fb = 0; fb < FBLK; fb++) {
-            acc[fb][pb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag[fb], bfrag, acc[fb][pb], 0, 0, 0);
+         for (int pb = 0; pb < MPB; pb++) {
+            bf16x8 bfrag;
+            if (ALIGN == 0) {
+#pragma unroll
+               for (int e = 0; e < 8; e++) {
+                  bfrag[e] = kos[e] >= 0 ? src[pbase[pb] + kos[e]] : (__bf16)0.0f;
+               }
+            }
+            else {
+               bfrag = ko >= 0 ? lds_frag<ALIGN>(src + pbase[pb] + ko) : bf16x8{};
+            }
+#pragma unroll
+            for (int fb = 0; fb < MFB; fb++) {
+               acc[fb][pb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                     __builtin_bit_cast(bf16x8, apre[st][fb]), bfrag, acc[fb][pb], 0, 0, 0);
+            }
+         }
          }
       }
    }
-   // Epilogue: lane holds D[4g+i][r] = feature fWave+fb*16+4g+i at position pb*16+r.
+   // Lane holds D[4g+i][r]: features fWave+fb*16+4g+i at position pb*16+r.
 #pragma unroll
-   for (int pb = 0; pb < PBLK; pb++) {
+   for (int pb = 0; pb < MPB; pb++) {
       int pj = pb * 16 + r;
-      int x  = x0 + pj % TX;
-      int y  = y0 + pj / TX;
-      if (x >= p.post.nx || y >= p.post.ny) {
-         continue;
-      }
+      int x  = x0 + pj % MTX;
+      int y  = y0 + pj / MTX;
       int kpos = (y * p.post.nx + x) * p.post.nf;
+      bool const inside = x < p.post.nx && y < p.post.ny; // no continue: keeps acc unrolled
 #pragma unroll
-      for (int fb = 0; fb < FBLK; fb++) {
+      for (int fb = 0; fb < MFB; fb++) {
+         int f = fWave + fb * 16 + 4 * g;
+         if (!inside) {
+            // position outside the layer: nothing to store
+         }
+         else if (f + 3 < p.post.nf && (p.post.nf & 3) == 0) {
+            post_epilogue4<EPI>(p, b, kpos + f, acc[fb][pb]);
+         }
+         else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+               if (f + i < p.post.nf) {
+                  float base = p.accumulate ? p.gsyn[(size_t)b * p.post.numRes() + kpos + f + i] : 0.0f;
+                  post_epilogue(p, b, kpos + f + i, base + p.dtf * acc[fb][pb][i]);
+               }
+            }
+         }
+      }
+   }
+}
+
+// ------------------------------------------------------------------ mfma (channel GEMM)
+// Implicit GEMM for sources with nfp % CK == 0 channels and any number of
+// data patches (targets larger than the source, e.g. the transposed form of
+// a one-to-many scatter): M = every kernel (nKx*nKy*post.nf, padded to 16),
+// N = distinct window starts (qx, qy) of a QX x QY tile, K = nyp*nxp*nfp
+// taken as (channel chunk, ky, kx, channel).  Output (kernel m, window q)
+// belongs to the target neuron whose kernel index and window start are
+// (m, q): the inverse tables map (kernel x-index, qx) -> target x.
+// T = __bf16 uses v_mfma_f32_16x16x32_bf16 (CK = 32); T = float uses
+// v_mfma_f32_16x16x4f32 (CK = 16).
+constexpr int CQX = 32, CQY = 8;        // window-start tile; wave w owns rows 2w, 2w+1
+
+template <typename T>
+struct CGemm;
+template <>
+struct CGemm<__bf16> {
+   static constexpr int CK = 32;
+};
+template <>
+struct CGemm<float> {
+   static constexpr int CK = 16;
+};
+
+template <typename T, int MB>
+__global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
+      PostParams p, const T *Wq, const int32_t *invX, const int32_t *invY, int qx0, int qy0,
+      int nqx, int nqy, int M) {
+   // Wq: weights in T, layout [M][nyp][nxp][C] (the post weights' own layout;
+   // the BF16 shadow of pvhip_recv_post_bf16_weights or the FP32 weights).
+   constexpr int CK = CGemm<T>::CK;
+   constexpr int U  = 8; // independent loads in flight per thread while staging
+   extern __shared__ float smem[];
+   int const C      = p.nfp;
+   int const tw     = CQX + p.nxp - 1;
+   int const th     = CQY + p.nyp - 1;
+   T *srcl          = reinterpret_cast<T *>(smem); // [th][tw][CK]
+   int const tilesX = (nqx + CQX - 1) / CQX;
+   int const qtx    = (blockIdx.x % tilesX) * CQX;
+   int const qty    = (blockIdx.x / tilesX) * CQY;
+   int const b      = blockIdx.z;
+   int const tid    = threadIdx.x;
+   int const lane   = tid % 64;
+   int const wave   = tid / 64;
+   int const r      = lane % 16;
+   int const g      = lane / 16;
+   int const nxExt  = p.pre.nxExt();
+   int const nyExt  = p.pre.nyExt();
+   float const *preB = p.pre_data + (size_t)b * p.pre.numExt();
+   int const patch  = p.nyp * p.nxp * C;
+   int const quads  = th * tw * (CK / 4); // float4 per staged chunk
+
+   f32x4 acc[MB][4];
+#pragma unroll
+   for (int mb = 0; mb < MB; mb++) {
+#pragma unroll
+      for (int nb = 0; nb < 4; nb++) {
+         acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      }
+   }
+   for (int c0 = 0; c0 < C; c0 += CK) {
+      __syncthreads();
+      for (int base = tid; base < quads; base += 256 * U) {
+         float4 v[U];
+#pragma unroll
+         for (int u = 0; u < U; u++) {
+            int i  = min(base + u * 256, quads - 1);
+            int q4 = i % (CK / 4);
+            int xx = (i / (CK / 4)) % tw;
+            int yy = i / ((CK / 4) * tw);
+            int ye = min(qy0 + qty + yy, nyExt - 1); // rows/cols past the grid are never used
+            int xe = min(qx0 + qtx + xx, nxExt - 1);
+            v[u]   = *reinterpret_cast<const float4 *>(preB + ((size_t)ye * nxExt + xe) * C + c0 + 4 * q4);
+         }
+#pragma unroll
+         for (int u = 0; u < U; u++) {
+            int i = base + u * 256;
+            if (i < quads) {
+               T *d = srcl + 4 * i;
+               d[0] = (T)v[u].x;
+               d[1] = (T)v[u].y;
+               d[2] = (T)v[u].z;
+               d[3] = (T)v[u].w;
+            }
+         }
+      }
+      __syncthreads();
+      for (int ky = 0; ky < p.nyp; ky++) {
+         for (int kx = 0; kx < p.nxp; kx++) {
+            int const wo = (ky * p.nxp + kx) * C + c0;
+            if constexpr (CK == 32) {
+               bf16x8 af[MB];
+#pragma unroll
+               for (int mb = 0; mb < MB; mb++) {
+                  int m  = mb * 16 + r;
+                  af[mb] = m < M ? *reinterpret_cast<const bf16x8 *>(Wq + (size_t)m * patch + wo + 8 * g)
+                                 : bf16x8{};
+               }
+#pragma unroll
+               for (int nb = 0; nb < 4; nb++) {
+                  int qyl = 2 * wave + nb / 2; // row within the tile
+                  int qxl = (nb % 2) * 16 + r; // column within the tile
+                  bf16x8 bf = *reinterpret_cast<const bf16x8 *>(
+                        srcl + ((qyl + ky) * tw + qxl + kx) * CK + 8 * g);
+#pragma unroll
+                  for (int mb = 0; mb < MB; mb++) {
+                     acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mb], bf, acc[mb][nb], 0, 0, 0);
+                  }
+               }
+            }
+            else {
+#pragma unroll
+               for (int cc = 0; cc < CK; cc += 4) {
+                  float af[MB];
+#pragma unroll
+                  for (int mb = 0; mb < MB; mb++) {
+                     int m  = mb * 16 + r;
+                     af[mb] = m < M ? (float)Wq[(size_t)m * patch + wo + cc + g] : 0.0f;
+                  }
+#pragma unroll
+                  for (int nb = 0; nb < 4; nb++) {
+                     int qyl  = 2 * wave + nb / 2;
+                     int qxl  = (nb % 2) * 16 + r;
+                     float bv = (float)srcl[((qyl + ky) * tw + qxl + kx) * CK + cc + g];
+#pragma unroll
+                     for (int mb = 0; mb < MB; mb++) {
+                        acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb], bv, acc[mb][nb], 0, 0, 0);
+                     }
+                  }
+               }
+            }
+         }
+      }
+   }
+   // Lane holds D[4g+i][r]: kernel mb*16+4g+i at window (qx, qy) of block nb.
+   int const nf = p.post.nf;
+#pragma unroll
+   for (int nb = 0; nb < 4; nb++) {
+      int qx = qtx + (nb % 2) * 16 + r;
+      int qy = qty + 2 * wave + nb / 2;
+      bool const inside = qx < nqx && qy < nqy;
+#pragma unroll
+      for (int mb = 0; mb < MB; mb++) {
 #pragma unroll
          for (int i = 0; i < 4; i++) {
-            int f = fWave + fb * 16 + 4 * g + i;
-            if (f < p.post.nf) {
-               float base = p.accumulate ? p.gsyn[(size_t)b * p.post.numRes() + kpos + f] : 0.0f;
-               float G    = base + p.dtf * acc[fb][pb][i];
-               post_epilogue(p, b, kpos + f, G);
+            int m  = mb * 16 + 4 * g + i;
+            int F  = m % nf;
+            int kx = (m / nf) % p.nKx;
+            int ky = m / (nf * p.nKx);
+            if (inside && m < M) {
+               int x = invX[kx * nqx + qx];
+               int y = invY[ky * nqy + qy];
+               if (x >= 0 && y >= 0) {
+                  size_t gi  = (size_t)b * p.post.numRes() + ((size_t)y * p.post.nx + x) * nf + F;
+                  float base = p.accumulate ? p.gsyn[gi] : 0.0f;
+                  p.gsyn[gi] = base + p.dtf * acc[mb][nb][i];
+               }
             }
          }
       }
@@ -437,40 +721,106 @@ int pvhip_recv_post(const pvhip_recv_post_args *args, pvhip_stream stream) {
    if (st) {
       return st;
    }
-   hipStream_t s = (hipStream_t)stream;
-   int const arith = args->arith;
-   bool const oneKernelPerFeature = p.nKx == 1 && p.nKy == 1;
-   // Source tile bound: the tables are monotone non-decreasing (calcPatchStartInPost),
-   // so a tile of TX columns spans at most the largest start difference + nxp.
-   int maxDX = 0, maxDY = 0;
-   if (oneKernelPerFeature) {
-      // The tables live on the device; the host side of the delivery passes
-      // their spans through the stride rule: span <= (T-1)*ceil(stride) + nxp.
-      int sx = p.pre.nx > p.post.nx ? p.pre.nx / p.post.nx : 1;
-      int sy = p.pre.ny > p.post.ny ? p.pre.ny / p.post.ny : 1;
-      maxDX  = (TX - 1) * sx;
-      maxDY  = (TY - 1) * sy;
-   }
-   int const srcColsMax = maxDX + p.nxp;
-   int const srcRowsMax = maxDY + p.nyp;
-   int const tilesX     = (p.post.nx + TX - 1) / TX;
-   int const tilesY     = (p.post.ny + TY - 1) / TY;
+   hipStream_t s     = (hipStream_t)stream;
+   int const arith   = args->arith;
+   bool const single = p.nKx == 1 && p.nKy == 1; // one data patch per feature
    PVHIP_REQUIRE(p.post.nbatch <= 65535, "recv_post: nbatch too large");
+   // Target <= source along both axes: the window starts advance by the
+   // source/target ratio, so a tile of T positions spans (T-1)*ratio + patch.
+   int const sx = p.pre.nx > p.post.nx ? p.pre.nx / p.post.nx : 1;
+   int const sy = p.pre.ny > p.post.ny ? p.pre.ny / p.post.ny : 1;
 
-   if (arith == PVHIP_ARITH_BF16 && oneKernelPerFeature) {
+   if ((arith == PVHIP_ARITH_BF16 || arith == PVHIP_ARITH_FP32) && !single) {
+      // Several data patches per feature: channel GEMM with inverse tables.
+      PVHIP_REQUIRE(args->inverse_x && args->inverse_y && args->window_nx > 0 && args->window_ny > 0,
+            "recv_post: targets larger than the source need inverse tables (pvhost_recv_post_inverse)");
+      PVHIP_REQUIRE(!p.fused, "recv_post: fused update needs one data patch per feature");
+      bool bf = arith == PVHIP_ARITH_BF16;
+      int CK  = bf ? 32 : 16;
+      PVHIP_REQUIRE(p.nfp % CK == 0, "recv_post: MFMA path with %d data patches needs nfp %% %d == 0",
+            p.nKx * p.nKy, CK);
+      int M  = p.nKx * p.nKy * p.post.nf;
+      int MB = (M + 15) / 16;
+      PVHIP_REQUIRE(MB <= 4, "recv_post: %d kernels exceed the MFMA tile (64)", M);
+      MB = MB <= 1 ? 1 : (MB <= 2 ? 2 : 4);
+      int tw = CQX + p.nxp - 1, th = CQY + p.nyp - 1;
+      size_t esz = bf ? 2 : 4;
+      size_t lds = (size_t)th * tw * CK * esz;
+      if (bf) {
+         PVHIP_REQUIRE(args->weights_lp != nullptr,
+               "recv_post: BF16 channel GEMM needs weights_lp (pvhip_recv_post_bf16_weights)");
+      }
+      const void *Wq = bf ? args->weights_lp : (const void *)args->weights;
+      PVHIP_REQUIRE(lds <= 160 * 1024, "recv_post: channel GEMM tile needs %zu bytes of LDS", lds);
+      int tilesX = (args->window_nx + CQX - 1) / CQX;
+      int tilesY = (args->window_ny + CQY - 1) / CQY;
+      dim3 grid(tilesX * tilesY, 1, p.post.nbatch);
+#define PVHIP_CGEMM(T, MBV)                                                                       \
+   hipLaunchKernelGGL((recv_post_cgemm_kernel<T, MBV>), grid, dim3(256), lds, s, p,              \
+         reinterpret_cast<const T *>(Wq), args->inverse_x, args->inverse_y, args->window_x0,        \
+         args->window_y0, args->window_nx, args->window_ny, M)
+      if (bf) {
+         if (MB == 1) PVHIP_CGEMM(__bf16, 1);
+         else if (MB == 2) PVHIP_CGEMM(__bf16, 2);
+         else PVHIP_CGEMM(__bf16, 4);
+      }
+      else {
+         if (MB == 1) PVHIP_CGEMM(float, 1);
+         else if (MB == 2) PVHIP_CGEMM(float, 2);
+         else PVHIP_CGEMM(float, 4);
+      }
+#undef PVHIP_CGEMM
+      PVHIP_LAUNCHED("recv_post_cgemm");
+      return PVHIP_OK;
+   }
+
+   if (arith == PVHIP_ARITH_BF16) {
       PVHIP_REQUIRE(args->weights_lp != nullptr, "recv_post: BF16 needs weights_lp (pvhip_recv_post_bf16_weights)");
-      int const K    = p.nxp * p.nyp * p.nfp;
+      int const rowK = p.nxp * p.nfp;
+      int const K    = rowK * p.nyp;
       int const Kpad = (K + 31) / 32 * 32;
-      size_t lds = (size_t)srcRowsMax * srcColsMax * p.nfp * sizeof(float) + (size_t)Kpad * sizeof(int);
+      int const srcColsMax = (MTX - 1) * sx + p.nxp;
+      int const srcRowsMax = (MTY - 1) * sy + p.nyp;
+      int const pitch      = (srcColsMax * p.nfp + 7) / 8 * 8;
+      size_t lds = ((size_t)srcRowsMax * pitch + 1) / 2 * 4 + (size_t)Kpad * sizeof(int);
       PVHIP_REQUIRE(lds <= 160 * 1024, "recv_post: BF16 tile needs %zu bytes of LDS", lds);
+      // alignment class of a fragment's LDS address: rows are 8-aligned, window
+      // starts differ by multiples of sx*nfp, fragments start at multiples of 8
+      int step  = sx * p.nfp;
+      int align = step % 8 == 0 ? 8 : (step % 4 == 0 ? 4 : (step % 2 == 0 ? 2 : 1));
+      if (rowK % 8 != 0) {
+         align = 0; // a fragment may straddle patch rows: per-element offsets
+      }
+      int const tilesX = (p.post.nx + MTX - 1) / MTX;
+      int const tilesY = (p.post.ny + MTY - 1) / MTY;
       dim3 grid(tilesX * tilesY, (p.post.nf + MF - 1) / MF, p.post.nbatch);
-      hipLaunchKernelGGL(recv_post_mfma_bf16_kernel, grid, dim3(256), lds, s, p,
-            (const __bf16 *)args->weights_lp, srcRowsMax, srcColsMax, Kpad);
+      const __bf16 *Wb = (const __bf16 *)args->weights_lp;
+      int const epi = !p.fused ? 0 : (p.lca.transfer.mode == PVHIP_TRANSFER_VTHRESH ? 1 : 2);
+#define PVHIP_MFMA(AL, EP)                                                                       \
+   hipLaunchKernelGGL((recv_post_mfma_bf16_kernel<AL, EP>), grid, dim3(256), lds, s, p, Wb,     \
+         srcRowsMax, srcColsMax, Kpad)
+#define PVHIP_MFMA_EPI(AL)                                                                       \
+   do {                                                                                          \
+      if (epi == 0) PVHIP_MFMA(AL, 0);                                                           \
+      else if (epi == 1) PVHIP_MFMA(AL, 1);                                                      \
+      else PVHIP_MFMA(AL, 2);                                                                    \
+   } while (0)
+      if (align == 8) PVHIP_MFMA_EPI(8);
+      else if (align == 4) PVHIP_MFMA_EPI(4);
+      else if (align == 2) PVHIP_MFMA_EPI(2);
+      else if (align == 1) PVHIP_MFMA_EPI(1);
+      else PVHIP_MFMA_EPI(0);
+#undef PVHIP_MFMA_EPI
+#undef PVHIP_MFMA
       PVHIP_LAUNCHED("recv_post_mfma_bf16");
       return PVHIP_OK;
    }
-   if ((arith == PVHIP_ARITH_EXACT || arith == PVHIP_ARITH_FP32) && oneKernelPerFeature) {
-      int const K = p.nxp * p.nfp;
+   if ((arith == PVHIP_ARITH_EXACT || arith == PVHIP_ARITH_FP32) && single) {
+      int const K          = p.nxp * p.nfp;
+      int const srcColsMax = (TX - 1) * sx + p.nxp;
+      int const srcRowsMax = (TY - 1) * sy + p.nyp;
+      int const tilesX     = (p.post.nx + TX - 1) / TX;
+      int const tilesY     = (p.post.ny + TY - 1) / TY;
       for (int FB : {64, 32}) {
          size_t lds = ((size_t)srcRowsMax * srcColsMax * p.nfp + (size_t)K * FB) * sizeof(float);
          if (lds > 160 * 1024) {

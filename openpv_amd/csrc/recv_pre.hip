@@ -16,6 +16,8 @@
 // and the weight reads are coalesced along the feature axis.
 #include "pvhip_internal.hpp"
 
+#include <algorithm>
+
 namespace pvhip {
 namespace {
 
@@ -114,6 +116,212 @@ __global__ void __launch_bounds__(256) recv_pre_gather_kernel(PreParams p) {
    p.gsyn[index] = g;
 }
 
+// ------------------------------------------------------------------ LDS-staged
+//
+// Same per-neuron order, organised so that the data a workgroup needs sits in
+// LDS: a workgroup owns R postsynaptic rows that share their sequence of
+// weight rows (rows rowStep apart), TX columns x all features, and G batch
+// elements.  For each patch row r (ascending) and each of its rows, the
+// presynaptic row ye that lands there is handled by
+//   1. staging the weight row wy of every data patch with that kernel row
+//      (only when it differs from the row already staged), and
+//   2. staging, column chunk by column chunk, the run of the ordered active
+//      list that belongs to pre row ye and the tile's pre columns -- one
+//      contiguous range of the list, since kex is position-major;
+// then every thread walks its own columns (ascending) and their active
+// features (ascending) out of LDS.  Work per thread and order per neuron are
+// those of recv_pre_gather_kernel; only where the operands come from changes.
+
+struct PreTile {
+   int TX;             // post columns per workgroup
+   int G;              // batch elements per workgroup
+   int rowStep;        // distance between the rows of a workgroup
+   int groupsPerRes;   // row groups per residue class
+   int cap;            // staged entries per batch element
+   int maxCols;        // widest pre-column union of a tile
+   int wlds;           // weight row staged in LDS
+   int sliceFloats;    // floats of one staged weight row
+};
+
+template <int R, bool EXACT>
+__global__ void __launch_bounds__(256) recv_pre_lds_kernel(PreParams p, PreTile t) {
+   extern __shared__ float smem[];
+   int const nfPost  = p.post.nf;
+   int const nfPre   = p.pre.nf;
+   int const nxExt   = p.pre.nxExt();
+   int const numPos  = nxExt * p.pre.nyExt();
+   int const nxpnfp  = p.nxp * p.nfp;
+   int const patchSize = nxpnfp * p.nyp;
+   float *Wl   = smem;
+   float *Ev   = smem + (t.wlds ? t.sliceFloats : 0);
+   int *Ef     = reinterpret_cast<int *>(Ev + t.G * t.cap);
+   int *cs     = Ef + t.G * t.cap;
+   int *shared = cs + t.G * (t.maxCols + 1); // [0] chunk end column, [1..2] column range
+
+   int const tid    = threadIdx.x;
+   int const perImg = t.TX * nfPost;
+   int const g      = tid / perImg;
+   int const rem    = tid % perImg;
+   int const X0     = blockIdx.x * t.TX;
+   int const X      = X0 + rem / nfPost;
+   int const F      = rem % nfPost;
+   int const b      = blockIdx.z * t.G + g;
+   bool const lane  = g < t.G && X < p.post.nx && b < p.post.nbatch;
+   int const residue = blockIdx.y / t.groupsPerRes;
+   int const gi      = blockIdx.y % t.groupsPerRes;
+   int Yj[R];
+#pragma unroll
+   for (int j = 0; j < R; j++) {
+      Yj[j] = residue + t.rowStep * (gi * R + j);
+   }
+   int const Xlast = min(X0 + t.TX, p.post.nx) - 1;
+   if (tid == 0) {
+      int lo = nxExt, hi = -1;
+      for (int ci = p.colPtr[X0]; ci < p.colPtr[Xlast + 1]; ci++) {
+         lo = min(lo, p.colXe[ci]);
+         hi = max(hi, p.colXe[ci]);
+      }
+      shared[1] = lo;
+      shared[2] = hi;
+   }
+   __syncthreads();
+   int const xeLo  = shared[1];
+   int const xeHi  = shared[2];
+   int const c0    = lane ? p.colPtr[X] : 0;
+   int const c1    = lane ? p.colPtr[X + 1] : 0;
+
+   float acc[R];
+#pragma unroll
+   for (int j = 0; j < R; j++) {
+      acc[j] = (lane && p.accumulate && Yj[j] < p.post.ny)
+                     ? p.gsyn[(size_t)b * p.post.numRes() + ((size_t)Yj[j] * p.post.nx + X) * nfPost + F]
+                     : 0.0f;
+   }
+   int curWy = -1, curKy = -1;
+   for (int r = 0; r < p.nyp; r++) {
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+         int const Y = Yj[j];
+         if (Y >= p.post.ny) {
+            continue; // uniform over the workgroup
+         }
+         int const r0 = p.rowPtr[Y * p.nyp + r];
+         int const r1 = p.rowPtr[Y * p.nyp + r + 1];
+         for (int ri = r0; ri < r1; ri++) {
+            int const ye  = p.rowYe[ri];
+            int const wy  = p.rowWy[ri];
+            int const kyK = p.rowKy[ri];
+            if (t.wlds && (wy != curWy || kyK != curKy)) {
+               __syncthreads();
+               int const nk = p.nKx * nfPre;
+               for (int i = tid; i < nk * nxpnfp; i += blockDim.x) {
+                  int kk = i / nxpnfp;
+                  int o  = i % nxpnfp;
+                  int kx = kk / nfPre;
+                  int f  = kk % nfPre;
+                  Wl[i]  = p.W[(size_t)((kx + kyK * p.nKx) * nfPre + f) * patchSize + wy * nxpnfp + o];
+               }
+               curWy = wy;
+               curKy = kyK;
+            }
+            int cA = xeLo;
+            while (cA <= xeHi) {
+               __syncthreads();
+               // Chunk end: the longest run of columns from cA whose entries fit
+               // the staging capacity for every batch element of the group
+               // (run length is monotone in the column count).
+               int width = min(min(xeHi - cA + 1, t.maxCols), (int)blockDim.x);
+               bool fits = false;
+               if (tid < width) {
+                  fits = true;
+                  for (int gg = 0; gg < t.G; gg++) {
+                     int bb = blockIdx.z * t.G + gg;
+                     if (bb >= p.post.nbatch) {
+                        break;
+                     }
+                     const uint32_t *ps = p.posStart + (size_t)bb * (numPos + 1) + ye * nxExt;
+                     if ((int)(ps[cA + tid + 1] - ps[cA]) > t.cap) {
+                        fits = false;
+                     }
+                  }
+               }
+               int nfit = __syncthreads_count(fits);
+               if (tid == 0) {
+                  shared[0] = cA + max(nfit, 1) - 1;
+               }
+               __syncthreads();
+               int const cB    = shared[0];
+               int const ncols = cB - cA + 1;
+               // stage the active-list run of pre row ye, columns [cA, cB]
+               for (int gg = 0; gg < t.G; gg++) {
+                  int bb = blockIdx.z * t.G + gg;
+                  if (bb >= p.post.nbatch) {
+                     break;
+                  }
+                  const uint32_t *ps = p.posStart + (size_t)bb * (numPos + 1) + ye * nxExt;
+                  uint32_t const e0  = ps[cA];
+                  int const cnt      = (int)(ps[cB + 1] - e0);
+                  const pvhip_sparse_entry *src = p.active + (size_t)bb * p.pre.numExt() + e0;
+                  for (int i = tid; i < cnt; i += blockDim.x) {
+                     pvhip_sparse_entry en   = src[i];
+                     Ev[gg * t.cap + i]      = en.value;
+                     Ef[gg * t.cap + i]      = (int)(en.index % (uint32_t)nfPre);
+                  }
+                  for (int c = tid; c <= ncols; c += blockDim.x) {
+                     cs[gg * (t.maxCols + 1) + c] = (int)(ps[cA + c] - e0);
+                  }
+               }
+               __syncthreads();
+               if (lane) {
+                  const int *csg  = cs + g * (t.maxCols + 1);
+                  const float *Eg = Ev + g * t.cap;
+                  const int *Fg   = Ef + g * t.cap;
+                  float a0        = acc[j];
+                  for (int ci = c0; ci < c1; ci++) {
+                     int const xe = p.colXe[ci];
+                     if (xe < cA) {
+                        continue;
+                     }
+                     if (xe > cB) {
+                        break;
+                     }
+                     int const wx   = p.colWx[ci];
+                     int const kx   = p.colKx[ci];
+                     int const s    = csg[xe - cA];
+                     int const e    = csg[xe - cA + 1];
+                     int const wofs = wx * p.nfp + F;
+                     for (int i = s; i < e; i++) {
+                        float a    = Eg[i] * p.dtf;
+                        int const f = Fg[i];
+                        float w = t.wlds ? Wl[(kx * nfPre + f) * nxpnfp + wofs]
+                                         : p.W[(size_t)((kx + kyK * p.nKx) * nfPre + f) * patchSize
+                                               + wy * nxpnfp + wofs];
+                        if (EXACT) {
+                           float prod = a * w;
+                           a0         = a0 + prod;
+                        }
+                        else {
+                           a0 = __builtin_fmaf(a, w, a0);
+                        }
+                     }
+                  }
+                  acc[j] = a0;
+               }
+               cA = cB + 1;
+            }
+         }
+      }
+   }
+   if (lane) {
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+         if (Yj[j] < p.post.ny) {
+            p.gsyn[(size_t)b * p.post.numRes() + ((size_t)Yj[j] * p.post.nx + X) * nfPost + F] = acc[j];
+         }
+      }
+   }
+}
+
 } // namespace
 } // namespace pvhip
 
@@ -161,13 +369,54 @@ int pvhip_recv_pre(const pvhip_recv_pre_args *a, pvhip_stream stream) {
    p.W          = a->weights;
    p.gsyn       = a->post_gsyn;
    p.accumulate = a->accumulate;
+   hipStream_t s = (hipStream_t)stream;
+   bool const exact = a->arith == PVHIP_ARITH_EXACT;
+
+   // LDS-staged form for sparse pre layers whose post features fit one workgroup.
+   if (p.posStart && p.post.nf <= 256 && p.pre.nf <= 4096) {
+      PreTile t;
+      t.TX      = std::max(1, std::min(p.post.nx, 256 / p.post.nf));
+      t.G       = std::max(1, std::min(p.post.nbatch, 256 / (t.TX * p.post.nf)));
+      t.rowStep = p.post.ny > p.pre.ny ? p.post.ny / p.pre.ny : 1;
+      int const R = 4;
+      int rowsPerRes = (p.post.ny + t.rowStep - 1) / t.rowStep;
+      t.groupsPerRes = (rowsPerRes + R - 1) / R;
+      // widest union of covering pre columns over the column tiles: bounded by
+      // the columns a tile of TX post columns can see (the patch reach in pre
+      // columns plus the tile span), computed from the stride rule
+      int sPre  = p.pre.nx > p.post.nx ? p.pre.nx / p.post.nx : 1;
+      int tPost = p.post.nx > p.pre.nx ? p.post.nx / p.pre.nx : 1;
+      int reach = (p.nxp + tPost - 1) / tPost * sPre + sPre;
+      t.maxCols = std::min(p.pre.nxExt(), (t.TX + tPost - 1) / tPost * sPre + reach + 2);
+      t.sliceFloats = p.nKx * p.pre.nf * p.nxp * p.nfp;
+      size_t const budget = 72 * 1024; // two workgroups per CU
+      t.wlds = (size_t)t.sliceFloats * 4 <= budget / 2;
+      size_t fixed = (t.wlds ? (size_t)t.sliceFloats * 4 : 0) + (size_t)t.G * (t.maxCols + 1) * 4 + 64;
+      size_t room  = budget > fixed ? budget - fixed : 0;
+      t.cap        = (int)std::min<size_t>(room / (8 * (size_t)t.G), 1u << 20);
+      if (t.cap >= p.pre.nf) {
+         size_t lds = fixed + (size_t)t.G * t.cap * 8;
+         int threads = t.TX * p.post.nf * t.G;
+         threads     = (threads + 63) / 64 * 64;
+         dim3 grid((p.post.nx + t.TX - 1) / t.TX, t.rowStep * t.groupsPerRes,
+               (p.post.nbatch + t.G - 1) / t.G);
+         if (exact) {
+            hipLaunchKernelGGL((recv_pre_lds_kernel<4, true>), grid, dim3(threads), lds, s, p, t);
+         }
+         else {
+            hipLaunchKernelGGL((recv_pre_lds_kernel<4, false>), grid, dim3(threads), lds, s, p, t);
+         }
+         PVHIP_LAUNCHED("recv_pre_lds");
+         return PVHIP_OK;
+      }
+   }
    size_t n     = (size_t)p.post.nbatch * p.post.numRes();
    dim3 grid((unsigned)((n + 255) / 256));
-   if (a->arith == PVHIP_ARITH_EXACT) {
-      hipLaunchKernelGGL(recv_pre_gather_kernel<true>, grid, dim3(256), 0, (hipStream_t)stream, p);
+   if (exact) {
+      hipLaunchKernelGGL(recv_pre_gather_kernel<true>, grid, dim3(256), 0, s, p);
    }
    else {
-      hipLaunchKernelGGL(recv_pre_gather_kernel<false>, grid, dim3(256), 0, (hipStream_t)stream, p);
+      hipLaunchKernelGGL(recv_pre_gather_kernel<false>, grid, dim3(256), 0, s, p);
    }
    PVHIP_LAUNCHED("recv_pre_gather");
    return PVHIP_OK;

@@ -213,6 +213,44 @@ int pvhost_recv_post_tables(const pvhip_loc *source, const pvhip_loc *target, in
    return 0;
 }
 
+int pvhost_recv_post_inverse(const pvhip_loc *source, const pvhip_loc *target, int nxp, int nyp,
+      int32_t *inverse_x, int32_t *inverse_y, int *window_x0, int *window_nx, int *window_y0,
+      int *window_ny) {
+   if (!locOk(source) || !locOk(target) || !window_x0 || !window_nx || !window_y0 || !window_ny) {
+      return -1;
+   }
+   std::vector<int32_t> sx(target->nx), sy(target->ny), kx(target->nx), ky(target->ny);
+   int nkx, nky;
+   int st = pvhost_recv_post_tables(source, target, nxp, nyp, sx.data(), sy.data(), kx.data(),
+         ky.data(), &nkx, &nky);
+   if (st != 0) {
+      return st;
+   }
+   auto fill = [](const std::vector<int32_t> &start, const std::vector<int32_t> &ker, int nk,
+                     int32_t *inv, int *w0, int *wn) {
+      int lo = *std::min_element(start.begin(), start.end());
+      int hi = *std::max_element(start.begin(), start.end());
+      *w0    = lo;
+      *wn    = hi - lo + 1;
+      if (inv) {
+         std::fill(inv, inv + (size_t)nk * (*wn), -1);
+         for (size_t t = 0; t < start.size(); t++) {
+            int32_t &slot = inv[(size_t)ker[t] * (*wn) + (start[t] - lo)];
+            if (slot >= 0) {
+               return -3; // two targets share (kernel, window): not a bijection
+            }
+            slot = (int32_t)t;
+         }
+      }
+      return 0;
+   };
+   st = fill(sx, kx, nkx, inverse_x, window_x0, window_nx);
+   if (st != 0) {
+      return st;
+   }
+   return fill(sy, ky, nky, inverse_y, window_y0, window_ny);
+}
+
 int pvhost_recv_pre_tables(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int nyp,
       int32_t *col_ptr, int32_t *col_xe, int32_t *col_wx, int32_t *col_kx, int *ncol,
       int32_t *row_ptr, int32_t *row_ye, int32_t *row_wy, int32_t *row_ky, int *nrow) {

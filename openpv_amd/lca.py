@@ -33,8 +33,7 @@ class LCAColumn:
 
     weights: V1->Residual pre-perspective shared weights [nf_v1][nyp][nxp][nf_in]
     (the HyPerConn the TransposeConn and CloneConn share).
-    arith: delivery arithmetic of the gather (EXACT, FP32, BF16); the scatter
-    runs EXACT for EXACT and FP32 (FMA) otherwise.
+    arith: delivery arithmetic of the gather and the scatter (EXACT, FP32, BF16).
     fused: apply the V1 update in the gather epilogue.
     """
 
@@ -82,9 +81,8 @@ class LCAColumn:
         # patch = nxp x nyp x nf_in (TransposeWeightsPair.cpp:141-147).
         self.gather = PostsynapticPerspectiveGPUDelivery(self.res_loc, self.v1_loc, nxp, nyp,
                                                          self.weights, arith=arith)
-        scatter_arith = abi.ARITH_EXACT if arith == abi.ARITH_EXACT else abi.ARITH_FP32
         self.scatter = PresynapticPerspectiveGPUDelivery(self.v1_loc, self.rec_loc, nxp, nyp,
-                                                         self.weights, arith=scatter_arith)
+                                                         self.weights, arith=arith)
         self.active = ActiveList(self.v1_loc, d)
         self.lca = HyPerLCAUpdate(self.v1_loc, self.v1_transfer, timeConstantTau, dt,
                                   self_interact, device=d)
@@ -99,7 +97,9 @@ class LCAColumn:
         self.active.compact(self.v1_A)
 
     # one timestep ---------------------------------------------------------
-    def step(self, stream=None):
+    def step(self, stream=None, gather_events=None):
+        """One timestep.  gather_events=(start, stop) torch events are recorded on
+        `stream` around the V1 gather (+ fused update) launch."""
         s = stream
         nb = self.in_loc.nbatch
         # phase 1: Residual = err_scale * (Input - Recon(t-1))
@@ -110,6 +110,8 @@ class LCAColumn:
         layer_update(self.res_loc, self.res_gsyn, self.res_V, self.res_A, self.none_transfer,
                      channel_coeffs=(1.0, -1.0), err_scale=self.err_scale, stream=s)
         # phase 2: V1 gather + LCA update (+ threshold), then its active list
+        if gather_events is not None:
+            gather_events[0].record(s)
         if self.fused:
             if self._fused_args is None:
                 self._fused_args = self.lca.args(None, self.v1_V, self.v1_A, num_channels=1)
@@ -117,9 +119,12 @@ class LCAColumn:
         else:
             self.gather.deliver(self.res_A, self.v1_gsyn, accumulate=False, stream=s)
             self.lca.update(self.v1_gsyn, self.v1_V, self.v1_A, stream=s)
+        if gather_events is not None:
+            gather_events[1].record(s)
         self.active.compact(self.v1_A, stream=s)
         # phase 3: Recon = scatter of the sparse V1 activity
-        self.scatter.deliver(self.rec_gsyn, active=self.active, accumulate=False, stream=s)
+        self.scatter.deliver(self.rec_gsyn, active=self.active, pre_activity=self.v1_A,
+                             accumulate=False, stream=s)
         layer_update(self.rec_loc, self.rec_gsyn, self.rec_V, self.rec_A, self.none_transfer,
                      channel_coeffs=(1.0,), err_scale=None, stream=s)
 

@@ -98,6 +98,25 @@ class PostsynapticPerspectiveGPUDelivery:
                                  % (weights.numel(), self.num_kernels, self.patch_size))
         self.weights = weights
         self.weights_lp = None
+        self.inverse = None
+        self.windows = (0, 0, 0, 0)
+        if self.nkx * self.nky > 1 and arith in (abi.ARITH_BF16, abi.ARITH_FP32):
+            # several data patches per feature on the MFMA path: inverse tables
+            x0, nx_, y0, ny_ = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+            H = abi.host()
+            r = H.pvhost_recv_post_inverse(C.byref(source), C.byref(target), nxp, nyp, None, None,
+                                           C.byref(x0), C.byref(nx_), C.byref(y0), C.byref(ny_))
+            if r != 0:
+                raise abi.PVHIPError("recv_post inverse tables failed (%d)" % r)
+            ix = np.zeros(self.nkx * nx_.value, np.int32)
+            iy = np.zeros(self.nky * ny_.value, np.int32)
+            r = H.pvhost_recv_post_inverse(C.byref(source), C.byref(target), nxp, nyp,
+                                           ix.ctypes.data_as(C.c_void_p), iy.ctypes.data_as(C.c_void_p),
+                                           C.byref(x0), C.byref(nx_), C.byref(y0), C.byref(ny_))
+            if r != 0:
+                raise abi.PVHIPError("recv_post inverse tables failed (%d)" % r)
+            self.inverse = (_dev_i32(ix, dev), _dev_i32(iy, dev))
+            self.windows = (x0.value, nx_.value, y0.value, ny_.value)
         if arith == abi.ARITH_BF16:
             self.refresh_weights()
 
@@ -125,6 +144,9 @@ class PostsynapticPerspectiveGPUDelivery:
         a.accumulate = int(accumulate)
         a.arith = self.arith
         a.weights_lp = _ptr(self.weights_lp)
+        if self.inverse is not None:
+            a.inverse_x, a.inverse_y = _ptr(self.inverse[0]), _ptr(self.inverse[1])
+            a.window_x0, a.window_nx, a.window_y0, a.window_ny = self.windows
         if fused is not None:
             a.fused_update = C.pointer(fused)
         return a
@@ -191,8 +213,62 @@ class PresynapticPerspectiveGPUDelivery:
         H.pvhost_num_kernels(C.byref(pre), C.byref(post), C.byref(nkx), C.byref(nky))
         self.nkx, self.nky = nkx.value, nky.value
         self.weights = weights
+        self.transposed = None
+        if arith in (abi.ARITH_BF16, abi.ARITH_FP32):
+            self.transposed = self._transposed_route()
+
+    def _transposed_route(self):
+        """BF16 / FP32 forms: the scatter as a post-perspective gather with the
+        transposed weights (TransposeWeights::transposeShared), which for a
+        target larger than the source is a stride-1 convolution over the pre
+        grid with (phase, feature) output channels -- an MFMA implicit GEMM.
+        Returns None when the geometry does not fit that route."""
+        pre, post = self.pre, self.post
+        H = abi.host()
+        ck = 32 if self.arith == abi.ARITH_BF16 else 16
+        nxq = H.pvhost_post_patch_size(self.nxp, pre.nx, post.nx)
+        nyq = H.pvhost_post_patch_size(self.nyp, pre.ny, post.ny)
+        pkx, pky = C.c_int(), C.c_int()
+        H.pvhost_num_kernels(C.byref(post), C.byref(pre), C.byref(pkx), C.byref(pky))
+        m = pkx.value * pky.value * post.nf
+        if nxq <= 0 or nyq <= 0 or pre.nf % ck or m > 64 or pkx.value * pky.value == 1:
+            return None
+        sx = np.zeros(post.nx, np.int32)
+        sy = np.zeros(post.ny, np.int32)
+        if H.pvhost_recv_post_tables(C.byref(pre), C.byref(post), nxq, nyq,
+                                     sx.ctypes.data_as(C.c_void_p), sy.ctypes.data_as(C.c_void_p),
+                                     None, None, None, None) != 0:
+            return None
+        self.transposed_shape = (m, nyq, nxq, pre.nf)
+        wt = torch.zeros(m * nyq * nxq * pre.nf, dtype=torch.float32, device=self.weights.device)
+        route = PostsynapticPerspectiveGPUDelivery(pre, post, nxq, nyq, wt, arith=self.arith,
+                                                   dt_factor=self.dt_factor)
+        self._route = route
+        self.refresh_weights()
+        return route
+
+    def refresh_weights(self, stream=None):
+        """Re-derive the transposed weights after a weight change."""
+        if self.transposed is None and getattr(self, "_route", None) is None:
+            return
+        route = self._route
+        w = self.weights.detach().float().cpu().numpy().ravel()
+        out = np.zeros(route.weights.numel(), np.float32)
+        r = abi.host().pvhost_transpose_weights(C.byref(self.pre), C.byref(self.post), self.nxp,
+                                                self.nyp, self.nfp, w.ctypes.data_as(C.c_void_p),
+                                                out.ctypes.data_as(C.c_void_p))
+        if r != 0:
+            raise abi.PVHIPError("pvhost_transpose_weights failed (%d)" % r)
+        route.weights.copy_(torch.as_tensor(out))
+        route.refresh_weights(stream)
 
     def deliver(self, gsyn, active=None, pre_activity=None, accumulate=False, stream=None):
+        if self.transposed is not None:
+            if pre_activity is None:
+                raise abi.PVHIPError("the %s scatter reads the dense pre activity"
+                                     % ("BF16" if self.arith == abi.ARITH_BF16 else "FP32"))
+            self.transposed.deliver(pre_activity, gsyn, accumulate=accumulate, stream=stream)
+            return
         a = abi.RecvPreArgs()
         a.pre, a.post = self.pre, self.post
         a.nxp, a.nyp, a.nfp = self.nxp, self.nyp, self.nfp

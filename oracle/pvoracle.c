@@ -932,7 +932,7 @@ void or_lca_run(const or_lca_config *cfg, const float *weights, const float *inp
 
       /* phase 3: Recon (ANNLayer) */
       memset(recG, 0, sizeof(float) * (size_t)nb * nR);
-      or_recv_pre(&L, &recL, nxp, nyp, v1A, ent, nAct, weights, recG, 1.0f, 1);
+      or_recv_pre(&L, &recL, nxp, nyp, v1A, ent, nAct, weights, recG, 1.0f, cfg->num_threads);
       or_gsyn_accumulate(nb * nR, 1, one, recG, recV);
       memcpy(recA, recV, sizeof(float) * (size_t)nb * nR);
 

@@ -164,7 +164,7 @@ typedef struct {
    double tau, dt; /* timeConstantTau and the column dt, read as double from params */
    int selfInteract;
    float errScale;
-   int num_threads;
+   int num_threads; /* OpenMP threads of the deliveries; 1 = the bit-reproducible order */
 } or_lca_config;
 
 /* Runs the loop. weights: V1->Residual pre-perspective [nf_v1][nyp][nxp][nf_in];

@@ -270,7 +270,7 @@ def test_recv_pre_exact_bit_exact(case):
     assert same(host(g2), ref)
     # FP32 (FMA) form
     d32 = R.PresynapticPerspectiveGPUDelivery(hp, hq, p, p, dev(W), arith=abi.ARITH_FP32)
-    d32.deliver(g2, active=al)
+    d32.deliver(g2, active=al, pre_activity=dev(A))
     assert np.abs(host(g2) - ref).max() <= 1e-5 * max(1e-30, np.abs(ref).max())
 
 
@@ -400,3 +400,50 @@ def test_lca_column_bf16_energy_and_sparsity():
     print("bf16 energy rel err", rel, "active", n_ex, n_bf, flush=True)
     assert rel.max() <= 1e-2
     assert (np.abs(n_bf - n_ex) <= 0.05 * n_ex + 2).all()
+
+
+@pytest.mark.parametrize("case", [
+    # source, target, post-perspective patch (target larger: 16 / 4 data patches per feature)
+    ((2, 16, 16, 64, 2), (2, 64, 64, 2, 0), 4),       # transpose of a stride-4 one-to-many
+    ((1, 16, 16, 128, 2), (1, 32, 32, 3, 0), 4),      # config[0] Recon <- V1 transposed
+    ((1, 8, 12, 32, 2), (1, 16, 24, 5, 1), 4),        # stride 2, ragged tiles
+])
+@pytest.mark.parametrize("arith", [abi.ARITH_FP32, abi.ARITH_BF16])
+def test_recv_post_several_patches_mfma(case, arith):
+    rng = np.random.default_rng(30)
+    hs, os_, ht, ot, p, W, A = gather_setup(case, rng, density=0.3)
+    ref = O.recv_post(os_, ot, p, p, A, W).astype(np.float64)
+    d = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=arith)
+    g = torch.full((ht.nbatch * ht.num_res,), 5.0, device=DEV)
+    d.deliver(dev(A), g)
+    err = np.abs(host(g) - ref)
+    scale = np.abs(ref).max()
+    if arith == abi.ARITH_FP32:
+        assert err.max() <= 1e-5 * scale
+    else:
+        assert np.sqrt((err ** 2).mean()) <= 1e-2 * np.sqrt((ref ** 2).mean())
+    # accumulate adds onto the existing GSyn
+    d.deliver(dev(A), g, accumulate=True)
+    assert np.abs(host(g) - 2 * ref).max() <= (2e-5 if arith == abi.ARITH_FP32 else 4e-2) * scale
+
+
+@pytest.mark.parametrize("case", SCATTER_CASES[:2])
+@pytest.mark.parametrize("arith", [abi.ARITH_FP32, abi.ARITH_BF16])
+def test_recv_pre_transposed_mfma_route(case, arith):
+    """BF16 / FP32 scatter as a post-perspective gather with transposed weights."""
+    rng = np.random.default_rng(31)
+    (hp, op), (hq, oq), p, dens = both(case[0]), both(case[1]), case[2], case[3]
+    kx, ky, kf = O.num_kernels(op, oq)
+    W = (rng.standard_normal((kx * ky * kf, p, p, oq.nf)) * 0.1).astype(np.float32)
+    A = ext_random(hp, rng, dens)
+    ent, cnt = O.active_indices(op, A)
+    ref = O.recv_pre(op, oq, p, p, W, entries=ent, num_active=cnt, num_threads=1).astype(np.float64)
+    d = R.PresynapticPerspectiveGPUDelivery(hp, hq, p, p, dev(W), arith=arith)
+    assert d.transposed is not None
+    g = torch.zeros(hq.nbatch * hq.num_res, device=DEV)
+    d.deliver(g, pre_activity=dev(A))
+    err = np.abs(host(g) - ref)
+    if arith == abi.ARITH_FP32:
+        assert err.max() <= 1e-5 * np.abs(ref).max()
+    else:
+        assert np.sqrt((err ** 2).mean()) <= 1e-2 * np.sqrt((ref ** 2).mean())
