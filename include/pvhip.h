@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PVHIP_ABI_VERSION 1
+#define PVHIP_ABI_VERSION 2
 
 enum {
    PVHIP_OK              = 0,
@@ -199,6 +199,12 @@ typedef struct {
    const float *exp_tau; /* device [nbatch] */
    int self_interact;
    pvhip_transfer transfer;
+   /* Nonzero: the caller guarantees A[kex] == transfer(V) bit for bit for every
+    * restricted neuron on entry -- the state this update leaves behind and the
+    * state the initial activity update (ANNActivityBuffer) gives.  A(t-1) in the
+    * self-interaction term is then recomputed from V instead of read, which
+    * saves one read of A per neuron with identical results. */
+   int activity_from_v;
 } pvhip_lca_update_args;
 
 /* Host helper: exp_tau[b] = (float)exp(-dt[b]/(double)tau), the reference's

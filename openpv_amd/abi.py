@@ -76,7 +76,7 @@ class Transfer(C.Structure):
 class LCAUpdateArgs(C.Structure):
     _fields_ = [("loc", Loc), ("num_channels", i32), ("channel_coeffs", f32 * MAX_CHANNELS),
                 ("gsyn", vp), ("V", vp), ("A", vp), ("exp_tau", vp), ("self_interact", i32),
-                ("transfer", Transfer)]
+                ("transfer", Transfer), ("activity_from_v", i32)]
 
 
 class RecvPostArgs(C.Structure):

@@ -26,6 +26,7 @@ int make_lca_params(const pvhip_lca_update_args &a, LCAParams *p) {
    p->A            = a.A;
    p->expTau       = a.exp_tau;
    p->selfInteract = a.self_interact ? 1.0f : 0.0f;
+   p->aFromV       = a.activity_from_v ? 1 : 0;
    return make_transfer(a.transfer, &p->transfer);
 }
 

@@ -191,18 +191,27 @@ struct LCAParams {
    const float *expTau;
    float selfInteract;
    Transfer transfer;
+   int aFromV; // A(t-1) = transfer(V(t-1)) is recomputed instead of read
 };
 
 int make_lca_params(const pvhip_lca_update_args &a, LCAParams *p);
 
+// Transfer selected at compile time: MODE 1 is the VTHRESH form only,
+// MODE 0 dispatches on t.mode.
+template <int MODE>
+__device__ __forceinline__ float transfer_as(const Transfer &t, float V) {
+   return MODE == 1 ? apply_vthresh(t, V) : apply_transfer(t, V);
+}
+
 // One neuron of the fused K6+K7 update given its accumulated input G
 // (HyPerLCAInternalStateBuffer.kpp:51-59, then the transfer).
+template <int MODE = 0>
 __device__ __forceinline__ void lca_update_one(const LCAParams &p, int b, int k, float G) {
    float const e   = p.expTau[b];
    size_t const vi = (size_t)b * p.loc.numRes() + k;
    size_t const ai = (size_t)b * p.loc.numExt() + p.loc.kex(k);
-   float const Aold = p.A[ai];
    float const Vold = p.V[vi];
+   float const Aold = p.aFromV ? transfer_as<MODE>(p.transfer, Vold) : p.A[ai];
    float t1    = e * Vold;
    float t2    = 1.0f - e;
    float si    = p.selfInteract * Aold;
@@ -210,7 +219,7 @@ __device__ __forceinline__ void lca_update_one(const LCAParams &p, int b, int k,
    float t4    = t2 * t3;
    float Vnew  = t1 + t4;
    p.V[vi]     = Vnew;
-   p.A[ai]     = apply_transfer(p.transfer, Vnew);
+   p.A[ai]     = transfer_as<MODE>(p.transfer, Vnew);
 }
 
 constexpr int kWave = 64;

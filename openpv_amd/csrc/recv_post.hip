@@ -264,7 +264,8 @@ __device__ __forceinline__ bf16x8 lds_frag(const __bf16 *p) {
 // LCA update with 16-byte loads and stores.  EPI: 0 GSyn only, 1 fused update
 // with the VTHRESH transfer, 2 fused update with either transfer.
 template <int EPI>
-__device__ __forceinline__ void post_epilogue4(const PostParams &p, int b, int k, f32x4 acc) {
+__device__ __forceinline__ void post_epilogue4(
+      const PostParams &p, int b, int k, f32x4 acc, float4 Vo) {
    size_t const gi = (size_t)b * p.post.numRes() + k;
    float G[4];
    if (p.accumulate) {
@@ -287,10 +288,24 @@ __device__ __forceinline__ void post_epilogue4(const PostParams &p, int b, int k
    const LCAParams &q = p.lca;
    float const e      = q.expTau[b];
    size_t const ai    = (size_t)b * q.loc.numExt() + q.loc.kex(k);
-   float4 Ao          = *reinterpret_cast<const float4 *>(q.A + ai);
-   float4 Vo          = *reinterpret_cast<const float4 *>(q.V + gi);
-   float const ao[4]  = {Ao.x, Ao.y, Ao.z, Ao.w};
+   if (EPI == 2) {
+      Vo = *reinterpret_cast<const float4 *>(q.V + gi); // not prefetched: see the kernel
+   }
    float const vo[4]  = {Vo.x, Vo.y, Vo.z, Vo.w};
+   float ao[4];
+   if (q.aFromV) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+         ao[i] = transfer_as<EPI == 1 ? 1 : 0>(q.transfer, vo[i]);
+      }
+   }
+   else {
+      float4 Ao = *reinterpret_cast<const float4 *>(q.A + ai);
+      ao[0] = Ao.x;
+      ao[1] = Ao.y;
+      ao[2] = Ao.z;
+      ao[3] = Ao.w;
+   }
    float vn[4], an[4];
 #pragma unroll
    for (int i = 0; i < 4; i++) {
@@ -300,14 +315,14 @@ __device__ __forceinline__ void post_epilogue4(const PostParams &p, int b, int k
       float t3 = G[i] + si;
       float t4 = t2 * t3;
       vn[i]    = t1 + t4;
-      an[i]    = EPI == 1 ? apply_vthresh(q.transfer, vn[i]) : apply_transfer(q.transfer, vn[i]);
+      an[i]    = transfer_as<EPI == 1 ? 1 : 0>(q.transfer, vn[i]);
    }
    *reinterpret_cast<float4 *>(q.V + gi) = make_float4(vn[0], vn[1], vn[2], vn[3]);
    *reinterpret_cast<float4 *>(q.A + ai) = make_float4(an[0], an[1], an[2], an[3]);
 }
 
 template <int ALIGN, int EPI>
-__global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) recv_post_mfma_bf16_kernel(
       PostParams p, const __bf16 *Wb, int srcRowsMax, int srcColsMax, int Kpad) {
    extern __shared__ float smem[];
    int const K       = p.nxp * p.nfp * p.nyp;
@@ -331,6 +346,33 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
    if (srcCols > srcColsMax || srcRows > srcRowsMax || srcX0 < 0 || srcY0 < 0
        || srcX0 + srcCols > p.pre.nxExt() || srcY0 + srcRows > p.pre.nyExt()) {
       return;
+   }
+   int const lane = tid % 64;
+   int const wave = tid / 64;
+   int const r    = lane % 16;
+   int const g    = lane / 16;
+   int const fWave = fBase + wave * (MF / 4);
+   bool const vec4 = (p.post.nf & 3) == 0;
+   // Fused VTHRESH update: the old V of the neurons this lane updates is
+   // fetched first so that its latency overlaps the staging and the MFMA
+   // loop.  (The vertex transfer needs those registers itself.)
+   float4 vpre[MPB][MFB];
+#pragma unroll
+   for (int pb = 0; pb < MPB; pb++) {
+#pragma unroll
+      for (int fb = 0; fb < MFB; fb++) {
+         vpre[pb][fb] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+         if (EPI == 1 && vec4) {
+            int pj = pb * 16 + r;
+            int x  = x0 + pj % MTX;
+            int y  = y0 + pj / MTX;
+            int f  = fWave + fb * 16 + 4 * g;
+            if (x < p.post.nx && y < p.post.ny && f < p.post.nf) {
+               vpre[pb][fb] = *reinterpret_cast<const float4 *>(
+                     p.lca.V + (size_t)b * p.post.numRes() + (y * p.post.nx + x) * p.post.nf + f);
+            }
+         }
+      }
    }
    int const preRow  = p.pre.nxExt() * p.pre.nf;
    float const *preB = p.pre_data + (size_t)b * p.pre.numExt();
@@ -360,10 +402,6 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
    }
    __syncthreads();
 
-   int const lane = tid % 64;
-   int const wave = tid / 64;
-   int const r    = lane % 16;
-   int const g    = lane / 16;
    int pbase[MPB];
 #pragma unroll
    for (int pb = 0; pb < MPB; pb++) {
@@ -392,7 +430,6 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
          acc[fb][pb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
       }
    }
-   int const fWave = fBase + wave * (MF / 4);
    constexpr int KPRE = 8; // k-steps whose A fragments are fetched together
    for (int kb = 0; kb < Kpad; kb += 32 * KPRE) {
       uint4 apre[KPRE][MFB]; // bf16x8 fragments kept as raw 128-bit registers
@@ -454,15 +491,21 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
          if (!inside) {
             // position outside the layer: nothing to store
          }
-         else if (f + 3 < p.post.nf && (p.post.nf & 3) == 0) {
-            post_epilogue4<EPI>(p, b, kpos + f, acc[fb][pb]);
+         else if (f + 3 < p.post.nf && vec4) {
+            post_epilogue4<EPI>(p, b, kpos + f, acc[fb][pb], vpre[pb][fb]);
          }
          else {
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                if (f + i < p.post.nf) {
-                  float base = p.accumulate ? p.gsyn[(size_t)b * p.post.numRes() + kpos + f + i] : 0.0f;
-                  post_epilogue(p, b, kpos + f + i, base + p.dtf * acc[fb][pb][i]);
+                  size_t const gi = (size_t)b * p.post.numRes() + kpos + f + i;
+                  float const G   = (p.accumulate ? p.gsyn[gi] : 0.0f) + p.dtf * acc[fb][pb][i];
+                  if (EPI == 0) {
+                     p.gsyn[gi] = G;
+                  }
+                  else {
+                     lca_update_one<EPI == 1 ? 1 : 0>(p.lca, b, kpos + f + i, G);
+                  }
                }
             }
          }

@@ -84,8 +84,9 @@ class LCAColumn:
         self.scatter = PresynapticPerspectiveGPUDelivery(self.v1_loc, self.rec_loc, nxp, nyp,
                                                          self.weights, arith=arith)
         self.active = ActiveList(self.v1_loc, d)
+        # V1's activity is written only by its own update and starts as T(V0).
         self.lca = HyPerLCAUpdate(self.v1_loc, self.v1_transfer, timeConstantTau, dt,
-                                  self_interact, device=d)
+                                  self_interact, activity_from_v=True, device=d)
         self.energy = torch.zeros(2, nb, dtype=torch.float64, device=d)
         self._graph = None
         self._fused_args = None

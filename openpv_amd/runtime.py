@@ -293,8 +293,12 @@ class HyPerLCAUpdate:
     ANNActivityBuffer in one kernel."""
 
     def __init__(self, loc_, transfer, timeConstantTau=1.0, dt=1.0, self_interact=True,
-                 channel_coeffs=(1.0,), device="cuda"):
+                 channel_coeffs=(1.0,), activity_from_v=False, device="cuda"):
+        """activity_from_v: A == transfer(V) holds on entry to every update (the layer's
+        activity is written only by this update and its initial A = T(V)), so the
+        kernels recompute A(t-1) from V instead of reading it."""
         self.loc = loc_
+        self.activity_from_v = bool(activity_from_v)
         self.transfer = transfer
         self.tau = scaled_tau(timeConstantTau, dt)
         self.exp_tau = torch.as_tensor(exp_tau(dt, self.tau, loc_.nbatch), device=device)
@@ -317,6 +321,7 @@ class HyPerLCAUpdate:
         a.exp_tau = _ptr(self.exp_tau)
         a.self_interact = int(self.self_interact)
         a.transfer = self.transfer
+        a.activity_from_v = int(self.activity_from_v)
         return a
 
     def update(self, gsyn, V, A, stream=None):

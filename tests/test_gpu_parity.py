@@ -88,7 +88,8 @@ def test_device_layer():
     (True, (0.025, 0.0, FMAX, 0.0, 0.025)),
     (False, (0.1, 0.0, 0.4, 0.05, 0.1)),
 ])
-def test_lca_update_bit_exact(vertices, params):
+@pytest.mark.parametrize("from_v", [False, True])
+def test_lca_update_bit_exact(vertices, params, from_v):
     rng = np.random.default_rng(21)
     hl, ol = both((3, 11, 7, 5, (2, 1, 1, 2)))
     n = hl.nbatch * hl.num_res
@@ -101,8 +102,9 @@ def test_lca_update_bit_exact(vertices, params):
     O.lib().or_gsyn_accumulate(n, 2, np.array([1.0, -0.5], np.float32), G, acc)
     Vr = O.update_lca(ol, True, np.array([1.0, 0.5, 2.0]), 100.0, acc, A, V)
     Ar = O.transfer(ol, Vr, *params, vertices=vertices, A=A)
+    # A == T(V) on entry, so recomputing A(t-1) from V must give the same bits
     upd = R.HyPerLCAUpdate(hl, R.make_transfer(*params, vertices=vertices), 100.0, 1.0, True,
-                           channel_coeffs=(1.0, -0.5), device=DEV)
+                           channel_coeffs=(1.0, -0.5), activity_from_v=from_v, device=DEV)
     upd.set_dt([1.0, 0.5, 2.0])
     Vd, Ad = dev(V), dev(A)
     upd.update(dev(G), Vd, Ad)
@@ -220,22 +222,25 @@ def test_recv_post_fp32_and_bf16(case):
 
 @pytest.mark.parametrize("case", GATHER_CASES[:3])
 @pytest.mark.parametrize("arith", [abi.ARITH_EXACT, abi.ARITH_BF16])
-def test_recv_post_fused_update(case, arith):
-    """Gather with the LCA update in its epilogue == gather then update."""
+@pytest.mark.parametrize("from_v,vertices", [(False, False), (True, False), (True, True)])
+def test_recv_post_fused_update(case, arith, from_v, vertices):
+    """Gather with the LCA update in its epilogue == gather then update; with
+    activity_from_v the epilogue recomputes A(t-1) = T(V(t-1)) instead of reading it."""
     rng = np.random.default_rng(26)
     hs, os_, ht, ot, p, W, A = gather_setup(case, rng)
     n = ht.nbatch * ht.num_res
     V0 = (rng.standard_normal(n) * 0.1).astype(np.float32)
-    tr = R.make_transfer(0.05, 0.0, FMAX, 0.05, 0.0)
-    A0 = O.transfer(ot, V0, 0.05, 0.0, FMAX, 0.05, 0.0)
+    tr = R.make_transfer(0.05, 0.0, FMAX, 0.05, 0.0, vertices=vertices)
+    A0 = O.transfer(ot, V0, 0.05, 0.0, FMAX, 0.05, 0.0, vertices=vertices)
     d = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=arith)
     upd = R.HyPerLCAUpdate(ht, tr, 100.0, 1.0, True, device=DEV)
+    fus = R.HyPerLCAUpdate(ht, tr, 100.0, 1.0, True, activity_from_v=from_v, device=DEV)
     g = torch.zeros(n, device=DEV)
     Va, Aa = dev(V0), dev(A0)
     d.deliver(dev(A), g)
     upd.update(g, Va, Aa)
     Vb, Ab = dev(V0), dev(A0)
-    d.deliver(dev(A), None, fused=upd.args(None, Vb, Ab, num_channels=1))
+    d.deliver(dev(A), None, fused=fus.args(None, Vb, Ab, num_channels=1))
     assert same(host(Va), host(Vb)) and same(host(Aa), host(Ab))
 
 
