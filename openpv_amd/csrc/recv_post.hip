@@ -517,142 +517,252 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
 // Implicit GEMM for sources with nfp % CK == 0 channels and any number of
 // data patches (targets larger than the source, e.g. the transposed form of
 // a one-to-many scatter): M = every kernel (nKx*nKy*post.nf, padded to 16),
-// N = distinct window starts (qx, qy) of a QX x QY tile, K = nyp*nxp*nfp
-// taken as (channel chunk, ky, kx, channel).  Output (kernel m, window q)
-// belongs to the target neuron whose kernel index and window start are
-// (m, q): the inverse tables map (kernel x-index, qx) -> target x.
+// N = distinct window starts (qx, qy), K = nyp*nxp*nfp taken as
+// (channel chunk, ky, kx, channel).  Output (kernel m, window q) belongs to
+// the target neuron whose kernel index and window start are (m, q): the
+// inverse tables map (kernel x-index, qx) -> target x.
+//
+// Tiling: the window-start grid is cut into strips of QW columns; a
+// workgroup takes CNT consecutive window starts of one strip in row-major
+// order, so a tile wastes at most one partial tile per strip, and stages
+// the source rows those starts touch (full strip width plus the patch
+// halo).  Per channel chunk the source (converted to T) and the weights of
+// every tap are staged in LDS with 16 bytes of padding per row of CK
+// values (conflict-free 16-byte fragment reads); the next chunk's global
+// loads are issued before the MFMAs of the current one.
 // T = __bf16 uses v_mfma_f32_16x16x32_bf16 (CK = 32); T = float uses
-// v_mfma_f32_16x16x4f32 (CK = 16).
-constexpr int CQX = 32, CQY = 8;        // window-start tile; wave w owns rows 2w, 2w+1
+// v_mfma_f32_16x16x4f32 (CK = 16, lane group g holds channels 4g..4g+3 and
+// MFMA j of a chunk sums channel 4g+j, the same permutation for A and B).
+constexpr int CNT  = 128; // window starts per workgroup
+constexpr int CNBW = 2;   // 16-start MFMA blocks per wave (4 waves * 2 * 16 = CNT)
+constexpr int CSV  = 12;  // source float4 per thread carried across a chunk in registers
+constexpr int CAV  = 4;   // weight 16-byte pieces per thread carried in registers
 
 template <typename T>
 struct CGemm;
 template <>
 struct CGemm<__bf16> {
    static constexpr int CK = 32;
+   static constexpr int PP = 40; // LDS pitch (elements) of one row of CK values
 };
 template <>
 struct CGemm<float> {
    static constexpr int CK = 16;
+   static constexpr int PP = 20;
 };
+
+struct CStrip {
+   int qx0, qy0, nqx, nqy; // window-start grid in extended source coordinates
+   int M;                  // kernels (GEMM rows) before padding
+   int QW, nStrips, tilesPerStrip;
+   int rowsMax, twMax;     // staged source tile bound (rows, columns)
+   int remap;              // 1: workgroups of one batch share an XCD (grid % 8 == 0)
+};
+
+__device__ __forceinline__ void store4(__bf16 *d, float4 v) {
+   bf16x4 h = bf16x4{(__bf16)v.x, (__bf16)v.y, (__bf16)v.z, (__bf16)v.w};
+   *reinterpret_cast<bf16x4 *>(d) = h;
+}
+__device__ __forceinline__ void store4(float *d, float4 v) { *reinterpret_cast<float4 *>(d) = v; }
 
 template <typename T, int MB>
 __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
-      PostParams p, const T *Wq, const int32_t *invX, const int32_t *invY, int qx0, int qy0,
-      int nqx, int nqy, int M) {
+      PostParams p, const T *Wq, const int32_t *invX, const int32_t *invY, CStrip cs) {
    // Wq: weights in T, layout [M][nyp][nxp][C] (the post weights' own layout;
    // the BF16 shadow of pvhip_recv_post_bf16_weights or the FP32 weights).
    constexpr int CK = CGemm<T>::CK;
-   constexpr int U  = 8; // independent loads in flight per thread while staging
+   constexpr int PP = CGemm<T>::PP;
+   constexpr int Mp = MB * 16;
+   constexpr int EP = 16 / sizeof(T); // elements per 16-byte piece
    extern __shared__ float smem[];
-   int const C      = p.nfp;
-   int const tw     = CQX + p.nxp - 1;
-   int const th     = CQY + p.nyp - 1;
-   T *srcl          = reinterpret_cast<T *>(smem); // [th][tw][CK]
-   int const tilesX = (nqx + CQX - 1) / CQX;
-   int const qtx    = (blockIdx.x % tilesX) * CQX;
-   int const qty    = (blockIdx.x / tilesX) * CQY;
-   int const b      = blockIdx.z;
-   int const tid    = threadIdx.x;
-   int const lane   = tid % 64;
-   int const wave   = tid / 64;
-   int const r      = lane % 16;
-   int const g      = lane / 16;
-   int const nxExt  = p.pre.nxExt();
-   int const nyExt  = p.pre.nyExt();
-   float const *preB = p.pre_data + (size_t)b * p.pre.numExt();
-   int const patch  = p.nyp * p.nxp * C;
-   int const quads  = th * tw * (CK / 4); // float4 per staged chunk
+   T *srcl = reinterpret_cast<T *>(smem);           // [rowsMax][twMax][PP]
+   T *wl   = srcl + (size_t)cs.rowsMax * cs.twMax * PP; // [taps][Mp][PP]
 
-   f32x4 acc[MB][4];
+   int L = blockIdx.x;
+   if (cs.remap) {
+      L = (L % 8) * (gridDim.x / 8) + L / 8; // hardware places workgroup L on XCD L % 8
+   }
+   int const perBatch = cs.nStrips * cs.tilesPerStrip;
+   int const b        = L / perBatch;
+   int const strip    = (L % perBatch) / cs.tilesPerStrip;
+   int const tile     = L % cs.tilesPerStrip;
+   int const qxS      = strip * cs.QW;
+   int const QWs      = min(cs.QW, cs.nqx - qxS);
+   int const nS       = QWs * cs.nqy;
+   int const n0       = tile * CNT;
+   if (b >= p.post.nbatch || n0 >= nS) {
+      return; // whole workgroup: no barrier is skipped by part of it
+   }
+   int const qyA  = n0 / QWs;
+   int const qyB  = (min(n0 + CNT, nS) - 1) / QWs;
+   int const rows = qyB - qyA + p.nyp;
+   int const tw   = QWs + p.nxp - 1;
+   int const tid  = threadIdx.x;
+   int const lane = tid % 64;
+   int const wave = tid / 64;
+   int const r    = lane % 16;
+   int const g    = lane / 16;
+   int const C    = p.nfp;
+   int const taps = p.nyp * p.nxp;
+   int const patch = taps * C;
+   int const nxExt = p.pre.nxExt();
+   int const nyExt = p.pre.nyExt();
+   float const *preB = p.pre_data + (size_t)b * p.pre.numExt();
+   int const srcItems = rows * tw * (CK / 4);
+   int const wItems   = taps * Mp * (CK / EP);
+
+   // LDS offset of each MFMA column (window start) of this lane at tap 0
+   int nbase[CNBW];
+#pragma unroll
+   for (int nb = 0; nb < CNBW; nb++) {
+      int n     = min(n0 + (wave * CNBW + nb) * 16 + r, nS - 1);
+      nbase[nb] = ((n / QWs - qyA) * cs.twMax + n % QWs) * PP;
+   }
+
+   // Item offsets do not depend on the chunk: computed once.  A source item
+   // is 4 channels of one staged position; a weight item 16 bytes of one
+   // (tap, kernel) row of CK channels.
+   auto src_goff = [&](int i) {
+      int q4 = i % (CK / 4);
+      int cc = (i / (CK / 4)) % tw;
+      int rr = i / ((CK / 4) * tw);
+      int ye = min(cs.qy0 + qyA + rr, nyExt - 1);
+      int xe = min(cs.qx0 + qxS + cc, nxExt - 1);
+      return (ye * nxExt + xe) * C + 4 * q4;
+   };
+   auto src_loff = [&](int i) {
+      int q4 = i % (CK / 4);
+      int cc = (i / (CK / 4)) % tw;
+      int rr = i / ((CK / 4) * tw);
+      return (rr * cs.twMax + cc) * PP + 4 * q4;
+   };
+   auto w_goff = [&](int j) { // -1: padding kernel row (zeros)
+      int pc = j % (CK / EP);
+      int m  = (j / (CK / EP)) % Mp;
+      int t  = j / ((CK / EP) * Mp);
+      return m < cs.M ? m * patch + t * C + pc * EP : -1;
+   };
+   auto w_loff = [&](int j) {
+      int pc = j % (CK / EP);
+      int m  = (j / (CK / EP)) % Mp;
+      int t  = j / ((CK / EP) * Mp);
+      return (t * Mp + m) * PP + pc * EP;
+   };
+   int sG[CSV], sL[CSV], wG[CAV], wL[CAV];
+#pragma unroll
+   for (int u = 0; u < CSV; u++) {
+      int i = tid + u * 256;
+      sG[u] = src_goff(min(i, srcItems - 1));
+      sL[u] = i < srcItems ? src_loff(i) : -1;
+   }
+#pragma unroll
+   for (int u = 0; u < CAV; u++) {
+      int j = tid + u * 256;
+      wG[u] = w_goff(min(j, wItems - 1));
+      wL[u] = j < wItems ? w_loff(j) : -1;
+   }
+   float4 sv[CSV];
+   uint4 wv[CAV];
+   // global -> registers for chunk c0 (the first CSV / CAV items per thread)
+   auto fetch = [&](int c0) {
+#pragma unroll
+      for (int u = 0; u < CSV; u++) {
+         sv[u] = *reinterpret_cast<const float4 *>(preB + sG[u] + c0);
+      }
+#pragma unroll
+      for (int u = 0; u < CAV; u++) {
+         wv[u] = wG[u] >= 0 ? *reinterpret_cast<const uint4 *>(Wq + (size_t)wG[u] + c0)
+                            : make_uint4(0u, 0u, 0u, 0u);
+      }
+   };
+   // registers (and any items beyond them, loaded now) -> LDS for chunk c0
+   auto stage = [&](int c0) {
+#pragma unroll
+      for (int u = 0; u < CSV; u++) {
+         if (sL[u] >= 0) {
+            store4(srcl + sL[u], sv[u]);
+         }
+      }
+      for (int i = tid + CSV * 256; i < srcItems; i += 256) {
+         store4(srcl + src_loff(i), *reinterpret_cast<const float4 *>(preB + src_goff(i) + c0));
+      }
+#pragma unroll
+      for (int u = 0; u < CAV; u++) {
+         if (wL[u] >= 0) {
+            *reinterpret_cast<uint4 *>(wl + wL[u]) = wv[u];
+         }
+      }
+      for (int j = tid + CAV * 256; j < wItems; j += 256) {
+         int go  = w_goff(j);
+         uint4 v = go >= 0 ? *reinterpret_cast<const uint4 *>(Wq + (size_t)go + c0) : make_uint4(0u, 0u, 0u, 0u);
+         *reinterpret_cast<uint4 *>(wl + w_loff(j)) = v;
+      }
+   };
+
+   f32x4 acc[MB][CNBW];
 #pragma unroll
    for (int mb = 0; mb < MB; mb++) {
 #pragma unroll
-      for (int nb = 0; nb < 4; nb++) {
+      for (int nb = 0; nb < CNBW; nb++) {
          acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
       }
    }
+   fetch(0);
    for (int c0 = 0; c0 < C; c0 += CK) {
+      __syncthreads(); // every wave is done reading the previous chunk
+      stage(c0);
       __syncthreads();
-      for (int base = tid; base < quads; base += 256 * U) {
-         float4 v[U];
-#pragma unroll
-         for (int u = 0; u < U; u++) {
-            int i  = min(base + u * 256, quads - 1);
-            int q4 = i % (CK / 4);
-            int xx = (i / (CK / 4)) % tw;
-            int yy = i / ((CK / 4) * tw);
-            int ye = min(qy0 + qty + yy, nyExt - 1); // rows/cols past the grid are never used
-            int xe = min(qx0 + qtx + xx, nxExt - 1);
-            v[u]   = *reinterpret_cast<const float4 *>(preB + ((size_t)ye * nxExt + xe) * C + c0 + 4 * q4);
-         }
-#pragma unroll
-         for (int u = 0; u < U; u++) {
-            int i = base + u * 256;
-            if (i < quads) {
-               T *d = srcl + 4 * i;
-               d[0] = (T)v[u].x;
-               d[1] = (T)v[u].y;
-               d[2] = (T)v[u].z;
-               d[3] = (T)v[u].w;
-            }
-         }
+      if (c0 + CK < C) {
+         fetch(c0 + CK); // in flight during this chunk's MFMAs
       }
-      __syncthreads();
-      for (int ky = 0; ky < p.nyp; ky++) {
-         for (int kx = 0; kx < p.nxp; kx++) {
-            int const wo = (ky * p.nxp + kx) * C + c0;
-            if constexpr (CK == 32) {
-               bf16x8 af[MB];
+      for (int t = 0; t < taps; t++) {
+         int const ky     = t / p.nxp;
+         int const kx     = t - ky * p.nxp;
+         int const tapOff = (ky * cs.twMax + kx) * PP;
+         T const *wt      = wl + t * Mp * PP;
+         if constexpr (CK == 32) {
+            bf16x8 af[MB];
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++) {
+               af[mb] = *reinterpret_cast<const bf16x8 *>(wt + (mb * 16 + r) * PP + 8 * g);
+            }
+#pragma unroll
+            for (int nb = 0; nb < CNBW; nb++) {
+               bf16x8 bf = *reinterpret_cast<const bf16x8 *>(srcl + nbase[nb] + tapOff + 8 * g);
 #pragma unroll
                for (int mb = 0; mb < MB; mb++) {
-                  int m  = mb * 16 + r;
-                  af[mb] = m < M ? *reinterpret_cast<const bf16x8 *>(Wq + (size_t)m * patch + wo + 8 * g)
-                                 : bf16x8{};
-               }
-#pragma unroll
-               for (int nb = 0; nb < 4; nb++) {
-                  int qyl = 2 * wave + nb / 2; // row within the tile
-                  int qxl = (nb % 2) * 16 + r; // column within the tile
-                  bf16x8 bf = *reinterpret_cast<const bf16x8 *>(
-                        srcl + ((qyl + ky) * tw + qxl + kx) * CK + 8 * g);
-#pragma unroll
-                  for (int mb = 0; mb < MB; mb++) {
-                     acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mb], bf, acc[mb][nb], 0, 0, 0);
-                  }
+                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mb], bf, acc[mb][nb], 0, 0, 0);
                }
             }
-            else {
+         }
+         else {
+            float4 af[MB];
 #pragma unroll
-               for (int cc = 0; cc < CK; cc += 4) {
-                  float af[MB];
+            for (int mb = 0; mb < MB; mb++) {
+               af[mb] = *reinterpret_cast<const float4 *>(wt + (mb * 16 + r) * PP + 4 * g);
+            }
 #pragma unroll
-                  for (int mb = 0; mb < MB; mb++) {
-                     int m  = mb * 16 + r;
-                     af[mb] = m < M ? (float)Wq[(size_t)m * patch + wo + cc + g] : 0.0f;
-                  }
+            for (int nb = 0; nb < CNBW; nb++) {
+               float4 bv = *reinterpret_cast<const float4 *>(srcl + nbase[nb] + tapOff + 4 * g);
 #pragma unroll
-                  for (int nb = 0; nb < 4; nb++) {
-                     int qyl  = 2 * wave + nb / 2;
-                     int qxl  = (nb % 2) * 16 + r;
-                     float bv = (float)srcl[((qyl + ky) * tw + qxl + kx) * CK + cc + g];
-#pragma unroll
-                     for (int mb = 0; mb < MB; mb++) {
-                        acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb], bv, acc[mb][nb], 0, 0, 0);
-                     }
-                  }
+               for (int mb = 0; mb < MB; mb++) {
+                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].x, bv.x, acc[mb][nb], 0, 0, 0);
+                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].y, bv.y, acc[mb][nb], 0, 0, 0);
+                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].z, bv.z, acc[mb][nb], 0, 0, 0);
+                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].w, bv.w, acc[mb][nb], 0, 0, 0);
                }
             }
          }
       }
    }
-   // Lane holds D[4g+i][r]: kernel mb*16+4g+i at window (qx, qy) of block nb.
+   // Lane holds D[4g+i][r]: kernel mb*16+4g+i at window start n of block nb.
    int const nf = p.post.nf;
 #pragma unroll
-   for (int nb = 0; nb < 4; nb++) {
-      int qx = qtx + (nb % 2) * 16 + r;
-      int qy = qty + 2 * wave + nb / 2;
-      bool const inside = qx < nqx && qy < nqy;
+   for (int nb = 0; nb < CNBW; nb++) {
+      int const n       = n0 + (wave * CNBW + nb) * 16 + r;
+      bool const inside = n < nS;
+      int const qx      = qxS + (inside ? n % QWs : 0);
+      int const qy      = inside ? n / QWs : 0;
 #pragma unroll
       for (int mb = 0; mb < MB; mb++) {
 #pragma unroll
@@ -661,9 +771,9 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
             int F  = m % nf;
             int kx = (m / nf) % p.nKx;
             int ky = m / (nf * p.nKx);
-            if (inside && m < M) {
-               int x = invX[kx * nqx + qx];
-               int y = invY[ky * nqy + qy];
+            if (inside && m < cs.M) {
+               int x = invX[kx * cs.nqx + qx];
+               int y = invY[ky * cs.nqy + qy];
                if (x >= 0 && y >= 0) {
                   size_t gi  = (size_t)b * p.post.numRes() + ((size_t)y * p.post.nx + x) * nf + F;
                   float base = p.accumulate ? p.gsyn[gi] : 0.0f;
@@ -786,22 +896,44 @@ int pvhip_recv_post(const pvhip_recv_post_args *args, pvhip_stream stream) {
       int MB = (M + 15) / 16;
       PVHIP_REQUIRE(MB <= 4, "recv_post: %d kernels exceed the MFMA tile (64)", M);
       MB = MB <= 1 ? 1 : (MB <= 2 ? 2 : 4);
-      int tw = CQX + p.nxp - 1, th = CQY + p.nyp - 1;
-      size_t esz = bf ? 2 : 4;
-      size_t lds = (size_t)th * tw * CK * esz;
       if (bf) {
          PVHIP_REQUIRE(args->weights_lp != nullptr,
                "recv_post: BF16 channel GEMM needs weights_lp (pvhip_recv_post_bf16_weights)");
       }
       const void *Wq = bf ? args->weights_lp : (const void *)args->weights;
+      CStrip cs;
+      cs.qx0 = args->window_x0;
+      cs.qy0 = args->window_y0;
+      cs.nqx = args->window_nx;
+      cs.nqy = args->window_ny;
+      cs.M   = M;
+      size_t const esz = bf ? 2 : 4;
+      int const PP     = bf ? CGemm<__bf16>::PP : CGemm<float>::PP;
+      size_t const wlds = (size_t)p.nyp * p.nxp * MB * 16 * PP * esz;
+      size_t lds        = 0;
+      // widest strip (at most 64 starts) whose staged tile fits the LDS budget
+      for (int strips = (cs.nqx + 63) / 64;; strips++) {
+         cs.QW      = (cs.nqx + strips - 1) / strips;
+         cs.nStrips = (cs.nqx + cs.QW - 1) / cs.QW;
+         // a tile of CNT starts in a strip of width w spans <= (w + CNT - 2) / w + 1
+         // start rows; the narrowest strip (the last, width >= 1) gives the bound
+         int const wmin = cs.nqx - (cs.nStrips - 1) * cs.QW;
+         cs.rowsMax = (wmin + CNT - 2) / wmin + p.nyp;
+         cs.twMax   = cs.QW + p.nxp - 1;
+         lds        = (size_t)cs.rowsMax * cs.twMax * PP * esz + wlds;
+         if (lds <= 96 * 1024 || cs.QW == 1) {
+            break;
+         }
+      }
       PVHIP_REQUIRE(lds <= 160 * 1024, "recv_post: channel GEMM tile needs %zu bytes of LDS", lds);
-      int tilesX = (args->window_nx + CQX - 1) / CQX;
-      int tilesY = (args->window_ny + CQY - 1) / CQY;
-      dim3 grid(tilesX * tilesY, 1, p.post.nbatch);
+      cs.tilesPerStrip  = (cs.QW * cs.nqy + CNT - 1) / CNT;
+      long long blocks  = (long long)p.post.nbatch * cs.nStrips * cs.tilesPerStrip;
+      PVHIP_REQUIRE(blocks < (1LL << 31), "recv_post: channel GEMM grid too large");
+      cs.remap = blocks % 8 == 0 ? 1 : 0;
+      dim3 grid((unsigned)blocks);
 #define PVHIP_CGEMM(T, MBV)                                                                       \
    hipLaunchKernelGGL((recv_post_cgemm_kernel<T, MBV>), grid, dim3(256), lds, s, p,              \
-         reinterpret_cast<const T *>(Wq), args->inverse_x, args->inverse_y, args->window_x0,        \
-         args->window_y0, args->window_nx, args->window_ny, M)
+         reinterpret_cast<const T *>(Wq), args->inverse_x, args->inverse_y, cs)
       if (bf) {
          if (MB == 1) PVHIP_CGEMM(__bf16, 1);
          else if (MB == 2) PVHIP_CGEMM(__bf16, 2);

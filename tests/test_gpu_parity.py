@@ -412,6 +412,8 @@ def test_lca_column_bf16_energy_and_sparsity():
     ((2, 16, 16, 64, 2), (2, 64, 64, 2, 0), 4),       # transpose of a stride-4 one-to-many
     ((1, 16, 16, 128, 2), (1, 32, 32, 3, 0), 4),      # config[0] Recon <- V1 transposed
     ((1, 8, 12, 32, 2), (1, 16, 24, 5, 1), 4),        # stride 2, ragged tiles
+    ((1, 80, 20, 32, 2), (1, 160, 40, 3, 0), 4),      # window grid wider than one strip
+    ((2, 20, 20, 32, 2), (2, 40, 40, 16, 0), 4),      # 64 kernels: four MFMA row blocks
 ])
 @pytest.mark.parametrize("arith", [abi.ARITH_FP32, abi.ARITH_BF16])
 def test_recv_post_several_patches_mfma(case, arith):

@@ -216,9 +216,10 @@ def test_recv_pre_equals_conv_transpose_and_post(case):
     g_sparse = O.recv_pre(pre, post, p, p, W, entries=ent, num_active=cnt)
     g_dense = O.recv_pre(pre, post, p, p, W, pre_activity=A)
     assert np.array_equal(g_sparse, g_dense)  # same order, same bits
-    # multithreaded form (per-thread GSyn) agrees to rounding
+    # multithreaded form (per-thread GSyn, guided schedule: the split varies from
+    # run to run) agrees to rounding relative to the largest sum
     g_mt = O.recv_pre(pre, post, p, p, W, entries=ent, num_active=cnt, num_threads=4)
-    assert np.allclose(g_mt, g_sparse, rtol=1e-5, atol=1e-5)
+    assert np.abs(g_mt - g_sparse).max() <= 1e-5 * np.abs(g_sparse).max()
     # equals float64 brute force over the patch tables
     pa, gs, _, _ = O.patch_geometry(pre, post, p, p, post.nf)
     ref = np.zeros((pre.nbatch, post.num_res))
