@@ -14,6 +14,8 @@
 //   for ky: dv = 0; for k < nxp*nfp: dv += a[k]*w[k];  GSyn += dtFactor*dv
 #include "pvhip_internal.hpp"
 
+#include <algorithm>
+
 namespace pvhip {
 namespace {
 
@@ -260,12 +262,20 @@ __device__ __forceinline__ bf16x8 lds_frag(const __bf16 *p) {
    return v;
 }
 
-// Four consecutive features of one neuron position: GSyn store or the fused
-// LCA update with 16-byte loads and stores.  EPI: 0 GSyn only, 1 fused update
-// with the VTHRESH transfer, 2 fused update with either transfer.
+// Output tile of the BF16 gather staged in LDS for the epilogue: one row
+// of MF features per position, padded by 4 floats (conflict-free float4
+// writes of the MFMA D fragments).
+constexpr int MOP = MF + 4;
+
+// Four consecutive features (k .. k+3) of one neuron position with the
+// partial sum acc (not yet scaled by dtf): GSyn store or the fused LCA
+// update with 16-byte loads and non-temporal 16-byte stores of V and A
+// (neither is read again by this step's gather; the stores stream past the
+// caches).  EPI: 0 GSyn only, 1 fused update with the VTHRESH transfer,
+// 2 fused update with either transfer.  Vo: the old V when EPI == 1.
 template <int EPI>
 __device__ __forceinline__ void post_epilogue4(
-      const PostParams &p, int b, int k, f32x4 acc, float4 Vo) {
+      const PostParams &p, int b, int k, f32x4 acc, f32x4 Vo) {
    size_t const gi = (size_t)b * p.post.numRes() + k;
    float G[4];
    if (p.accumulate) {
@@ -289,14 +299,13 @@ __device__ __forceinline__ void post_epilogue4(
    float const e      = q.expTau[b];
    size_t const ai    = (size_t)b * q.loc.numExt() + q.loc.kex(k);
    if (EPI == 2) {
-      Vo = *reinterpret_cast<const float4 *>(q.V + gi); // not prefetched: see the kernel
+      Vo = *reinterpret_cast<const f32x4 *>(q.V + gi); // not prefetched: see the kernel
    }
-   float const vo[4]  = {Vo.x, Vo.y, Vo.z, Vo.w};
    float ao[4];
    if (q.aFromV) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-         ao[i] = transfer_as<EPI == 1 ? 1 : 0>(q.transfer, vo[i]);
+         ao[i] = transfer_as<EPI == 1 ? 1 : 0>(q.transfer, Vo[i]);
       }
    }
    else {
@@ -306,10 +315,10 @@ __device__ __forceinline__ void post_epilogue4(
       ao[2] = Ao.z;
       ao[3] = Ao.w;
    }
-   float vn[4], an[4];
+   f32x4 vn, an;
 #pragma unroll
    for (int i = 0; i < 4; i++) {
-      float t1 = e * vo[i];
+      float t1 = e * Vo[i];
       float t2 = 1.0f - e;
       float si = q.selfInteract * ao[i];
       float t3 = G[i] + si;
@@ -317,8 +326,8 @@ __device__ __forceinline__ void post_epilogue4(
       vn[i]    = t1 + t4;
       an[i]    = transfer_as<EPI == 1 ? 1 : 0>(q.transfer, vn[i]);
    }
-   *reinterpret_cast<float4 *>(q.V + gi) = make_float4(vn[0], vn[1], vn[2], vn[3]);
-   *reinterpret_cast<float4 *>(q.A + ai) = make_float4(an[0], an[1], an[2], an[3]);
+   __builtin_nontemporal_store(vn, reinterpret_cast<f32x4 *>(q.V + gi));
+   __builtin_nontemporal_store(an, reinterpret_cast<f32x4 *>(q.A + ai));
 }
 
 template <int ALIGN, int EPI>
@@ -353,24 +362,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
    int const g    = lane / 16;
    int const fWave = fBase + wave * (MF / 4);
    bool const vec4 = (p.post.nf & 3) == 0;
-   // Fused VTHRESH update: the old V of the neurons this lane updates is
-   // fetched first so that its latency overlaps the staging and the MFMA
-   // loop.  (The vertex transfer needs those registers itself.)
-   float4 vpre[MPB][MFB];
+   // Epilogue layout (row layout): thread t owns features fBase + 4*(t % 32) ..
+   // +3 of positions (t / 32) + 8 * it, it < MEP.  The old V of those neurons
+   // (fused VTHRESH update) is fetched first so that its latency overlaps the
+   // staging and the MFMA loop; the vertex transfer needs those registers itself.
+   constexpr int MEP = MTX * MTY / 8;
+   int const eq  = tid % 32;
+   int const epr = tid / 32;
+   int const ef  = fBase + 4 * eq;
+   f32x4 vpre[MEP];
 #pragma unroll
-   for (int pb = 0; pb < MPB; pb++) {
-#pragma unroll
-      for (int fb = 0; fb < MFB; fb++) {
-         vpre[pb][fb] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-         if (EPI == 1 && vec4) {
-            int pj = pb * 16 + r;
-            int x  = x0 + pj % MTX;
-            int y  = y0 + pj / MTX;
-            int f  = fWave + fb * 16 + 4 * g;
-            if (x < p.post.nx && y < p.post.ny && f < p.post.nf) {
-               vpre[pb][fb] = *reinterpret_cast<const float4 *>(
-                     p.lca.V + (size_t)b * p.post.numRes() + (y * p.post.nx + x) * p.post.nf + f);
-            }
+   for (int it = 0; it < MEP; it++) {
+      vpre[it] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      if (EPI == 1 && vec4) {
+         int pj = it * 8 + epr;
+         int x  = x0 + pj % MTX;
+         int y  = y0 + pj / MTX;
+         if (x < p.post.nx && y < p.post.ny && ef < p.post.nf) {
+            vpre[it] = *reinterpret_cast<const f32x4 *>(
+                  p.lca.V + (size_t)b * p.post.numRes() + (y * p.post.nx + x) * p.post.nf + ef);
          }
       }
    }
@@ -478,34 +488,43 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
       }
    }
    // Lane holds D[4g+i][r]: features fWave+fb*16+4g+i at position pb*16+r.
+   // Transpose through LDS (the staged source is dead now) so that each
+   // position's MF features are written as one contiguous run.
+   __syncthreads();
+   float *outl = smem; // [MTX*MTY][MOP]
 #pragma unroll
    for (int pb = 0; pb < MPB; pb++) {
-      int pj = pb * 16 + r;
-      int x  = x0 + pj % MTX;
-      int y  = y0 + pj / MTX;
-      int kpos = (y * p.post.nx + x) * p.post.nf;
-      bool const inside = x < p.post.nx && y < p.post.ny; // no continue: keeps acc unrolled
 #pragma unroll
       for (int fb = 0; fb < MFB; fb++) {
-         int f = fWave + fb * 16 + 4 * g;
-         if (!inside) {
-            // position outside the layer: nothing to store
-         }
-         else if (f + 3 < p.post.nf && vec4) {
-            post_epilogue4<EPI>(p, b, kpos + f, acc[fb][pb], vpre[pb][fb]);
-         }
-         else {
+         *reinterpret_cast<f32x4 *>(outl + (pb * 16 + r) * MOP + wave * (MF / 4) + fb * 16 + 4 * g) =
+               acc[fb][pb];
+      }
+   }
+   __syncthreads();
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-               if (f + i < p.post.nf) {
-                  size_t const gi = (size_t)b * p.post.numRes() + kpos + f + i;
-                  float const G   = (p.accumulate ? p.gsyn[gi] : 0.0f) + p.dtf * acc[fb][pb][i];
-                  if (EPI == 0) {
-                     p.gsyn[gi] = G;
-                  }
-                  else {
-                     lca_update_one<EPI == 1 ? 1 : 0>(p.lca, b, kpos + f + i, G);
-                  }
+   for (int it = 0; it < MEP; it++) {
+      int const pj   = it * 8 + epr;
+      int const x    = x0 + pj % MTX;
+      int const y    = y0 + pj / MTX;
+      int const kpos = (y * p.post.nx + x) * p.post.nf;
+      f32x4 const a4 = *reinterpret_cast<const f32x4 *>(outl + pj * MOP + 4 * eq);
+      if (!(x < p.post.nx && y < p.post.ny)) {
+         // position outside the layer: nothing to store
+      }
+      else if (ef + 3 < p.post.nf && vec4) {
+         post_epilogue4<EPI>(p, b, kpos + ef, a4, vpre[it]);
+      }
+      else {
+#pragma unroll
+         for (int i = 0; i < 4; i++) {
+            if (ef + i < p.post.nf) {
+               size_t const gi = (size_t)b * p.post.numRes() + kpos + ef + i;
+               float const G   = (p.accumulate ? p.gsyn[gi] : 0.0f) + p.dtf * a4[i];
+               if (EPI == 0) {
+                  p.gsyn[gi] = G;
+               }
+               else {
+                  lca_update_one<EPI == 1 ? 1 : 0>(p.lca, b, kpos + ef + i, G);
                }
             }
          }
@@ -958,6 +977,7 @@ int pvhip_recv_post(const pvhip_recv_post_args *args, pvhip_stream stream) {
       int const srcRowsMax = (MTY - 1) * sy + p.nyp;
       int const pitch      = (srcColsMax * p.nfp + 7) / 8 * 8;
       size_t lds = ((size_t)srcRowsMax * pitch + 1) / 2 * 4 + (size_t)Kpad * sizeof(int);
+      lds        = std::max(lds, (size_t)MTX * MTY * MOP * sizeof(float)); // epilogue tile
       PVHIP_REQUIRE(lds <= 160 * 1024, "recv_post: BF16 tile needs %zu bytes of LDS", lds);
       // alignment class of a fragment's LDS address: rows are 8-aligned, window
       // starts differ by multiples of sx*nfp, fragments start at multiples of 8
