@@ -119,6 +119,23 @@ def kernel_timing(col, s, n, torch):
     return float(np.mean(d)), float(np.min(d))
 
 
+TRAFFIC_SUMMARY = os.path.join("profiles", "recv_post_mfma_bf16_lca64_traffic.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the fused BF16 gather on this workload, measured
+    by two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the same
+    counters on a 256 MiB copy; the summary file records the commands."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_SUMMARY)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(images, W, budget_s=12.0):
     """The oracle (C restatement of the reference loops, OpenMP over the host
     cores) on a bounded sample of the workload: a few images, a few steps."""
@@ -217,10 +234,12 @@ def main():
     alg_bytes = n_v1 * 12 + src_bytes + w_bytes   # V read+write, A write, source, weights
     achieved = alg_bytes / (kavg * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": profiled_traffic() if head == "bf16" else None,
             "kernel": "recv_post_mfma_bf16 (fused LCA epilogue)" if head == "bf16" else "recv_post",
             "kernel_ms": round(kavg, 5), "kernel_ms_min": round(kmin, 5),
             "algorithmic_bytes": int(alg_bytes),
+            "traffic_source": TRAFFIC_SUMMARY if head == "bf16" else None,
             "mfma_tflops": round(2.0 * gather_macs / (kavg * 1e-3) / 1e12, 1)}
     line = {
         "metric": "LCA steps/s + synaptic ops/s, BF16-MFMA vs FP32 GPU at <=1e-3 rel energy err",

@@ -384,3 +384,74 @@ def lca_cfg(nb, n, nf, nv, nfv, p, vt=0.1, tau=100.0):
     cfg.errScale = 1.0
     cfg.num_threads = 1
     return cfg
+
+
+def test_hebbian_normalize_apply_closed_form():
+    """normalize_dW divides by the activation count (0 where the count is 0), then W += dW;
+    without normalizeDw, W += dW unchanged."""
+    rng = np.random.default_rng(14)
+    n = 64
+    w = rng.standard_normal(n).astype(np.float32)
+    dw = rng.standard_normal(n).astype(np.float32)
+    cnt = rng.integers(0, 4, n).astype(np.int64)
+    w1, dw1 = w.copy(), dw.copy()
+    O.lib().or_hebbian_normalize_apply(w1, dw1, cnt.ctypes.data, n, 1)
+    expect_dw = np.where(cnt > 0, dw / np.maximum(cnt, 1).astype(np.float32), np.float32(0))
+    assert np.array_equal(dw1, expect_dw.astype(np.float32))
+    assert np.array_equal(w1, (w + expect_dw).astype(np.float32))
+    w2, dw2 = w.copy(), dw.copy()
+    O.lib().or_hebbian_normalize_apply(w2, dw2, cnt.ctypes.data, n, 0)
+    assert np.array_equal(dw2, dw) and np.array_equal(w2, w + dw)
+
+
+def test_ident_deliver_and_errscale_closed_form():
+    loc = O.Loc.make(2, 5, 3, 4, (1, 2, 2, 1))
+    rng = np.random.default_rng(15)
+    A = rng.standard_normal(loc.nbatch * loc.num_ext).astype(np.float32)
+    g = np.zeros(loc.nbatch * loc.num_res, np.float32)
+    O.lib().or_ident_deliver(O.C.byref(loc), A, g)
+    # restricted window of the extended buffer: rows up.., columns lt..
+    Ar = A.reshape(2, loc.ny_ext, loc.nx_ext, 4)[:, loc.up:loc.up + 3,
+                                                 loc.lt:loc.lt + 5]
+    assert np.array_equal(g, np.ascontiguousarray(Ar).ravel())
+    V = rng.standard_normal(30).astype(np.float32)
+    V1 = V.copy()
+    O.lib().or_errscale(30, np.float32(0.5), V1)
+    assert np.array_equal(V1, (V * np.float32(0.5)).astype(np.float32))
+
+
+def test_lca_loop_fixed_point():
+    """The converged loop satisfies the LCA fixed point V = G + A with
+    G = gather(x - scatter(A)), the scatter being the gather's adjoint; both maps
+    here come from float64 torch conv2d (and its autograd), not from the oracle."""
+    rng = np.random.default_rng(16)
+    nb, n, nf, nv, nfv, p = 1, 16, 3, 8, 24, 8
+    img = rng.standard_normal((nb, n, n, nf)).astype(np.float32)
+    img -= img.mean()
+    W = rng.standard_normal((nfv, p, p, nf)).astype(np.float32)
+    W /= np.sqrt((W.reshape(nfv, -1) ** 2).sum(1)).reshape(-1, 1, 1, 1)
+    cfg = lca_cfg(nb, n, nf, nv, nfv, p, vt=0.1, tau=4.0)
+    out = O.lca_run(cfg, W, img, None, 400)
+    S, T = cfg.res, cfg.v1
+    ux = O.patch_axis(T, S, p, p, 0)["unshrunken"][T.lt:T.lt + T.nx]
+    uy = O.patch_axis(T, S, p, p, 1)["unshrunken"][T.up:T.up + T.ny]
+    stride = int(ux[1] - ux[0])
+    w = torch.tensor(W, dtype=torch.float64).permute(0, 3, 1, 2)
+
+    def gather(r):  # r: restricted residual [nb, nf, n, n] -> V1 input [nb, nfv, nv, nv]
+        ext = torch.zeros((nb, nf, S.ny_ext, S.nx_ext), dtype=torch.float64)
+        ext[:, :, S.up:S.up + n, S.lt:S.lt + n] = r
+        return torch.nn.functional.conv2d(ext[:, :, uy[0]:, ux[0]:], w, stride=stride)[:, :, :nv, :nv]
+
+    A = out["A"].reshape(nb, T.ny_ext, T.nx_ext, nfv)[:, T.up:T.up + nv, T.lt:T.lt + nv]
+    a = torch.tensor(np.ascontiguousarray(A), dtype=torch.float64).permute(0, 3, 1, 2)
+    r0 = torch.zeros((nb, nf, n, n), dtype=torch.float64, requires_grad=True)
+    (gather(r0) * a).sum().backward()
+    recon = r0.grad                                  # scatter(A) = gather^T A
+    rec_or = torch.tensor(out["recon"].reshape(nb, n, n, nf)).permute(0, 3, 1, 2).double()
+    assert (recon - rec_or).abs().max() <= 1e-5 * recon.abs().max()
+    x = torch.tensor(img, dtype=torch.float64).permute(0, 3, 1, 2)
+    G = gather(x - recon)
+    V = torch.tensor(out["V"].reshape(nb, nv, nv, nfv), dtype=torch.float64).permute(0, 3, 1, 2)
+    assert (V - (G + a)).abs().max() <= 2e-3 * V.abs().max()
+    assert 0 < out["num_active"][-1, 0] < nv * nv * nfv
