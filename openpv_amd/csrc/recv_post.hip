@@ -330,6 +330,83 @@ __device__ __forceinline__ void post_epilogue4(
    __builtin_nontemporal_store(an, reinterpret_cast<f32x4 *>(q.A + ai));
 }
 
+// Fused VTHRESH update of a full tile in the row layout (the common case):
+// thread (epr, eq) updates features ef .. ef+3 of positions epr + 8*it,
+// it < MEP, whose partial sums sit at out[it * 8 * MOP].  Same arithmetic,
+// bit for bit, as post_epilogue4<1>; index math without divisions and the
+// threshold as selects (the ramp of a firm threshold only when it exists).
+constexpr int MEP = MTX * MTY / 8;
+static_assert(MTX == 16 && MEP == 2 * MTY, "row-layout epilogue assumes 16-wide position tiles");
+
+__device__ __forceinline__ float vthresh_select(
+      float v, bool thr, bool ramp, bool clamp, float VT, float AMin, float AShift, float AMax,
+      float VW) {
+   float a = v;
+   if (thr) {
+      a = v < VT ? AMin : v - AShift;
+      if (ramp && !(v < VT) && v < VT + VW) {
+         float span = VT + VW;
+         span       = span - AShift;
+         span       = span - AMin;
+         float num  = span * (v - VT);
+         a          = AMin + num / VW;
+      }
+   }
+   if (clamp) {
+      a = a > AMax ? AMax : a;
+   }
+   return a;
+}
+
+__device__ __forceinline__ void lca_epilogue_vthresh(
+      const PostParams &p, int b, int x0, int y0, int epr, int ef, const float *out,
+      const f32x4 *vpre) {
+   const LCAParams &q = p.lca;
+   float const e      = q.expTau[b];
+   float const ome    = 1.0f - e;
+   float const si     = q.selfInteract;
+   Transfer const &t  = q.transfer;
+   bool const thr     = t.VThresh > -FLT_MAX;
+   bool const ramp    = t.VThresh + t.VWidth > t.VThresh;
+   bool const clamp   = t.AMax < FLT_MAX;
+   int const nf       = p.post.nf;
+   int const rowV     = p.post.nx * nf;
+   int const rowA     = q.loc.nxExt() * nf;
+   float *Vp = q.V + (size_t)b * p.post.numRes() + (size_t)(y0 * p.post.nx + x0 + epr) * nf + ef;
+   float *Ap = q.A + (size_t)b * q.loc.numExt()
+               + (size_t)((y0 + q.loc.up) * q.loc.nxExt() + x0 + epr + q.loc.lt) * nf + ef;
+#pragma unroll
+   for (int it = 0; it < MEP; it++) {
+      int const yl    = it >> 1;
+      int const xo    = (it & 1) * 8 * nf;
+      f32x4 const acc = *reinterpret_cast<const f32x4 *>(out + it * 8 * MOP);
+      f32x4 const vo  = vpre[it];
+      f32x4 ao;
+      if (q.aFromV) {
+#pragma unroll
+         for (int i = 0; i < 4; i++) {
+            ao[i] = vthresh_select(vo[i], thr, ramp, clamp, t.VThresh, t.AMin, t.AShift, t.AMax, t.VWidth);
+         }
+      }
+      else {
+         ao = *reinterpret_cast<const f32x4 *>(Ap + yl * rowA + xo);
+      }
+      f32x4 vn, an;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+         float G  = 0.0f + p.dtf * acc[i];
+         float t1 = e * vo[i];
+         float sa = si * ao[i];
+         float t3 = G + sa;
+         float t4 = ome * t3;
+         vn[i]    = t1 + t4;
+         an[i]    = vthresh_select(vn[i], thr, ramp, clamp, t.VThresh, t.AMin, t.AShift, t.AMax, t.VWidth);
+      }
+      __builtin_nontemporal_store(vn, reinterpret_cast<f32x4 *>(Vp + yl * rowV + xo));
+      __builtin_nontemporal_store(an, reinterpret_cast<f32x4 *>(Ap + yl * rowA + xo));
+   }
+}
+
 template <int ALIGN, int EPI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) recv_post_mfma_bf16_kernel(
       PostParams p, const __bf16 *Wb, int srcRowsMax, int srcColsMax, int Kpad) {
@@ -366,7 +443,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
    // +3 of positions (t / 32) + 8 * it, it < MEP.  The old V of those neurons
    // (fused VTHRESH update) is fetched first so that its latency overlaps the
    // staging and the MFMA loop; the vertex transfer needs those registers itself.
-   constexpr int MEP = MTX * MTY / 8;
    int const eq  = tid % 32;
    int const epr = tid / 32;
    int const ef  = fBase + 4 * eq;
@@ -388,22 +464,34 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
    float const *preB = p.pre_data + (size_t)b * p.pre.numExt();
    int const rowElems = srcCols * p.nfp;
    int const total    = srcRows * rowElems;
+   // Element i = r * rowElems + c of the window, stepped by 256 with a carry
+   // instead of a division per element.
+   int const dR = 256 / rowElems;
+   int const dC = 256 - dR * rowElems;
+   int sr       = tid / rowElems;
+   int sc       = tid - sr * rowElems;
+   float const *winB = preB + (size_t)srcY0 * preRow + (size_t)srcX0 * p.nfp;
    for (int base = tid; base < total; base += 256 * 8) {
-      // unconditional loads (clamped index) so all eight are in flight together
+      // unconditional loads (clamped position) so all eight are in flight together
       float v[8];
+      int rr[8], cc[8];
 #pragma unroll
       for (int u = 0; u < 8; u++) {
-         int i = min(base + u * 256, total - 1);
-         int r = i / rowElems;
-         int c = i - r * rowElems;
-         v[u]  = preB[(size_t)(srcY0 + r) * preRow + (size_t)srcX0 * p.nfp + c];
+         rr[u] = sr;
+         cc[u] = sc;
+         bool const ok = sr < srcRows;
+         v[u] = winB[(size_t)(ok ? sr : srcRows - 1) * preRow + (ok ? sc : rowElems - 1)];
+         sc += dC;
+         sr += dR;
+         if (sc >= rowElems) {
+            sc -= rowElems;
+            sr++;
+         }
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
-         int i = base + u * 256;
-         if (i < total) {
-            int r = i / rowElems;
-            src[r * pitch + (i - r * rowElems)] = (__bf16)v[u];
+         if (rr[u] < srcRows) {
+            src[rr[u] * pitch + cc[u]] = (__bf16)v[u];
          }
       }
    }
@@ -501,6 +589,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
       }
    }
    __syncthreads();
+   bool const fullTile = vec4 && fBase + MF <= p.post.nf && x0 + MTX <= p.post.nx
+                         && y0 + MTY <= p.post.ny;
+   if (EPI == 1 && fullTile) {
+      lca_epilogue_vthresh(p, b, x0, y0, epr, ef, outl + epr * MOP + 4 * eq, vpre);
+      return;
+   }
 #pragma unroll
    for (int it = 0; it < MEP; it++) {
       int const pj   = it * 8 + epr;
