@@ -246,9 +246,31 @@ typedef struct {
     * coeff 1): the gather result is the accumulated G of the update.  post_gsyn
     * may then be NULL (GSyn is not materialised). */
    const pvhip_lca_update_args *fused_update;
+   /* Optional, with fused_update: also publish the active list of the updated
+    * activity, as pvhip_active_compact would after the update (HyPerLayer
+    * publish of a sparseLayer: Publisher::updateActiveIndices,
+    * src/columns/Publisher.cpp:85-134).  NULL: no active list. */
+   const struct pvhip_active_out *active_out;
 } pvhip_recv_post_args;
 
+/* Active-list outputs of a fused delivery (same layout and contents as
+ * pvhip_active_compact over fused_update->loc and fused_update->A). */
+typedef struct pvhip_active_out {
+   pvhip_sparse_entry *entries; /* [nbatch][numExt] */
+   int64_t *num_active;         /* [nbatch] */
+   uint32_t *pos_start;         /* [nbatch][nxExt*nyExt + 1], or NULL */
+   void *scratch;               /* device memory, pvhip_recv_post_active_scratch_bytes() bytes */
+   size_t scratch_bytes;
+   /* 1: the caller guarantees the activity halo holds only zeros (no border
+    * exchange or mirror boundary writes it; one rank with valueBC 0).  The
+    * list is then emitted from the delivery's epilogue; otherwise it is built
+    * by a separate ordered compaction of the whole extended buffer. */
+   int halo_is_zero;
+} pvhip_active_out;
+
 int pvhip_recv_post(const pvhip_recv_post_args *args, pvhip_stream stream);
+/* Scratch bytes pvhip_active_out needs for these arguments (fused_update set). */
+size_t pvhip_recv_post_active_scratch_bytes(const pvhip_recv_post_args *args);
 
 /* Low-precision weight shadow for the BF16 form; call after every weight change. */
 int pvhip_recv_post_bf16_weights(

@@ -79,6 +79,11 @@ class LCAUpdateArgs(C.Structure):
                 ("transfer", Transfer), ("activity_from_v", i32)]
 
 
+class ActiveOut(C.Structure):
+    _fields_ = [("entries", vp), ("num_active", vp), ("pos_start", vp), ("scratch", vp),
+                ("scratch_bytes", C.c_size_t), ("halo_is_zero", i32)]
+
+
 class RecvPostArgs(C.Structure):
     _fields_ = [("pre", Loc), ("post", Loc), ("nxp", i32), ("nyp", i32), ("nfp", i32),
                 ("num_kernels_x", i32), ("num_kernels_y", i32),
@@ -87,7 +92,8 @@ class RecvPostArgs(C.Structure):
                 ("accumulate", i32), ("arith", i32), ("weights_lp", vp),
                 ("inverse_x", vp), ("inverse_y", vp), ("window_x0", i32), ("window_nx", i32),
                 ("window_y0", i32), ("window_ny", i32),
-                ("fused_update", C.POINTER(LCAUpdateArgs))]
+                ("fused_update", C.POINTER(LCAUpdateArgs)),
+                ("active_out", C.POINTER(ActiveOut))]
 
 
 class RecvPreArgs(C.Structure):
@@ -141,6 +147,7 @@ PVHIP_SYMBOLS = {
     "pvhip_graph_destroy": (i32, [vp]),
     "pvhip_lca_exp_tau": (None, [C.POINTER(C.c_double), i32, f32, C.POINTER(f32)]),
     "pvhip_recv_post": (i32, [C.POINTER(RecvPostArgs), vp]),
+    "pvhip_recv_post_active_scratch_bytes": (C.c_size_t, [C.POINTER(RecvPostArgs)]),
     "pvhip_recv_post_bf16_weights": (i32, [vp, vp, i32, i32, vp]),
     "pvhip_recv_post_bf16_weights_bytes": (C.c_size_t, [i32, i32]),
     "pvhip_recv_pre": (i32, [C.POINTER(RecvPreArgs), vp]),

@@ -395,7 +395,92 @@ __global__ void __launch_bounds__(256) hebbian_apply_kernel(
 
 inline unsigned blocks_for(size_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
 
+// ------------------------------------------- active list from epilogue segments
+//
+// A delivery epilogue that applies the update can also emit the active
+// entries of each (restricted position, feature block of segW features)
+// segment into a staging slot of segW entries, in ascending feature order,
+// and the segment's count.  Segments in (position, feature block) order are
+// in ascending extended index order when the halo holds no actives, so an
+// exclusive scan of the counts (compact_scan_kernel) and a copy give the
+// ordered list of DataStore::updateActiveIndices.
+
+// Eight lanes per segment (a segment holds a few entries on average):
+// segment s = 32 * blockIdx.x + threadIdx.x / 8.
+constexpr int kSegLanes = 8;
+__global__ void __launch_bounds__(256) segment_copy_kernel(
+      const pvhip_sparse_entry *stg, const uint32_t *segOff, int segsPerBatch, int segW,
+      const int64_t *numActive, pvhip_sparse_entry *entries, int numExt) {
+   int const b = blockIdx.y;
+   int const s = blockIdx.x * (256 / kSegLanes) + threadIdx.x / kSegLanes;
+   if (s >= segsPerBatch) {
+      return;
+   }
+   uint32_t const *Ob = segOff + (size_t)b * segsPerBatch;
+   uint32_t const off = Ob[s];
+   uint32_t const n   = (s + 1 < segsPerBatch ? Ob[s + 1] : (uint32_t)numActive[b]) - off;
+   uint2 const *src   = reinterpret_cast<const uint2 *>(stg + ((size_t)b * segsPerBatch + s) * segW);
+   uint2 *dst         = reinterpret_cast<uint2 *>(entries + (size_t)b * numExt + off);
+   for (uint32_t j = threadIdx.x % kSegLanes; j < n; j += kSegLanes) {
+      dst[j] = src[j];
+   }
+}
+
+// pos_start[b][p] for every extended position p: the actives before the
+// first restricted position at or after p in raster order.
+__global__ void __launch_bounds__(256) segment_pos_kernel(
+      Geo g, const uint32_t *segOff, int segsPerPos, const int64_t *numActive, uint32_t *posStart) {
+   int const b      = blockIdx.y;
+   int const numPos = g.nxExt() * g.nyExt();
+   int const p      = blockIdx.x * blockDim.x + threadIdx.x;
+   if (p > numPos) {
+      return;
+   }
+   int const numPosR = g.nx * g.ny;
+   int r             = numPosR;
+   if (p < numPos) {
+      int const xe = p % g.nxExt();
+      int const ye = p / g.nxExt();
+      if (ye < g.up) {
+         r = 0;
+      }
+      else if (ye < g.up + g.ny) {
+         int const y = ye - g.up;
+         r           = xe < g.lt ? y * g.nx : (xe < g.lt + g.nx ? y * g.nx + xe - g.lt : (y + 1) * g.nx);
+      }
+   }
+   uint32_t const v = r < numPosR ? segOff[((size_t)b * numPosR + r) * segsPerPos] : (uint32_t)numActive[b];
+   posStart[(size_t)b * (numPos + 1) + p] = v;
+}
+
 } // namespace
+
+size_t compact_scratch_bytes(const Geo &g) {
+   return (size_t)g.nbatch * (size_t)((g.numExt() + kChunk - 1) / kChunk) * sizeof(uint32_t);
+}
+
+int active_from_segments(
+      const Geo &g, uint32_t *segCnt, const pvhip_sparse_entry *stg, int segsPerPos, int segW,
+      pvhip_sparse_entry *entries, int64_t *numActive, uint32_t *posStart, hipStream_t s) {
+   PVHIP_REQUIRE(segW >= 1, "active list: segment width %d", segW);
+   int const segsPerBatch = g.nx * g.ny * segsPerPos;
+   hipLaunchKernelGGL(compact_scan_kernel, dim3(g.nbatch), dim3(1024), 0, s, segCnt, segsPerBatch,
+         numActive);
+   PVHIP_LAUNCHED("compact_scan");
+   int const segsPerBlock = 256 / kSegLanes;
+   hipLaunchKernelGGL(segment_copy_kernel, dim3((segsPerBatch + segsPerBlock - 1) / segsPerBlock, g.nbatch),
+         dim3(256), 0, s,
+         stg, segCnt, segsPerBatch, segW, numActive, entries, g.numExt());
+   PVHIP_LAUNCHED("segment_copy");
+   if (posStart) {
+      int const numPos = g.nxExt() * g.nyExt();
+      hipLaunchKernelGGL(segment_pos_kernel, dim3((numPos + 1 + 255) / 256, g.nbatch), dim3(256), 0,
+            s, g, segCnt, segsPerPos, numActive, posStart);
+      PVHIP_LAUNCHED("segment_pos");
+   }
+   return PVHIP_OK;
+}
+
 } // namespace pvhip
 
 using namespace pvhip;
@@ -574,8 +659,7 @@ size_t pvhip_active_compact_scratch_bytes(const pvhip_loc *loc) {
    if (!loc || !loc_valid(*loc)) {
       return 0;
    }
-   Geo g = geo(*loc);
-   return (size_t)g.nbatch * (size_t)((g.numExt() + kChunk - 1) / kChunk) * sizeof(uint32_t);
+   return compact_scratch_bytes(geo(*loc));
 }
 
 } // extern "C"

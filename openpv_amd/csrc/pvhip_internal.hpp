@@ -224,4 +224,16 @@ __device__ __forceinline__ void lca_update_one(const LCAParams &p, int b, int k,
 
 constexpr int kWave = 64;
 
+// Scratch of the ordered compaction (chunk counts) for a layer.
+size_t compact_scratch_bytes(const Geo &g);
+
+// Ordered active list of layer g from per-segment staging: segCnt holds the
+// entry count of each (restricted position, feature block) segment in raster
+// order (segsPerPos blocks of segW features per position) and is scanned in
+// place; stg holds each segment's entries in slots of segW.  Requires an
+// activity halo of zeros.
+int active_from_segments(
+      const Geo &g, uint32_t *segCnt, const pvhip_sparse_entry *stg, int segsPerPos, int segW,
+      pvhip_sparse_entry *entries, int64_t *numActive, uint32_t *posStart, hipStream_t s);
+
 } // namespace pvhip

@@ -31,6 +31,12 @@ struct PostParams {
    int accumulate;
    int fused;
    LCAParams lca;
+   // Active-list emission of the fused epilogue (full tiles only): entries of
+   // each (restricted position, MF-feature block) segment into a slot of MF
+   // entries of stg, and the segment's count into segCnt.
+   int emit;
+   pvhip_sparse_entry *stg;
+   uint32_t *segCnt;
 };
 
 __device__ __forceinline__ void post_epilogue(const PostParams &p, int b, int k, float G) {
@@ -232,7 +238,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // the widest load the tile's alignment class ALIGN (elements) allows.
 constexpr int MTX = 16, MTY = 8;        // position tile
 constexpr int MPB = MTX * MTY / 16;     // 16-position blocks per wave
-constexpr int MF  = 128;                // features per workgroup
+constexpr int MF  = 64;                 // features per workgroup
 constexpr int MFB = MF / 4 / 16;        // 16-feature blocks per wave
 
 template <int ALIGN>
@@ -330,13 +336,18 @@ __device__ __forceinline__ void post_epilogue4(
    __builtin_nontemporal_store(an, reinterpret_cast<f32x4 *>(q.A + ai));
 }
 
+// Epilogue row layout: MEQ threads cover the MF features of one position
+// (4 each); a pass covers MEPP positions; MEP passes cover the tile.
+constexpr int MEQ  = MF / 4;
+constexpr int MEPP = 256 / MEQ;
+constexpr int MEP  = MTX * MTY / MEPP;
+static_assert(MTX == 16 && 16 % MEPP == 0, "row-layout epilogue assumes 16-wide position tiles");
+
 // Fused VTHRESH update of a full tile in the row layout (the common case):
-// thread (epr, eq) updates features ef .. ef+3 of positions epr + 8*it,
-// it < MEP, whose partial sums sit at out[it * 8 * MOP].  Same arithmetic,
+// thread (epr, eq) updates features ef .. ef+3 of positions epr + MEPP*it,
+// it < MEP, whose partial sums sit at out[it * MEPP * MOP].  Same arithmetic,
 // bit for bit, as post_epilogue4<1>; index math without divisions and the
 // threshold as selects (the ramp of a firm threshold only when it exists).
-constexpr int MEP = MTX * MTY / 8;
-static_assert(MTX == 16 && MEP == 2 * MTY, "row-layout epilogue assumes 16-wide position tiles");
 
 __device__ __forceinline__ float vthresh_select(
       float v, bool thr, bool ramp, bool clamp, float VT, float AMin, float AShift, float AMax,
@@ -358,8 +369,9 @@ __device__ __forceinline__ float vthresh_select(
    return a;
 }
 
+template <bool EMIT>
 __device__ __forceinline__ void lca_epilogue_vthresh(
-      const PostParams &p, int b, int x0, int y0, int epr, int ef, const float *out,
+      const PostParams &p, int b, int x0, int y0, int epr, int eq, int ef, const float *out,
       const f32x4 *vpre) {
    const LCAParams &q = p.lca;
    float const e      = q.expTau[b];
@@ -377,9 +389,9 @@ __device__ __forceinline__ void lca_epilogue_vthresh(
                + (size_t)((y0 + q.loc.up) * q.loc.nxExt() + x0 + epr + q.loc.lt) * nf + ef;
 #pragma unroll
    for (int it = 0; it < MEP; it++) {
-      int const yl    = it >> 1;
-      int const xo    = (it & 1) * 8 * nf;
-      f32x4 const acc = *reinterpret_cast<const f32x4 *>(out + it * 8 * MOP);
+      int const yl    = it * MEPP / MTX;
+      int const xo    = it * MEPP % MTX * nf;
+      f32x4 const acc = *reinterpret_cast<const f32x4 *>(out + it * MEPP * MOP);
       f32x4 const vo  = vpre[it];
       f32x4 ao;
       if (q.aFromV) {
@@ -404,11 +416,41 @@ __device__ __forceinline__ void lca_epilogue_vthresh(
       }
       __builtin_nontemporal_store(vn, reinterpret_cast<f32x4 *>(Vp + yl * rowV + xo));
       __builtin_nontemporal_store(an, reinterpret_cast<f32x4 *>(Ap + yl * rowA + xo));
+      if (EMIT) {
+         // Rank of this thread's actives within its position's segment: the
+         // actives of lower lanes of the same MEQ-lane group, from ballots.
+         uint64_t const m0 = __ballot(an[0] != 0.0f);
+         uint64_t const m1 = __ballot(an[1] != 0.0f);
+         uint64_t const m2 = __ballot(an[2] != 0.0f);
+         uint64_t const m3 = __ballot(an[3] != 0.0f);
+         int const lane      = threadIdx.x % kWave;
+         uint64_t const grp  = ((MEQ == 64 ? ~0ull : ((1ull << MEQ) - 1)) << (lane - eq));
+         uint64_t const lowr = grp & ((1ull << lane) - 1);
+         int rank = __popcll(m0 & lowr) + __popcll(m1 & lowr) + __popcll(m2 & lowr) + __popcll(m3 & lowr);
+         int const x      = x0 + epr + it * MEPP % MTX;
+         int const y      = y0 + yl;
+         size_t const seg = ((size_t)b * p.post.ny * p.post.nx + (size_t)y * p.post.nx + x)
+                                  * (nf / MF) + ef / MF;
+         uint2 *e         = reinterpret_cast<uint2 *>(p.stg + seg * MF);
+         uint32_t kex     = (uint32_t)(((y + q.loc.up) * q.loc.nxExt() + x + q.loc.lt) * nf + ef);
+#pragma unroll
+         for (int i = 0; i < 4; i++) {
+            if (an[i] != 0.0f) {
+               e[rank++] = make_uint2(kex + i, __float_as_uint(an[i]));
+            }
+         }
+         if (eq == MEQ - 1) {
+            p.segCnt[seg] = (uint32_t)(__popcll(m0 & grp) + __popcll(m1 & grp) + __popcll(m2 & grp)
+                                       + __popcll(m3 & grp));
+         }
+      }
    }
 }
 
+// EPI: 0 GSyn only, 1 fused VTHRESH update, 2 fused update with either
+// transfer, 3 as 1 plus the active-list emission of full tiles.
 template <int ALIGN, int EPI>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) recv_post_mfma_bf16_kernel(
+__global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
       PostParams p, const __bf16 *Wb, int srcRowsMax, int srcColsMax, int Kpad) {
    extern __shared__ float smem[];
    int const K       = p.nxp * p.nfp * p.nyp;
@@ -439,19 +481,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
    int const g    = lane / 16;
    int const fWave = fBase + wave * (MF / 4);
    bool const vec4 = (p.post.nf & 3) == 0;
-   // Epilogue layout (row layout): thread t owns features fBase + 4*(t % 32) ..
-   // +3 of positions (t / 32) + 8 * it, it < MEP.  The old V of those neurons
+   // Epilogue layout (row layout): thread t owns features fBase + 4*(t % MEQ) ..
+   // +3 of positions (t / MEQ) + MEPP * it, it < MEP.  The old V of those neurons
    // (fused VTHRESH update) is fetched first so that its latency overlaps the
    // staging and the MFMA loop; the vertex transfer needs those registers itself.
-   int const eq  = tid % 32;
-   int const epr = tid / 32;
+   int const eq  = tid % MEQ;
+   int const epr = tid / MEQ;
    int const ef  = fBase + 4 * eq;
    f32x4 vpre[MEP];
 #pragma unroll
    for (int it = 0; it < MEP; it++) {
       vpre[it] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-      if (EPI == 1 && vec4) {
-         int pj = it * 8 + epr;
+      if ((EPI == 1 || EPI == 3) && vec4) {
+         int pj = it * MEPP + epr;
          int x  = x0 + pj % MTX;
          int y  = y0 + pj / MTX;
          if (x < p.post.nx && y < p.post.ny && ef < p.post.nf) {
@@ -528,7 +570,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
          acc[fb][pb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
       }
    }
-   constexpr int KPRE = 8; // k-steps whose A fragments are fetched together
+   constexpr int KPRE = 4; // k-steps whose A fragments are fetched together
    for (int kb = 0; kb < Kpad; kb += 32 * KPRE) {
       uint4 apre[KPRE][MFB]; // bf16x8 fragments kept as raw 128-bit registers
 #pragma unroll
@@ -591,13 +633,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
    __syncthreads();
    bool const fullTile = vec4 && fBase + MF <= p.post.nf && x0 + MTX <= p.post.nx
                          && y0 + MTY <= p.post.ny;
-   if (EPI == 1 && fullTile) {
-      lca_epilogue_vthresh(p, b, x0, y0, epr, ef, outl + epr * MOP + 4 * eq, vpre);
+   if ((EPI == 1 || EPI == 3) && fullTile) {
+      lca_epilogue_vthresh<EPI == 3>(p, b, x0, y0, epr, eq, ef, outl + epr * MOP + 4 * eq, vpre);
       return;
    }
 #pragma unroll
    for (int it = 0; it < MEP; it++) {
-      int const pj   = it * 8 + epr;
+      int const pj   = it * MEPP + epr;
       int const x    = x0 + pj % MTX;
       int const y    = y0 + pj / MTX;
       int const kpos = (y * p.post.nx + x) * p.post.nf;
@@ -606,7 +648,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
          // position outside the layer: nothing to store
       }
       else if (ef + 3 < p.post.nf && vec4) {
-         post_epilogue4<EPI>(p, b, kpos + ef, a4, vpre[it]);
+         post_epilogue4<EPI == 3 ? 1 : EPI>(p, b, kpos + ef, a4, vpre[it]);
       }
       else {
 #pragma unroll
@@ -618,7 +660,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))
                   p.gsyn[gi] = G;
                }
                else {
-                  lca_update_one<EPI == 1 ? 1 : 0>(p.lca, b, kpos + ef + i, G);
+                  lca_update_one<EPI == 1 || EPI == 3 ? 1 : 0>(p.lca, b, kpos + ef + i, G);
                }
             }
          }
@@ -933,6 +975,9 @@ int build_params(const pvhip_recv_post_args &a, PostParams *p) {
    p->W        = a.weights;
    p->gsyn     = a.post_gsyn;
    p->accumulate = a.accumulate;
+   p->emit       = 0;
+   p->stg        = nullptr;
+   p->segCnt     = nullptr;
    p->fused      = a.fused_update != nullptr;
    if (p->fused) {
       PVHIP_REQUIRE(!a.accumulate, "recv_post: fused update requires accumulate == 0");
@@ -958,12 +1003,31 @@ int build_params(const pvhip_recv_post_args &a, PostParams *p) {
    return PVHIP_OK;
 }
 
+// Staging of the epilogue-emitted active list: MF-entry slots for every
+// restricted neuron, then one count per segment (256-byte aligned).
+size_t segment_stage_bytes(const pvhip_loc &l) {
+   return ((size_t)l.nbatch * l.nx * l.ny * l.nf * sizeof(pvhip_sparse_entry) + 255) / 256 * 256;
+}
+size_t segment_scratch_bytes(const pvhip_loc &l) {
+   return segment_stage_bytes(l) + (size_t)l.nbatch * l.nx * l.ny * ((l.nf + MF - 1) / MF) * sizeof(uint32_t);
+}
+
+int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_t s);
+
 } // namespace
 } // namespace pvhip
 
 using namespace pvhip;
 
 extern "C" {
+
+size_t pvhip_recv_post_active_scratch_bytes(const pvhip_recv_post_args *args) {
+   if (!args || !args->fused_update || !loc_valid(args->fused_update->loc)) {
+      return 0;
+   }
+   const pvhip_loc &l = args->fused_update->loc;
+   return std::max(segment_scratch_bytes(l), compact_scratch_bytes(geo(l)));
+}
 
 int pvhip_recv_post_bf16_weights(
       const float *weights, void *weights_bf16, int num_kernels, int patch_size, pvhip_stream stream) {
@@ -987,7 +1051,35 @@ int pvhip_recv_post(const pvhip_recv_post_args *args, pvhip_stream stream) {
    if (st) {
       return st;
    }
-   hipStream_t s     = (hipStream_t)stream;
+   hipStream_t s                = (hipStream_t)stream;
+   const pvhip_active_out *aout = args->active_out;
+   if (aout) {
+      PVHIP_REQUIRE(p.fused, "recv_post: active_out needs fused_update");
+      PVHIP_REQUIRE(aout->entries && aout->num_active && aout->scratch,
+            "recv_post: active_out needs entries, num_active and scratch");
+      PVHIP_REQUIRE(aout->scratch_bytes >= pvhip_recv_post_active_scratch_bytes(args),
+            "recv_post: active_out scratch of %zu bytes, %zu needed", aout->scratch_bytes,
+            pvhip_recv_post_active_scratch_bytes(args));
+   }
+   st = recv_post_launch(args, p, s);
+   if (st || !aout) {
+      return st;
+   }
+   const pvhip_lca_update_args &u = *args->fused_update;
+   if (p.emit) {
+      return active_from_segments(geo(u.loc), p.segCnt, p.stg, u.loc.nf / MF, MF, aout->entries,
+            aout->num_active, aout->pos_start, s);
+   }
+   return pvhip_active_compact(&u.loc, u.A, aout->entries, aout->num_active, aout->pos_start,
+         static_cast<uint32_t *>(aout->scratch), stream);
+}
+
+} // extern "C"
+
+namespace pvhip {
+namespace {
+
+int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_t s) {
    int const arith   = args->arith;
    bool const single = p.nKx == 1 && p.nKy == 1; // one data patch per feature
    PVHIP_REQUIRE(p.post.nbatch <= 65535, "recv_post: nbatch too large");
@@ -1084,7 +1176,17 @@ int pvhip_recv_post(const pvhip_recv_post_args *args, pvhip_stream stream) {
       int const tilesY = (p.post.ny + MTY - 1) / MTY;
       dim3 grid(tilesX * tilesY, (p.post.nf + MF - 1) / MF, p.post.nbatch);
       const __bf16 *Wb = (const __bf16 *)args->weights_lp;
-      int const epi = !p.fused ? 0 : (p.lca.transfer.mode == PVHIP_TRANSFER_VTHRESH ? 1 : 2);
+      int epi = !p.fused ? 0 : (p.lca.transfer.mode == PVHIP_TRANSFER_VTHRESH ? 1 : 2);
+      const pvhip_active_out *aout = args->active_out;
+      if (aout && aout->halo_is_zero && epi == 1 && p.post.nf % MF == 0 && p.post.nx % MTX == 0
+          && p.post.ny % MTY == 0) {
+         // every tile is full: the epilogue emits the active list
+         p.emit   = 1;
+         p.stg    = static_cast<pvhip_sparse_entry *>(aout->scratch);
+         p.segCnt = reinterpret_cast<uint32_t *>(
+               static_cast<char *>(aout->scratch) + segment_stage_bytes(args->fused_update->loc));
+         epi      = 3;
+      }
 #define PVHIP_MFMA(AL, EP)                                                                       \
    hipLaunchKernelGGL((recv_post_mfma_bf16_kernel<AL, EP>), grid, dim3(256), lds, s, p, Wb,     \
          srcRowsMax, srcColsMax, Kpad)
@@ -1092,6 +1194,7 @@ int pvhip_recv_post(const pvhip_recv_post_args *args, pvhip_stream stream) {
    do {                                                                                          \
       if (epi == 0) PVHIP_MFMA(AL, 0);                                                           \
       else if (epi == 1) PVHIP_MFMA(AL, 1);                                                      \
+      else if (epi == 3) PVHIP_MFMA(AL, 3);                                                      \
       else PVHIP_MFMA(AL, 2);                                                                    \
    } while (0)
       if (align == 8) PVHIP_MFMA_EPI(8);
@@ -1145,4 +1248,5 @@ int pvhip_recv_post(const pvhip_recv_post_args *args, pvhip_stream stream) {
    return set_error(PVHIP_ERR_UNSUPPORTED, "recv_post: arith %d not supported for this geometry", arith);
 }
 
-} // extern "C"
+} // namespace
+} // namespace pvhip

@@ -89,7 +89,12 @@ class LCAColumn:
                                   self_interact, activity_from_v=True, device=d)
         self.energy = torch.zeros(2, nb, dtype=torch.float64, device=d)
         self._graph = None
-        self._fused_args = None
+        self._fused_args = self._gather_args = None
+        if fused:
+            # argument blocks (and the delivery's scratch) prepared outside any graph capture
+            self._fused_args = self.lca.args(None, self.v1_V, self.v1_A, num_channels=1)
+            self._gather_args = self.gather.args(self.res_A, None, False, fused=self._fused_args,
+                                                 active=self.active, halo_is_zero=True)
         # InitializeState: V1 V from V0, A = T(V); Residual and Recon start at 0.
         if V0 is not None:
             self.v1_V.copy_(torch.as_tensor(V0, dtype=torch.float32).reshape(-1).to(d))
@@ -114,15 +119,16 @@ class LCAColumn:
         if gather_events is not None:
             gather_events[0].record(s)
         if self.fused:
-            if self._fused_args is None:
-                self._fused_args = self.lca.args(None, self.v1_V, self.v1_A, num_channels=1)
-            self.gather.deliver(self.res_A, None, accumulate=False, fused=self._fused_args, stream=s)
+            # gather + LCA update + V1's active list (its halo is never written) in one call
+            self.gather.launch(self._gather_args, stream=s)
+            if gather_events is not None:
+                gather_events[1].record(s)
         else:
             self.gather.deliver(self.res_A, self.v1_gsyn, accumulate=False, stream=s)
             self.lca.update(self.v1_gsyn, self.v1_V, self.v1_A, stream=s)
-        if gather_events is not None:
-            gather_events[1].record(s)
-        self.active.compact(self.v1_A, stream=s)
+            if gather_events is not None:
+                gather_events[1].record(s)
+            self.active.compact(self.v1_A, stream=s)
         # phase 3: Recon = scatter of the sparse V1 activity
         self.scatter.deliver(self.rec_gsyn, active=self.active, pre_activity=self.v1_A,
                              accumulate=False, stream=s)

@@ -131,7 +131,8 @@ class PostsynapticPerspectiveGPUDelivery:
             _ptr(self.weights), _ptr(self.weights_lp), self.num_kernels, self.patch_size,
             _stream(stream)), "pvhip_recv_post_bf16_weights")
 
-    def args(self, pre_activity, gsyn, accumulate=False, fused=None):
+    def args(self, pre_activity, gsyn, accumulate=False, fused=None, active=None,
+             halo_is_zero=False):
         a = abi.RecvPostArgs()
         a.pre, a.post = self.source, self.target
         a.nxp, a.nyp, a.nfp = self.nxp, self.nyp, self.nfp
@@ -149,11 +150,33 @@ class PostsynapticPerspectiveGPUDelivery:
             a.window_x0, a.window_nx, a.window_y0, a.window_ny = self.windows
         if fused is not None:
             a.fused_update = C.pointer(fused)
+        if active is not None:
+            if fused is None:
+                raise abi.PVHIPError("an active list needs the fused update")
+            nbytes = abi.hip().pvhip_recv_post_active_scratch_bytes(C.byref(a))
+            scratch = active.delivery_scratch(nbytes)
+            o = abi.ActiveOut()
+            o.entries, o.num_active = _ptr(active.entries), _ptr(active.num_active)
+            o.pos_start = _ptr(active.pos_start)
+            o.scratch, o.scratch_bytes = _ptr(scratch), scratch.numel() * 4
+            o.halo_is_zero = int(halo_is_zero)
+            a.active_out = C.pointer(o)
+            a._keep = o  # the struct must outlive the call
         return a
 
-    def deliver(self, pre_activity, gsyn, accumulate=False, fused=None, stream=None):
-        a = self.args(pre_activity, gsyn, accumulate, fused)
-        abi.check(abi.hip().pvhip_recv_post(C.byref(a), _stream(stream)), "pvhip_recv_post")
+    def deliver(self, pre_activity, gsyn, accumulate=False, fused=None, active=None,
+                halo_is_zero=False, stream=None):
+        """GSyn (+)= gather; with `fused` the LCA update is applied instead of
+        writing GSyn, and with `active` (an ActiveList of the updated layer) the
+        active list is published in the same call (halo_is_zero: the layer's
+        activity halo holds only zeros, which lets the epilogue emit it)."""
+        self.launch(self.args(pre_activity, gsyn, accumulate, fused, active, halo_is_zero), stream)
+
+    @staticmethod
+    def launch(args, stream=None):
+        """Runs a prepared argument block (see args()); used where the block is
+        built once, e.g. outside a hipGraph capture."""
+        abi.check(abi.hip().pvhip_recv_post(C.byref(args), _stream(stream)), "pvhip_recv_post")
 
 
 class ActiveList:
@@ -170,6 +193,13 @@ class ActiveList:
                                      device=device)
         nscr = abi.hip().pvhip_active_compact_scratch_bytes(C.byref(loc_))
         self.scratch = torch.zeros(max(1, nscr // 4), dtype=torch.int32, device=device)
+
+    def delivery_scratch(self, nbytes):
+        """Device scratch of a delivery that publishes this list (grown on demand)."""
+        n = max(1, (int(nbytes) + 3) // 4)
+        if getattr(self, "_dscratch", None) is None or self._dscratch.numel() < n:
+            self._dscratch = torch.empty(n, dtype=torch.int32, device=self.entries.device)
+        return self._dscratch
 
     def compact(self, A, stream=None):
         abi.check(abi.hip().pvhip_active_compact(

@@ -454,3 +454,40 @@ def test_recv_pre_transposed_mfma_route(case, arith):
         assert err.max() <= 1e-5 * np.abs(ref).max()
     else:
         assert np.sqrt((err ** 2).mean()) <= 1e-2 * np.sqrt((ref ** 2).mean())
+
+
+@pytest.mark.parametrize("case,arith,halo_zero", [
+    (GATHER_CASES[0], abi.ARITH_BF16, True),    # full tiles: active list emitted by the epilogue
+    (((3, 64, 32, 1, 6), (3, 32, 16, 64, 1), 8), abi.ARITH_BF16, True),
+    (GATHER_CASES[0], abi.ARITH_BF16, False),   # halo not guaranteed zero: ordered compaction
+    (GATHER_CASES[5], abi.ARITH_BF16, True),    # ragged tiles: ordered compaction
+    (GATHER_CASES[0], abi.ARITH_EXACT, True),   # tiled EXACT kernel: ordered compaction
+])
+def test_recv_post_fused_active_list(case, arith, halo_zero):
+    """Gather + LCA update + published active list in one call == the same
+    update followed by pvhip_active_compact, bit for bit (V, A, entries, counts,
+    position segments)."""
+    rng = np.random.default_rng(27)
+    hs, os_, ht, ot, p, W, A = gather_setup(case, rng)
+    n = ht.nbatch * ht.num_res
+    V0 = (rng.standard_normal(n) * 0.1).astype(np.float32)
+    tr = R.make_transfer(0.05, 0.0, FMAX, 0.05, 0.0)
+    A0 = O.transfer(ot, V0, 0.05, 0.0, FMAX, 0.05, 0.0)
+    d = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=arith)
+    upd = R.HyPerLCAUpdate(ht, tr, 100.0, 1.0, True, activity_from_v=True, device=DEV)
+    Va, Aa = dev(V0), dev(A0)
+    d.deliver(dev(A), None, fused=upd.args(None, Va, Aa, num_channels=1))
+    ref = R.ActiveList(ht, DEV)
+    ref.compact(Aa)
+    Vb, Ab = dev(V0), dev(A0)
+    act = R.ActiveList(ht, DEV)
+    act.entries.fill_(-7)
+    d.deliver(dev(A), None, fused=upd.args(None, Vb, Ab, num_channels=1), active=act,
+              halo_is_zero=halo_zero)
+    assert same(host(Va), host(Vb)) and same(host(Aa), host(Ab))
+    e_ref, n_ref = ref.host_entries()
+    e_got, n_got = act.host_entries()
+    assert (n_ref == n_got).all() and n_ref.sum() > 0
+    for b in range(ht.nbatch):
+        assert np.array_equal(e_ref[b, :n_ref[b]], e_got[b, :n_got[b]])
+    assert np.array_equal(host(ref.pos_start), host(act.pos_start))
