@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PVHIP_ABI_VERSION 2
+#define PVHIP_ABI_VERSION 3
 
 enum {
    PVHIP_OK              = 0,
@@ -405,9 +405,21 @@ typedef struct {
    float *dw;              /* [kernels][nyp][nxp][nfp], accumulated into (caller clears) */
    int64_t *counts;        /* same shape, or NULL when normalizeDw is false */
    int arith;              /* EXACT (reference order) or FP32/BF16 (MFMA correlation) */
+   /* MFMA forms (FP32, BF16): dW = dWMax * sum over batch and pre extended
+    * positions of a_pre (x) the post window, a dense correlation on the matrix
+    * cores, for connections with one data patch per feature (pre no larger
+    * than post).  The window of pre extended (xe, ye) starts at post extended
+    * (window_start_x + window_stride_x*xe, window_start_y + window_stride_y*ye)
+    * and is clipped to the post restricted region (pvhost_hebbian_windows).
+    * Partial sums over split-K chunks go to scratch
+    * (pvhip_hebbian_dw_scratch_bytes) and are added in chunk order. */
+   int window_stride_x, window_start_x, window_stride_y, window_start_y;
+   void *scratch;
+   size_t scratch_bytes;
 } pvhip_hebbian_args;
 
 int pvhip_hebbian_dw(const pvhip_hebbian_args *args, pvhip_stream stream);
+size_t pvhip_hebbian_dw_scratch_bytes(const pvhip_hebbian_args *args);
 
 /* HebbianUpdater::normalize_dW (HebbianUpdater.cpp:773-807): dw /= count, 0
  * where count == 0; then updateWeights (:822-833): w += dw. */

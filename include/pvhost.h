@@ -33,6 +33,15 @@ int pvhost_patch_axis(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int 
       int32_t *dim, int32_t *patch_start, int32_t *post_start_res, int32_t *post_start_ext,
       int32_t *unshrunken_start);
 
+/* Window geometry of the MFMA forms of pvhip_hebbian_dw: the unshrunken
+ * patch of pre extended position e starts at post extended coordinate
+ * start + stride * e along each axis, and its shrunken patch (PatchGeometry)
+ * is that window clipped to the post restricted region.  Returns 0 when both
+ * hold for every pre extended position, a negative value otherwise (the EXACT
+ * form then applies). */
+int pvhost_hebbian_windows(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int nyp,
+      int *stride_x, int *start_x, int *stride_y, int *start_y);
+
 /* Tables of pvhip_recv_post_args for a post-perspective delivery from
  * `source` (pre layer) into `target` (post layer) with post-perspective patch
  * nxp x nyp: start_x/kernel_x [target.nx], start_y/kernel_y [target.ny].

@@ -116,7 +116,9 @@ class HebbianArgs(C.Structure):
                 ("num_kernels_x", i32), ("num_kernels_y", i32),
                 ("dim_x", vp), ("dim_y", vp), ("pstart_x", vp), ("pstart_y", vp),
                 ("apost_x", vp), ("apost_y", vp), ("dw_max", f32), ("pre_data", vp),
-                ("post_data", vp), ("dw", vp), ("counts", vp), ("arith", i32)]
+                ("post_data", vp), ("dw", vp), ("counts", vp), ("arith", i32),
+                ("window_stride_x", i32), ("window_start_x", i32), ("window_stride_y", i32),
+                ("window_start_y", i32), ("scratch", vp), ("scratch_bytes", C.c_size_t)]
 
 
 # Every symbol include/pvhip.h declares, with its ctypes signature.
@@ -159,6 +161,7 @@ PVHIP_SYMBOLS = {
     "pvhip_active_compact": (i32, [C.POINTER(Loc), vp, vp, vp, vp, vp, vp]),
     "pvhip_active_compact_scratch_bytes": (C.c_size_t, [C.POINTER(Loc)]),
     "pvhip_hebbian_dw": (i32, [C.POINTER(HebbianArgs), vp]),
+    "pvhip_hebbian_dw_scratch_bytes": (C.c_size_t, [C.POINTER(HebbianArgs)]),
     "pvhip_hebbian_apply": (i32, [vp, vp, vp, C.c_size_t, i32, vp]),
     "pvhip_normalize_l2_f64": (i32, [vp, i32, i32, C.c_double, C.c_double, vp]),
     "pvhip_cost_sum_f64": (i32, [C.POINTER(Loc), vp, i32, i32, C.c_double, C.c_double, vp, vp]),
@@ -179,6 +182,8 @@ PVHOST_SYMBOLS = {
                                      C.POINTER(i32), vp, vp, vp, vp, C.POINTER(i32)]),
     "pvhost_make_transfer": (i32, [f32, f32, f32, f32, f32, i32, C.POINTER(Transfer)]),
     "pvhost_transpose_weights": (i32, [C.POINTER(Loc), C.POINTER(Loc), i32, i32, i32, vp, vp]),
+    "pvhost_hebbian_windows": (i32, [C.POINTER(Loc), C.POINTER(Loc), i32, i32, C.POINTER(i32),
+                                     C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
 }
 
 _libs = {}

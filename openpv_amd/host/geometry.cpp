@@ -178,6 +178,43 @@ int pvhost_patch_axis(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int 
    return 0;
 }
 
+int pvhost_hebbian_windows(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int nyp,
+      int *stride_x, int *start_x, int *stride_y, int *start_y) {
+   if (!locOk(pre) || !locOk(post) || nxp <= 0 || nyp <= 0) {
+      return -1;
+   }
+   for (int axis = 0; axis < 2; axis++) {
+      Axis a      = axisOf(*pre, *post, nxp, nyp, axis);
+      int n       = axis == 0 ? pre->nx + pre->halo.lt + pre->halo.rt : pre->ny + pre->halo.dn + pre->halo.up;
+      int p       = axis == 0 ? nxp : nyp;
+      int lo_res  = axis == 0 ? post->halo.lt : post->halo.up;
+      int n_res   = axis == 0 ? post->nx : post->ny;
+      int u0      = axisEntry(a, 0).unshrunk;
+      int stride  = n > 1 ? axisEntry(a, 1).unshrunk - u0 : 1;
+      for (int e = 0; e < n; e++) {
+         AxisEntry r = axisEntry(a, e);
+         if (r.unshrunk != u0 + stride * e) {
+            return -2; // window starts are not affine in the extended index
+         }
+         int lo = std::max(r.unshrunk, lo_res);
+         int hi = std::min(r.unshrunk + p, lo_res + n_res);
+         int d  = std::max(0, hi - lo);
+         if (d != r.dim || (d > 0 && (r.postStartExt != lo || r.patchStart != lo - r.unshrunk))) {
+            return -3; // shrunken patch is not the window clipped to the restricted layer
+         }
+      }
+      if (axis == 0) {
+         if (stride_x) *stride_x = stride;
+         if (start_x) *start_x = u0;
+      }
+      else {
+         if (stride_y) *stride_y = stride;
+         if (start_y) *start_y = u0;
+      }
+   }
+   return 0;
+}
+
 int pvhost_recv_post_tables(const pvhip_loc *source, const pvhip_loc *target, int nxp, int nyp,
       int32_t *start_x, int32_t *start_y, int32_t *kernel_x, int32_t *kernel_y, int *nkx, int *nky) {
    if (!locOk(source) || !locOk(target) || nxp <= 0 || nyp <= 0) {

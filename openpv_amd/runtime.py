@@ -427,6 +427,11 @@ class HebbianUpdater:
         self.size = self.nkx * self.nky * pre.nf * nxp * nyp * self.nfp
         self.dw = torch.zeros(self.size, dtype=torch.float32, device=device)
         self.counts = torch.zeros(self.size, dtype=torch.int64, device=device) if normalize_dw else None
+        # MFMA forms: window geometry (None when only the EXACT form applies)
+        w = [C.c_int() for _ in range(4)]
+        r = H.pvhost_hebbian_windows(C.byref(pre), C.byref(post), nxp, nyp, *[C.byref(v) for v in w])
+        self.windows = tuple(v.value for v in w) if r == 0 and self.nkx * self.nky == 1 else None
+        self._scratch = None
 
     def update_dw(self, pre_activity, post_activity, clear=True, arith=abi.ARITH_EXACT, stream=None):
         if clear:
@@ -443,6 +448,16 @@ class HebbianUpdater:
         a.dw = _ptr(self.dw)
         a.counts = _ptr(self.counts)
         a.arith = arith
+        if arith in (abi.ARITH_FP32, abi.ARITH_BF16):
+            if self.windows is None:
+                raise abi.PVHIPError("the MFMA forms of the Hebbian update need one data patch per "
+                                     "feature and windows clipped to the post layer")
+            a.window_stride_x, a.window_start_x, a.window_stride_y, a.window_start_y = self.windows
+            n = abi.hip().pvhip_hebbian_dw_scratch_bytes(C.byref(a))
+            if self._scratch is None or self._scratch.numel() * 4 < n:
+                self._scratch = torch.empty(max(1, (n + 3) // 4), dtype=torch.float32,
+                                            device=self.dw.device)
+            a.scratch, a.scratch_bytes = _ptr(self._scratch), self._scratch.numel() * 4
         abi.check(abi.hip().pvhip_hebbian_dw(C.byref(a), _stream(stream)), "pvhip_hebbian_dw")
 
     def apply(self, weights, stream=None):

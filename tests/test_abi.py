@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol(header, loader, table):
 
 
 def test_abi_version_and_error_string():
-    assert abi.hip().pvhip_abi_version() == 2
+    assert abi.hip().pvhip_abi_version() == 3
     assert isinstance(abi.hip().pvhip_last_error(), bytes)
 
 
@@ -170,3 +170,37 @@ def test_transpose_weights_matches_oracle(case):
                                                W.ctypes.data_as(C.c_void_p),
                                                b.ctypes.data_as(C.c_void_p)) == 0
     assert np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.mark.parametrize("case", [
+    ((1, 16, 16, 128, 2), (1, 32, 32, 3, 3), 8),
+    ((1, 32, 32, 512, 2), (1, 128, 128, 1, 6), 16),
+    ((1, 128, 128, 16, 3), (1, 512, 512, 1, 6), 16),
+    ((1, 8, 8, 5, 1), (1, 8, 8, 2, 1), 3),
+])
+def test_hebbian_windows_match_patch_geometry(case):
+    """pvhost_hebbian_windows: affine unshrunken windows, and the oracle's
+    shrunken patches are those windows clipped to the post restricted region."""
+    pre, post, p = abi.Loc.make(*case[0][:4], halo=case[0][4]), abi.Loc.make(*case[1][:4], halo=case[1][4]), case[2]
+    w = [C.c_int() for _ in range(4)]
+    assert abi.host().pvhost_hebbian_windows(C.byref(pre), C.byref(post), p, p, *[C.byref(v) for v in w]) == 0
+    sx, cx, sy, cy = (v.value for v in w)
+    opre = O.Loc.make(*case[0][:4], halo=case[0][4])
+    opost = O.Loc.make(*case[1][:4], halo=case[1][4])
+    for axis, (s, c, lo_res, n_res) in enumerate([(sx, cx, post.halo.lt, post.nx), (sy, cy, post.halo.up, post.ny)]):
+        t = O.patch_axis(opre, opost, p, p, axis)
+        for e in range(len(t["dim"])):
+            u = c + s * e
+            lo, hi = max(u, lo_res), min(u + p, lo_res + n_res)
+            assert t["dim"][e] == max(0, hi - lo)
+            if hi > lo:
+                assert t["post_start_ext"][e] == lo and t["pstart"][e] == lo - u
+
+
+def test_hebbian_windows_refuse_unclipped_geometry():
+    """Asymmetric halos: the reference's shrunken patches are not the clipped
+    windows (its per-axis border quirks), so only the EXACT Hebbian form applies."""
+    pre = abi.Loc.make(1, 12, 10, 16, (2, 1, 3, 2))
+    post = abi.Loc.make(1, 24, 20, 5, (3, 4, 4, 3))
+    w = [C.c_int() for _ in range(4)]
+    assert abi.host().pvhost_hebbian_windows(C.byref(pre), C.byref(post), 6, 6, *[C.byref(v) for v in w]) < 0

@@ -491,3 +491,38 @@ def test_recv_post_fused_active_list(case, arith, halo_zero):
     for b in range(ht.nbatch):
         assert np.array_equal(e_ref[b, :n_ref[b]], e_got[b, :n_got[b]])
     assert np.array_equal(host(ref.pos_start), host(act.pos_start))
+
+
+@pytest.mark.parametrize("case", [
+    ((2, 16, 16, 40, 2), (2, 32, 32, 3, 3), 8, True),      # V1 -> Residual (config[0] shape)
+    ((2, 16, 16, 96, 2), (2, 64, 64, 1, 6), 16, True),     # config[2] / config[4] geometry
+    ((1, 8, 8, 70, 1), (1, 8, 8, 2, 1), 3, False),         # one-to-one, ragged feature tile
+    ((1, 12, 10, 16, 2), (1, 24, 20, 5, 3), 6, True),      # non-square, even patch
+])
+@pytest.mark.parametrize("arith", [abi.ARITH_FP32, abi.ARITH_BF16])
+def test_hebbian_dw_mfma(case, arith):
+    """MFMA correlation form of the Hebbian dW against the oracle: dW to
+    rounding (FP32) or to the BF16 input bound, activation counts exactly.
+    Both halos hold nonzero values: the shrunken patches must ignore the post
+    halo and include every pre extended neuron."""
+    rng = np.random.default_rng(32)
+    (hp, op), (hq, oq), p, norm = both(case[0]), both(case[1]), case[2], case[3]
+    a = rng.standard_normal(hp.nbatch * hp.num_ext).astype(np.float32)
+    a[rng.random(a.shape) < 0.7] = 0
+    r = rng.standard_normal(hq.nbatch * hq.num_ext).astype(np.float32)
+    dw, cnt = O.hebbian_dw(op, oq, p, p, oq.nf, 0.25, a, r, counts=norm)
+    hb = R.HebbianUpdater(hp, hq, p, p, 0.25, normalize_dw=norm, device=DEV)
+    assert hb.windows is not None
+    hb.update_dw(dev(a), dev(r), arith=arith)
+    got = host(hb.dw)
+    scale = np.abs(dw).max()
+    if arith == abi.ARITH_FP32:
+        assert np.abs(got - dw).max() <= 2e-5 * scale
+    else:
+        assert np.sqrt(((got - dw) ** 2).mean()) <= 1e-2 * np.sqrt((dw ** 2).mean())
+        assert np.abs(got - dw).max() <= 3e-2 * scale
+    if norm:
+        assert (host(hb.counts) == cnt).all()
+    # dw accumulates across calls when not cleared
+    hb.update_dw(dev(a), dev(r), clear=False, arith=arith)
+    assert np.abs(host(hb.dw) - 2 * got).max() <= 1e-6 * scale
