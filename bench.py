@@ -179,6 +179,119 @@ def cpu_baseline(images, W, budget_s=12.0):
                       "scaled to the 64-image batch" % (nb, steps, elapsed)}
 
 
+def _time_events(fn, s, reps, torch):
+    """Mean and min duration (ms) of fn() on stream s, each call bracketed by HIP events."""
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(reps)]
+    with torch.cuda.stream(s):
+        fn()
+        for a, b in evs:
+            a.record(s)
+            fn()
+            b.record(s)
+    s.synchronize()
+    d = [a.elapsed_time(b) for a, b in evs]
+    return float(np.mean(d)), float(np.min(d))
+
+
+def c1_delivery(torch, reps=10):
+    """Delivery microbench: pre 256x256x16 halo 6 -> post 128x128x256,
+    pre-perspective patch 6x6x256 (post-perspective 12x12x16), W ~ N(0, 0.05^2).
+    Dense gather (pre ~N(0,1)) and sparse scatter (2 % of pre neurons, Exp(1))."""
+    import ctypes as C
+    from openpv_amd import abi, runtime as R
+    dev = "cuda"
+    rng = np.random.default_rng(3)
+    pre = R.loc(1, 256, 256, 16, 6)
+    post = R.loc(1, 128, 128, 256, 0)
+    nkx, nky = C.c_int(), C.c_int()
+    abi.host().pvhost_num_kernels(C.byref(pre), C.byref(post), C.byref(nkx), C.byref(nky))
+    nk = nkx.value * nky.value * pre.nf
+    Wpre = (rng.standard_normal((nk, 6, 6, 256)) * 0.05).astype(np.float32)
+    Wpost = np.zeros((256, 12, 12, 16), np.float32)
+    abi.host().pvhost_transpose_weights(C.byref(pre), C.byref(post), 6, 6, 256,
+                                        Wpre.ctypes.data_as(C.c_void_p), Wpost.ctypes.data_as(C.c_void_p))
+    A = np.zeros((1, pre.ny_ext, pre.nx_ext, 16), np.float32)
+    A[:, 6:262, 6:262] = rng.standard_normal((1, 256, 256, 16))
+    As = np.zeros_like(A)
+    v = np.zeros(256 * 256 * 16, np.float32)
+    idx = rng.choice(v.size, size=v.size // 50, replace=False)
+    v[idx] = rng.exponential(1.0, idx.size).astype(np.float32)
+    As[:, 6:262, 6:262] = v.reshape(1, 256, 256, 16)
+    dA = torch.as_tensor(A.ravel()).to(dev)
+    dAs = torch.as_tensor(As.ravel()).to(dev)
+    g = torch.zeros(post.num_res, device=dev)
+    s = torch.cuda.Stream()
+    out = {"workload": "pre 256x256x16 (halo 6) -> post 128x128x256, 12x12x16 post patch",
+           "gather_macs": 128 * 128 * 256 * 12 * 12 * 16, "gather_ms": {}, "scatter_ms": {}}
+    ref = None
+    for name, ar in (("exact", abi.ARITH_EXACT), ("fp32", abi.ARITH_FP32), ("bf16", abi.ARITH_BF16)):
+        d = R.PostsynapticPerspectiveGPUDelivery(pre, post, 12, 12, torch.as_tensor(Wpost.ravel()).to(dev),
+                                                 arith=ar)
+        ms, _ = _time_events(lambda: d.deliver(dA, g, stream=s), s, reps, torch)
+        out["gather_ms"][name] = round(ms, 4)
+        gh = g.cpu().numpy().astype(np.float64)
+        if ref is None:
+            ref = gh
+        else:
+            out.setdefault("gather_rel_err_vs_exact", {})[name] = float(
+                np.sqrt(((gh - ref) ** 2).mean() / (ref ** 2).mean()))
+    act = R.ActiveList(pre, dev)
+    act.compact(dAs)
+    torch.cuda.synchronize()
+    out["scatter_active"] = int(act.num_active.cpu().numpy()[0])
+    out["scatter_macs"] = out["scatter_active"] * 6 * 6 * 256
+    for name, ar in (("exact", abi.ARITH_EXACT), ("bf16", abi.ARITH_BF16)):
+        d = R.PresynapticPerspectiveGPUDelivery(pre, post, 6, 6, torch.as_tensor(Wpre.ravel()).to(dev),
+                                                arith=ar)
+        ms, _ = _time_events(lambda: d.deliver(g, active=act, pre_activity=dAs, stream=s), s, reps, torch)
+        out["scatter_ms"][name] = round(ms, 4)
+    ms_b = out["gather_ms"]["bf16"]
+    out["gather_bf16_tflops"] = round(2.0 * out["gather_macs"] / (ms_b * 1e-3) / 1e12, 1)
+    # compulsory bytes of the sparse scatter: active list, weights, GSyn read + write
+    sbytes = 8 * out["scatter_active"] + 4 * Wpre.size + 8 * post.num_res
+    out["scatter_exact_gbs"] = round(sbytes / (out["scatter_ms"]["exact"] * 1e-3) / 1e9, 1)
+    return out
+
+
+def c4_hebbian(torch, reps=5):
+    """HebbianUpdater dW (SURVEY C4 shape): V1 8 x 128x128x1024 (5 % active, Exp(1))
+    x Residual 8 x 512x512x1 (~N(0,1)), 16x16 patches at stride 4, MFMA forms."""
+    import ctypes as C
+    from openpv_amd import abi, runtime as R
+    dev = "cuda"
+    H = abi.host()
+    mv = H.pvhost_required_margin(128, 512, 16)
+    mr = H.pvhost_required_margin(512, 128, H.pvhost_post_patch_size(16, 128, 512))
+    pre = R.loc(8, 128, 128, 1024, mv)
+    post = R.loc(8, 512, 512, 1, mr)
+    g = torch.Generator(device=dev)
+    g.manual_seed(4)
+    a = torch.zeros(pre.nbatch * pre.num_ext, device=dev)
+    a.view(8, pre.ny_ext, pre.nx_ext, 1024)[:, mv:mv + 128, mv:mv + 128] = torch.where(
+        torch.rand(8, 128, 128, 1024, device=dev, generator=g) < 0.05,
+        torch.empty(8, 128, 128, 1024, device=dev).exponential_(1.0, generator=g), 0.0)
+    r = torch.zeros(post.nbatch * post.num_ext, device=dev)
+    r.view(8, post.ny_ext, post.nx_ext)[:, mr:mr + 512, mr:mr + 512] = torch.randn(
+        8, 512, 512, device=dev, generator=g)
+    hb = R.HebbianUpdater(pre, post, 16, 16, 1e-3, normalize_dw=False, device=dev)
+    s = torch.cuda.Stream()
+    flops = 2.0 * 1024 * 256 * pre.nbatch * pre.nx_ext * pre.ny_ext
+    out = {"workload": "V1 8x128x128x1024 (halo %d) x Residual 8x512x512x1, 16x16 stride 4" % mv,
+           "gflop_per_update": round(flops / 1e9, 1)}
+    dws = {}
+    for name, ar in (("fp32", abi.ARITH_FP32), ("bf16", abi.ARITH_BF16)):
+        ms, _ = _time_events(lambda: hb.update_dw(a, r, arith=ar, stream=s), s, reps, torch)
+        out[name + "_ms"] = round(ms, 4)
+        out[name + "_tflops"] = round(flops / (ms * 1e-3) / 1e12, 1)
+        dws[name] = hb.dw.cpu().numpy().astype(np.float64)
+    out["bf16_rel_err_vs_fp32"] = float(np.sqrt(((dws["bf16"] - dws["fp32"]) ** 2).mean()
+                                                / (dws["fp32"] ** 2).mean()))
+    # activity read once per update in FP32: the HBM floor of this kernel
+    out["pre_activity_bytes"] = int(4 * pre.nbatch * pre.num_ext)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +299,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--forms", default="exact,fp32,bf16")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the delivery (C1) and Hebbian (C4) measurements")
     ap.add_argument("--profile-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
@@ -266,6 +381,9 @@ def main():
         line["bf16_vs_exact"] = round(res["exact"]["ms_per_step"] / res["bf16"]["ms_per_step"], 3)
     line.update(extra)
     line["roofline"] = roof
+    if not args.no_extras:
+        line["delivery_c1"] = c1_delivery(torch)
+        line["hebbian_c4"] = c4_hebbian(torch)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(images, W)
         line["gpu_vs_cpu_exact"] = (round(res["exact"]["steps_per_s"] / line["cpu_baseline"]["value"], 1)

@@ -261,7 +261,8 @@ class PresynapticPerspectiveGPUDelivery:
         pkx, pky = C.c_int(), C.c_int()
         H.pvhost_num_kernels(C.byref(post), C.byref(pre), C.byref(pkx), C.byref(pky))
         m = pkx.value * pky.value * post.nf
-        if nxq <= 0 or nyq <= 0 or pre.nf % ck or m > 64 or pkx.value * pky.value == 1:
+        single = pkx.value * pky.value == 1  # target no larger than the source: a plain gather
+        if nxq <= 0 or nyq <= 0 or (not single and (pre.nf % ck or m > 64)):
             return None
         sx = np.zeros(post.nx, np.int32)
         sy = np.zeros(post.ny, np.int32)

@@ -526,3 +526,49 @@ def test_hebbian_dw_mfma(case, arith):
     # dw accumulates across calls when not cleared
     hb.update_dw(dev(a), dev(r), clear=False, arith=arith)
     assert np.abs(host(hb.dw) - 2 * got).max() <= 1e-6 * scale
+
+
+def test_config1_delivery_full_size():
+    """Delivery drop-in at its full size (pre 256x256x16 halo 6 -> post 128x128x256,
+    6x6x256 pre-perspective / 12x12x16 post-perspective patches): the EXACT gather
+    over dense N(0,1) activity and the EXACT scatter over a 2 % Exp(1) active list
+    are bit-identical to the oracle; the BF16 forms stay within their bounds."""
+    import ctypes as C
+    rng = np.random.default_rng(33)
+    (hp, op), (hq, oq) = both((1, 256, 256, 16, 6)), both((1, 128, 128, 256, 0))
+    nkx, nky, nkf = O.num_kernels(op, oq)
+    Wpre = (rng.standard_normal((nkx * nky * nkf, 6, 6, 256)) * 0.05).astype(np.float32)
+    Wpost = np.zeros((256, 12, 12, 16), np.float32)
+    assert abi.host().pvhost_transpose_weights(C.byref(hp), C.byref(hq), 6, 6, 256,
+                                               Wpre.ctypes.data_as(C.c_void_p),
+                                               Wpost.ctypes.data_as(C.c_void_p)) == 0
+    A = ext_random(hp, rng)
+    print("config1 gather: oracle", flush=True)
+    ref = O.recv_post(op, oq, 12, 12, A, Wpost)
+    g = torch.zeros(hq.num_res, device=DEV)
+    R.PostsynapticPerspectiveGPUDelivery(hp, hq, 12, 12, dev(Wpost)).deliver(dev(A), g)
+    assert same(host(g), ref)
+    R.PostsynapticPerspectiveGPUDelivery(hp, hq, 12, 12, dev(Wpost), arith=abi.ARITH_BF16).deliver(dev(A), g)
+    err = host(g).astype(np.float64) - ref
+    assert np.sqrt((err ** 2).mean()) <= 1e-2 * np.sqrt((ref.astype(np.float64) ** 2).mean())
+    # sparse scatter
+    As = np.zeros((1, hp.ny_ext, hp.nx_ext, 16), np.float32)
+    v = np.zeros(256 * 256 * 16, np.float32)
+    idx = rng.choice(v.size, size=v.size // 50, replace=False)
+    v[idx] = rng.exponential(1.0, idx.size).astype(np.float32)
+    As[:, 6:262, 6:262] = v.reshape(1, 256, 256, 16)
+    As = As.ravel()
+    ent, cnt = O.active_indices(op, As)
+    print("config1 scatter: oracle", flush=True)
+    ref = O.recv_pre(op, oq, 6, 6, Wpre, entries=ent, num_active=cnt, num_threads=1)
+    act = R.ActiveList(hp, DEV)
+    dA = dev(As)
+    act.compact(dA)
+    d = R.PresynapticPerspectiveGPUDelivery(hp, hq, 6, 6, dev(Wpre))
+    g.zero_()
+    d.deliver(g, active=act)
+    assert same(host(g), ref)
+    db = R.PresynapticPerspectiveGPUDelivery(hp, hq, 6, 6, dev(Wpre), arith=abi.ARITH_BF16)
+    db.deliver(g, active=act, pre_activity=dA)
+    err = host(g).astype(np.float64) - ref
+    assert np.sqrt((err ** 2).mean()) <= 1e-2 * np.sqrt((ref.astype(np.float64) ** 2).mean())
