@@ -322,6 +322,126 @@ __global__ void __launch_bounds__(256) recv_pre_lds_kernel(PreParams p, PreTile 
    }
 }
 
+// ------------------------------------------------------------------ wave form
+//
+// For postsynaptic layers with many features (nf % 4 == 0, nf >= 64): one
+// wave per postsynaptic position, lane l summing features 4l..4l+3 of each
+// 256-feature chunk.  The walk over patch rows, presynaptic rows, columns
+// and active features is the same for every feature, so it is wave-uniform:
+// when the covering presynaptic columns of the position are consecutive
+// (at most 64 of them), the entries of one presynaptic row form a single
+// run of the ordered active list, read 64 at a time by the lanes and then
+// broadcast one by one in list order; otherwise the run of each column is
+// read separately.  Each lane keeps the one-thread reference order for its
+// features; the weight rows of an entry are 16-byte loads along the feature
+// axis.
+__device__ __forceinline__ int bcast_i(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+__device__ __forceinline__ float bcast_f(float v, int j) {
+   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+template <bool EXACT>
+__device__ __forceinline__ void acc4(float4 &acc, float a, float4 w) {
+   if (EXACT) {
+      float p0 = a * w.x, p1 = a * w.y, p2 = a * w.z, p3 = a * w.w;
+      acc.x    = acc.x + p0;
+      acc.y    = acc.y + p1;
+      acc.z    = acc.z + p2;
+      acc.w    = acc.w + p3;
+   }
+   else {
+      acc.x = __builtin_fmaf(a, w.x, acc.x);
+      acc.y = __builtin_fmaf(a, w.y, acc.y);
+      acc.z = __builtin_fmaf(a, w.z, acc.z);
+      acc.w = __builtin_fmaf(a, w.w, acc.w);
+   }
+}
+
+template <bool EXACT>
+__global__ void __launch_bounds__(256) recv_pre_wave_kernel(PreParams p) {
+   int const lane    = threadIdx.x % kWave;
+   size_t const wpos = (size_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+   size_t const nPos = (size_t)p.post.nbatch * p.post.ny * p.post.nx;
+   if (wpos >= nPos) {
+      return; // the whole wave
+   }
+   int const perImg  = p.post.ny * p.post.nx;
+   int const b       = (int)(wpos / perImg);
+   int const Y       = (int)(wpos % perImg) / p.post.nx;
+   int const X       = (int)(wpos % perImg) % p.post.nx;
+   int const nfPost  = p.post.nf;
+   int const nfPre   = p.pre.nf;
+   int const nxExt   = p.pre.nxExt();
+   int const numPos  = nxExt * p.pre.nyExt();
+   int const patchSize = p.nxp * p.nyp * p.nfp;
+   const uint32_t *ps  = p.posStart + (size_t)b * (numPos + 1);
+   const pvhip_sparse_entry *ent = p.active + (size_t)b * p.pre.numExt();
+   int const cA     = p.colPtr[X];
+   int const cB     = p.colPtr[X + 1];
+   int const ncol   = cB - cA;
+   int const xeA    = ncol > 0 ? p.colXe[cA] : 0;
+   bool const contig = ncol > 0 && ncol <= kWave && p.colXe[cB - 1] - xeA == ncol - 1;
+   int const colwx  = lane < ncol ? p.colWx[cA + lane] : 0;
+   int const colkx  = lane < ncol ? p.colKx[cA + lane] : 0;
+   float *gout      = p.gsyn + (size_t)b * p.post.numRes() + ((size_t)Y * p.post.nx + X) * nfPost;
+   for (int fc = 0; fc < nfPost; fc += 4 * kWave) {
+      int const F    = fc + 4 * lane;
+      bool const fOK = F < nfPost;
+      int const Fc   = fOK ? F : 0;
+      float4 acc     = (p.accumulate && fOK) ? *reinterpret_cast<const float4 *>(gout + F)
+                                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      for (int r = 0; r < p.nyp; r++) {
+         int const r0 = p.rowPtr[Y * p.nyp + r];
+         int const r1 = p.rowPtr[Y * p.nyp + r + 1];
+         for (int ri = r0; ri < r1; ri++) {
+            int const ye   = p.rowYe[ri];
+            int const wy   = p.rowWy[ri];
+            int const kyK  = p.rowKy[ri];
+            float const *Wr = p.W + (size_t)(kyK * p.nKx) * nfPre * patchSize + (size_t)wy * p.nxp * p.nfp + Fc;
+            int const pbase = ye * nxExt;
+            int const nruns = contig ? 1 : ncol;
+            for (int run = 0; run < nruns; run++) {
+               int const xe0 = contig ? xeA : p.colXe[cA + run];
+               uint32_t const s = ps[pbase + xe0];
+               uint32_t const e = ps[pbase + xe0 + (contig ? ncol : 1)];
+               for (uint32_t base = s; base < e; base += kWave) {
+                  uint32_t const i = base + lane;
+                  pvhip_sparse_entry en = i < e ? ent[i] : pvhip_sparse_entry{0u, 0.0f};
+                  int const fl = (int)(en.index % (uint32_t)nfPre);
+                  int const cl = (int)((en.index / (uint32_t)nfPre) % (uint32_t)nxExt) - xeA;
+                  int const n  = (int)min((uint32_t)kWave, e - base);
+                  for (int j0 = 0; j0 < n; j0 += 4) {
+                     float4 w[4];
+                     float av[4];
+#pragma unroll
+                     for (int u = 0; u < 4; u++) {
+                        int const j = min(j0 + u, n - 1);
+                        float a     = bcast_f(en.value, j);
+                        int const f = bcast_i(fl, j);
+                        int const c = contig ? bcast_i(cl, j) : run;
+                        int const wx = contig ? bcast_i(colwx, c) : p.colWx[cA + run];
+                        int const kx = contig ? bcast_i(colkx, c) : p.colKx[cA + run];
+                        av[u] = (j0 + u < n) ? a : 0.0f;
+                        w[u]  = *reinterpret_cast<const float4 *>(
+                              Wr + ((size_t)kx * nfPre + f) * patchSize + (size_t)wx * p.nfp);
+                     }
+#pragma unroll
+                     for (int u = 0; u < 4; u++) {
+                        if (j0 + u < n && av[u] != 0.0f) {
+                           acc4<EXACT>(acc, av[u] * p.dtf, w[u]);
+                        }
+                     }
+                  }
+               }
+            }
+         }
+      }
+      if (fOK) {
+         *reinterpret_cast<float4 *>(gout + F) = acc;
+      }
+   }
+}
+
 } // namespace
 } // namespace pvhip
 
@@ -372,6 +492,19 @@ int pvhip_recv_pre(const pvhip_recv_pre_args *a, pvhip_stream stream) {
    hipStream_t s = (hipStream_t)stream;
    bool const exact = a->arith == PVHIP_ARITH_EXACT;
 
+   // Wave form for sparse pre layers into many postsynaptic features.
+   if (p.posStart && p.post.nf % 4 == 0 && p.post.nf >= 64) {
+      size_t const nPos = (size_t)p.post.nbatch * p.post.ny * p.post.nx;
+      dim3 grid((unsigned)((nPos + 3) / 4));
+      if (exact) {
+         hipLaunchKernelGGL(recv_pre_wave_kernel<true>, grid, dim3(256), 0, s, p);
+      }
+      else {
+         hipLaunchKernelGGL(recv_pre_wave_kernel<false>, grid, dim3(256), 0, s, p);
+      }
+      PVHIP_LAUNCHED("recv_pre_wave");
+      return PVHIP_OK;
+   }
    // LDS-staged form for sparse pre layers whose post features fit one workgroup.
    if (p.posStart && p.post.nf <= 256 && p.pre.nf <= 4096) {
       PreTile t;
