@@ -444,6 +444,39 @@ int pvhip_cost_sum_f64(
       const pvhip_loc *loc, const float *buffer, int extended, int kind, double p0, double p1,
       double *out, pvhip_stream stream);
 
+/* ------------------------------------------------------------------------ */
+/* Adjoint of the unrolled LCA loop (an extension: the reference has no      */
+/* backward pass).  One backward step on the LCA layer's grid:               */
+/*   gVt = gV + T'(V_t) * (gA + l1*sign(T(V_t)))                             */
+/*   gG[kex] = (1-e)*gVt   (extended, the gather's output gradient)          */
+/*   gA_prev = selfInteract*(1-e)*gVt   (restricted; may be NULL)            */
+/*   gV = e*gVt                                                              */
+/* the derivatives of HyPerLCAInternalStateBuffer.kpp:48-59 and of the       */
+/* transfer (ANNActivityBuffer.cpp:227-556).                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+   pvhip_loc loc;             /* the LCA layer (halo of its activity) */
+   pvhip_transfer transfer;
+   const float *exp_tau;      /* device [nbatch] */
+   int self_interact;
+   const float *V;            /* V_t, restricted */
+   const float *gA;           /* dE/dA_t from the later steps, restricted; may be NULL */
+   float l1;                  /* coefficient of sum |A_t| in E (0: none) */
+   float *gV;                 /* in: dE/dV_t carried from step t+1 (e*gVt); out: e*gVt */
+   float *gG;                 /* out: dE/dG_t, extended (halo left as it is) */
+   float *gA_prev;            /* out: self-interaction part of dE/dA_{t-1}, restricted */
+} pvhip_lca_adjoint_args;
+
+int pvhip_lca_adjoint_step(const pvhip_lca_adjoint_args *args, pvhip_stream stream);
+
+/* dst[b][kex(k)] (+)= scale * src[b][k]: restricted buffer into an extended one. */
+int pvhip_scale_to_extended(
+      const pvhip_loc *loc, float scale, const float *src, float *dst, int accumulate,
+      pvhip_stream stream);
+
+/* y = a*x + b*y over n floats. */
+int pvhip_axpby(size_t n, float a, const float *x, float b, float *y, pvhip_stream stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -111,6 +111,11 @@ class LayerUpdateArgs(C.Structure):
                 ("transfer", Transfer)]
 
 
+class LCAAdjointArgs(C.Structure):
+    _fields_ = [("loc", Loc), ("transfer", Transfer), ("exp_tau", vp), ("self_interact", i32),
+                ("V", vp), ("gA", vp), ("l1", f32), ("gV", vp), ("gG", vp), ("gA_prev", vp)]
+
+
 class HebbianArgs(C.Structure):
     _fields_ = [("pre", Loc), ("post", Loc), ("nxp", i32), ("nyp", i32), ("nfp", i32),
                 ("num_kernels_x", i32), ("num_kernels_y", i32),
@@ -165,6 +170,9 @@ PVHIP_SYMBOLS = {
     "pvhip_hebbian_apply": (i32, [vp, vp, vp, C.c_size_t, i32, vp]),
     "pvhip_normalize_l2_f64": (i32, [vp, i32, i32, C.c_double, C.c_double, vp]),
     "pvhip_cost_sum_f64": (i32, [C.POINTER(Loc), vp, i32, i32, C.c_double, C.c_double, vp, vp]),
+    "pvhip_lca_adjoint_step": (i32, [C.POINTER(LCAAdjointArgs), vp]),
+    "pvhip_scale_to_extended": (i32, [C.POINTER(Loc), f32, vp, vp, i32, vp]),
+    "pvhip_axpby": (i32, [C.c_size_t, f32, vp, f32, vp, vp]),
 }
 
 i32p = C.POINTER(C.c_int32)

@@ -14,7 +14,7 @@ LIBDIR = os.path.join(HERE, "lib")
 CSRC = os.path.join(HERE, "csrc")
 HOSTSRC = os.path.join(HERE, "host")
 
-HIP_SOURCES = ["device.hip", "layer.hip", "recv_post.hip", "recv_pre.hip", "hebbian.hip"]
+HIP_SOURCES = ["device.hip", "layer.hip", "recv_post.hip", "recv_pre.hip", "hebbian.hip", "adjoint.hip"]
 HOST_SOURCES = ["geometry.cpp"]
 
 # -ffp-contract=off keeps mul and add separately rounded in the EXACT kernels

@@ -572,3 +572,121 @@ def test_config1_delivery_full_size():
     db.deliver(g, active=act, pre_activity=dA)
     err = host(g).astype(np.float64) - ref
     assert np.sqrt((err ** 2).mean()) <= 1e-2 * np.sqrt((ref.astype(np.float64) ** 2).mean())
+
+
+# ---------------------------------------------------------------- adjoint
+def _adjoint_setup(T=12, checkpoint=5, arith=abi.ARITH_FP32, seed=40):
+    from openpv_amd.adjoint import LCAAdjoint
+    rng = np.random.default_rng(seed)
+    nb, n, nf, nv, nfv, p = 2, 16, 3, 8, 16, 8
+    x = rng.standard_normal((nb, n, n, nf)).astype(np.float32)
+    x -= x.mean(axis=(1, 2, 3), keepdims=True)
+    W = rng.standard_normal((nfv, p, p, nf))
+    W /= np.sqrt((W.reshape(nfv, -1) ** 2).sum(1)).reshape(-1, 1, 1, 1)
+    W = W.astype(np.float32)
+    adj = LCAAdjoint(nb, n, n, nf, nv, nv, nfv, p, p, VThresh=0.1, timeConstantTau=10.0,
+                     arith=arith, checkpoint=checkpoint, device=DEV)
+    return adj, x, W, T
+
+
+def _energy_f64(adj, x, W, T):
+    """The same unrolled loop in float64 torch (the FD checker): gather as conv2d on
+    the residual's extended grid, scatter as its adjoint, soft threshold."""
+    S, Tl = adj.res_loc, adj.v1_loc
+    oS = O.Loc.make(S.nbatch, S.nx, S.ny, S.nf, S.halo_tuple())
+    oT = O.Loc.make(Tl.nbatch, Tl.nx, Tl.ny, Tl.nf, Tl.halo_tuple())
+    p = adj.nxp
+    ux = O.patch_axis(oT, oS, p, p, 0)["unshrunken"][Tl.halo.lt:Tl.halo.lt + Tl.nx]
+    uy = O.patch_axis(oT, oS, p, p, 1)["unshrunken"][Tl.halo.up:Tl.halo.up + Tl.ny]
+    stride = int(ux[1] - ux[0])
+    w = torch.tensor(W, dtype=torch.float64).permute(0, 3, 1, 2)
+    nb, nf = S.nbatch, S.nf
+
+    def gather(r):
+        ext = torch.zeros((nb, nf, S.ny_ext, S.nx_ext), dtype=torch.float64)
+        ext[:, :, S.halo.up:S.halo.up + S.ny, S.halo.lt:S.halo.lt + S.nx] = r
+        return torch.nn.functional.conv2d(ext[:, :, uy[0]:, ux[0]:], w, stride=stride)[:, :, :Tl.ny, :Tl.nx]
+
+    def scatter(a):
+        r0 = torch.zeros((nb, nf, S.ny, S.nx), dtype=torch.float64, requires_grad=True)
+        (g,) = torch.autograd.grad((gather(r0) * a).sum(), r0)
+        return g
+
+    e = float(adj.lca.exp_tau[0].item())
+    vt = 0.1
+    xx = torch.tensor(x, dtype=torch.float64).permute(0, 3, 1, 2)
+    V = torch.zeros((nb, Tl.nf, Tl.ny, Tl.nx), dtype=torch.float64)
+    A = torch.zeros_like(V)
+    rec = torch.zeros_like(xx)
+    for _ in range(T):
+        G = gather(xx - rec)
+        V = e * V + (1 - e) * (G + A)
+        A = torch.where(V >= vt, V - vt, torch.zeros_like(V))
+        rec = scatter(A)
+    return float(0.5 * ((xx - rec) ** 2).sum() + float(np.float32(vt)) * A.abs().sum())
+
+
+def test_adjoint_matches_fp64_finite_differences():
+    """dE/dW and dE/dx of 12 unrolled LCA steps (checkpoint every 5) against
+    central finite differences of the float64 loop along random directions."""
+    adj, x, W, T = _adjoint_setup()
+    E, ck = adj.forward(x, W, T)
+    gW, gx = adj.backward(x, T, ck)
+    gW, gx = host(gW).astype(np.float64), host(gx).astype(np.float64)
+    e32 = float(E.sum().item())
+    e64 = _energy_f64(adj, x, W, T)
+    assert abs(e32 - e64) <= 1e-4 * abs(e64)
+    rng = np.random.default_rng(41)
+    eps = 1e-4
+    for _ in range(2):
+        d = rng.standard_normal(W.shape)
+        fd = (_energy_f64(adj, x, W + eps * d, T) - _energy_f64(adj, x, W - eps * d, T)) / (2 * eps)
+        ad = float((gW * d.ravel()).sum())
+        print("dW direction: fd", fd, "adjoint", ad, flush=True)
+        assert abs(fd - ad) <= 2e-3 * abs(fd) + 1e-6
+        dx = rng.standard_normal(x.shape)
+        fdx = (_energy_f64(adj, x + eps * dx, W, T) - _energy_f64(adj, x - eps * dx, W, T)) / (2 * eps)
+        adx = float((gx * dx.ravel()).sum())
+        print("dx direction: fd", fdx, "adjoint", adx, flush=True)
+        assert abs(fdx - adx) <= 2e-3 * abs(fdx) + 1e-6
+
+
+def test_adjoint_checkpoint_interval_does_not_change_gradients():
+    outs = []
+    for k in (1, 5, 100):
+        adj, x, W, T = _adjoint_setup(checkpoint=k)
+        E, ck = adj.forward(x, W, T)
+        gW, gx = adj.backward(x, T, ck)
+        outs.append((host(gW), host(gx)))
+    for gW, gx in outs[1:]:
+        assert same(gW, outs[0][0]) and same(gx, outs[0][1])
+
+
+def test_adjoint_fixed_activity_equals_hebbian_dw():
+    """With A_T held fixed the weight gradient is the reconstruction term,
+    corr(A_T, -(x - Rec_T)) = minus HebbianUpdater's dW (dWMax 1, normalizeDw off),
+    checked against the oracle's Hebbian restatement (EXACT forms: bit for bit)."""
+    adj, x, W, T = _adjoint_setup(T=1, arith=abi.ARITH_EXACT)
+    E, ck = adj.forward(x, W, T)
+    gW, _ = adj.backward(x, T, ck, fixed_activity=True)
+    S, Tl = adj.res_loc, adj.v1_loc
+    oS = O.Loc.make(S.nbatch, S.nx, S.ny, S.nf, S.halo_tuple())
+    oT = O.Loc.make(Tl.nbatch, Tl.nx, Tl.ny, Tl.nf, Tl.halo_tuple())
+    A_T = host(adj.A)
+    negR = -host(adj.R_ext)
+    dw, _ = O.hebbian_dw(oT, oS, adj.nxp, adj.nyp, S.nf, 1.0, A_T, negR, counts=False)
+    assert np.abs(A_T).sum() > 0
+    assert same(host(gW), dw)
+
+
+def test_adjoint_autograd_function():
+    from openpv_amd.adjoint import LCAEnergyFunction
+    adj, x, W, T = _adjoint_setup()
+    xt = torch.tensor(x, device=DEV, requires_grad=True)
+    Wt = torch.tensor(W, device=DEV, requires_grad=True)
+    E = LCAEnergyFunction.apply(xt, Wt, adj, T)
+    (2.0 * E).backward()
+    E2, ck = adj.forward(x, W, T)
+    gW, gx = adj.backward(x, T, ck)
+    assert np.allclose(host(Wt.grad).ravel(), 2.0 * host(gW), rtol=1e-6, atol=1e-7)
+    assert np.allclose(host(xt.grad).ravel(), 2.0 * host(gx), rtol=1e-6, atol=1e-7)
