@@ -292,6 +292,34 @@ def c4_hebbian(torch, reps=5):
     return out
 
 
+def c3_adjoint(torch, T=100, checkpoint=10):
+    """Adjoint check workload: 16 images 64x64x3 ~N(0,1), 256 features
+    8x8x3 stride 2 (V1 32x32x256), 100 unrolled steps, checkpoint every 10;
+    E = 0.5||x - W^T a||^2 + 0.1||a||_1.  FP32 forms; forward and backward timed."""
+    from openpv_amd import abi
+    from openpv_amd.adjoint import LCAAdjoint
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((16, 64, 64, 3)).astype(np.float32)
+    x -= x.mean(axis=(1, 2, 3), keepdims=True)
+    W = rng.standard_normal((256, 8, 8, 3))
+    W = (W / np.sqrt((W.reshape(256, -1) ** 2).sum(1)).reshape(-1, 1, 1, 1)).astype(np.float32)
+    adj = LCAAdjoint(16, 64, 64, 3, 32, 32, 256, 8, 8, VThresh=0.1, timeConstantTau=100.0,
+                     arith=abi.ARITH_FP32, checkpoint=checkpoint, device="cuda")
+    adj.forward(x, W, 5)  # warm-up (tables, shadows, scratch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E, ck = adj.forward(x, W, T)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    gW, gx = adj.backward(x, T, ck)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return {"workload": "16 x 64x64x3 -> 32x32x256, 8x8 stride 2, %d steps, checkpoint %d"
+                        % (T, checkpoint),
+            "forward_ms": round((t1 - t0) * 1e3, 2), "backward_ms": round((t2 - t1) * 1e3, 2),
+            "energy": float(E.sum().item()), "grad_w_norm": float(gW.norm().item())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,6 +412,7 @@ def main():
     if not args.no_extras:
         line["delivery_c1"] = c1_delivery(torch)
         line["hebbian_c4"] = c4_hebbian(torch)
+        line["adjoint_c3"] = c3_adjoint(torch)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(images, W)
         line["gpu_vs_cpu_exact"] = (round(res["exact"]["steps_per_s"] / line["cpu_baseline"]["value"], 1)

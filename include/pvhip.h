@@ -427,6 +427,14 @@ int pvhip_hebbian_apply(
       float *weights, float *dw, const int64_t *counts, size_t n, int normalize_dw,
       pvhip_stream stream);
 
+/* NormalizeL2 per data patch in the reference's float arithmetic and order
+ * (src/normalizers/NormalizeL2.cpp:93-125): sumsq += w*w, w *= strength /
+ * sqrtf(sumsq); a patch with sumsq <= min_l2 ends the loop, leaving it and all
+ * later patches unchanged, as the reference does. */
+int pvhip_normalize_l2(
+      float *weights, int num_patches, int patch_size, float strength, float min_l2,
+      pvhip_stream stream);
+
 /* NormalizeL2 per data patch (src/normalizers/NormalizeL2.cpp:36-120) with the
  * sum of squares and scale in FP64: w *= strength / sqrt(sum w^2). Patches
  * whose sum of squares is <= min_l2 are left unchanged. */

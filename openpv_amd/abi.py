@@ -169,6 +169,7 @@ PVHIP_SYMBOLS = {
     "pvhip_hebbian_dw_scratch_bytes": (C.c_size_t, [C.POINTER(HebbianArgs)]),
     "pvhip_hebbian_apply": (i32, [vp, vp, vp, C.c_size_t, i32, vp]),
     "pvhip_normalize_l2_f64": (i32, [vp, i32, i32, C.c_double, C.c_double, vp]),
+    "pvhip_normalize_l2": (i32, [vp, i32, i32, f32, f32, vp]),
     "pvhip_cost_sum_f64": (i32, [C.POINTER(Loc), vp, i32, i32, C.c_double, C.c_double, vp, vp]),
     "pvhip_lca_adjoint_step": (i32, [C.POINTER(LCAAdjointArgs), vp]),
     "pvhip_scale_to_extended": (i32, [C.POINTER(Loc), f32, vp, vp, i32, vp]),

@@ -342,6 +342,49 @@ __global__ void __launch_bounds__(1024) cost_sum_kernel(
    }
 }
 
+// NormalizeL2 in the reference's float arithmetic (NormalizeL2.cpp:93-125,
+// NormalizeBase.cpp:184-193, NormalizeMultiply.cpp:251-254): per data patch
+// sumsq += w*w in order, l2 = sqrtf(sumsq), w *= strength / l2.  A patch
+// whose sumsq is within minL2 of zero ends the loop (the reference breaks out
+// of its patch loop), so it and every later patch stay unchanged.  One
+// workgroup; thread p owns patch p (strided).
+__global__ void __launch_bounds__(1024) normalize_l2_exact_kernel(
+      float *w, int numPatches, int patchSize, float strength, float minL2) {
+   __shared__ int firstBad;
+   if (threadIdx.x == 0) {
+      firstBad = numPatches;
+   }
+   __syncthreads();
+   for (int p = threadIdx.x; p < numPatches; p += blockDim.x) {
+      float const *d = w + (size_t)p * patchSize;
+      float sumsq    = 0.0f;
+      for (int k = 0; k < patchSize; k++) {
+         float const x  = d[k];
+         float const sq = x * x;
+         sumsq          = sumsq + sq;
+      }
+      if (fabsf(sumsq) <= minL2) {
+         atomicMin(&firstBad, p);
+      }
+   }
+   __syncthreads();
+   int const stop = firstBad;
+   for (int p = threadIdx.x; p < stop; p += blockDim.x) {
+      float *d    = w + (size_t)p * patchSize;
+      float sumsq = 0.0f;
+      for (int k = 0; k < patchSize; k++) {
+         float const x  = d[k];
+         float const sq = x * x;
+         sumsq          = sumsq + sq;
+      }
+      float const l2   = sqrtf(sumsq);
+      float const mult = strength / l2;
+      for (int k = 0; k < patchSize; k++) {
+         d[k] = d[k] * mult;
+      }
+   }
+}
+
 __global__ void __launch_bounds__(256) normalize_l2_f64_kernel(
       float *w, int patchSize, double strength, double minL2) {
    float *data = w + (size_t)blockIdx.x * patchSize;
@@ -638,6 +681,16 @@ int pvhip_normalize_l2_f64(
    hipLaunchKernelGGL(normalize_l2_f64_kernel, dim3(num_patches), dim3(256), 0,
          (hipStream_t)stream, weights, patch_size, strength, min_l2);
    PVHIP_LAUNCHED("normalize_l2_f64");
+   return PVHIP_OK;
+}
+
+int pvhip_normalize_l2(
+      float *weights, int num_patches, int patch_size, float strength, float min_l2,
+      pvhip_stream stream) {
+   PVHIP_REQUIRE(weights && num_patches > 0 && patch_size > 0, "normalize_l2: bad argument");
+   hipLaunchKernelGGL(normalize_l2_exact_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream,
+         weights, num_patches, patch_size, strength, min_l2);
+   PVHIP_LAUNCHED("normalize_l2");
    return PVHIP_OK;
 }
 

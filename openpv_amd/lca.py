@@ -40,7 +40,10 @@ class LCAColumn:
     def __init__(self, images, weights, nx_v1, ny_v1, nf_v1, nxp, nyp, VThresh=0.1, AMin=0.0,
                  AMax=FLT_MAX, AShift=None, VWidth=0.0, timeConstantTau=100.0, dt=1.0,
                  self_interact=True, err_scale=1.0, vertices=False, arith=abi.ARITH_EXACT,
-                 fused=False, V0=None, res_halo=None, v1_halo=None, device="cuda"):
+                 fused=False, V0=None, res_halo=None, v1_halo=None, normalize=None,
+                 device="cuda"):
+        """normalize: NormalizeL2 of the weights on initialisation (strength 1):
+        "exact" (the reference's float arithmetic), "fp64", or None."""
         images = torch.as_tensor(images, dtype=torch.float32)
         nb, ny, nx, nf = images.shape
         self.device = torch.device(device)
@@ -68,10 +71,24 @@ class LCAColumn:
         self.rec_V = torch.zeros(nb * self.rec_loc.num_res, **f)
         self.rec_A = torch.zeros(nb * self.rec_loc.num_res, **f)
         self.weights = torch.as_tensor(weights, dtype=torch.float32).reshape(-1).to(d).contiguous()
+        if normalize is not None:
+            npatch = nf_v1
+            psize = self.weights.numel() // npatch
+            H2 = abi.hip()
+            if normalize == "exact":
+                abi.check(H2.pvhip_normalize_l2(C.c_void_p(self.weights.data_ptr()), npatch, psize,
+                                                1.0, 0.0, None), "pvhip_normalize_l2")
+            elif normalize == "fp64":
+                abi.check(H2.pvhip_normalize_l2_f64(C.c_void_p(self.weights.data_ptr()), npatch,
+                                                    psize, 1.0, 0.0, None), "pvhip_normalize_l2_f64")
+            else:
+                raise ValueError("normalize must be 'exact', 'fp64' or None")
+            torch.cuda.synchronize()
         if AShift is None:
             AShift = VThresh
         # probe coefficient: the layer's float VThresh widened to double (L1NormLCAProbe)
         self.VThresh = float(np.float32(VThresh))
+        self.VWidth = float(np.float32(VWidth))
         self.v1_transfer = make_transfer(VThresh, AMin, AMax, AShift, VWidth, vertices)
         self.none_transfer = make_transfer(-FLT_MAX, -FLT_MAX, FLT_MAX, 0.0, 0.0, False)
         self.err_scale = float(err_scale)
@@ -135,11 +152,25 @@ class LCAColumn:
         layer_update(self.rec_loc, self.rec_gsyn, self.rec_V, self.rec_A, self.none_transfer,
                      channel_coeffs=(1.0,), err_scale=None, stream=s)
 
-    def probe_energy(self, stream=None):
-        """E_b = 0.5*sum r^2 + VThresh*sum |a| in FP64 (L2NormProbe + L1NormLCAProbe)."""
+    def probe_energy(self, stream=None, cost="l1"):
+        """ColumnEnergyProbe in FP64: E_b = 0.5*sum r^2 (L2NormProbe, exponent 2) +
+        VThresh*cost(A) with cost "l1" (L1NormLCAProbe: sum |a|) or "firm"
+        (FirmThresholdCostFnLCAProbe with the layer's VThresh and VWidth)."""
         cost_sum_f64(self.res_loc, self.res_A, True, 0, out=self.energy[0], stream=stream)
-        cost_sum_f64(self.v1_loc, self.v1_A, True, 1, out=self.energy[1], stream=stream)
+        if cost == "firm":
+            cost_sum_f64(self.v1_loc, self.v1_A, True, 2, self.VThresh, self.VWidth,
+                         out=self.energy[1], stream=stream)
+        else:
+            cost_sum_f64(self.v1_loc, self.v1_A, True, 1, out=self.energy[1], stream=stream)
         return 0.5 * self.energy[0] + self.VThresh * self.energy[1]
+
+    def set_input(self, images):
+        """New input frame(s) [nb][ny][nx][nf] (an input layer's retrieveInput)."""
+        self.inputA.copy_(torch.as_tensor(images, dtype=torch.float32).reshape(-1))
+
+    def set_dt(self, dt):
+        """Per-batch dt of the V1 update for the coming steps (adaptive time scales)."""
+        self.lca.set_dt(dt)
 
     def capture(self, stream):
         """Capture one step as a hipGraph on `stream` (a non-default torch stream)."""
@@ -169,3 +200,41 @@ class LCAColumn:
 
     def num_active(self):
         return self.active.num_active.cpu().numpy()
+
+
+def run_column(col, steps, frames=None, display_period=0, adaptive=None, energy_cost="l1",
+               on_step=None, stream=None):
+    """HyPerCol::advanceTime for the column (src/columns/HyPerCol.cpp:659-791), one
+    step per timestep t = 1 .. steps (dt 1):
+      1. AdaptTimestep: with `adaptive` (timescale.AdaptiveTimeScaleController),
+         the per-batch dt of the V1 update from the energy of step t-1, or from
+         -1 when the input layer updates this step (its trigger);
+      2. the input layer: every `display_period` steps starting at t = 1, the
+         next frame of `frames` [n][nb][ny][nx][nf] (retrieveInput, then the
+         index advances; InputActivityBuffer.cpp:614-660);
+      3. the layer phases (LCAColumn.step);
+      4. the column energy probe (ColProbeOutputState), kept for step t+1.
+    on_step(t, col, energy) is called after each step.  Returns the energies."""
+    from .timescale import PeriodicUpdate
+    nb = col.in_loc.nbatch
+    trigger = PeriodicUpdate(display_period) if (frames is not None and display_period > 0) else None
+    index = 0
+    if frames is not None:
+        col.set_input(frames[0])
+    energy_prev = np.zeros(nb)
+    energies = []
+    for t in range(1, steps + 1):
+        updating = trigger is not None and trigger.need_update(t)
+        if adaptive is not None:
+            vals = np.full(nb, -1.0) if updating else energy_prev
+            col.set_dt(adaptive.calc_timesteps(vals))
+        if updating:
+            col.set_input(frames[index])
+            index = (index + 1) % len(frames)
+            trigger.updated(t)
+        col.step(stream=stream)
+        energy_prev = col.probe_energy(stream=stream, cost=energy_cost).cpu().numpy()
+        energies.append(energy_prev)
+        if on_step is not None:
+            on_step(t, col, energy_prev)
+    return np.array(energies)

@@ -315,7 +315,9 @@ class PresynapticPerspectiveGPUDelivery:
         a.weights = _ptr(self.weights)
         a.post_gsyn = _ptr(gsyn)
         a.accumulate = int(accumulate)
-        a.arith = self.arith
+        # the ordered scatter has EXACT and FP32 forms; a BF16 delivery whose
+        # geometry has no transposed MFMA route runs it in FP32
+        a.arith = abi.ARITH_EXACT if self.arith == abi.ARITH_EXACT else abi.ARITH_FP32
         abi.check(abi.hip().pvhip_recv_pre(C.byref(a), _stream(stream)), "pvhip_recv_pre")
 
 

@@ -5,6 +5,7 @@ lists, counts); FP32 (FMA) forms within 1e-5 relative to the largest
 magnitude; BF16 forms within the tolerance written at each test.
 """
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -690,3 +691,95 @@ def test_adjoint_autograd_function():
     gW, gx = adj.backward(x, T, ck)
     assert np.allclose(host(Wt.grad).ravel(), 2.0 * host(gW), rtol=1e-6, atol=1e-7)
     assert np.allclose(host(xt.grad).ravel(), 2.0 * host(gx), rtol=1e-6, atol=1e-7)
+
+
+# ------------------------------------------------- end-to-end: reference fixture
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gpulca")
+
+
+def _gpulca_run(arith, fused=True, steps=40):
+    """The CPU column of the reference's GPULCATest (tests/GPULCATest): Input
+    128x64x1 (two frames, displayPeriod 20) -> Residual <-> V1 64x32x24 (18x18
+    patches, firm threshold VThresh 0.025 / VWidth 0.025, tau 100, V from file)
+    -> Reconstruction, weights from file L2-normalised on initialisation,
+    adaptive time scales (baseMax 0.55, baseMin 0.5, tauFactor 0.1, growth
+    0.01) on the total energy 0.5|r|^2 + VThresh*firm(a); 40 steps."""
+    from openpv_amd import pvp
+    from openpv_amd.lca import LCAColumn, run_column
+    from openpv_amd.timescale import AdaptiveTimeScaleController
+    _, _, frames = pvp.read_activity(os.path.join(GOLDEN, "inputImages.pvp"))
+    _, _, V0 = pvp.read_activity(os.path.join(GOLDEN, "initialV1.pvp"))
+    _, _, W = pvp.read_weights(os.path.join(GOLDEN, "weights.pvp"))
+    col = LCAColumn(frames[:1], W[0], 64, 32, 24, 18, 18, VThresh=0.025, AMin=0.0, AMax=FMAX,
+                    AShift=0.0, VWidth=0.025, timeConstantTau=100.0, dt=1.0, arith=arith,
+                    fused=fused, V0=V0[0], normalize="exact", device=DEV)
+    ctrl = AdaptiveTimeScaleController(1, 0.55, 0.5, 0.1, 0.01)
+    recon = []
+    run_column(col, steps, frames=[f[None] for f in frames], display_period=20, adaptive=ctrl,
+               energy_cost="firm", on_step=lambda t, c, e: recon.append(c.rec_A.cpu().numpy().copy()))
+    return np.stack(recon).reshape(steps, 64, 128, 1)
+
+
+def _gpulca_oracle(steps):
+    """The same column on the oracle (one-thread CPU restatement), reusing the
+    host timescale controller: Reconstruction of each step."""
+    from openpv_amd import pvp, abi as A
+    from openpv_amd.timescale import AdaptiveTimeScaleController, PeriodicUpdate
+    _, _, frames = pvp.read_activity(os.path.join(GOLDEN, "inputImages.pvp"))
+    _, _, V0 = pvp.read_activity(os.path.join(GOLDEN, "initialV1.pvp"))
+    _, _, W = pvp.read_weights(os.path.join(GOLDEN, "weights.pvp"))
+    W = W[0].copy()
+    O.lib().or_normalize_l2(W.ravel(), 24, 18 * 18, 1.0, 0.0)  # in place (contiguous)
+    H = A.host()
+    mres = H.pvhost_required_margin(128, 64, H.pvhost_post_patch_size(18, 64, 128))
+    mv1 = H.pvhost_required_margin(64, 128, 18)
+    res, v1, rec = O.Loc.make(1, 128, 64, 1, mres), O.Loc.make(1, 64, 32, 24, mv1), O.Loc.make(1, 128, 64, 1, 0)
+    tr = (0.025, 0.0, FMAX, 0.0, 0.025)
+    V = V0[0].ravel().copy()
+    Av = O.transfer(v1, V, *tr)
+    rc = np.zeros(rec.num_res, np.float32)
+    ctrl, trig = AdaptiveTimeScaleController(1, 0.55, 0.5, 0.1, 0.01), PeriodicUpdate(20)
+    x, idx, e_prev, out = frames[0], 0, np.zeros(1), []
+    for t in range(1, steps + 1):
+        up = trig.need_update(t)
+        ts = ctrl.calc_timesteps(np.full(1, -1.0) if up else e_prev)
+        if up:
+            x, idx = frames[idx], (idx + 1) % 2
+            trig.updated(t)
+        Rres = np.zeros(res.num_res, np.float32)
+        O.lib().or_gsyn_accumulate(res.num_res, 2, np.array([1.0, -1.0], np.float32),
+                                   np.concatenate([x.ravel(), rc]), Rres)
+        Rext = np.zeros((res.ny_ext, res.nx_ext), np.float32)
+        Rext[mres:mres + 64, mres:mres + 128] = (Rres * np.float32(1.0)).reshape(64, 128)
+        G = O.recv_post(res, v1, 18, 18, Rext.ravel(), W)
+        V = O.update_lca(v1, True, ts, 100.0, G, Av, V)
+        Av = O.transfer(v1, V, *tr)
+        ent, cnt = O.active_indices(v1, Av)
+        rc = O.recv_pre(v1, rec, 18, 18, W, entries=ent, num_active=cnt, num_threads=1)
+        vt = float(np.float32(0.025))
+        e_prev = np.array([0.5 * O.cost_sum(res, Rext.ravel(), True, 0, 0, 0, 0)
+                           + vt * O.cost_sum(v1, Av, True, 0, 2, vt, vt)])
+        out.append(rc.copy())
+    return np.stack(out).reshape(steps, 64, 128, 1)
+
+
+def test_gpulca_inputs_adaptive_timescales_exact():
+    """The reference's GPULCATest inputs (images, initial V1, weights from its
+    input/ folder) through the whole column with adaptive time scales: EXACT
+    forms on the GPU == the oracle, bit for bit, for 25 steps (an input change
+    and its timescale reset at step 21 included)."""
+    steps = 25
+    ref = _gpulca_oracle(steps)
+    got = _gpulca_run(abi.ARITH_EXACT, steps=steps)
+    for t in range(steps):
+        assert same(got[t], ref[t]), "step %d differs (max %g)" % (t + 1, np.abs(got[t] - ref[t]).max())
+
+
+@pytest.mark.parametrize("arith,tol", [(abi.ARITH_FP32, 1e-4), (abi.ARITH_BF16, 3e-2)])
+def test_gpulca_inputs_adaptive_timescales_fast_forms(arith, tol):
+    steps = 25
+    ref = _gpulca_oracle(steps)
+    got = _gpulca_run(arith, steps=steps)
+    err = np.abs(got - ref).max()
+    print("max abs error", err, "scale", np.abs(ref).max(), flush=True)
+    assert err <= tol * np.abs(ref).max()

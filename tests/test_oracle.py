@@ -455,3 +455,41 @@ def test_lca_loop_fixed_point():
     V = torch.tensor(out["V"].reshape(nb, nv, nv, nfv), dtype=torch.float64).permute(0, 3, 1, 2)
     assert (V - (G + a)).abs().max() <= 2e-3 * V.abs().max()
     assert 0 < out["num_active"][-1, 0] < nv * nv * nfv
+
+
+def test_adaptive_timescale_controller_cases():
+    """AdaptiveTimeScaleController::calcTimesteps by cases: the first value and any
+    rise of the energy reset to baseMin; a fall sets tauFactor*E0/((E0-E)/ts),
+    capped at timeScaleMax which then grows by growthFactor; equal values keep
+    the state."""
+    from openpv_amd.timescale import AdaptiveTimeScaleController, PeriodicUpdate
+    c = AdaptiveTimeScaleController(1, 0.55, 0.5, 0.1, 0.01)
+    assert c.calc_timesteps([-1.0])[0] == 0.5          # equal to the initial -1: unchanged
+    assert c.calc_timesteps([10.0])[0] == 0.5          # E_0 = -1 <= 0: reset
+    ts = c.calc_timesteps([9.0])[0]                     # fall: 0.1 * 10 / ((10-9)/0.5) = 0.5
+    assert ts == 0.1 * 10.0 / ((10.0 - 9.0) / 0.5)
+    ts = c.calc_timesteps([8.99])[0]                    # steep: capped at 0.55, max grows
+    assert ts == 0.55 and c.time_scale_max[0] == 0.55 * 1.01
+    assert c.calc_timesteps([9.5])[0] == 0.5 and c.time_scale_max[0] == 0.55   # rise: reset
+    p = PeriodicUpdate(20)
+    hits = [t for t in range(1, 60) if p.need_update(t) and (p.updated(t) or True)]
+    assert hits == [1, 21, 41]
+
+
+def test_pvp_roundtrip_and_reference_fixtures(tmp_path):
+    """PVP reader on the reference's own fixture files (headers and sizes) and a
+    write/read round trip."""
+    from openpv_amd import pvp
+    g = os.path.join(HERE, "golden", "gpulca")
+    h, t, d = pvp.read_activity(os.path.join(g, "inputImages.pvp"))
+    assert (h["fileType"], h["nx"], h["ny"], h["nf"]) == (4, 128, 64, 1) and d.shape == (2, 64, 128, 1)
+    h, geo, w = pvp.read_weights(os.path.join(g, "weights.pvp"))
+    assert (h["nxp"], h["nyp"], h["nfp"], h["numPatches"]) == (18, 18, 1, 24)
+    assert (geo[0, :, :2] == 18).all() and w.shape == (1, 24, 18, 18, 1)
+    assert np.allclose(np.sqrt((w[0].reshape(24, -1).astype(np.float64) ** 2).sum(1)), 1.0, atol=1e-5)
+    assert w.min() >= h["wMin"] - 1e-7 and w.max() <= h["wMax"] + 1e-7
+    rng = np.random.default_rng(17)
+    fr = rng.standard_normal((3, 5, 7, 2)).astype(np.float32)
+    pvp.write_activity(str(tmp_path / "a.pvp"), fr, [1.0, 2.0, 3.5])
+    h, t, d = pvp.read_activity(str(tmp_path / "a.pvp"))
+    assert list(t) == [1.0, 2.0, 3.5] and np.array_equal(d, fr)
