@@ -1,7 +1,9 @@
 /*
  * pvhost.h -- C-ABI of libpvhost, the C++ host side above libpvhip: patch
  * geometry and the device tables of the deliveries, weight transposition,
- * transfer-function vertices, and the params-driven LCA column runner.
+ * transfer-function vertices.  The .params reader (pvparams.h) is part of the
+ * same library; the pvrun executable builds and runs a column from a .params
+ * file over both libraries.
  *
  * The reference computes these tables on the host as well
  * (PatchGeometry::setPatchGeometry, src/components/PatchGeometry.cpp:179-279;
@@ -51,7 +53,7 @@ int pvhost_recv_post_tables(const pvhip_loc *source, const pvhip_loc *target, in
       int32_t *start_x, int32_t *start_y, int32_t *kernel_x, int32_t *kernel_y, int *nkx, int *nky);
 
 /* Inverse tables of pvhip_recv_post_args for the MFMA path (any number of
- * data patches per feature): *window_x0/*window_nx (and y) receive the range
+ * data patches per feature): *window_x0 and *window_nx (and y) receive the range
  * of window starts; inverse_x (nkx*window_nx ints) and inverse_y may be NULL
  * on a first call that only queries the ranges. */
 int pvhost_recv_post_inverse(const pvhip_loc *source, const pvhip_loc *target, int nxp, int nyp,

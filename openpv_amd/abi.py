@@ -195,6 +195,22 @@ PVHOST_SYMBOLS = {
                                      C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
 }
 
+# the .params reader (pvparams.h), exported by libpvhost.so
+PVPARAMS_SYMBOLS = {
+    "pvparams_load": (i32, [C.c_char_p, C.POINTER(vp)]),
+    "pvparams_parse": (i32, [C.c_char_p, C.POINTER(vp)]),
+    "pvparams_error": (C.c_char_p, []),
+    "pvparams_free": (None, [vp]),
+    "pvparams_num_groups": (i32, [vp]),
+    "pvparams_group_keyword": (C.c_char_p, [vp, i32]),
+    "pvparams_group_name": (C.c_char_p, [vp, i32]),
+    "pvparams_find_group": (i32, [vp, C.c_char_p]),
+    "pvparams_number": (i32, [vp, i32, C.c_char_p, C.POINTER(C.c_double)]),
+    "pvparams_string": (i32, [vp, i32, C.c_char_p, C.POINTER(C.c_char_p)]),
+    "pvparams_array": (i32, [vp, i32, C.c_char_p, C.POINTER(C.c_double), i32]),
+    "pvparams_directive": (i32, [vp, C.c_char_p, C.POINTER(C.c_double)]),
+}
+
 _libs = {}
 
 
@@ -224,8 +240,17 @@ def hip():
 
 
 def host():
-    """libpvhost.so (host geometry / tables)."""
-    return _load("libpvhost.so", PVHOST_SYMBOLS)
+    """libpvhost.so (host geometry / tables, and the .params reader)."""
+    return _load("libpvhost.so", dict(PVHOST_SYMBOLS, **PVPARAMS_SYMBOLS))
+
+
+def pvrun_path():
+    """The pvrun executable (runs a .params file on the device)."""
+    path = os.path.join(LIBDIR, "pvrun")
+    if not os.path.exists(path):
+        raise PVHIPError("%s is not built (run __graft_entry__.build() or python -m openpv_amd.build)"
+                         % path)
+    return path
 
 
 def check(status, what=""):

@@ -1,6 +1,7 @@
-"""Builds libpvhip.so (HIP, gfx950) and libpvhost.so (C++ host) in-tree.
+"""Builds libpvhip.so (HIP, gfx950), libpvhost.so (C++ host) and the pvrun
+executable in-tree.
 
-    python -m openpv_amd.build          # both libraries
+    python -m openpv_amd.build          # the libraries and pvrun
 """
 import os
 import shutil
@@ -15,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 HOSTSRC = os.path.join(HERE, "host")
 
 HIP_SOURCES = ["device.hip", "layer.hip", "recv_post.hip", "recv_pre.hip", "hebbian.hip", "adjoint.hip"]
-HOST_SOURCES = ["geometry.cpp"]
+HOST_SOURCES = ["geometry.cpp", "params.cpp"]
 
 # -ffp-contract=off keeps mul and add separately rounded in the EXACT kernels
 # (FMA and MFMA paths call the builtins explicitly).  Packed FP32 instructions
@@ -64,8 +65,21 @@ def build_host(verbose=False):
     return out
 
 
+def build_pvrun(verbose=False):
+    """The pvrun executable (runs a .params file) next to the libraries it links."""
+    out = os.path.join(LIBDIR, "pvrun")
+    tmp = out + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-o", tmp,
+           os.path.join(HOSTSRC, "pvrun.cpp"), "-L" + LIBDIR, "-lpvhip", "-lpvhost",
+           "-Wl,-rpath,$ORIGIN"]
+    _run(cmd, verbose)
+    os.replace(tmp, out)
+    return out
+
+
 def build(verbose=False):
-    return build_hip(verbose), build_host(verbose)
+    hip, host = build_hip(verbose), build_host(verbose)
+    return hip, host, build_pvrun(verbose)
 
 
 if __name__ == "__main__":

@@ -17,11 +17,12 @@ FMAX = float(np.finfo(np.float32).max)
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(pv(?:hip|host)_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(pv(?:hip|host|params)_[a-z0-9_]+)\s*\(", text)))
 
 
 @pytest.mark.parametrize("header,loader,table", [
-    ("pvhip.h", abi.hip, abi.PVHIP_SYMBOLS), ("pvhost.h", abi.host, abi.PVHOST_SYMBOLS)])
+    ("pvhip.h", abi.hip, abi.PVHIP_SYMBOLS), ("pvhost.h", abi.host, abi.PVHOST_SYMBOLS),
+    ("pvparams.h", abi.host, abi.PVPARAMS_SYMBOLS)])
 def test_library_exports_every_declared_symbol(header, loader, table):
     lib = loader()
     names = declared(header)

@@ -5,7 +5,9 @@ lists, counts); FP32 (FMA) forms within 1e-5 relative to the largest
 magnitude; BF16 forms within the tolerance written at each test.
 """
 import ctypes as C
+import functools
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -721,8 +723,15 @@ def _gpulca_run(arith, fused=True, steps=40):
 
 
 def _gpulca_oracle(steps):
+    """Reconstruction of each step of the column on the oracle."""
+    return _gpulca_oracle_run(steps)[0]
+
+
+@functools.lru_cache(maxsize=None)
+def _gpulca_oracle_run(steps):
     """The same column on the oracle (one-thread CPU restatement), reusing the
-    host timescale controller: Reconstruction of each step."""
+    host timescale controller: (Reconstruction of each step, total energy of
+    each step).  Cached: callers must not modify the arrays."""
     from openpv_amd import pvp, abi as A
     from openpv_amd.timescale import AdaptiveTimeScaleController, PeriodicUpdate
     _, _, frames = pvp.read_activity(os.path.join(GOLDEN, "inputImages.pvp"))
@@ -739,7 +748,7 @@ def _gpulca_oracle(steps):
     Av = O.transfer(v1, V, *tr)
     rc = np.zeros(rec.num_res, np.float32)
     ctrl, trig = AdaptiveTimeScaleController(1, 0.55, 0.5, 0.1, 0.01), PeriodicUpdate(20)
-    x, idx, e_prev, out = frames[0], 0, np.zeros(1), []
+    x, idx, e_prev, out, energies = frames[0], 0, np.zeros(1), [], []
     for t in range(1, steps + 1):
         up = trig.need_update(t)
         ts = ctrl.calc_timesteps(np.full(1, -1.0) if up else e_prev)
@@ -760,7 +769,8 @@ def _gpulca_oracle(steps):
         e_prev = np.array([0.5 * O.cost_sum(res, Rext.ravel(), True, 0, 0, 0, 0)
                            + vt * O.cost_sum(v1, Av, True, 0, 2, vt, vt)])
         out.append(rc.copy())
-    return np.stack(out).reshape(steps, 64, 128, 1)
+        energies.append(e_prev[0])
+    return np.stack(out).reshape(steps, 64, 128, 1), np.array(energies)
 
 
 def test_gpulca_inputs_adaptive_timescales_exact():
@@ -783,3 +793,64 @@ def test_gpulca_inputs_adaptive_timescales_fast_forms(arith, tol):
     err = np.abs(got - ref).max()
     print("max abs error", err, "scale", np.abs(ref).max(), flush=True)
     assert err <= tol * np.abs(ref).max()
+
+
+# ------------------------------------------------- the C++ host: pvrun on a .params file
+def _pvrun(params, outdir, steps):
+    """Runs pvrun on `params` in the fixture directory (the params' relative
+    input paths resolve there); returns the completed process."""
+    print("pvrun", params, "steps", steps, flush=True)
+    r = subprocess.run([abi.pvrun_path(), "-p", str(params), "-o", str(outdir), "--steps", str(steps)],
+                       cwd=GOLDEN, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    return r
+
+
+def _energy_file(path):
+    return np.array([float(line.split(",")[2]) for line in open(path) if line.strip()])
+
+
+def test_pvrun_two_columns_from_params(tmp_path):
+    """pvrun builds both columns of the committed params file (a host column with
+    receiveGpu = false and a device column with receiveGpu = true on the gather)
+    and runs 25 steps: the host column's reconstructions equal the oracle's bit
+    for bit and its energy file matches the oracle's energies; the device column
+    (FP32) is within 1e-4 of the oracle's scale; the Difference probe's verdict
+    agrees with the two reconstructions."""
+    from openpv_amd import pvp
+    steps = 25
+    r = _pvrun("LCAColumn.params", tmp_path, steps)
+    ref, energy = _gpulca_oracle_run(steps)
+    _, times, host = pvp.read_activity(str(tmp_path / "ReconHost.pvp"))
+    _, _, dev = pvp.read_activity(str(tmp_path / "ReconDevice.pvp"))
+    assert host.shape == (steps, 64, 128, 1) and list(times) == list(range(1, steps + 1))
+    for t in range(steps):
+        assert same(host[t], ref[t]), "host column, step %d (max %g)" % (t + 1, np.abs(host[t] - ref[t]).max())
+    e = _energy_file(tmp_path / "EnergyHost.txt")
+    assert np.allclose(e, energy, rtol=1e-8, atol=1e-9), (e[:4], energy[:4])
+    err = np.abs(dev - ref).max()
+    diff = np.abs(dev - host).max()
+    print("device column: max |dev - oracle| %g (scale %g); max |dev - host| %g" % (err, np.abs(ref).max(), diff))
+    assert err <= 1e-4 * np.abs(ref).max()
+    # RequireAllZeroActivityProbe at nnzThreshold 2e-7 on Difference = ReconDevice - ReconHost
+    assert (r.returncode == 0) == (diff <= np.float32(2e-7)), r.stderr[-500:]
+
+
+def test_pvrun_bf16_device_column(tmp_path):
+    """gpuArithmetic = "bf16": the device column's gather runs on the BF16 MFMA
+    path (FP32 state); the host column is unchanged (bit-exact)."""
+    from openpv_amd import pvp
+    steps = 25
+    text = open(os.path.join(GOLDEN, "LCAColumn.params")).read()
+    assert 'gpuArithmetic = "fp32";' in text
+    params = tmp_path / "bf16.params"
+    params.write_text(text.replace('gpuArithmetic = "fp32";', 'gpuArithmetic = "bf16";'))
+    r = _pvrun(params, tmp_path, steps)
+    assert r.returncode in (0, 1), r.stderr[-500:]   # 1: the 2e-7 comparison probe fails, as expected
+    ref = _gpulca_oracle(steps)
+    _, _, host = pvp.read_activity(str(tmp_path / "ReconHost.pvp"))
+    _, _, dev = pvp.read_activity(str(tmp_path / "ReconDevice.pvp"))
+    assert same(host, ref)
+    err = np.abs(dev - ref).max()
+    print("bf16 device column: max |dev - oracle| %g (scale %g)" % (err, np.abs(ref).max()))
+    assert err <= 3e-2 * np.abs(ref).max()

@@ -1,0 +1,171 @@
+"""The .params reader of the C++ host (pvparams.h) and pvrun's refusal paths.
+CPU only: nothing here touches a device.
+
+The language is that of the reference's parser (src/io/parser/params.l and
+params.y, with PVParams's include / overwrite semantics, src/io/PVParams.cpp).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from openpv_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FMAX = float(np.finfo(np.float32).max)
+
+
+class Params:
+    """Thin reader over the C-ABI, for the tests."""
+
+    def __init__(self, text):
+        self.H = abi.host()
+        self.p = C.c_void_p()
+        self.status = self.H.pvparams_parse(text.encode(), C.byref(self.p))
+        self.error = self.H.pvparams_error().decode() if self.status != 0 else ""
+
+    def __del__(self):
+        if getattr(self, "p", None) and self.p.value:
+            self.H.pvparams_free(self.p)
+
+    def groups(self):
+        n = self.H.pvparams_num_groups(self.p)
+        return [(self.H.pvparams_group_keyword(self.p, g).decode(),
+                 self.H.pvparams_group_name(self.p, g).decode()) for g in range(n)]
+
+    def g(self, name):
+        return self.H.pvparams_find_group(self.p, name.encode())
+
+    def num(self, group, key):
+        v = C.c_double()
+        return v.value if self.H.pvparams_number(self.p, self.g(group), key.encode(), C.byref(v)) else None
+
+    def str(self, group, key):
+        v = C.c_char_p()
+        if not self.H.pvparams_string(self.p, self.g(group), key.encode(), C.byref(v)):
+            return "<absent>"
+        return None if v.value is None else v.value.decode()
+
+    def arr(self, group, key):
+        buf = (C.c_double * 16)()
+        n = self.H.pvparams_array(self.p, self.g(group), key.encode(), buf, 16)
+        return list(buf[:n])
+
+
+TEXT = r"""
+// a comment line
+debugParsing = true;   // top-level directive
+
+HyPerCol "column" = {
+    nx = 128;  ny = 64;          // two on a line
+    dt = 1.0e0;
+    stopTime = -2.5E1;
+    verifyWrites = true;
+    checkpointWrite = false;
+    outputPath = "out/dir";
+    printParamsFilename = NULL;
+    delay = [0.0, 1.5, 3];
+    empty = [];
+    bound = infinity;
+    lower = -infinity;
+    small = .5;
+};
+
+ANNLayer "A" = {
+    nf = 4;
+    VThresh = 0.1;
+    phase = 2;
+};
+
+ANNLayer "B" = {
+    #include "A";
+    @phase = 3;
+    extra = "x";
+};
+"""
+
+
+def test_values_of_every_kind():
+    P = Params(TEXT)
+    assert P.status == 0, P.error
+    assert P.groups() == [("HyPerCol", "column"), ("ANNLayer", "A"), ("ANNLayer", "B")]
+    assert P.num("column", "nx") == 128 and P.num("column", "ny") == 64
+    assert P.num("column", "dt") == 1.0
+    assert P.num("column", "stopTime") == -25.0
+    assert P.num("column", "verifyWrites") == 1.0 and P.num("column", "checkpointWrite") == 0.0
+    assert P.str("column", "outputPath") == "out/dir"
+    assert P.str("column", "printParamsFilename") is None          # NULL
+    assert P.str("column", "nx") == "<absent>"                     # a number is not a string
+    assert P.arr("column", "delay") == [0.0, 1.5, 3.0]
+    assert P.num("column", "delay") == 0.0                          # first element
+    assert P.arr("column", "empty") == []
+    assert P.arr("column", "nx") == [128.0]                         # a scalar as a 1-array
+    assert P.num("column", "bound") == FMAX and P.num("column", "lower") == -FMAX
+    assert P.num("column", "small") == 0.5
+    assert P.num("column", "absent") is None
+    v = C.c_double()
+    assert P.H.pvparams_directive(P.p, b"debugParsing", C.byref(v)) == 1 and v.value == 1.0
+    assert P.H.pvparams_directive(P.p, b"nx", C.byref(v)) == 0      # group keys are not directives
+
+
+def test_include_copies_and_overwrite_replaces():
+    P = Params(TEXT)
+    assert P.num("B", "nf") == 4 and P.num("B", "VThresh") == 0.1
+    assert P.num("B", "phase") == 3 and P.num("A", "phase") == 2    # the source group is unchanged
+    assert P.str("B", "extra") == "x"
+    assert P.g("missing") == -1
+
+
+@pytest.mark.parametrize("text,msg", [
+    ('G "a" = { x = 1; x = 2; };', "defined twice"),
+    ('G "a" = { @x = 1; };', "overwrites nothing"),
+    ('G "a" = { #include "b"; };', "unknown group"),
+    ('G "a" = { x = 1; }; G "a" = { y = 1; };', "used twice"),
+    ('G "a" = { x = "open; };', "unterminated"),
+    ('G "a" = { x = 1 };', "expected ';'"),
+    ('G "a" = { x = [1, "s"]; };', "number in an array"),
+    ('G "a" = { x = 1; ', "parameter name"),
+    ('ParameterSweep "a" = { x = 1; };', "not supported"),
+    ('G "a" = { x = $; };', "unexpected character"),
+])
+def test_errors_are_reported_with_the_line(text, msg):
+    P = Params("\n" + text)
+    assert P.status != 0
+    assert msg in P.error and "line 2" in P.error, P.error
+
+
+def test_the_column_params_parse():
+    """The committed two-column params file: every group, the includes and overwrites."""
+    H = abi.host()
+    p = C.c_void_p()
+    path = os.path.join(HERE, "golden", "gpulca", "LCAColumn.params")
+    assert H.pvparams_load(path.encode(), C.byref(p)) == 0, H.pvparams_error()
+    try:
+        names = [H.pvparams_group_name(p, g).decode() for g in range(H.pvparams_num_groups(p))]
+        assert "SparseCodeDevice" in names and "DifferenceIsZero" in names
+        g = H.pvparams_find_group(p, b"SparseCodeDevice")
+        s, v = C.c_char_p(), C.c_double()
+        assert H.pvparams_string(p, g, b"adaptiveTimeScaleProbe", C.byref(s)) == 1
+        assert s.value == b"TimeScalesDevice"
+        assert H.pvparams_number(p, g, b"VThresh", C.byref(v)) == 1 and v.value == 0.025
+        g = H.pvparams_find_group(p, b"ResidualToCodeDevice")
+        assert H.pvparams_number(p, g, b"receiveGpu", C.byref(v)) == 1 and v.value == 1.0
+        g = H.pvparams_find_group(p, b"ResidualToCodeHost")
+        assert H.pvparams_number(p, g, b"receiveGpu", C.byref(v)) == 1 and v.value == 0.0
+    finally:
+        H.pvparams_free(p)
+
+
+def test_pvrun_refuses_bad_params_before_touching_a_device(tmp_path):
+    exe = abi.pvrun_path()
+    bad = tmp_path / "bad.params"
+    bad.write_text('HyPerCol "c" = { nx = 1; nx = 2; };\n')
+    r = subprocess.run([exe, "-p", str(bad)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "defined twice" in r.stderr, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([exe, "-p", str(tmp_path / "missing.params")], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "cannot open" in r.stderr
