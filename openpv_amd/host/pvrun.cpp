@@ -27,6 +27,7 @@
 #include "pvparams.h"
 
 #include <algorithm>
+#include <cctype>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -386,8 +387,6 @@ void Net::build() {
    if (dtCol != 1.0) {
       fatal("only dt = 1 is supported");
    }
-   check(pvhip_device_create(0, &D.dev), "pvhip_device_create");
-   D.stream = pvhip_device_stream(D.dev);
    for (int g = 0; g < pvparams_num_groups(P); g++) {
       std::string kw = pvparams_group_keyword(P, g);
       Params q{P, g};
@@ -456,6 +455,28 @@ void Net::build() {
       }
       bool postPersp = C.prm.num("updateGSynFromPostPerspective", 0) != 0;
       bool gpu       = C.prm.num("receiveGpu", 0) != 0;
+      // HyPerDeliveryCreator::ioParam_pvpatchAccumulateType (HyPerDeliveryCreator.cpp:60-100)
+      std::string acc = C.prm.str("pvpatchAccumulateType", "convolve");
+      std::transform(acc.begin(), acc.end(), acc.begin(), [](unsigned char ch) { return (char)std::tolower(ch); });
+      if (acc == "stochastic") {
+         fatal(gpu ? C.name + " sets receiveGpu to true and pvpatchAccumulateType to stochastic, but stochastic "
+                              "release has not been implemented on the GPU"
+                   : C.name + ": stochastic release is not supported");
+      }
+      if (acc != "convolve") {
+         fatal(C.name + ": pvpatchAccumulateType \"" + acc + "\" is unrecognized (\"convolve\" or \"stochastic\")");
+      }
+      if (C.prm.num("convertRateToSpikeCount", 0) != 0) {
+         fatal(C.name + ": convertRateToSpikeCount is not supported");
+      }
+      if (C.prm.num("numAxonalArbors", 1) != 1) {
+         fatal(C.name + ": only one arbor is supported");
+      }
+      if (C.keyword == "HyPerConn" && C.prm.num("sharedWeights", 1) == 0) {
+         fatal(C.name + ": only shared weights are supported");
+      }
+      std::printf("pvrun: %s \"%s\": %ssynapticPerspective%sDelivery\n", C.keyword.c_str(), C.name.c_str(),
+            postPersp ? "Post" : "Pre", gpu ? "GPU" : "Convolve");
       C.arith        = !gpu ? PVHIP_ARITH_EXACT : (gpuArith == "bf16" ? PVHIP_ARITH_BF16 : PVHIP_ARITH_FP32);
       if (C.keyword == "HyPerConn") {
          C.nxp = (int)C.prm.num("nxp", 1);
@@ -488,6 +509,9 @@ void Net::build() {
       int my = pvhost_required_margin(pre.loc.ny, post.loc.ny, C.nyp);
       pre.halo = std::max(pre.halo, std::max(mx, my));
    }
+   // parameters checked: from here on the device is used
+   check(pvhip_device_create(0, &D.dev), "pvhip_device_create");
+   D.stream = pvhip_device_stream(D.dev);
    for (auto &L : layers) {
       L.loc.halo = {L.halo, L.halo, L.halo, L.halo};
       size_t nres = (size_t)nbatch * L.loc.nx * L.loc.ny * L.loc.nf;

@@ -854,3 +854,25 @@ def test_pvrun_bf16_device_column(tmp_path):
     err = np.abs(dev - ref).max()
     print("bf16 device column: max |dev - oracle| %g (scale %g)" % (err, np.abs(ref).max()))
     assert err <= 3e-2 * np.abs(ref).max()
+
+
+def test_cost_sum_firm_threshold_matches_reference_fixture():
+    """pvhip_cost_sum_f64 (firm threshold, VThresh 0.5, VWidth 0.3) over the
+    reference FirmThresholdCostTest inputs placed in an extended buffer whose
+    halo holds junk: per batch element equal to the sum of the reference's
+    expected per-neuron costs (FP64 vs the reference layer's float values)."""
+    from openpv_amd import pvp
+    g = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "firmthresh")
+    _, _, x = pvp.read_activity(os.path.join(g, "firmthreshinput.pvp"))
+    _, _, y = pvp.read_activity(os.path.join(g, "correctoutput.pvp"))
+    h = 2
+    ext = np.full((2, 16 + 2 * h, 16 + 2 * h, 3), 0.7, np.float32)   # junk in the halo
+    ext[:, h:h + 16, h:h + 16, :] = x[:2]
+    loc = abi.Loc.make(2, 16, 16, 3, halo=h)
+    out = R.cost_sum_f64(loc, dev(ext.ravel()), True, 2, 0.5, 0.3)
+    got = out.cpu().numpy()
+    want = y[:2].astype(np.float64).reshape(2, -1).sum(1)
+    np.testing.assert_allclose(got, want, rtol=1e-6)
+    # L0 count above a threshold (kind 3), the all-zero probe's test
+    cnt = R.cost_sum_f64(loc, dev(ext.ravel()), True, 3, 0.5).cpu().numpy()
+    assert list(cnt) == [float((np.abs(x[b]) > 0.5).sum()) for b in range(2)]

@@ -493,3 +493,32 @@ def test_pvp_roundtrip_and_reference_fixtures(tmp_path):
     pvp.write_activity(str(tmp_path / "a.pvp"), fr, [1.0, 2.0, 3.5])
     h, t, d = pvp.read_activity(str(tmp_path / "a.pvp"))
     assert list(t) == [1.0, 2.0, 3.5] and np.array_equal(d, fr)
+
+
+def _firm_fixture():
+    """tests/FirmThresholdCostTest: 16x16x3 inputs (batch 2, displayPeriod 0, so
+    batch element b shows frame b throughout) and FirmThresholdCostLayer's
+    per-neuron cost with VThresh 0.5, VWidth 0.3 (in float), 10 steps."""
+    from openpv_amd import pvp
+    g = os.path.join(HERE, "golden", "firmthresh")
+    _, _, x = pvp.read_activity(os.path.join(g, "firmthreshinput.pvp"))
+    _, t, y = pvp.read_activity(os.path.join(g, "correctoutput.pvp"))
+    return x, t, y
+
+
+def test_firm_threshold_cost_matches_reference_fixture():
+    """The oracle's firm-threshold cost (CostFunctions.hpp:37-50, FP64), element
+    by element and summed, against the reference's own expected output: equal
+    up to the float rounding of the reference layer's evaluation."""
+    x, t, y = _firm_fixture()
+    assert y.shape == (20, 16, 16, 3) and list(t) == [s for s in range(1, 11) for _ in range(2)]
+    one = O.Loc.make(1, 1, 1, 1, 0)
+    for b in range(2):
+        xb = x[b].ravel()
+        cost = np.array([O.cost_sum(one, xb[i:i + 1], False, 0, 2, 0.5, 0.3) for i in range(xb.size)])
+        for step in range(10):
+            np.testing.assert_allclose(cost, y[2 * step + b].ravel(), rtol=0, atol=1e-7)
+    loc = O.Loc.make(2, 16, 16, 3, 0)
+    for b in range(2):
+        total = O.cost_sum(loc, x[:2].ravel(), False, b, 2, 0.5, 0.3)
+        assert abs(total - y[b].astype(np.float64).sum()) <= 1e-6 * total

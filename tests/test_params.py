@@ -169,3 +169,36 @@ def test_pvrun_refuses_bad_params_before_touching_a_device(tmp_path):
     r = subprocess.run([exe, "-p", str(tmp_path / "missing.params")], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+NET = """
+HyPerCol "c" = { nx = 8; ny = 8; nbatch = 1; dt = 1; stopTime = 2; };
+ANNLayer "Pre" = { nxScale = 1; nyScale = 1; nf = 2; phase = 0; };
+ANNLayer "Post" = { nxScale = 1; nyScale = 1; nf = 2; phase = 1; };
+HyPerConn "W" = {
+    preLayerName = "Pre"; postLayerName = "Post"; channelCode = 0;
+    nxp = 3; nyp = 3; nfp = 2;
+    weightInitType = "FileWeight"; initWeightsFile = "w.pvp";
+    %s
+};
+"""
+
+
+@pytest.mark.parametrize("extra,msg", [
+    ('receiveGpu = true; pvpatchAccumulateType = "Stochastic";',
+     "stochastic release has not been implemented on the GPU"),
+    ('receiveGpu = false; pvpatchAccumulateType = "stochastic";', "stochastic release is not supported"),
+    ('pvpatchAccumulateType = "sum";', "is unrecognized"),
+    ('numAxonalArbors = 2;', "one arbor"),
+    ('sharedWeights = false;', "shared weights"),
+    ('delay = [1.0];', "delays are not supported"),
+])
+def test_pvrun_selection_errors_follow_the_delivery_creator(tmp_path, extra, msg):
+    """HyPerDeliveryCreator's fatal cases (and what pvrun leaves out) are reported
+    while the params are read, before any device call."""
+    path = tmp_path / "net.params"
+    path.write_text(NET % extra)
+    r = subprocess.run([abi.pvrun_path(), "-p", str(path), "-o", str(tmp_path / "out")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and msg in r.stderr, r.stderr
+    assert "pvhip_device_create" not in r.stderr
