@@ -876,3 +876,23 @@ def test_cost_sum_firm_threshold_matches_reference_fixture():
     # L0 count above a threshold (kind 3), the all-zero probe's test
     cnt = R.cost_sum_f64(loc, dev(ext.ravel()), True, 3, 0.5).cpu().numpy()
     assert list(cnt) == [float((np.abs(x[b]) > 0.5).sum()) for b in range(2)]
+
+
+@pytest.mark.parametrize("zero_patch", [None, 5])
+def test_normalize_l2_exact_bit_exact(zero_patch):
+    """pvhip_normalize_l2 (the reference's float sum of squares, sqrtf and
+    multiply per data patch) equals the oracle bit for bit on 24 patches of
+    18x18 and on 512 patches of 256, including a degenerate patch that ends
+    the loop."""
+    rng = np.random.default_rng(31)
+    for npatch, psize in ((24, 324), (512, 256)):
+        w = rng.standard_normal((npatch, psize)).astype(np.float32)
+        if zero_patch is not None:
+            w[zero_patch] = 0.0
+        ref = w.copy()
+        O.lib().or_normalize_l2(ref.ravel(), npatch, psize, 0.75, 0.0)
+        d = dev(w.ravel())
+        abi.check(abi.hip().pvhip_normalize_l2(C.c_void_p(d.data_ptr()), npatch, psize, 0.75, 0.0, None),
+                  "pvhip_normalize_l2")
+        torch.cuda.synchronize()
+        assert same(d.cpu().numpy(), ref.ravel())

@@ -355,6 +355,28 @@ def test_normalize_l2():
                        np.sqrt((w.astype(np.float64).reshape(24, 50) ** 2).sum(1))[:, None], atol=1e-6)
 
 
+def test_normalize_l2_degenerate_patch_ends_the_loop():
+    """NormalizeL2.cpp:107-116 (shared weights, arbors together): a patch whose
+    sum of squares is within minL2NormTolerated of zero breaks the patch loop,
+    so that patch and every later one keep their weights; earlier patches are
+    normalised."""
+    rng = np.random.default_rng(21)
+    w = rng.standard_normal((6, 40)).astype(np.float32)
+    w[3] = 0.0
+    got = w.copy()
+    O.lib().or_normalize_l2(got.ravel(), 6, 40, 1.5, 0.0)
+    n = np.sqrt((got[:3].astype(np.float64) ** 2).sum(1))
+    assert np.allclose(n, 1.5, rtol=1e-6)
+    assert np.array_equal(got[3:], w[3:])
+    # a tolerance above a patch's sum of squares acts the same way
+    w2 = rng.standard_normal((4, 40)).astype(np.float32)
+    w2[1] *= 1e-3
+    got2 = w2.copy()
+    O.lib().or_normalize_l2(got2.ravel(), 4, 40, 1.0, 1e-3)
+    assert np.allclose(np.sqrt((got2[0].astype(np.float64) ** 2).sum()), 1.0, rtol=1e-6)
+    assert np.array_equal(got2[1:], w2[1:])
+
+
 def test_lca_loop_descends_energy_and_is_sparse():
     rng = np.random.default_rng(13)
     nb, n, nf, nv, nfv, p = 1, 16, 3, 8, 24, 8
