@@ -80,11 +80,13 @@ def scatter_macs(col):
     return total
 
 
-def time_form(arith, images, W, steps, warmup, torch):
+def time_form(arith, images, W, steps, warmup, torch, tuner=None):
     from openpv_amd.lca import LCAColumn
     col = LCAColumn(images, W, NXV, NYV, NFV, NXP, NXP, VThresh=VTHRESH, timeConstantTau=TAU,
                     dt=DT, arith=arith, fused=True, device="cuda")
     s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    shapes = col.autotune(tuner, stream=s) if tuner is not None else {}
     torch.cuda.synchronize()
     col.capture(s)
     for _ in range(warmup):
@@ -102,7 +104,7 @@ def time_form(arith, images, W, steps, warmup, torch):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1)
-    return col, s, ms / steps, wall
+    return col, s, ms / steps, wall, shapes
 
 
 def kernel_timing(col, s, n, torch):
@@ -254,7 +256,7 @@ def c1_delivery(torch, reps=10):
     return out
 
 
-def c4_hebbian(torch, reps=5):
+def c4_hebbian(torch, reps=5, tuner=None):
     """HebbianUpdater dW (SURVEY C4 shape): V1 8 x 128x128x1024 (5 % active, Exp(1))
     x Residual 8 x 512x512x1 (~N(0,1)), 16x16 patches at stride 4, MFMA forms."""
     import ctypes as C
@@ -281,6 +283,9 @@ def c4_hebbian(torch, reps=5):
            "gflop_per_update": round(flops / 1e9, 1)}
     dws = {}
     for name, ar in (("fp32", abi.ARITH_FP32), ("bf16", abi.ARITH_BF16)):
+        if tuner is not None:
+            shape = tuner.tune_hebbian(hb, a, r, ar, stream=s)
+            out[name + "_rows_per_chunk"] = shape[0] if shape else None
         ms, _ = _time_events(lambda: hb.update_dw(a, r, arith=ar, stream=s), s, reps, torch)
         out[name + "_ms"] = round(ms, 4)
         out[name + "_tflops"] = round(flops / (ms * 1e-3) / 1e12, 1)
@@ -327,6 +332,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--forms", default="exact,fp32,bf16")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch shapes (no autotuner)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the delivery (C1) and Hebbian (C4) measurements")
     ap.add_argument("--profile-copy", type=int, default=0, metavar="MIB",
@@ -343,15 +350,20 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
     abi.hip()
+    from openpv_amd.autotune import LaunchTuner
     images = one_over_f_images(NB, NY, NX, seed=1)
     W = dictionary(NFV, NXP, NF, seed=2)
     forms = [f for f in args.forms.split(",") if f]
     arith = {"exact": abi.ARITH_EXACT, "fp32": abi.ARITH_FP32, "bf16": abi.ARITH_BF16}
     res = {}
     cols = {}
+    # launch shapes measured afresh on this machine (no cached choices)
+    tuner = None if args.no_autotune else LaunchTuner(cache=None)
     for f in forms:
-        col, s, ms, wall = time_form(arith[f], images, W, args.steps, args.warmup, torch)
+        col, s, ms, wall, shapes = time_form(arith[f], images, W, args.steps, args.warmup, torch, tuner)
         res[f] = {"ms_per_step": ms, "steps_per_s": 1000.0 / ms, "wall_s": wall}
+        if shapes:
+            res[f]["launch_shapes"] = shapes
         cols[f] = (col, s)
         print("# %s: %.4f ms/step (%.1f steps/s)" % (f, ms, 1000.0 / ms), file=sys.stderr,
               flush=True)
@@ -411,7 +423,7 @@ def main():
     line["roofline"] = roof
     if not args.no_extras:
         line["delivery_c1"] = c1_delivery(torch)
-        line["hebbian_c4"] = c4_hebbian(torch)
+        line["hebbian_c4"] = c4_hebbian(torch, tuner=tuner)
         line["adjoint_c3"] = c3_adjoint(torch)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(images, W)

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PVHIP_ABI_VERSION 3
+#define PVHIP_ABI_VERSION 4
 
 enum {
    PVHIP_OK              = 0,
@@ -251,6 +251,19 @@ typedef struct {
     * publish of a sparseLayer: Publisher::updateActiveIndices,
     * src/columns/Publisher.cpp:85-134).  NULL: no active list. */
    const struct pvhip_active_out *active_out;
+   /* Optional device scratch for the channel GEMM (FP32/BF16 forms with
+    * several data patches per feature): the channels are split over
+    * workgroups and their FP32 partial sums added in split order.  Size from
+    * pvhip_recv_post_scratch_bytes(args); NULL or smaller runs unsplit.  The
+    * library allocates nothing itself. */
+   void *scratch;
+   size_t scratch_bytes;
+   /* Channel GEMM launch shape, 0 = the library's default: window starts per
+    * workgroup (64, 128 or 256) and channel ranges (1 .. nfp / 32 for BF16,
+    * nfp / 16 for FP32; more than one needs the scratch).  Picked by an
+    * autotuner; results differ only in the FP32 summation order. */
+   int cgemm_tile;
+   int cgemm_splits;
 } pvhip_recv_post_args;
 
 /* Active-list outputs of a fused delivery (same layout and contents as
@@ -271,6 +284,12 @@ typedef struct pvhip_active_out {
 int pvhip_recv_post(const pvhip_recv_post_args *args, pvhip_stream stream);
 /* Scratch bytes pvhip_active_out needs for these arguments (fused_update set). */
 size_t pvhip_recv_post_active_scratch_bytes(const pvhip_recv_post_args *args);
+/* Scratch bytes of pvhip_recv_post_args.scratch for these arguments (0: the
+ * launch is not split).  Depends on the geometry and the form only; the data
+ * pointers may still be NULL.  An extension of the reference interface: the
+ * reference's cuDNN call takes its workspace the same way
+ * (CudaRecvPost.cpp:227-249). */
+size_t pvhip_recv_post_scratch_bytes(const pvhip_recv_post_args *args);
 
 /* Low-precision weight shadow for the BF16 form; call after every weight change. */
 int pvhip_recv_post_bf16_weights(
@@ -416,6 +435,10 @@ typedef struct {
    int window_stride_x, window_start_x, window_stride_y, window_start_y;
    void *scratch;
    size_t scratch_bytes;
+   /* MFMA forms: pre rows (batch x extended y) per split-K chunk, 0 = the
+    * library's default (about 512 workgroups, at least 4 rows).  Picked by an
+    * autotuner; results differ only in the FP32 summation order. */
+   int rows_per_chunk;
 } pvhip_hebbian_args;
 
 int pvhip_hebbian_dw(const pvhip_hebbian_args *args, pvhip_stream stream);

@@ -93,7 +93,8 @@ class RecvPostArgs(C.Structure):
                 ("inverse_x", vp), ("inverse_y", vp), ("window_x0", i32), ("window_nx", i32),
                 ("window_y0", i32), ("window_ny", i32),
                 ("fused_update", C.POINTER(LCAUpdateArgs)),
-                ("active_out", C.POINTER(ActiveOut))]
+                ("active_out", C.POINTER(ActiveOut)), ("scratch", vp), ("scratch_bytes", C.c_size_t),
+                ("cgemm_tile", i32), ("cgemm_splits", i32)]
 
 
 class RecvPreArgs(C.Structure):
@@ -123,7 +124,8 @@ class HebbianArgs(C.Structure):
                 ("apost_x", vp), ("apost_y", vp), ("dw_max", f32), ("pre_data", vp),
                 ("post_data", vp), ("dw", vp), ("counts", vp), ("arith", i32),
                 ("window_stride_x", i32), ("window_start_x", i32), ("window_stride_y", i32),
-                ("window_start_y", i32), ("scratch", vp), ("scratch_bytes", C.c_size_t)]
+                ("window_start_y", i32), ("scratch", vp), ("scratch_bytes", C.c_size_t),
+                ("rows_per_chunk", i32)]
 
 
 # Every symbol include/pvhip.h declares, with its ctypes signature.
@@ -155,6 +157,7 @@ PVHIP_SYMBOLS = {
     "pvhip_lca_exp_tau": (None, [C.POINTER(C.c_double), i32, f32, C.POINTER(f32)]),
     "pvhip_recv_post": (i32, [C.POINTER(RecvPostArgs), vp]),
     "pvhip_recv_post_active_scratch_bytes": (C.c_size_t, [C.POINTER(RecvPostArgs)]),
+    "pvhip_recv_post_scratch_bytes": (C.c_size_t, [C.POINTER(RecvPostArgs)]),
     "pvhip_recv_post_bf16_weights": (i32, [vp, vp, i32, i32, vp]),
     "pvhip_recv_post_bf16_weights_bytes": (C.c_size_t, [i32, i32]),
     "pvhip_recv_pre": (i32, [C.POINTER(RecvPreArgs), vp]),

@@ -330,8 +330,10 @@ HebbPlan hebb_plan(const pvhip_hebbian_args &a, bool bf) {
    int nt     = (N + HNT - 1) / HNT;
    int rows   = pre.nbatch * pre.nyExt();
    // enough workgroups to fill the chip twice, at least 4 rows per chunk
+   // (or the caller's choice)
    int want   = std::max(1, 512 / std::max(1, mt * nt));
-   pl.rowsPerChunk = std::max(4, (rows + want - 1) / want);
+   pl.rowsPerChunk = a.rows_per_chunk > 0 ? std::min(a.rows_per_chunk, rows)
+                                          : std::max(4, (rows + want - 1) / want);
    pl.chunks       = (rows + pl.rowsPerChunk - 1) / pl.rowsPerChunk;
    pl.kxPad        = (pre.nxExt() + 31) / 32 * 32;
    int sx          = std::max(1, a.window_stride_x);
@@ -365,6 +367,7 @@ int pvhip_hebbian_dw(const pvhip_hebbian_args *a, pvhip_stream stream) {
          a->dim_x && a->dim_y && a->pstart_x && a->pstart_y && a->apost_x && a->apost_y,
          "hebbian_dw: NULL geometry table");
    PVHIP_REQUIRE(a->pre_data && a->post_data && a->dw, "hebbian_dw: NULL buffer");
+   PVHIP_REQUIRE(a->rows_per_chunk >= 0, "hebbian_dw: negative rows_per_chunk");
    if (a->arith == PVHIP_ARITH_FP32 || a->arith == PVHIP_ARITH_BF16) {
       bool const bf = a->arith == PVHIP_ARITH_BF16;
       PVHIP_REQUIRE(a->num_kernels_x == 1 && a->num_kernels_y == 1,

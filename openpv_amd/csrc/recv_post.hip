@@ -688,8 +688,8 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
 // T = __bf16 uses v_mfma_f32_16x16x32_bf16 (CK = 32); T = float uses
 // v_mfma_f32_16x16x4f32 (CK = 16, lane group g holds channels 4g..4g+3 and
 // MFMA j of a chunk sums channel 4g+j, the same permutation for A and B).
-constexpr int CNT  = 128; // window starts per workgroup
-constexpr int CNBW = 2;   // 16-start MFMA blocks per wave (4 waves * 2 * 16 = CNT)
+// Window starts per workgroup: CNT = 4 waves * NBW blocks of 16 (NBW = 1, 2,
+// 4: 64, 128 or 256 starts), chosen per launch (default 128).
 constexpr int CSV  = 12;  // source float4 per thread carried across a chunk in registers
 constexpr int CAV  = 4;   // weight 16-byte pieces per thread carried in registers
 
@@ -699,11 +699,15 @@ template <>
 struct CGemm<__bf16> {
    static constexpr int CK = 32;
    static constexpr int PP = 40; // LDS pitch (elements) of one row of CK values
+   static constexpr int KL = 8;  // k values of one lane group in a fragment
+   using Frag              = bf16x8;
 };
 template <>
 struct CGemm<float> {
    static constexpr int CK = 16;
    static constexpr int PP = 20;
+   static constexpr int KL = 4;
+   using Frag              = float4;
 };
 
 struct CStrip {
@@ -712,6 +716,9 @@ struct CStrip {
    int QW, nStrips, tilesPerStrip;
    int rowsMax, twMax;     // staged source tile bound (rows, columns)
    int remap;              // 1: workgroups of one batch share an XCD (grid % 8 == 0)
+   int splits;             // channel ranges (gridDim.y); > 1: partial sums to part
+   float *part;            // [splits][nOut] FP32 partials, or NULL
+   size_t nOut;            // nbatch * post restricted size
 };
 
 __device__ __forceinline__ void store4(__bf16 *d, float4 v) {
@@ -720,9 +727,10 @@ __device__ __forceinline__ void store4(__bf16 *d, float4 v) {
 }
 __device__ __forceinline__ void store4(float *d, float4 v) { *reinterpret_cast<float4 *>(d) = v; }
 
-template <typename T, int MB>
+template <typename T, int MB, int CNBW>
 __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
       PostParams p, const T *Wq, const int32_t *invX, const int32_t *invY, CStrip cs) {
+   constexpr int CNT = 4 * CNBW * 16;
    // Wq: weights in T, layout [M][nyp][nxp][C] (the post weights' own layout;
    // the BF16 shadow of pvhip_recv_post_bf16_weights or the FP32 weights).
    constexpr int CK = CGemm<T>::CK;
@@ -731,7 +739,7 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
    constexpr int EP = 16 / sizeof(T); // elements per 16-byte piece
    extern __shared__ float smem[];
    T *srcl = reinterpret_cast<T *>(smem);           // [rowsMax][twMax][PP]
-   T *wl   = srcl + (size_t)cs.rowsMax * cs.twMax * PP; // [taps][Mp][PP]
+   T *wl   = srcl + (size_t)cs.rowsMax * cs.twMax * PP; // [taps + 1][Mp][PP], the last all zero
 
    int L = blockIdx.x;
    if (cs.remap) {
@@ -765,6 +773,16 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
    float const *preB = p.pre_data + (size_t)b * p.pre.numExt();
    int const srcItems = rows * tw * (CK / 4);
    int const wItems   = taps * Mp * (CK / EP);
+   // the zero weight tap read by the tap loop past the last tap (ordered
+   // before any read by the first chunk's barriers)
+   for (int i = tid; i < Mp * PP; i += 256) {
+      wl[(size_t)taps * Mp * PP + i] = T(0.0f);
+   }
+   // this workgroup's channel range: chunks [ks*chunks/splits, (ks+1)*chunks/splits)
+   int const chunks = C / CK;
+   int const ks     = blockIdx.y;
+   int const cBeg   = (ks * chunks / cs.splits) * CK;
+   int const cEnd   = ((ks + 1) * chunks / cs.splits) * CK;
 
    // LDS offset of each MFMA column (window start) of this lane at tap 0
    int nbase[CNBW];
@@ -862,52 +880,68 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
          acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
       }
    }
-   fetch(0);
-   for (int c0 = 0; c0 < C; c0 += CK) {
+   fetch(cBeg);
+   for (int c0 = cBeg; c0 < cEnd; c0 += CK) {
       __syncthreads(); // every wave is done reading the previous chunk
       stage(c0);
       __syncthreads();
-      if (c0 + CK < C) {
+      if (c0 + CK < cEnd) {
          fetch(c0 + CK); // in flight during this chunk's MFMAs
       }
-      for (int t = 0; t < taps; t++) {
-         int const ky     = t / p.nxp;
-         int const kx     = t - ky * p.nxp;
-         int const tapOff = (ky * cs.twMax + kx) * PP;
-         T const *wt      = wl + t * Mp * PP;
-         if constexpr (CK == 32) {
-            bf16x8 af[MB];
+      // Software-pipelined over the taps, two register sets in turn: the
+      // fragments of the next tap are read from LDS while the MFMAs of the
+      // current one run.  Loads are issued unconditionally (no branch, so each
+      // MFMA waits only for its own set): a tap index past the last one reads
+      // the zero weight tap at index `taps` (adds nothing).  Tap offsets step
+      // (ky, kx) without a division.
+      using Frag = typename CGemm<T>::Frag;
+      Frag afC[MB], bfC[CNBW], afN[MB], bfN[CNBW];
+      auto load_frags = [&](Frag *af, Frag *bfr, int u, int ky, int kx) {
+         bool const real  = u < taps;
+         T const *wt      = wl + (real ? u : taps) * Mp * PP;
+         int const tapOff = real ? (ky * cs.twMax + kx) * PP : 0;
+#pragma unroll
+         for (int mb = 0; mb < MB; mb++) {
+            af[mb] = *reinterpret_cast<const Frag *>(wt + (mb * 16 + r) * PP + CGemm<T>::KL * g);
+         }
+#pragma unroll
+         for (int nb = 0; nb < CNBW; nb++) {
+            bfr[nb] = *reinterpret_cast<const Frag *>(srcl + nbase[nb] + tapOff + CGemm<T>::KL * g);
+         }
+      };
+      auto mfma_tap = [&](const Frag *af, const Frag *bfr) {
+#pragma unroll
+         for (int nb = 0; nb < CNBW; nb++) {
 #pragma unroll
             for (int mb = 0; mb < MB; mb++) {
-               af[mb] = *reinterpret_cast<const bf16x8 *>(wt + (mb * 16 + r) * PP + 8 * g);
-            }
-#pragma unroll
-            for (int nb = 0; nb < CNBW; nb++) {
-               bf16x8 bf = *reinterpret_cast<const bf16x8 *>(srcl + nbase[nb] + tapOff + 8 * g);
-#pragma unroll
-               for (int mb = 0; mb < MB; mb++) {
-                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mb], bf, acc[mb][nb], 0, 0, 0);
+               if constexpr (CK == 32) {
+                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mb], bfr[nb], acc[mb][nb], 0, 0, 0);
+               }
+               else {
+                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].x, bfr[nb].x, acc[mb][nb], 0, 0, 0);
+                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].y, bfr[nb].y, acc[mb][nb], 0, 0, 0);
+                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].z, bfr[nb].z, acc[mb][nb], 0, 0, 0);
+                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].w, bfr[nb].w, acc[mb][nb], 0, 0, 0);
                }
             }
          }
-         else {
-            float4 af[MB];
-#pragma unroll
-            for (int mb = 0; mb < MB; mb++) {
-               af[mb] = *reinterpret_cast<const float4 *>(wt + (mb * 16 + r) * PP + 4 * g);
-            }
-#pragma unroll
-            for (int nb = 0; nb < CNBW; nb++) {
-               float4 bv = *reinterpret_cast<const float4 *>(srcl + nbase[nb] + tapOff + 4 * g);
-#pragma unroll
-               for (int mb = 0; mb < MB; mb++) {
-                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].x, bv.x, acc[mb][nb], 0, 0, 0);
-                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].y, bv.y, acc[mb][nb], 0, 0, 0);
-                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].z, bv.z, acc[mb][nb], 0, 0, 0);
-                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].w, bv.w, acc[mb][nb], 0, 0, 0);
-               }
-            }
+      };
+      int ky = 0, kx = 0; // tap of the next load
+      auto advance = [&]() {
+         if (++kx == p.nxp) {
+            kx = 0;
+            ky++;
          }
+      };
+      load_frags(afC, bfC, 0, 0, 0);
+      advance();
+      for (int t = 0; t < taps; t += 2) {
+         load_frags(afN, bfN, t + 1, ky, kx);
+         advance();
+         mfma_tap(afC, bfC);
+         load_frags(afC, bfC, t + 2, ky, kx);
+         advance();
+         mfma_tap(afN, bfN);
       }
    }
    // Lane holds D[4g+i][r]: kernel mb*16+4g+i at window start n of block nb.
@@ -930,13 +964,57 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
                int x = invX[kx * cs.nqx + qx];
                int y = invY[ky * cs.nqy + qy];
                if (x >= 0 && y >= 0) {
-                  size_t gi  = (size_t)b * p.post.numRes() + ((size_t)y * p.post.nx + x) * nf + F;
-                  float base = p.accumulate ? p.gsyn[gi] : 0.0f;
-                  p.gsyn[gi] = base + p.dtf * acc[mb][nb][i];
+                  size_t gi = (size_t)b * p.post.numRes() + ((size_t)y * p.post.nx + x) * nf + F;
+                  if (cs.part) {
+                     cs.part[(size_t)ks * cs.nOut + gi] = acc[mb][nb][i];
+                  }
+                  else {
+                     float base = p.accumulate ? p.gsyn[gi] : 0.0f;
+                     p.gsyn[gi] = base + p.dtf * acc[mb][nb][i];
+                  }
                }
             }
          }
       }
+   }
+}
+
+// Split channel GEMM: GSyn (+)= dtf * sum of the split partials, added in
+// split order (deterministic).  VEC: n % 4 == 0, one float4 per thread.
+template <bool VEC>
+__global__ void __launch_bounds__(256) cgemm_reduce_kernel(
+      const float *part, int splits, size_t n, float *gsyn, int accumulate, float dtf) {
+   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+   if constexpr (VEC) {
+      if (i >= n / 4) {
+         return;
+      }
+      float4 acc = reinterpret_cast<const float4 *>(part)[i];
+      for (int k = 1; k < splits; k++) {
+         float4 v = reinterpret_cast<const float4 *>(part + (size_t)k * n)[i];
+         acc.x += v.x;
+         acc.y += v.y;
+         acc.z += v.z;
+         acc.w += v.w;
+      }
+      float4 base = accumulate ? reinterpret_cast<const float4 *>(gsyn)[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      float4 out;
+      out.x = base.x + dtf * acc.x;
+      out.y = base.y + dtf * acc.y;
+      out.z = base.z + dtf * acc.z;
+      out.w = base.w + dtf * acc.w;
+      reinterpret_cast<float4 *>(gsyn)[i] = out;
+   }
+   else {
+      if (i >= n) {
+         return;
+      }
+      float acc = part[i];
+      for (int k = 1; k < splits; k++) {
+         acc += part[(size_t)k * n + i];
+      }
+      float base = accumulate ? gsyn[i] : 0.0f;
+      gsyn[i]    = base + dtf * acc;
    }
 }
 
@@ -1014,12 +1092,128 @@ size_t segment_scratch_bytes(const pvhip_loc &l) {
 
 int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_t s);
 
+// Channel GEMM launch shape: strips, LDS, grid, and the channel split that
+// gives the launch about CGEMM_WG_TARGET workgroups (a workgroup runs its
+// channel chunks one after another, each staged through LDS, so a short
+// grid leaves the chip waiting on those loads).
+constexpr long long CGEMM_WG_TARGET = 2048;
+
+struct CGemmPlan {
+   CStrip cs;
+   size_t lds;
+   long long blocks; // workgroups per channel range
+   int MB;
+   int NBW; // 16-start MFMA blocks per wave: CNT = 64 * NBW window starts per workgroup
+   bool bf;
+};
+
+int cgemm_plan(const pvhip_recv_post_args *args, const PostParams &p, CGemmPlan *pl) {
+   PVHIP_REQUIRE(args->inverse_x && args->inverse_y && args->window_nx > 0 && args->window_ny > 0,
+         "recv_post: targets larger than the source need inverse tables (pvhost_recv_post_inverse)");
+   PVHIP_REQUIRE(!p.fused, "recv_post: fused update needs one data patch per feature");
+   bool bf = args->arith == PVHIP_ARITH_BF16;
+   int CK  = bf ? 32 : 16;
+   PVHIP_REQUIRE(p.nfp % CK == 0, "recv_post: MFMA path with %d data patches needs nfp %% %d == 0",
+         p.nKx * p.nKy, CK);
+   int M  = p.nKx * p.nKy * p.post.nf;
+   int MB = (M + 15) / 16;
+   PVHIP_REQUIRE(MB <= 4, "recv_post: %d kernels exceed the MFMA tile (64)", M);
+   MB = MB <= 1 ? 1 : (MB <= 2 ? 2 : 4);
+   if (bf) {
+      PVHIP_REQUIRE(args->weights_lp != nullptr,
+            "recv_post: BF16 channel GEMM needs weights_lp (pvhip_recv_post_bf16_weights)");
+   }
+   int const tile = args->cgemm_tile;
+   PVHIP_REQUIRE(tile == 0 || tile == 64 || tile == 128 || tile == 256,
+         "recv_post: cgemm_tile %d is not 0, 64, 128 or 256", tile);
+   int const NBW = tile == 0 ? 2 : tile / 64;
+   int const CNT = 64 * NBW;
+   CStrip cs;
+   cs.qx0 = args->window_x0;
+   cs.qy0 = args->window_y0;
+   cs.nqx = args->window_nx;
+   cs.nqy = args->window_ny;
+   cs.M   = M;
+   size_t const esz  = bf ? 2 : 4;
+   int const PP      = bf ? CGemm<__bf16>::PP : CGemm<float>::PP;
+   size_t const wlds = ((size_t)p.nyp * p.nxp + 1) * MB * 16 * PP * esz; // + the zero tap
+   size_t lds        = 0;
+   // widest strip (at most 64 starts) whose staged tile fits the LDS budget
+   for (int strips = (cs.nqx + 63) / 64;; strips++) {
+      cs.QW      = (cs.nqx + strips - 1) / strips;
+      cs.nStrips = (cs.nqx + cs.QW - 1) / cs.QW;
+      // a tile of CNT starts in a strip of width w spans <= (w + CNT - 2) / w + 1
+      // start rows; the narrowest strip (the last, width >= 1) gives the bound
+      int const wmin = cs.nqx - (cs.nStrips - 1) * cs.QW;
+      cs.rowsMax = (wmin + CNT - 2) / wmin + p.nyp;
+      cs.twMax   = cs.QW + p.nxp - 1;
+      lds        = (size_t)cs.rowsMax * cs.twMax * PP * esz + wlds;
+      if (lds <= 96 * 1024 || cs.QW == 1) {
+         break;
+      }
+   }
+   PVHIP_REQUIRE(lds <= 160 * 1024, "recv_post: channel GEMM tile needs %zu bytes of LDS", lds);
+   cs.tilesPerStrip = (cs.QW * cs.nqy + CNT - 1) / CNT;
+   long long blocks = (long long)p.post.nbatch * cs.nStrips * cs.tilesPerStrip;
+   PVHIP_REQUIRE(blocks < (1LL << 31), "recv_post: channel GEMM grid too large");
+   cs.remap     = blocks % 8 == 0 ? 1 : 0;
+   int const chunks = p.nfp / CK;
+   PVHIP_REQUIRE(args->cgemm_splits >= 0, "recv_post: negative cgemm_splits");
+   if (args->cgemm_splits > 0) {
+      cs.splits = std::min(args->cgemm_splits, std::min(chunks, 65535));
+   }
+   else if (bf) {
+      cs.splits = 1; // measured: the BF16 chunks do not gain from a split
+   }
+   else {
+      // ranges of at least 8 chunks, enough of them for about CGEMM_WG_TARGET workgroups
+      long long sp = (CGEMM_WG_TARGET + blocks - 1) / blocks;
+      cs.splits    = (int)std::min<long long>(std::max<long long>(sp, 1), std::max(chunks / 8, 1));
+   }
+   cs.part      = nullptr;
+   cs.nOut      = 0;
+   pl->cs       = cs;
+   pl->lds      = lds;
+   pl->blocks   = blocks;
+   pl->MB       = MB;
+   pl->NBW      = NBW;
+   pl->bf       = bf;
+   return PVHIP_OK;
+}
+
+size_t cgemm_scratch_bytes(const PostParams &p, const CGemmPlan &pl) {
+   return pl.cs.splits > 1 ? (size_t)pl.cs.splits * p.post.nbatch * p.post.numRes() * sizeof(float) : 0;
+}
+
 } // namespace
 } // namespace pvhip
 
 using namespace pvhip;
 
 extern "C" {
+
+size_t pvhip_recv_post_scratch_bytes(const pvhip_recv_post_args *args) {
+   if (!args || (args->arith != PVHIP_ARITH_BF16 && args->arith != PVHIP_ARITH_FP32)) {
+      return 0;
+   }
+   // the size depends on the geometry and the form only: data pointers may be
+   // NULL here, and stand-ins (never dereferenced) let the checks pass
+   static float unused;
+   pvhip_recv_post_args a = *args;
+   a.pre_data             = a.pre_data ? a.pre_data : &unused;
+   a.weights              = a.weights ? a.weights : &unused;
+   a.weights_lp           = a.weights_lp ? a.weights_lp : &unused;
+   a.post_gsyn            = a.post_gsyn ? a.post_gsyn : &unused;
+   PostParams p;
+   if (build_params(a, &p) != PVHIP_OK || (p.nKx == 1 && p.nKy == 1)) {
+      return 0;
+   }
+   CGemmPlan pl;
+   if (cgemm_plan(&a, p, &pl) != PVHIP_OK) {
+      return 0;
+   }
+   return cgemm_scratch_bytes(p, pl);
+}
 
 size_t pvhip_recv_post_active_scratch_bytes(const pvhip_recv_post_args *args) {
    if (!args || !args->fused_update || !loc_valid(args->fused_update->loc)) {
@@ -1090,67 +1284,54 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
 
    if ((arith == PVHIP_ARITH_BF16 || arith == PVHIP_ARITH_FP32) && !single) {
       // Several data patches per feature: channel GEMM with inverse tables.
-      PVHIP_REQUIRE(args->inverse_x && args->inverse_y && args->window_nx > 0 && args->window_ny > 0,
-            "recv_post: targets larger than the source need inverse tables (pvhost_recv_post_inverse)");
-      PVHIP_REQUIRE(!p.fused, "recv_post: fused update needs one data patch per feature");
-      bool bf = arith == PVHIP_ARITH_BF16;
-      int CK  = bf ? 32 : 16;
-      PVHIP_REQUIRE(p.nfp % CK == 0, "recv_post: MFMA path with %d data patches needs nfp %% %d == 0",
-            p.nKx * p.nKy, CK);
-      int M  = p.nKx * p.nKy * p.post.nf;
-      int MB = (M + 15) / 16;
-      PVHIP_REQUIRE(MB <= 4, "recv_post: %d kernels exceed the MFMA tile (64)", M);
-      MB = MB <= 1 ? 1 : (MB <= 2 ? 2 : 4);
-      if (bf) {
-         PVHIP_REQUIRE(args->weights_lp != nullptr,
-               "recv_post: BF16 channel GEMM needs weights_lp (pvhip_recv_post_bf16_weights)");
+      CGemmPlan pl;
+      int st = cgemm_plan(args, p, &pl);
+      if (st) {
+         return st;
       }
-      const void *Wq = bf ? args->weights_lp : (const void *)args->weights;
-      CStrip cs;
-      cs.qx0 = args->window_x0;
-      cs.qy0 = args->window_y0;
-      cs.nqx = args->window_nx;
-      cs.nqy = args->window_ny;
-      cs.M   = M;
-      size_t const esz = bf ? 2 : 4;
-      int const PP     = bf ? CGemm<__bf16>::PP : CGemm<float>::PP;
-      size_t const wlds = (size_t)p.nyp * p.nxp * MB * 16 * PP * esz;
-      size_t lds        = 0;
-      // widest strip (at most 64 starts) whose staged tile fits the LDS budget
-      for (int strips = (cs.nqx + 63) / 64;; strips++) {
-         cs.QW      = (cs.nqx + strips - 1) / strips;
-         cs.nStrips = (cs.nqx + cs.QW - 1) / cs.QW;
-         // a tile of CNT starts in a strip of width w spans <= (w + CNT - 2) / w + 1
-         // start rows; the narrowest strip (the last, width >= 1) gives the bound
-         int const wmin = cs.nqx - (cs.nStrips - 1) * cs.QW;
-         cs.rowsMax = (wmin + CNT - 2) / wmin + p.nyp;
-         cs.twMax   = cs.QW + p.nxp - 1;
-         lds        = (size_t)cs.rowsMax * cs.twMax * PP * esz + wlds;
-         if (lds <= 96 * 1024 || cs.QW == 1) {
-            break;
-         }
+      if (pl.cs.splits > 1 && (!args->scratch || args->scratch_bytes < cgemm_scratch_bytes(p, pl))) {
+         pl.cs.splits = 1; // no room for the partials: one range of channels
       }
-      PVHIP_REQUIRE(lds <= 160 * 1024, "recv_post: channel GEMM tile needs %zu bytes of LDS", lds);
-      cs.tilesPerStrip  = (cs.QW * cs.nqy + CNT - 1) / CNT;
-      long long blocks  = (long long)p.post.nbatch * cs.nStrips * cs.tilesPerStrip;
-      PVHIP_REQUIRE(blocks < (1LL << 31), "recv_post: channel GEMM grid too large");
-      cs.remap = blocks % 8 == 0 ? 1 : 0;
-      dim3 grid((unsigned)blocks);
+      pl.cs.part = pl.cs.splits > 1 ? static_cast<float *>(args->scratch) : nullptr;
+      pl.cs.nOut = (size_t)p.post.nbatch * p.post.numRes();
+      const void *Wq = pl.bf ? args->weights_lp : (const void *)args->weights;
+      dim3 grid((unsigned)pl.blocks, (unsigned)pl.cs.splits);
+#define PVHIP_CGEMM3(T, MBV, NB)                                                                  \
+   hipLaunchKernelGGL((recv_post_cgemm_kernel<T, MBV, NB>), grid, dim3(256), pl.lds, s, p,        \
+         reinterpret_cast<const T *>(Wq), args->inverse_x, args->inverse_y, pl.cs)
 #define PVHIP_CGEMM(T, MBV)                                                                       \
-   hipLaunchKernelGGL((recv_post_cgemm_kernel<T, MBV>), grid, dim3(256), lds, s, p,              \
-         reinterpret_cast<const T *>(Wq), args->inverse_x, args->inverse_y, cs)
-      if (bf) {
-         if (MB == 1) PVHIP_CGEMM(__bf16, 1);
-         else if (MB == 2) PVHIP_CGEMM(__bf16, 2);
+   do {                                                                                          \
+      if (pl.NBW == 1) PVHIP_CGEMM3(T, MBV, 1);                                                  \
+      else if (pl.NBW == 2) PVHIP_CGEMM3(T, MBV, 2);                                             \
+      else PVHIP_CGEMM3(T, MBV, 4);                                                              \
+   } while (0)
+      if (pl.bf) {
+         if (pl.MB == 1) PVHIP_CGEMM(__bf16, 1);
+         else if (pl.MB == 2) PVHIP_CGEMM(__bf16, 2);
          else PVHIP_CGEMM(__bf16, 4);
       }
       else {
-         if (MB == 1) PVHIP_CGEMM(float, 1);
-         else if (MB == 2) PVHIP_CGEMM(float, 2);
+         if (pl.MB == 1) PVHIP_CGEMM(float, 1);
+         else if (pl.MB == 2) PVHIP_CGEMM(float, 2);
          else PVHIP_CGEMM(float, 4);
       }
+#undef PVHIP_CGEMM3
 #undef PVHIP_CGEMM
       PVHIP_LAUNCHED("recv_post_cgemm");
+      if (pl.cs.splits > 1) {
+         size_t n     = pl.cs.nOut;
+         bool aligned = (reinterpret_cast<uintptr_t>(p.gsyn) % 16 == 0)
+                        && (reinterpret_cast<uintptr_t>(pl.cs.part) % 16 == 0);
+         if (n % 4 == 0 && aligned) {
+            hipLaunchKernelGGL(cgemm_reduce_kernel<true>, dim3((unsigned)((n / 4 + 255) / 256)), dim3(256), 0, s,
+                  pl.cs.part, pl.cs.splits, n, p.gsyn, p.accumulate, p.dtf);
+         }
+         else {
+            hipLaunchKernelGGL(cgemm_reduce_kernel<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                  pl.cs.part, pl.cs.splits, n, p.gsyn, p.accumulate, p.dtf);
+         }
+         PVHIP_LAUNCHED("recv_post_cgemm_reduce");
+      }
       return PVHIP_OK;
    }
 

@@ -2,7 +2,7 @@
 // on the GPU through libpvhip (the drop-in for OpenPV's run of the same file
 // for the delivery + LCA path).
 //
-//   pvrun -p file.params [-o outputPath] [--steps N]
+//   pvrun -p file.params [-o outputPath] [--steps N] [--autotune]
 //
 // Supported objects (the LCA column of tests/GPULCATest and LCATest):
 //   HyPerCol; PvpLayer, ANNErrorLayer, HyPerLCALayer, ANNLayer, HyPerLayer;
@@ -267,6 +267,9 @@ struct Conn {
    int32_t *tab[8] = {};
    int32_t *inv[2] = {};
    int win[4] = {};
+   void *scratch = nullptr; // channel-GEMM split partials (pvhip_recv_post_scratch_bytes)
+   size_t scratchBytes = 0;
+   int tile = 0, splits = 0; // channel-GEMM launch shape (0: the library's default)
 };
 
 struct Probe {
@@ -328,6 +331,8 @@ struct Net {
    bool transposedRoute(Conn &c);
    void buildConn(Conn &c);
    void step(double t);
+   pvhip_recv_post_args gatherArgs(Conn &c);
+   void autotune();
    void deliver(Conn &c);
    void receive(Layer &L);
    void updateLayer(Layer &L, double t);
@@ -753,6 +758,11 @@ void Net::buildConn(Conn &C) {
          C.Wlp = D.alloc(pvhip_recv_post_bf16_weights_bytes(nK, psize));
          check(pvhip_recv_post_bf16_weights(C.W, C.Wlp, nK, psize, D.stream), "bf16 weights");
       }
+      pvhip_recv_post_args a = gatherArgs(C);
+      C.scratchBytes         = pvhip_recv_post_scratch_bytes(&a);
+      if (C.scratchBytes > 0) {
+         C.scratch = D.alloc(C.scratchBytes);
+      }
    }
    else {
       if (C.keyword == "TransposeConn") {
@@ -788,6 +798,100 @@ void Net::buildConn(Conn &C) {
    }
 }
 
+pvhip_recv_post_args Net::gatherArgs(Conn &C) {
+   Layer &pre = layers[C.pre], &post = layers[C.post];
+   size_t nres = (size_t)nbatch * post.loc.nx * post.loc.ny * post.loc.nf;
+   pvhip_recv_post_args a{};
+   a.pre           = pre.loc;
+   a.post          = post.loc;
+   a.nxp           = C.nxq;
+   a.nyp           = C.nyq;
+   a.nfp           = pre.loc.nf;
+   a.num_kernels_x = C.nkx;
+   a.num_kernels_y = C.nky;
+   a.start_x       = C.tab[0];
+   a.start_y       = C.tab[1];
+   a.kernel_x      = C.tab[2];
+   a.kernel_y      = C.tab[3];
+   a.dt_factor     = 1.0f;
+   a.pre_data      = pre.A;
+   a.weights       = C.W;
+   a.post_gsyn     = post.gsyn ? post.gsyn + (size_t)C.channel * nres : nullptr;
+   a.accumulate    = 1;
+   a.arith         = C.arith;
+   a.weights_lp    = C.Wlp;
+   a.inverse_x     = C.inv[0];
+   a.inverse_y     = C.inv[1];
+   a.window_x0     = C.win[0];
+   a.window_nx     = C.win[1];
+   a.window_y0     = C.win[2];
+   a.window_ny     = C.win[3];
+   a.scratch       = C.scratch;
+   a.scratch_bytes = C.scratchBytes;
+   a.cgemm_tile    = C.tile;
+   a.cgemm_splits  = C.splits;
+   return a;
+}
+
+// --autotune: the channel GEMM's launch shape (window starts per workgroup,
+// channel ranges) of every FP32/BF16 gather with several data patches per
+// feature, timed with HIP events on the column's own buffers (median of 5
+// launches after 2); the fastest is kept.  Deliveries write only GSyn, which
+// every step resets.
+void Net::autotune() {
+   pvhip_event *e0 = nullptr, *e1 = nullptr;
+   check(pvhip_event_create(&e0), "pvhip_event_create");
+   check(pvhip_event_create(&e1), "pvhip_event_create");
+   for (auto &C : conns) {
+      if (C.mode != C_GATHER || !C.inv[0] || C.arith == PVHIP_ARITH_EXACT) {
+         continue;
+      }
+      int chunks   = layers[C.pre].loc.nf / (C.arith == PVHIP_ARITH_BF16 ? 32 : 16);
+      float bestMs = 0.0f;
+      int bestT = 0, bestK = 0;
+      for (int tile : {64, 128, 256}) {
+         for (int k : {1, 2, 4, 8}) {
+            if (k > chunks) {
+               continue;
+            }
+            C.tile                 = tile;
+            C.splits               = k;
+            pvhip_recv_post_args a = gatherArgs(C);
+            size_t need            = pvhip_recv_post_scratch_bytes(&a);
+            if (need > C.scratchBytes) {
+               C.scratch      = D.alloc(need);
+               C.scratchBytes = need;
+               a              = gatherArgs(C);
+            }
+            std::vector<float> ms;
+            for (int rep = 0; rep < 7; rep++) {
+               check(pvhip_event_record(e0, D.stream), "pvhip_event_record");
+               check(pvhip_recv_post(&a, D.stream), "pvhip_recv_post");
+               check(pvhip_event_record(e1, D.stream), "pvhip_event_record");
+               check(pvhip_stream_sync(D.stream), "pvhip_stream_sync");
+               float t = 0.0f;
+               check(pvhip_event_elapsed_ms(e0, e1, &t), "pvhip_event_elapsed_ms");
+               if (rep >= 2) {
+                  ms.push_back(t);
+               }
+            }
+            std::sort(ms.begin(), ms.end());
+            float med = ms[ms.size() / 2];
+            if (bestT == 0 || med < bestMs) {
+               bestMs = med;
+               bestT  = tile;
+               bestK  = k;
+            }
+         }
+      }
+      C.tile   = bestT;
+      C.splits = bestK;
+      std::printf("pvrun: %s: channel GEMM tile %d, splits %d (%.4f ms)\n", C.name.c_str(), bestT, bestK, bestMs);
+   }
+   pvhip_event_destroy(e0);
+   pvhip_event_destroy(e1);
+}
+
 void Net::deliver(Conn &C) {
    Layer &pre = layers[C.pre], &post = layers[C.post];
    size_t nres = (size_t)nbatch * post.loc.nx * post.loc.ny * post.loc.nf;
@@ -796,31 +900,7 @@ void Net::deliver(Conn &C) {
       check(pvhip_ident_deliver(&pre.loc, pre.A, g, 1, D.stream), "pvhip_ident_deliver");
    }
    else if (C.mode == C_GATHER) {
-      pvhip_recv_post_args a{};
-      a.pre = pre.loc;
-      a.post = post.loc;
-      a.nxp = C.nxq;
-      a.nyp = C.nyq;
-      a.nfp = pre.loc.nf;
-      a.num_kernels_x = C.nkx;
-      a.num_kernels_y = C.nky;
-      a.start_x = C.tab[0];
-      a.start_y = C.tab[1];
-      a.kernel_x = C.tab[2];
-      a.kernel_y = C.tab[3];
-      a.dt_factor = 1.0f;
-      a.pre_data = pre.A;
-      a.weights = C.W;
-      a.post_gsyn = g;
-      a.accumulate = 1;
-      a.arith = C.arith;
-      a.weights_lp = C.Wlp;
-      a.inverse_x = C.inv[0];
-      a.inverse_y = C.inv[1];
-      a.window_x0 = C.win[0];
-      a.window_nx = C.win[1];
-      a.window_y0 = C.win[2];
-      a.window_ny = C.win[3];
+      pvhip_recv_post_args a = gatherArgs(C);
       check(pvhip_recv_post(&a, D.stream), "pvhip_recv_post");
    }
    else if (C.mode == C_SCATTER) {
@@ -1134,6 +1214,7 @@ int Net::finish() {
 int main(int argc, char **argv) {
    std::string paramsPath, outDir;
    long steps = -1;
+   bool tune  = false;
    for (int i = 1; i < argc; i++) {
       std::string a = argv[i];
       if (a == "-p" && i + 1 < argc) {
@@ -1145,13 +1226,16 @@ int main(int argc, char **argv) {
       else if (a == "--steps" && i + 1 < argc) {
          steps = std::atol(argv[++i]);
       }
+      else if (a == "--autotune") {
+         tune = true;
+      }
       else {
-         std::fprintf(stderr, "usage: pvrun -p file.params [-o outputPath] [--steps N]\n");
+         std::fprintf(stderr, "usage: pvrun -p file.params [-o outputPath] [--steps N] [--autotune]\n");
          return 2;
       }
    }
    if (paramsPath.empty()) {
-      std::fprintf(stderr, "usage: pvrun -p file.params [-o outputPath] [--steps N]\n");
+      std::fprintf(stderr, "usage: pvrun -p file.params [-o outputPath] [--steps N] [--autotune]\n");
       return 2;
    }
    pvparams *P = nullptr;
@@ -1165,6 +1249,9 @@ int main(int argc, char **argv) {
       net.P          = P;
       net.outputPath = outDir;
       net.build();
+      if (tune) {
+         net.autotune();
+      }
       long n = steps >= 0 ? steps : (long)std::lround(net.stopTime / net.dtCol);
       for (long s = 1; s <= n; s++) {
          net.step((double)s * net.dtCol);

@@ -119,6 +119,19 @@ class LCAColumn:
         ann_activity(self.v1_loc, self.v1_transfer, self.v1_V, self.v1_A)
         self.active.compact(self.v1_A)
 
+    def autotune(self, tuner, stream=None):
+        """Launch shapes of the column's tunable kernels (the FP32/BF16 scatter's
+        channel GEMM), measured on the column's own buffers by an
+        autotune.LaunchTuner; call before capture().  Returns {kernel: shape}."""
+        out = {}
+        shape = tuner.tune_recv_post(self.scatter, self.v1_A, self.rec_gsyn, stream=stream)
+        if shape is not None:
+            out["scatter_cgemm(tile,splits)"] = list(shape)
+        shape = tuner.tune_recv_post(self.gather, self.res_A, self.v1_gsyn, stream=stream)
+        if shape is not None:
+            out["gather_cgemm(tile,splits)"] = list(shape)
+        return out
+
     # one timestep ---------------------------------------------------------
     def step(self, stream=None, gather_events=None):
         """One timestep.  gather_events=(start, stop) torch events are recorded on

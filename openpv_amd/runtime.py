@@ -71,7 +71,10 @@ class PostsynapticPerspectiveGPUDelivery:
     patch; weights: device tensor [kernels][nyp][nxp][source.nf].
     """
 
-    def __init__(self, source, target, nxp, nyp, weights, arith=abi.ARITH_EXACT, dt_factor=1.0):
+    def __init__(self, source, target, nxp, nyp, weights, arith=abi.ARITH_EXACT, dt_factor=1.0,
+                 split=True):
+        """split: give the channel GEMM (several data patches per feature) its
+        scratch, so the channels are split over workgroups."""
         self.source, self.target = source, target
         self.nxp, self.nyp, self.nfp = nxp, nyp, source.nf
         self.arith = arith
@@ -119,6 +122,25 @@ class PostsynapticPerspectiveGPUDelivery:
             self.windows = (x0.value, nx_.value, y0.value, ny_.value)
         if arith == abi.ARITH_BF16:
             self.refresh_weights()
+        self.scratch = None
+        self.split = bool(split)
+        self.cgemm_tile, self.cgemm_splits = 0, 0   # channel GEMM launch shape, 0 = default
+        self._alloc_scratch()
+
+    def _alloc_scratch(self):
+        """(Re)allocate the channel GEMM's split scratch for the current launch
+        shape; call outside graph capture."""
+        if not self.split or self.inverse is None:
+            return
+        n = abi.hip().pvhip_recv_post_scratch_bytes(C.byref(self.args(None, None)))
+        if n and (self.scratch is None or self.scratch.numel() * 4 < n):
+            self.scratch = torch.empty((n + 3) // 4, dtype=torch.float32, device=self.weights.device)
+
+    def set_launch(self, cgemm_tile=0, cgemm_splits=0):
+        """Channel GEMM launch shape (window starts per workgroup, channel
+        ranges); 0 keeps the library's default.  Used by the autotuner."""
+        self.cgemm_tile, self.cgemm_splits = int(cgemm_tile), int(cgemm_splits)
+        self._alloc_scratch()
 
     def refresh_weights(self, stream=None):
         """Re-derive the low-precision shadow after a weight change."""
@@ -148,6 +170,10 @@ class PostsynapticPerspectiveGPUDelivery:
         if self.inverse is not None:
             a.inverse_x, a.inverse_y = _ptr(self.inverse[0]), _ptr(self.inverse[1])
             a.window_x0, a.window_nx, a.window_y0, a.window_ny = self.windows
+        if getattr(self, "scratch", None) is not None:
+            a.scratch, a.scratch_bytes = _ptr(self.scratch), self.scratch.numel() * 4
+        a.cgemm_tile = getattr(self, "cgemm_tile", 0)
+        a.cgemm_splits = getattr(self, "cgemm_splits", 0)
         if fused is not None:
             a.fused_update = C.pointer(fused)
         if active is not None:
@@ -435,6 +461,7 @@ class HebbianUpdater:
         r = H.pvhost_hebbian_windows(C.byref(pre), C.byref(post), nxp, nyp, *[C.byref(v) for v in w])
         self.windows = tuple(v.value for v in w) if r == 0 and self.nkx * self.nky == 1 else None
         self._scratch = None
+        self.rows_per_chunk = 0   # MFMA split-K chunk rows, 0 = the library's default (autotuner)
 
     def update_dw(self, pre_activity, post_activity, clear=True, arith=abi.ARITH_EXACT, stream=None):
         if clear:
@@ -451,6 +478,7 @@ class HebbianUpdater:
         a.dw = _ptr(self.dw)
         a.counts = _ptr(self.counts)
         a.arith = arith
+        a.rows_per_chunk = self.rows_per_chunk
         if arith in (abi.ARITH_FP32, abi.ARITH_BF16):
             if self.windows is None:
                 raise abi.PVHIPError("the MFMA forms of the Hebbian update need one data patch per "

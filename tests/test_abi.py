@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol(header, loader, table):
 
 
 def test_abi_version_and_error_string():
-    assert abi.hip().pvhip_abi_version() == 3
+    assert abi.hip().pvhip_abi_version() == 4
     assert isinstance(abi.hip().pvhip_last_error(), bytes)
 
 
@@ -45,6 +45,43 @@ def test_invalid_arguments_are_rejected_without_a_device():
     assert H.pvhip_recv_pre(C.byref(args), None) == 1  # invalid loc
     bad = abi.Loc.make(0, 4, 4, 1)
     assert H.pvhip_active_compact_scratch_bytes(C.byref(bad)) == 0
+
+
+def test_channel_gemm_scratch_query_follows_the_launch_shape():
+    """pvhip_recv_post_scratch_bytes (host logic only): the stride-4 transposed
+    gather (32x32x512 -> 128x128x1) needs splits x GSyn floats of scratch:
+    FP32 by default splits into ranges of >= 8 of its 32 chunks (4), BF16 does
+    not split by default, an explicit cgemm_splits is honoured (clamped to the
+    chunk count), EXACT has no channel GEMM, and a bad tile is refused."""
+    H = abi.hip()
+    stub = np.zeros(64, np.int32)          # table pointers: never read by the query
+    P = stub.ctypes.data_as(C.c_void_p)
+    a = abi.RecvPostArgs()
+    a.pre, a.post = abi.Loc.make(1, 32, 32, 512, 2), abi.Loc.make(1, 128, 128, 1, 0)
+    a.nxp, a.nyp, a.nfp = 4, 4, 512
+    a.num_kernels_x, a.num_kernels_y = 4, 4
+    a.start_x = a.start_y = a.kernel_x = a.kernel_y = P
+    a.inverse_x = a.inverse_y = P
+    a.window_x0, a.window_nx, a.window_y0, a.window_ny = 0, 33, 0, 33
+    a.dt_factor = 1.0
+    gsyn = 128 * 128 * 4
+    q = lambda: H.pvhip_recv_post_scratch_bytes(C.byref(a))
+    a.arith = abi.ARITH_FP32
+    assert q() == 4 * gsyn
+    a.cgemm_splits = 2
+    assert q() == 2 * gsyn
+    a.cgemm_splits = 1000                  # clamped to 512 / 16 chunks
+    assert q() == 32 * gsyn
+    a.cgemm_splits, a.cgemm_tile = 0, 256
+    assert q() == 4 * gsyn
+    a.arith, a.cgemm_tile = abi.ARITH_BF16, 0
+    assert q() == 0
+    a.cgemm_splits = 8
+    assert q() == 8 * gsyn
+    a.arith = abi.ARITH_EXACT
+    assert q() == 0
+    a.arith, a.cgemm_tile = abi.ARITH_FP32, 100
+    assert q() == 0
 
 
 CASES = [

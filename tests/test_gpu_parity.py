@@ -437,6 +437,36 @@ def test_recv_post_several_patches_mfma(case, arith):
     assert np.abs(host(g) - 2 * ref).max() <= (2e-5 if arith == abi.ARITH_FP32 else 4e-2) * scale
 
 
+@pytest.mark.parametrize("arith", [abi.ARITH_FP32, abi.ARITH_BF16])
+def test_channel_gemm_split_equals_unsplit(arith):
+    """The channel GEMM with its channels split over workgroups (partials in the
+    caller's scratch, added in split order) against the same GEMM unsplit and
+    the oracle, at the transposed stride-4 scatter of the LCA bench (4 images
+    32x32x512 -> 128x128x1, 4x4 post patch); accumulate adds onto GSyn."""
+    rng = np.random.default_rng(33)
+    case = ((4, 32, 32, 512, 2), (4, 128, 128, 1, 0), 4)
+    hs, os_, ht, ot, p, W, A = gather_setup(case, rng, density=0.1)
+    ref = O.recv_post(os_, ot, p, p, A, W).astype(np.float64)
+    split = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=arith)
+    split.set_launch(cgemm_tile=0, cgemm_splits=4)
+    whole = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=arith, split=False)
+    assert split.scratch is not None and whole.scratch is None
+    nbytes = split.scratch.numel() * 4
+    assert nbytes % (ht.nbatch * ht.num_res * 4) == 0 and nbytes // (ht.nbatch * ht.num_res * 4) > 1
+    gs = torch.full((ht.nbatch * ht.num_res,), 3.0, device=DEV)
+    gw = torch.full((ht.nbatch * ht.num_res,), 3.0, device=DEV)
+    split.deliver(dev(A), gs, accumulate=True)
+    whole.deliver(dev(A), gw, accumulate=True)
+    a, b = host(gs) - 3.0, host(gw) - 3.0
+    scale = np.abs(ref).max()
+    assert np.abs(a - b).max() <= 1e-5 * scale          # only the order of the FP32 sums differs
+    tol = 1e-5 if arith == abi.ARITH_FP32 else 1e-2
+    assert np.abs(a - ref).max() <= tol * scale
+    # the EXACT form has no channel GEMM: no scratch
+    ex = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=abi.ARITH_EXACT)
+    assert ex.scratch is None
+
+
 @pytest.mark.parametrize("case", SCATTER_CASES[:2])
 @pytest.mark.parametrize("arith", [abi.ARITH_FP32, abi.ARITH_BF16])
 def test_recv_pre_transposed_mfma_route(case, arith):
@@ -529,6 +559,46 @@ def test_hebbian_dw_mfma(case, arith):
     # dw accumulates across calls when not cleared
     hb.update_dw(dev(a), dev(r), clear=False, arith=arith)
     assert np.abs(host(hb.dw) - 2 * got).max() <= 1e-6 * scale
+
+
+def test_autotuner_shapes_keep_results(tmp_path):
+    """The launch-shape autotuner: for the channel GEMM (FP32 and BF16) and the
+    Hebbian MFMA split-K it measures every candidate, applies the fastest and
+    caches it; the tuned launch gives the oracle's result within each form's
+    bar and a second tuner reads the cache without measuring."""
+    from openpv_amd.autotune import LaunchTuner
+    rng = np.random.default_rng(34)
+    cache = str(tmp_path / "shapes.json")
+    case = ((2, 32, 32, 512, 2), (2, 128, 128, 1, 0), 4)
+    hs, os_, ht, ot, p, W, A = gather_setup(case, rng, density=0.1)
+    ref = O.recv_post(os_, ot, p, p, A, W).astype(np.float64)
+    scale = np.abs(ref).max()
+    for arith, tol in ((abi.ARITH_FP32, 1e-5), (abi.ARITH_BF16, 1e-2)):
+        d = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=arith)
+        g = torch.zeros(ht.nbatch * ht.num_res, device=DEV)
+        tuner = LaunchTuner(cache=cache)
+        shape = tuner.tune_recv_post(d, dev(A), g)
+        chunks = 512 // (32 if arith == abi.ARITH_BF16 else 16)
+        assert shape[0] in (64, 128, 256) and 1 <= shape[1] <= min(8, chunks)
+        assert (d.cgemm_tile, d.cgemm_splits) == tuple(shape) and len(tuner.log) == 1
+        print("arith", arith, "tuned", shape, {k: round(v, 4) for k, v in tuner.log[0][1].items()}, flush=True)
+        g.zero_()
+        d.deliver(dev(A), g)
+        assert np.abs(host(g) - ref).max() <= tol * scale
+        d2 = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=arith)
+        t2 = LaunchTuner(cache=cache)
+        assert tuple(t2.tune_recv_post(d2, dev(A), g)) == tuple(shape) and t2.log == []
+    # Hebbian split-K rows
+    (hp, op), (hq, oq), p = both((2, 16, 16, 96, 2)), both((2, 64, 64, 1, 6)), 16
+    a = rng.standard_normal(hp.nbatch * hp.num_ext).astype(np.float32)
+    a[rng.random(a.shape) < 0.7] = 0
+    r = rng.standard_normal(hq.nbatch * hq.num_ext).astype(np.float32)
+    dw, _ = O.hebbian_dw(op, oq, p, p, oq.nf, 0.25, a, r, counts=False)
+    hb = R.HebbianUpdater(hp, hq, p, p, 0.25, normalize_dw=False, device=DEV)
+    shape = LaunchTuner(cache=None).tune_hebbian(hb, dev(a), dev(r), abi.ARITH_FP32)
+    assert hb.rows_per_chunk == shape[0]
+    hb.update_dw(dev(a), dev(r), arith=abi.ARITH_FP32)
+    assert np.abs(host(hb.dw) - dw).max() <= 2e-5 * np.abs(dw).max()
 
 
 def test_config1_delivery_full_size():
