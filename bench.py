@@ -227,9 +227,11 @@ def c1_delivery(torch, reps=10):
     out = {"workload": "pre 256x256x16 (halo 6) -> post 128x128x256, 12x12x16 post patch",
            "gather_macs": 128 * 128 * 256 * 12 * 12 * 16, "gather_ms": {}, "scatter_ms": {}}
     ref = None
-    for name, ar in (("exact", abi.ARITH_EXACT), ("fp32", abi.ARITH_FP32), ("bf16", abi.ARITH_BF16)):
+    for name, ar in (("exact", abi.ARITH_EXACT), ("fp32", abi.ARITH_FP32), ("bf16", abi.ARITH_BF16),
+                     ("fp8", abi.ARITH_FP8)):
         d = R.PostsynapticPerspectiveGPUDelivery(pre, post, 12, 12, torch.as_tensor(Wpost.ravel()).to(dev),
                                                  arith=ar)
+        d.set_source_scale(R.fp8_scale_for(np.abs(A).max()))
         ms, _ = _time_events(lambda: d.deliver(dA, g, stream=s), s, reps, torch)
         out["gather_ms"][name] = round(ms, 4)
         gh = g.cpu().numpy().astype(np.float64)
@@ -250,6 +252,7 @@ def c1_delivery(torch, reps=10):
         out["scatter_ms"][name] = round(ms, 4)
     ms_b = out["gather_ms"]["bf16"]
     out["gather_bf16_tflops"] = round(2.0 * out["gather_macs"] / (ms_b * 1e-3) / 1e12, 1)
+    out["gather_fp8_tflops"] = round(2.0 * out["gather_macs"] / (out["gather_ms"]["fp8"] * 1e-3) / 1e12, 1)
     # compulsory bytes of the sparse scatter: active list, weights, GSyn read + write
     sbytes = 8 * out["scatter_active"] + 4 * Wpre.size + 8 * post.num_res
     out["scatter_exact_gbs"] = round(sbytes / (out["scatter_ms"]["exact"] * 1e-3) / 1e9, 1)
@@ -331,7 +334,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--forms", default="exact,fp32,bf16")
+    ap.add_argument("--forms", default="exact,fp32,bf16,fp8")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch shapes (no autotuner)")
     ap.add_argument("--no-extras", action="store_true",
@@ -354,7 +357,8 @@ def main():
     images = one_over_f_images(NB, NY, NX, seed=1)
     W = dictionary(NFV, NXP, NF, seed=2)
     forms = [f for f in args.forms.split(",") if f]
-    arith = {"exact": abi.ARITH_EXACT, "fp32": abi.ARITH_FP32, "bf16": abi.ARITH_BF16}
+    arith = {"exact": abi.ARITH_EXACT, "fp32": abi.ARITH_FP32, "bf16": abi.ARITH_BF16,
+             "fp8": abi.ARITH_FP8}
     res = {}
     cols = {}
     # launch shapes measured afresh on this machine (no cached choices)
@@ -376,12 +380,17 @@ def main():
     extra = {}
     if "exact" in cols and head != "exact":
         e_ref = cols["exact"][0].probe_energy().cpu().numpy()
-        e_lp = col.probe_energy().cpu().numpy()
         n_ref = cols["exact"][0].num_active().astype(np.float64)
-        n_lp = col.num_active().astype(np.float64)
-        extra["energy_rel_err_vs_exact"] = float(np.max(np.abs(e_lp - e_ref) / np.abs(e_ref)))
-        extra["active_rel_diff_vs_exact"] = float(np.max(np.abs(n_lp - n_ref) / np.maximum(n_ref, 1)))
-        extra["active_mean"] = float(n_lp.mean())
+        for f in res:
+            if f == "exact":
+                continue
+            e_lp = cols[f][0].probe_energy().cpu().numpy()
+            n_lp = cols[f][0].num_active().astype(np.float64)
+            res[f]["energy_rel_err_vs_exact"] = float(np.max(np.abs(e_lp - e_ref) / np.abs(e_ref)))
+            res[f]["active_rel_diff_vs_exact"] = float(np.max(np.abs(n_lp - n_ref) / np.maximum(n_ref, 1)))
+        extra["energy_rel_err_vs_exact"] = res[head]["energy_rel_err_vs_exact"]
+        extra["active_rel_diff_vs_exact"] = res[head]["active_rel_diff_vs_exact"]
+        extra["active_mean"] = float(col.num_active().astype(np.float64).mean())
     for f in res:
         res[f]["steps_per_s"] = round(res[f]["steps_per_s"], 2)
         res[f]["ms_per_step"] = round(res[f]["ms_per_step"], 5)

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PVHIP_ABI_VERSION 4
+#define PVHIP_ABI_VERSION 5
 
 enum {
    PVHIP_OK              = 0,
@@ -264,6 +264,11 @@ typedef struct {
     * autotuner; results differ only in the FP32 summation order. */
    int cgemm_tile;
    int cgemm_splits;
+   /* FP8 form (one data patch per feature): the source activity is
+    * multiplied by this power of two before its e4m3 conversion, so that it
+    * uses the format's range (0 = 1).  weights_lp then holds the FP8 shadow of
+    * pvhip_recv_post_fp8_weights, which carries its own per-kernel scales. */
+   float lp_source_scale;
 } pvhip_recv_post_args;
 
 /* Active-list outputs of a fused delivery (same layout and contents as
@@ -295,6 +300,13 @@ size_t pvhip_recv_post_scratch_bytes(const pvhip_recv_post_args *args);
 int pvhip_recv_post_bf16_weights(
       const float *weights, void *weights_bf16, int num_kernels, int patch_size, pvhip_stream stream);
 size_t pvhip_recv_post_bf16_weights_bytes(int num_kernels, int patch_size);
+/* FP8 (e4m3) weight shadow: per kernel (data patch), the weights times the
+ * power of two that brings max |w| into [120, 240), as [kernels][Kpad] bytes
+ * followed by the float scales.  An extension of the reference interface (its
+ * GPU path has no FP8 form); call after every weight change. */
+int pvhip_recv_post_fp8_weights(
+      const float *weights, void *weights_fp8, int num_kernels, int patch_size, pvhip_stream stream);
+size_t pvhip_recv_post_fp8_weights_bytes(int num_kernels, int patch_size);
 
 /* ------------------------------------------------------------------------ */
 /* Pre-perspective delivery: replaces CudaRecvPre / HyPerLayer_recv_pre      */

@@ -94,7 +94,7 @@ class RecvPostArgs(C.Structure):
                 ("window_y0", i32), ("window_ny", i32),
                 ("fused_update", C.POINTER(LCAUpdateArgs)),
                 ("active_out", C.POINTER(ActiveOut)), ("scratch", vp), ("scratch_bytes", C.c_size_t),
-                ("cgemm_tile", i32), ("cgemm_splits", i32)]
+                ("cgemm_tile", i32), ("cgemm_splits", i32), ("lp_source_scale", f32)]
 
 
 class RecvPreArgs(C.Structure):
@@ -158,6 +158,8 @@ PVHIP_SYMBOLS = {
     "pvhip_recv_post": (i32, [C.POINTER(RecvPostArgs), vp]),
     "pvhip_recv_post_active_scratch_bytes": (C.c_size_t, [C.POINTER(RecvPostArgs)]),
     "pvhip_recv_post_scratch_bytes": (C.c_size_t, [C.POINTER(RecvPostArgs)]),
+    "pvhip_recv_post_fp8_weights": (i32, [vp, vp, i32, i32, vp]),
+    "pvhip_recv_post_fp8_weights_bytes": (C.c_size_t, [i32, i32]),
     "pvhip_recv_post_bf16_weights": (i32, [vp, vp, i32, i32, vp]),
     "pvhip_recv_post_bf16_weights_bytes": (C.c_size_t, [i32, i32]),
     "pvhip_recv_pre": (i32, [C.POINTER(RecvPreArgs), vp]),

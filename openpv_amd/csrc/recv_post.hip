@@ -37,6 +37,9 @@ struct PostParams {
    int emit;
    pvhip_sparse_entry *stg;
    uint32_t *segCnt;
+   // FP8 form: per-kernel weight scales (after the FP8 shadow) and the source scale
+   const float *lpWScale;
+   float lpSrcScale;
 };
 
 __device__ __forceinline__ void post_epilogue(const PostParams &p, int b, int k, float G) {
@@ -240,6 +243,58 @@ constexpr int MTX = 16, MTY = 8;        // position tile
 constexpr int MPB = MTX * MTY / 16;     // 16-position blocks per wave
 constexpr int MF  = 64;                 // features per workgroup
 constexpr int MFB = MF / 4 / 16;        // 16-feature blocks per wave
+
+// Element types of the gather's low-precision forms.  A fragment is 8
+// consecutive k (v_mfma_f32_16x16x32_bf16 and v_mfma_f32_16x16x32_fp8_fp8
+// share the lane layout: A[r][8g..8g+7], B[8g..8g+7][r]).  FP8 values are
+// scaled (weights per kernel, source per call, powers of two) so that they
+// sit in the e4m3 range; the accumulators are scaled back before the
+// epilogue.
+struct GatherBF16 {
+   using T    = __bf16;
+   using Frag = bf16x8;
+   using RawA = uint4; // 8 elements as raw 128-bit registers
+   static constexpr bool SCALED = false;
+   __device__ static T cvt(float x) { return (__bf16)x; }
+   __device__ static f32x4 mfma(RawA a, Frag b, f32x4 c) {
+      return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), b, c, 0, 0, 0);
+   }
+};
+struct GatherFP8 {
+   using T    = uint8_t;
+   using Frag = long;
+   using RawA = uint2;
+   static constexpr bool SCALED = true;
+   __device__ static T cvt(float x) {
+      return (T)(__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false) & 0xff);
+   }
+   __device__ static f32x4 mfma(RawA a, Frag b, f32x4 c) {
+      return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(__builtin_bit_cast(long, a), b, c, 0, 0, 0);
+   }
+};
+
+// 8 consecutive FP8 values at p, read with the widest load ALIGN allows.
+template <int ALIGN>
+__device__ __forceinline__ long lds_frag8(const uint8_t *p) {
+   if (ALIGN >= 8) {
+      return *reinterpret_cast<const long *>(p);
+   }
+   uint32_t lo, hi;
+   if (ALIGN >= 4) {
+      lo = *reinterpret_cast<const uint32_t *>(p);
+      hi = *reinterpret_cast<const uint32_t *>(p + 4);
+   }
+   else if (ALIGN >= 2) {
+      const uint16_t *q = reinterpret_cast<const uint16_t *>(p);
+      lo = (uint32_t)q[0] | ((uint32_t)q[1] << 16);
+      hi = (uint32_t)q[2] | ((uint32_t)q[3] << 16);
+   }
+   else {
+      lo = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+      hi = (uint32_t)p[4] | ((uint32_t)p[5] << 8) | ((uint32_t)p[6] << 16) | ((uint32_t)p[7] << 24);
+   }
+   return (long)(((uint64_t)hi << 32) | lo);
+}
 
 template <int ALIGN>
 __device__ __forceinline__ bf16x8 lds_frag(const __bf16 *p) {
@@ -449,15 +504,16 @@ __device__ __forceinline__ void lca_epilogue_vthresh(
 
 // EPI: 0 GSyn only, 1 fused VTHRESH update, 2 fused update with either
 // transfer, 3 as 1 plus the active-list emission of full tiles.
-template <int ALIGN, int EPI>
+template <int ALIGN, int EPI, typename E = GatherBF16>
 __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
-      PostParams p, const __bf16 *Wb, int srcRowsMax, int srcColsMax, int Kpad) {
+      PostParams p, const typename E::T *Wb, int srcRowsMax, int srcColsMax, int Kpad) {
+   using T = typename E::T;
    extern __shared__ float smem[];
    int const K       = p.nxp * p.nfp * p.nyp;
    int const rowK    = p.nxp * p.nfp;
-   int const pitch   = (srcColsMax * p.nfp + 7) / 8 * 8; // bf16 elements per LDS row
-   __bf16 *src       = reinterpret_cast<__bf16 *>(smem);
-   int *koff         = reinterpret_cast<int *>(smem + (srcRowsMax * pitch + 1) / 2); // [Kpad]
+   int const pitch   = (srcColsMax * p.nfp + 7) / 8 * 8; // elements per LDS row
+   T *src            = reinterpret_cast<T *>(smem);
+   int *koff         = reinterpret_cast<int *>(smem + (srcRowsMax * pitch * sizeof(T) + 3) / 4); // [Kpad]
    int const tilesX  = (p.post.nx + MTX - 1) / MTX;
    int const tile    = blockIdx.x;
    int const x0      = (tile % tilesX) * MTX;
@@ -533,7 +589,7 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
 #pragma unroll
       for (int u = 0; u < 8; u++) {
          if (rr[u] < srcRows) {
-            src[rr[u] * pitch + cc[u]] = (__bf16)v[u];
+            src[rr[u] * pitch + cc[u]] = E::cvt(E::SCALED ? v[u] * p.lpSrcScale : v[u]);
          }
       }
    }
@@ -571,8 +627,9 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
       }
    }
    constexpr int KPRE = 4; // k-steps whose A fragments are fetched together
+   using RawA         = typename E::RawA;
    for (int kb = 0; kb < Kpad; kb += 32 * KPRE) {
-      uint4 apre[KPRE][MFB]; // bf16x8 fragments kept as raw 128-bit registers
+      RawA apre[KPRE][MFB]; // fragments kept as raw registers
 #pragma unroll
       for (int st = 0; st < KPRE; st++) {
          int const kk = kb + 32 * st + 8 * g;
@@ -580,8 +637,8 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
          for (int fb = 0; fb < MFB; fb++) {
             int f = fWave + fb * 16 + r;
             apre[st][fb] = (f < p.post.nf && kk < Kpad)
-                                 ? *reinterpret_cast<const uint4 *>(Wb + (size_t)f * Kpad + kk)
-                                 : make_uint4(0u, 0u, 0u, 0u);
+                                 ? *reinterpret_cast<const RawA *>(Wb + (size_t)f * Kpad + kk)
+                                 : RawA{};
          }
       }
 #pragma unroll
@@ -598,20 +655,34 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
          }
 #pragma unroll
          for (int pb = 0; pb < MPB; pb++) {
-            bf16x8 bfrag;
-            if (ALIGN == 0) {
+            typename E::Frag bfrag;
+            if constexpr (E::SCALED) {
+               if (ALIGN == 0) {
+                  uint64_t v = 0;
 #pragma unroll
-               for (int e = 0; e < 8; e++) {
-                  bfrag[e] = kos[e] >= 0 ? src[pbase[pb] + kos[e]] : (__bf16)0.0f;
+                  for (int e = 0; e < 8; e++) {
+                     v |= (uint64_t)(kos[e] >= 0 ? src[pbase[pb] + kos[e]] : (T)0) << (8 * e);
+                  }
+                  bfrag = (long)v;
+               }
+               else {
+                  bfrag = ko >= 0 ? lds_frag8<ALIGN>(src + pbase[pb] + ko) : 0L;
                }
             }
             else {
-               bfrag = ko >= 0 ? lds_frag<ALIGN>(src + pbase[pb] + ko) : bf16x8{};
+               if (ALIGN == 0) {
+#pragma unroll
+                  for (int e = 0; e < 8; e++) {
+                     bfrag[e] = kos[e] >= 0 ? src[pbase[pb] + kos[e]] : (__bf16)0.0f;
+                  }
+               }
+               else {
+                  bfrag = ko >= 0 ? lds_frag<ALIGN>(src + pbase[pb] + ko) : bf16x8{};
+               }
             }
 #pragma unroll
             for (int fb = 0; fb < MFB; fb++) {
-               acc[fb][pb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                     __builtin_bit_cast(bf16x8, apre[st][fb]), bfrag, acc[fb][pb], 0, 0, 0);
+               acc[fb][pb] = E::mfma(apre[st][fb], bfrag, acc[fb][pb]);
             }
          }
          }
@@ -620,6 +691,21 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
    // Lane holds D[4g+i][r]: features fWave+fb*16+4g+i at position pb*16+r.
    // Transpose through LDS (the staged source is dead now) so that each
    // position's MF features are written as one contiguous run.
+   if constexpr (E::SCALED) {
+      // back from the scaled operands: D[f][pos] / (weight scale of f * source scale)
+#pragma unroll
+      for (int fb = 0; fb < MFB; fb++) {
+#pragma unroll
+         for (int i = 0; i < 4; i++) {
+            int const f       = fWave + fb * 16 + 4 * g + i;
+            float const unscl = f < p.post.nf ? 1.0f / (p.lpWScale[f] * p.lpSrcScale) : 0.0f;
+#pragma unroll
+            for (int pb = 0; pb < MPB; pb++) {
+               acc[fb][pb][i] *= unscl;
+            }
+         }
+      }
+   }
    __syncthreads();
    float *outl = smem; // [MTX*MTY][MOP]
 #pragma unroll
@@ -1018,6 +1104,43 @@ __global__ void __launch_bounds__(256) cgemm_reduce_kernel(
    }
 }
 
+// FP8 weight shadow: one workgroup per kernel (data patch); its scale is the
+// power of two that brings max |w| into [120, 240) (the e4m3 range with
+// headroom), stored after the [kernels][Kpad] bytes.
+__global__ void __launch_bounds__(256) weights_to_fp8_kernel(
+      const float *W, uint8_t *Wq, float *scales, int K, int Kpad) {
+   int const f = blockIdx.x;
+   float m     = 0.0f;
+   for (int k = threadIdx.x; k < K; k += 256) {
+      m = fmaxf(m, fabsf(W[(size_t)f * K + k]));
+   }
+   for (int off = kWave / 2; off > 0; off >>= 1) {
+      m = fmaxf(m, __shfl_down(m, off, kWave));
+   }
+   __shared__ float red[256 / kWave];
+   if (threadIdx.x % kWave == 0) {
+      red[threadIdx.x / kWave] = m;
+   }
+   __syncthreads();
+   m = red[0];
+   for (int w = 1; w < 256 / kWave; w++) {
+      m = fmaxf(m, red[w]);
+   }
+   float sc = 1.0f;
+   if (m > 0.0f) {
+      int e;
+      frexpf(240.0f / m, &e); // 240/m = q * 2^e, q in [0.5, 1)
+      sc = ldexpf(1.0f, e - 1);
+   }
+   if (threadIdx.x == 0) {
+      scales[f] = sc;
+   }
+   for (int k = threadIdx.x; k < Kpad; k += 256) {
+      float v               = k < K ? W[(size_t)f * K + k] * sc : 0.0f;
+      Wq[(size_t)f * Kpad + k] = GatherFP8::cvt(v);
+   }
+}
+
 __global__ void weights_to_bf16_kernel(
       const float *W, __bf16 *Wb, int numKernels, int K, int Kpad) {
    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1238,6 +1361,23 @@ size_t pvhip_recv_post_bf16_weights_bytes(int num_kernels, int patch_size) {
    return (size_t)num_kernels * ((patch_size + 31) / 32 * 32) * 2;
 }
 
+int pvhip_recv_post_fp8_weights(
+      const float *weights, void *weights_fp8, int num_kernels, int patch_size, pvhip_stream stream) {
+   PVHIP_REQUIRE(weights && weights_fp8 && num_kernels > 0 && patch_size > 0, "fp8 weights: bad argument");
+   int Kpad     = (patch_size + 31) / 32 * 32;
+   uint8_t *q   = static_cast<uint8_t *>(weights_fp8);
+   float *scale = reinterpret_cast<float *>(q + (size_t)num_kernels * Kpad);
+   hipLaunchKernelGGL(weights_to_fp8_kernel, dim3((unsigned)num_kernels), dim3(256), 0, (hipStream_t)stream,
+         weights, q, scale, patch_size, Kpad);
+   PVHIP_LAUNCHED("weights_to_fp8");
+   return PVHIP_OK;
+}
+
+size_t pvhip_recv_post_fp8_weights_bytes(int num_kernels, int patch_size) {
+   // bytes [kernels][Kpad] (Kpad a multiple of 32, so the scales stay aligned), then float scales
+   return (size_t)num_kernels * ((patch_size + 31) / 32 * 32) + (size_t)num_kernels * sizeof(float);
+}
+
 int pvhip_recv_post(const pvhip_recv_post_args *args, pvhip_stream stream) {
    PVHIP_REQUIRE(args != nullptr, "pvhip_recv_post: NULL args");
    PostParams p;
@@ -1335,15 +1475,21 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
       return PVHIP_OK;
    }
 
-   if (arith == PVHIP_ARITH_BF16) {
-      PVHIP_REQUIRE(args->weights_lp != nullptr, "recv_post: BF16 needs weights_lp (pvhip_recv_post_bf16_weights)");
+   PVHIP_REQUIRE(arith != PVHIP_ARITH_FP8 || single,
+         "recv_post: the FP8 form needs one data patch per feature (the channel GEMM has BF16 and FP32 forms)");
+   if (arith == PVHIP_ARITH_BF16 || arith == PVHIP_ARITH_FP8) {
+      bool const fp8 = arith == PVHIP_ARITH_FP8;
+      PVHIP_REQUIRE(args->weights_lp != nullptr,
+            "recv_post: %s needs weights_lp (pvhip_recv_post_%s_weights)", fp8 ? "FP8" : "BF16",
+            fp8 ? "fp8" : "bf16");
       int const rowK = p.nxp * p.nfp;
       int const K    = rowK * p.nyp;
       int const Kpad = (K + 31) / 32 * 32;
       int const srcColsMax = (MTX - 1) * sx + p.nxp;
       int const srcRowsMax = (MTY - 1) * sy + p.nyp;
       int const pitch      = (srcColsMax * p.nfp + 7) / 8 * 8;
-      size_t lds = ((size_t)srcRowsMax * pitch + 1) / 2 * 4 + (size_t)Kpad * sizeof(int);
+      size_t const esz = fp8 ? 1 : 2;
+      size_t lds = ((size_t)srcRowsMax * pitch * esz + 3) / 4 * 4 + (size_t)Kpad * sizeof(int);
       lds        = std::max(lds, (size_t)MTX * MTY * MOP * sizeof(float)); // epilogue tile
       PVHIP_REQUIRE(lds <= 160 * 1024, "recv_post: BF16 tile needs %zu bytes of LDS", lds);
       // alignment class of a fragment's LDS address: rows are 8-aligned, window
@@ -1356,7 +1502,13 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
       int const tilesX = (p.post.nx + MTX - 1) / MTX;
       int const tilesY = (p.post.ny + MTY - 1) / MTY;
       dim3 grid(tilesX * tilesY, (p.post.nf + MF - 1) / MF, p.post.nbatch);
-      const __bf16 *Wb = (const __bf16 *)args->weights_lp;
+      if (fp8) {
+         float const ss = args->lp_source_scale;
+         PVHIP_REQUIRE(ss >= 0.0f && ss == ss && ss < 3.0e38f, "recv_post: bad lp_source_scale %g", ss);
+         p.lpSrcScale = ss > 0.0f ? ss : 1.0f;
+         p.lpWScale   = reinterpret_cast<const float *>(static_cast<const uint8_t *>(args->weights_lp)
+                                                      + (size_t)(p.nKx * p.nKy * p.post.nf) * Kpad);
+      }
       int epi = !p.fused ? 0 : (p.lca.transfer.mode == PVHIP_TRANSFER_VTHRESH ? 1 : 2);
       const pvhip_active_out *aout = args->active_out;
       if (aout && aout->halo_is_zero && epi == 1 && p.post.nf % MF == 0 && p.post.nx % MTX == 0
@@ -1369,8 +1521,16 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
          epi      = 3;
       }
 #define PVHIP_MFMA(AL, EP)                                                                       \
-   hipLaunchKernelGGL((recv_post_mfma_bf16_kernel<AL, EP>), grid, dim3(256), lds, s, p, Wb,     \
-         srcRowsMax, srcColsMax, Kpad)
+   do {                                                                                          \
+      if (fp8) {                                                                                 \
+         hipLaunchKernelGGL((recv_post_mfma_bf16_kernel<AL, EP, GatherFP8>), grid, dim3(256), lds, \
+               s, p, static_cast<const uint8_t *>(args->weights_lp), srcRowsMax, srcColsMax, Kpad); \
+      }                                                                                          \
+      else {                                                                                     \
+         hipLaunchKernelGGL((recv_post_mfma_bf16_kernel<AL, EP, GatherBF16>), grid, dim3(256), lds, \
+               s, p, static_cast<const __bf16 *>(args->weights_lp), srcRowsMax, srcColsMax, Kpad); \
+      }                                                                                          \
+   } while (0)
 #define PVHIP_MFMA_EPI(AL)                                                                       \
    do {                                                                                          \
       if (epi == 0) PVHIP_MFMA(AL, 0);                                                           \
@@ -1385,7 +1545,7 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
       else PVHIP_MFMA_EPI(0);
 #undef PVHIP_MFMA_EPI
 #undef PVHIP_MFMA
-      PVHIP_LAUNCHED("recv_post_mfma_bf16");
+      PVHIP_LAUNCHED(fp8 ? "recv_post_mfma_fp8" : "recv_post_mfma_bf16");
       return PVHIP_OK;
    }
    if ((arith == PVHIP_ARITH_EXACT || arith == PVHIP_ARITH_FP32) && single) {

@@ -98,6 +98,12 @@ class LCAColumn:
         # patch = nxp x nyp x nf_in (TransposeWeightsPair.cpp:141-147).
         self.gather = PostsynapticPerspectiveGPUDelivery(self.res_loc, self.v1_loc, nxp, nyp,
                                                          self.weights, arith=arith)
+        if arith == abi.ARITH_FP8:
+            # the residual err_scale*(x - recon) stays within err_scale*2*max|x| once the
+            # reconstruction approximates x; its FP8 scale is fixed from that bound
+            from .runtime import fp8_scale_for
+            bound = 2.0 * abs(self.err_scale) * float(images.abs().max())
+            self.gather.set_source_scale(fp8_scale_for(bound))
         self.scatter = PresynapticPerspectiveGPUDelivery(self.v1_loc, self.rec_loc, nxp, nyp,
                                                          self.weights, arith=arith)
         self.active = ActiveList(self.v1_loc, d)

@@ -59,6 +59,16 @@ def exp_tau(dt, tau, nbatch):
     return np.array(out[:], np.float32)
 
 
+def fp8_scale_for(bound):
+    """The power of two s with bound*s in [120, 240): the scale that puts values
+    of magnitude up to `bound` into the e4m3 range with headroom."""
+    bound = float(bound)
+    if not bound > 0.0 or not np.isfinite(bound):
+        return 1.0
+    m, e = np.frexp(240.0 / bound)   # 240/bound = m * 2^e, m in [0.5, 1)
+    return float(np.ldexp(1.0, int(e) - 1))
+
+
 def scaled_tau(timeConstantTau, deltaTime):
     """mScaledTimeConstantTau = (float)(tau / dt) (HyPerLCAInternalStateBuffer.cpp:104-112)."""
     return float(np.float32(float(timeConstantTau) / float(deltaTime)))
@@ -120,7 +130,8 @@ class PostsynapticPerspectiveGPUDelivery:
                 raise abi.PVHIPError("recv_post inverse tables failed (%d)" % r)
             self.inverse = (_dev_i32(ix, dev), _dev_i32(iy, dev))
             self.windows = (x0.value, nx_.value, y0.value, ny_.value)
-        if arith == abi.ARITH_BF16:
+        self.lp_source_scale = 0.0   # FP8 form: power-of-two source scale (0 = 1)
+        if arith in (abi.ARITH_BF16, abi.ARITH_FP8):
             self.refresh_weights()
         self.scratch = None
         self.split = bool(split)
@@ -143,15 +154,24 @@ class PostsynapticPerspectiveGPUDelivery:
         self._alloc_scratch()
 
     def refresh_weights(self, stream=None):
-        """Re-derive the low-precision shadow after a weight change."""
-        if self.arith != abi.ARITH_BF16:
+        """Re-derive the low-precision shadow (BF16, or FP8 with its per-kernel
+        scales) after a weight change."""
+        if self.arith not in (abi.ARITH_BF16, abi.ARITH_FP8):
             return
-        nbytes = abi.hip().pvhip_recv_post_bf16_weights_bytes(self.num_kernels, self.patch_size)
+        H = abi.hip()
+        kind = "bf16" if self.arith == abi.ARITH_BF16 else "fp8"
+        nbytes = getattr(H, "pvhip_recv_post_%s_weights_bytes" % kind)(self.num_kernels, self.patch_size)
         if self.weights_lp is None:
-            self.weights_lp = torch.empty(nbytes // 2, dtype=torch.int16, device=self.weights.device)
-        abi.check(abi.hip().pvhip_recv_post_bf16_weights(
+            self.weights_lp = torch.empty((nbytes + 15) // 16 * 4, dtype=torch.int32,
+                                          device=self.weights.device)
+        abi.check(getattr(H, "pvhip_recv_post_%s_weights" % kind)(
             _ptr(self.weights), _ptr(self.weights_lp), self.num_kernels, self.patch_size,
-            _stream(stream)), "pvhip_recv_post_bf16_weights")
+            _stream(stream)), "pvhip_recv_post_%s_weights" % kind)
+
+    def set_source_scale(self, scale):
+        """FP8 form: the power of two the source activity is multiplied by before
+        its e4m3 conversion (fp8_scale_for(bound on |source|))."""
+        self.lp_source_scale = float(scale)
 
     def args(self, pre_activity, gsyn, accumulate=False, fused=None, active=None,
              halo_is_zero=False):
@@ -174,6 +194,7 @@ class PostsynapticPerspectiveGPUDelivery:
             a.scratch, a.scratch_bytes = _ptr(self.scratch), self.scratch.numel() * 4
         a.cgemm_tile = getattr(self, "cgemm_tile", 0)
         a.cgemm_splits = getattr(self, "cgemm_splits", 0)
+        a.lp_source_scale = getattr(self, "lp_source_scale", 0.0)
         if fused is not None:
             a.fused_update = C.pointer(fused)
         if active is not None:
@@ -245,6 +266,9 @@ class PresynapticPerspectiveGPUDelivery:
     def __init__(self, pre, post, nxp, nyp, weights, arith=abi.ARITH_EXACT, dt_factor=1.0):
         self.pre, self.post = pre, post
         self.nxp, self.nyp, self.nfp = nxp, nyp, post.nf
+        # the scatter's MFMA route (channel GEMM) has BF16 and FP32 forms: an
+        # FP8 request runs it in BF16
+        arith = abi.ARITH_BF16 if arith == abi.ARITH_FP8 else arith
         self.arith = arith
         self.dt_factor = dt_factor
         dev = weights.device

@@ -437,6 +437,70 @@ def test_recv_post_several_patches_mfma(case, arith):
     assert np.abs(host(g) - 2 * ref).max() <= (2e-5 if arith == abi.ARITH_FP32 else 4e-2) * scale
 
 
+@pytest.mark.parametrize("case", [GATHER_CASES[0], GATHER_CASES[1], GATHER_CASES[2], GATHER_CASES[5]])
+def test_recv_post_fp8_exact_on_representable_values(case):
+    """FP8 (e4m3) gather on values that survive the conversion exactly: weights in
+    {0, +-1/32, +-1/16} (per-kernel scale 2^11 -> {0, 64, 128}), sources in
+    {0, +-1/4, +-1, +-2} (source scale 2^6 -> {0, 16, 64, 128}).  Every product
+    and partial sum is exact in FP32, so the result equals the oracle's float64
+    sum bit for bit: the FP8 path's fragment layout, conversion and scaling."""
+    rng = np.random.default_rng(40)
+    hs, os_, ht, ot, p, W, A = gather_setup(case, rng)
+    W = (rng.choice([-2, -1, 0, 1, 2], size=W.shape) / 32.0).astype(np.float32)
+    ext = A.reshape(hs.nbatch, hs.ny_ext, hs.nx_ext, hs.nf) != 0
+    vals = rng.choice([-2.0, -1.0, -0.25, 0.0, 0.25, 1.0, 2.0], size=ext.shape).astype(np.float32)
+    A = np.where(ext, vals, 0.0).astype(np.float32).ravel()
+    ref = O.recv_post(os_, ot, p, p, A, W).astype(np.float64)
+    # the float64 sum of exact products is itself exact here (small integers x 2^-7)
+    d = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=abi.ARITH_FP8)
+    d.set_source_scale(R.fp8_scale_for(2.0))
+    assert R.fp8_scale_for(2.0) == 64.0
+    g = torch.full((ht.nbatch * ht.num_res,), 9.0, device=DEV)
+    d.deliver(dev(A), g)
+    assert np.array_equal(host(g).astype(np.float64), ref)
+    d.deliver(dev(A), g, accumulate=True)
+    assert np.array_equal(host(g).astype(np.float64), 2 * ref)
+
+
+@pytest.mark.parametrize("case", [GATHER_CASES[0], GATHER_CASES[1]])
+def test_recv_post_fp8_error_bound(case):
+    """FP8 gather on random data against the oracle: e4m3 keeps 3 mantissa bits
+    (relative rounding <= 2^-4 per operand), so the RMS error of a dot product
+    is a few percent of the RMS result; the largest error stays within 0.15 of
+    the largest magnitude."""
+    rng = np.random.default_rng(41)
+    hs, os_, ht, ot, p, W, A = gather_setup(case, rng, density=0.5)
+    ref = O.recv_post(os_, ot, p, p, A, W).astype(np.float64)
+    d = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=abi.ARITH_FP8)
+    d.set_source_scale(R.fp8_scale_for(np.abs(A).max()))
+    g = torch.zeros(ht.nbatch * ht.num_res, device=DEV)
+    d.deliver(dev(A), g)
+    err = host(g) - ref
+    rms = np.sqrt((err ** 2).mean()) / np.sqrt((ref ** 2).mean())
+    print("fp8 rms rel err", rms, "max", np.abs(err).max() / np.abs(ref).max(), flush=True)
+    assert rms <= 5e-2
+    assert np.abs(err).max() <= 0.15 * np.abs(ref).max()
+
+
+def test_lca_column_fp8_energy_and_sparsity():
+    """The config-0 column with the FP8 gather (the scatter runs its BF16 route),
+    200 steps against the EXACT column: energy within 2 % and active counts
+    within 10 % (measured values printed)."""
+    img, W = c0_problem(nb=2, seed=5)
+    cols = {}
+    for name, ar in (("exact", abi.ARITH_EXACT), ("fp8", abi.ARITH_FP8)):
+        c = LCAColumn(img, W, 16, 16, 128, 8, 8, VThresh=0.1, arith=ar, fused=True, device=DEV)
+        for _ in range(200):
+            c.step()
+        cols[name] = c
+    e_ex, e_lp = host(cols["exact"].probe_energy()), host(cols["fp8"].probe_energy())
+    n_ex, n_lp = cols["exact"].num_active(), cols["fp8"].num_active()
+    rel = np.abs(e_lp - e_ex) / np.abs(e_ex)
+    print("fp8 column energy rel err", rel, "active", n_ex, n_lp, flush=True)
+    assert rel.max() <= 2e-2
+    assert (np.abs(n_lp - n_ex) <= 0.1 * n_ex + 2).all()
+
+
 @pytest.mark.parametrize("arith", [abi.ARITH_FP32, abi.ARITH_BF16])
 def test_channel_gemm_split_equals_unsplit(arith):
     """The channel GEMM with its channels split over workgroups (partials in the
