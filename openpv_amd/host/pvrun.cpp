@@ -13,7 +13,9 @@
 // updateGSynFromPostPerspective picks the post-perspective gather or the
 // pre-perspective scatter; receiveGpu = false gives the EXACT form (the
 // reference's CPU summation order, bit for bit), receiveGpu = true the form
-// named by the HyPerCol parameter gpuArithmetic ("fp32" default, "bf16").
+// named by the HyPerCol parameter gpuArithmetic ("fp32" default, "bf16",
+// "fp8": the gather in FP8 with a source scale from fp8SourceBound or the
+// input range; scatters and channel GEMMs run FP8 requests in BF16).
 // A GPU scatter runs as a gather with transposed weights where the geometry
 // allows (as the Python runtime does), else as the ordered scatter in FP32.
 // updateGpu selects the transfer form (false: applyVThresh/applyAMax,
@@ -270,6 +272,7 @@ struct Conn {
    void *scratch = nullptr; // channel-GEMM split partials (pvhip_recv_post_scratch_bytes)
    size_t scratchBytes = 0;
    int tile = 0, splits = 0; // channel-GEMM launch shape (0: the library's default)
+   float srcScale = 0.0f;    // FP8 gather: power-of-two source scale
 };
 
 struct Probe {
@@ -332,6 +335,7 @@ struct Net {
    void buildConn(Conn &c);
    void step(double t);
    pvhip_recv_post_args gatherArgs(Conn &c);
+   double sourceBound(const Layer &pre);
    void autotune();
    void deliver(Conn &c);
    void receive(Layer &L);
@@ -482,7 +486,16 @@ void Net::build() {
       }
       std::printf("pvrun: %s \"%s\": %ssynapticPerspective%sDelivery\n", C.keyword.c_str(), C.name.c_str(),
             postPersp ? "Post" : "Pre", gpu ? "GPU" : "Convolve");
-      C.arith        = !gpu ? PVHIP_ARITH_EXACT : (gpuArith == "bf16" ? PVHIP_ARITH_BF16 : PVHIP_ARITH_FP32);
+      C.arith        = !gpu                 ? PVHIP_ARITH_EXACT
+                       : gpuArith == "bf16" ? PVHIP_ARITH_BF16
+                       : gpuArith == "fp8"  ? PVHIP_ARITH_FP8
+                                            : PVHIP_ARITH_FP32;
+      if (gpu && gpuArith != "fp32" && gpuArith != "bf16" && gpuArith != "fp8") {
+         fatal("gpuArithmetic \"" + gpuArith + "\" is not fp32, bf16 or fp8");
+      }
+      if (C.arith == PVHIP_ARITH_FP8 && !postPersp) {
+         C.arith = PVHIP_ARITH_BF16; // the scatter's MFMA route has BF16 and FP32 forms
+      }
       if (C.keyword == "HyPerConn") {
          C.nxp = (int)C.prm.num("nxp", 1);
          C.nyp = (int)C.prm.num("nyp", 1);
@@ -659,6 +672,38 @@ void Net::build() {
    check(pvhip_stream_sync(D.stream), "initialisation");
 }
 
+// The power of two s with bound*s in [120, 240): FP8 (e4m3) range with headroom.
+float fp8Scale(double bound) {
+   if (!(bound > 0.0) || !std::isfinite(bound)) {
+      return 1.0f;
+   }
+   int e;
+   std::frexp(240.0 / bound, &e);
+   return std::ldexp(1.0f, e - 1);
+}
+
+// Bound on |activity| of a gather's source for its FP8 scale: an error layer
+// fed by input layers holds errScale*(input - reconstruction), within
+// 2*|errScale|*max|input| once the reconstruction approximates the input;
+// another source has no such bound and uses 1 (a connection may set
+// fp8SourceBound).
+double Net::sourceBound(const Layer &pre) {
+   if (pre.kind != L_ERROR) {
+      return 1.0;
+   }
+   double m = 0.0;
+   for (auto &C : conns) {
+      if (&layers[C.post] == &pre && C.mode == C_IDENT && layers[C.pre].kind == L_INPUT) {
+         for (auto &fr : layers[C.pre].input.frames) {
+            for (float v : fr) {
+               m = std::max(m, (double)std::fabs(v));
+            }
+         }
+      }
+   }
+   return m > 0.0 ? 2.0 * std::fabs(pre.errScale) * m : 1.0;
+}
+
 // Pre-perspective weights of C (its own pre -> post) on the host: a HyPerConn's
 // or CloneConn's are the original's; a TransposeConn's are the transpose of
 // its original's (TransposeWeights::transposeShared).
@@ -754,9 +799,17 @@ void Net::buildConn(Conn &C) {
          C.win[2] = y0;
          C.win[3] = ny;
       }
+      if (C.arith == PVHIP_ARITH_FP8 && C.nkx * C.nky > 1) {
+         C.arith = PVHIP_ARITH_BF16; // the channel GEMM has BF16 and FP32 forms
+      }
       if (C.arith == PVHIP_ARITH_BF16) {
          C.Wlp = D.alloc(pvhip_recv_post_bf16_weights_bytes(nK, psize));
          check(pvhip_recv_post_bf16_weights(C.W, C.Wlp, nK, psize, D.stream), "bf16 weights");
+      }
+      if (C.arith == PVHIP_ARITH_FP8) {
+         C.Wlp = D.alloc(pvhip_recv_post_fp8_weights_bytes(nK, psize));
+         check(pvhip_recv_post_fp8_weights(C.W, C.Wlp, nK, psize, D.stream), "fp8 weights");
+         C.srcScale = fp8Scale(C.prm.num("fp8SourceBound", sourceBound(pre)));
       }
       pvhip_recv_post_args a = gatherArgs(C);
       C.scratchBytes         = pvhip_recv_post_scratch_bytes(&a);
@@ -828,8 +881,9 @@ pvhip_recv_post_args Net::gatherArgs(Conn &C) {
    a.window_ny     = C.win[3];
    a.scratch       = C.scratch;
    a.scratch_bytes = C.scratchBytes;
-   a.cgemm_tile    = C.tile;
-   a.cgemm_splits  = C.splits;
+   a.cgemm_tile      = C.tile;
+   a.cgemm_splits    = C.splits;
+   a.lp_source_scale = C.srcScale;
    return a;
 }
 

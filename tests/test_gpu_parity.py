@@ -970,15 +970,17 @@ def test_pvrun_two_columns_from_params(tmp_path):
     assert (r.returncode == 0) == (diff <= np.float32(2e-7)), r.stderr[-500:]
 
 
-def test_pvrun_bf16_device_column(tmp_path):
-    """gpuArithmetic = "bf16": the device column's gather runs on the BF16 MFMA
-    path (FP32 state); the host column is unchanged (bit-exact)."""
+@pytest.mark.parametrize("form,tol", [("bf16", 3e-2), ("fp8", 6e-2)])
+def test_pvrun_low_precision_device_column(tmp_path, form, tol):
+    """gpuArithmetic = "bf16" / "fp8": the device column's gather runs on the
+    BF16 or FP8 MFMA path (FP32 state; the FP8 source scale comes from the
+    input range); the host column is unchanged (bit-exact)."""
     from openpv_amd import pvp
     steps = 25
     text = open(os.path.join(GOLDEN, "LCAColumn.params")).read()
     assert 'gpuArithmetic = "fp32";' in text
-    params = tmp_path / "bf16.params"
-    params.write_text(text.replace('gpuArithmetic = "fp32";', 'gpuArithmetic = "bf16";'))
+    params = tmp_path / (form + ".params")
+    params.write_text(text.replace('gpuArithmetic = "fp32";', 'gpuArithmetic = "%s";' % form))
     r = _pvrun(params, tmp_path, steps)
     assert r.returncode in (0, 1), r.stderr[-500:]   # 1: the 2e-7 comparison probe fails, as expected
     ref = _gpulca_oracle(steps)
@@ -986,8 +988,8 @@ def test_pvrun_bf16_device_column(tmp_path):
     _, _, dev = pvp.read_activity(str(tmp_path / "ReconDevice.pvp"))
     assert same(host, ref)
     err = np.abs(dev - ref).max()
-    print("bf16 device column: max |dev - oracle| %g (scale %g)" % (err, np.abs(ref).max()))
-    assert err <= 3e-2 * np.abs(ref).max()
+    print("%s device column: max |dev - oracle| %g (scale %g)" % (form, err, np.abs(ref).max()))
+    assert err <= tol * np.abs(ref).max()
 
 
 def test_cost_sum_firm_threshold_matches_reference_fixture():

@@ -202,3 +202,11 @@ def test_pvrun_selection_errors_follow_the_delivery_creator(tmp_path, extra, msg
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and msg in r.stderr, r.stderr
     assert "pvhip_device_create" not in r.stderr
+
+
+def test_pvrun_refuses_an_unknown_gpu_arithmetic(tmp_path):
+    path = tmp_path / "net.params"
+    path.write_text((NET % "receiveGpu = true;").replace("stopTime = 2;", 'stopTime = 2; gpuArithmetic = "fp4";'))
+    r = subprocess.run([abi.pvrun_path(), "-p", str(path), "-o", str(tmp_path / "out")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "is not fp32, bf16 or fp8" in r.stderr, r.stderr
