@@ -357,6 +357,26 @@ __device__ __forceinline__ void acc4(float4 &acc, float a, float4 w) {
    }
 }
 
+// Wide targets (post.nf >= 64, a multiple of 4), sparse source: one wave per
+// target position, lanes over 4 features each.  The terms of the position are
+// the active entries of its coverage segments -- (patch row, covering pre
+// row, column run) in the reference's order -- each in active-list order.
+// Per chunk of 64 segments: lane i loads segment i's pos_start range, a wave
+// prefix sum of the lengths numbers every entry, lane j loads entry j (a
+// binary search over the prefix finds its segment) and computes its weight
+// row offset; the wave then adds the entries in that order, 8 weight loads in
+// flight at a time.  Same terms and order as the one-thread delivery.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+#pragma unroll
+   for (int off = 1; off < kWave; off <<= 1) {
+      int u = __shfl_up(v, off, kWave);
+      if ((int)(threadIdx.x % kWave) >= off) {
+         v += u;
+      }
+   }
+   return v;
+}
+
 template <bool EXACT>
 __global__ void __launch_bounds__(256) recv_pre_wave_kernel(PreParams p) {
    int const lane    = threadIdx.x % kWave;
@@ -383,6 +403,10 @@ __global__ void __launch_bounds__(256) recv_pre_wave_kernel(PreParams p) {
    bool const contig = ncol > 0 && ncol <= kWave && p.colXe[cB - 1] - xeA == ncol - 1;
    int const colwx  = lane < ncol ? p.colWx[cA + lane] : 0;
    int const colkx  = lane < ncol ? p.colKx[cA + lane] : 0;
+   int const nruns  = contig ? 1 : ncol;
+   int const rA     = p.rowPtr[Y * p.nyp];
+   int const rB     = p.rowPtr[(Y + 1) * p.nyp];
+   int const nseg   = (rB - rA) * nruns;
    float *gout      = p.gsyn + (size_t)b * p.post.numRes() + ((size_t)Y * p.post.nx + X) * nfPost;
    for (int fc = 0; fc < nfPost; fc += 4 * kWave) {
       int const F    = fc + 4 * lane;
@@ -390,47 +414,70 @@ __global__ void __launch_bounds__(256) recv_pre_wave_kernel(PreParams p) {
       int const Fc   = fOK ? F : 0;
       float4 acc     = (p.accumulate && fOK) ? *reinterpret_cast<const float4 *>(gout + F)
                                              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      for (int r = 0; r < p.nyp; r++) {
-         int const r0 = p.rowPtr[Y * p.nyp + r];
-         int const r1 = p.rowPtr[Y * p.nyp + r + 1];
-         for (int ri = r0; ri < r1; ri++) {
-            int const ye   = p.rowYe[ri];
-            int const wy   = p.rowWy[ri];
-            int const kyK  = p.rowKy[ri];
-            float const *Wr = p.W + (size_t)(kyK * p.nKx) * nfPre * patchSize + (size_t)wy * p.nxp * p.nfp + Fc;
-            int const pbase = ye * nxExt;
-            int const nruns = contig ? 1 : ncol;
-            for (int run = 0; run < nruns; run++) {
-               int const xe0 = contig ? xeA : p.colXe[cA + run];
-               uint32_t const s = ps[pbase + xe0];
-               uint32_t const e = ps[pbase + xe0 + (contig ? ncol : 1)];
-               for (uint32_t base = s; base < e; base += kWave) {
-                  uint32_t const i = base + lane;
-                  pvhip_sparse_entry en = i < e ? ent[i] : pvhip_sparse_entry{0u, 0.0f};
-                  int const fl = (int)(en.index % (uint32_t)nfPre);
-                  int const cl = (int)((en.index / (uint32_t)nfPre) % (uint32_t)nxExt) - xeA;
-                  int const n  = (int)min((uint32_t)kWave, e - base);
-                  for (int j0 = 0; j0 < n; j0 += 4) {
-                     float4 w[4];
-                     float av[4];
+      for (int s0 = 0; s0 < nseg; s0 += kWave) {
+         // lane's segment: covering row ri, column run
+         int const sid  = s0 + lane;
+         bool const sOK = sid < nseg;
+         int const ri   = rA + (sOK ? sid / nruns : 0);
+         int const run  = sOK ? sid % nruns : 0;
+         int const ye   = p.rowYe[ri];
+         int const xe0  = contig ? xeA : p.colXe[cA + run];
+         uint32_t const sst = sOK ? ps[ye * nxExt + xe0] : 0u;
+         uint32_t const sen = sOK ? ps[ye * nxExt + xe0 + (contig ? ncol : 1)] : 0u;
+         int const len      = (int)(sen - sst);
+         // weight offset of the segment's patch row (kernel y index, weight y)
+         int const rowOff = (p.rowKy[ri] * p.nKx) * nfPre * patchSize + p.rowWy[ri] * p.nxp * p.nfp;
+         int const runWx  = contig ? 0 : p.colWx[cA + run];
+         int const runKx  = contig ? 0 : p.colKx[cA + run];
+         int const incl   = wave_incl_scan(len);
+         int const excl   = incl - len;
+         int const total  = __builtin_amdgcn_readlane(incl, kWave - 1);
+         int const nsegC  = min(nseg - s0, kWave);
+         for (int k0 = 0; k0 < total; k0 += kWave) {
+            int const k  = k0 + lane;
+            bool const kOK = k < total;
+            // segment t of entry k: the last segment whose exclusive prefix is <=
+            // k (excl[0] = 0 <= k; that segment is never empty).  A branchless
+            // search of fixed length: every lane takes every shuffle, because a
+            // shuffle reads zero from a lane that is not active.
+            int t = 0;
 #pragma unroll
-                     for (int u = 0; u < 4; u++) {
-                        int const j = min(j0 + u, n - 1);
-                        float a     = bcast_f(en.value, j);
-                        int const f = bcast_i(fl, j);
-                        int const c = contig ? bcast_i(cl, j) : run;
-                        int const wx = contig ? bcast_i(colwx, c) : p.colWx[cA + run];
-                        int const kx = contig ? bcast_i(colkx, c) : p.colKx[cA + run];
-                        av[u] = (j0 + u < n) ? a : 0.0f;
-                        w[u]  = *reinterpret_cast<const float4 *>(
-                              Wr + ((size_t)kx * nfPre + f) * patchSize + (size_t)wx * p.nfp);
-                     }
+            for (int step = kWave / 2; step > 0; step >>= 1) {
+               int const mid = t + step;
+               int const em  = __shfl(excl, min(mid, nsegC - 1), kWave);
+               if (mid < nsegC && em <= k) {
+                  t = mid;
+               }
+            }
+            int const segS   = __shfl((int)sst, t, kWave);
+            int const segE   = __shfl((int)sen, t, kWave);
+            uint32_t const e = (uint32_t)segS + (uint32_t)(k - __shfl(excl, t, kWave));
+            // an entry must lie in its segment's range (true by construction;
+            // anything else is dropped rather than read out of bounds)
+            bool const eOK = kOK && e >= (uint32_t)segS && e < (uint32_t)segE;
+            pvhip_sparse_entry en = eOK ? ent[e] : pvhip_sparse_entry{0u, 0.0f};
+            int const fl   = (int)(en.index % (uint32_t)nfPre);
+            int const cl   = (int)((en.index / (uint32_t)nfPre) % (uint32_t)nxExt) - xeA;
+            int const wx   = contig ? __shfl(colwx, cl, kWave) : __shfl(runWx, t, kWave);
+            int const kx   = contig ? __shfl(colkx, cl, kWave) : __shfl(runKx, t, kWave);
+            int const rowT = __shfl(rowOff, t, kWave);
+            int const woff = eOK ? rowT + (kx * nfPre + fl) * patchSize + wx * p.nfp : 0;
+            float const av = eOK ? en.value : 0.0f;
+            int const n    = min(kWave, total - k0);
+            for (int j0 = 0; j0 < n; j0 += 8) {
+               float4 w[8];
+               float a[8];
 #pragma unroll
-                     for (int u = 0; u < 4; u++) {
-                        if (j0 + u < n && av[u] != 0.0f) {
-                           acc4<EXACT>(acc, av[u] * p.dtf, w[u]);
-                        }
-                     }
+               for (int u = 0; u < 8; u++) {
+                  int const j  = min(j0 + u, n - 1);
+                  a[u]         = (j0 + u < n) ? bcast_f(av, j) : 0.0f;
+                  int const wo = bcast_i(woff, j);
+                  w[u]         = *reinterpret_cast<const float4 *>(p.W + (size_t)wo + Fc);
+               }
+#pragma unroll
+               for (int u = 0; u < 8; u++) {
+                  if (j0 + u < n && a[u] != 0.0f) {
+                     acc4<EXACT>(acc, a[u] * p.dtf, w[u]);
                   }
                }
             }
