@@ -5,7 +5,11 @@ results beyond the order of FP32 sums:
 - the channel GEMM of `pvhip_recv_post` (FP32/BF16 forms with several data
   patches per feature): `cgemm_tile` window starts per workgroup (64, 128 or
   256) and `cgemm_splits` channel ranges;
-- the MFMA forms of `pvhip_hebbian_dw`: `rows_per_chunk` split-K rows.
+- the MFMA forms of `pvhip_hebbian_dw`: `rows_per_chunk` split-K rows;
+- the FP32/BF16 forms of a pre-perspective delivery: its dense MFMA route
+  (the transposed gather) or the ordered sparse scatter over the active list
+  in FP32, whichever is faster on the caller's current activity (the choice
+  depends on the density, so it is made on representative data).
 
 The tuner times each candidate on the caller's own buffers with HIP events
 (a few warm-up launches, then the median of the timed ones), keeps the
@@ -126,3 +130,22 @@ class LaunchTuner:
                           lambda: updater.update_dw(pre_activity, post_activity, clear=True,
                                                     arith=arith, stream=stream), stream)
         return best
+
+    def tune_recv_pre(self, delivery, gsyn, active, pre_activity, stream=None):
+        """Dense MFMA route or ordered sparse scatter for a
+        PresynapticPerspectiveGPUDelivery (FP32/BF16 forms with a route).
+        Returns "route", "ordered", or None when there is no choice."""
+        if delivery.transposed is None or delivery.arith == abi.ARITH_EXACT or active is None:
+            return None
+        stream = stream or torch.cuda.current_stream()
+        self.tune_recv_post(delivery.transposed, pre_activity, gsyn, stream=stream)
+        key = "recv_pre|%s|abi%d|arith%d|%s|%s|%dx%d" % (
+            self._device(), abi.hip().pvhip_abi_version(), delivery.arith, _loc_key(delivery.pre),
+            _loc_key(delivery.post), delivery.nxp, delivery.nyp)
+
+        def apply(mode):
+            delivery.use_route = mode == "route"
+        best = self._pick(key, [("route",), ("ordered",)], apply,
+                          lambda: delivery.deliver(gsyn, active=active, pre_activity=pre_activity,
+                                                   stream=stream), stream)
+        return best[0]

@@ -125,14 +125,34 @@ class LCAColumn:
         ann_activity(self.v1_loc, self.v1_transfer, self.v1_V, self.v1_A)
         self.active.compact(self.v1_A)
 
-    def autotune(self, tuner, stream=None):
+    def autotune(self, tuner, stream=None, warm_steps=20):
         """Launch shapes of the column's tunable kernels (the FP32/BF16 scatter's
-        channel GEMM), measured on the column's own buffers by an
-        autotune.LaunchTuner; call before capture().  Returns {kernel: shape}."""
+        channel GEMM, and its dense route against the ordered sparse scatter),
+        measured on the column's own buffers by an autotune.LaunchTuner after
+        `warm_steps` steps (representative sparsity); the state those steps
+        changed is restored afterwards.  Call before capture().  Returns
+        {kernel: shape}."""
+        saved = [t.clone() for t in (self.v1_V, self.v1_A, self.rec_A)]
+        for _ in range(warm_steps):
+            self.step(stream)
+        if stream is not None:
+            stream.synchronize()
+        out = self._tune(tuner, stream)
+        torch.cuda.synchronize()
+        for t, v in zip((self.v1_V, self.v1_A, self.rec_A), saved):
+            t.copy_(v)
+        self.active.compact(self.v1_A)
+        torch.cuda.synchronize()
+        return out
+
+    def _tune(self, tuner, stream):
         out = {}
         shape = tuner.tune_recv_post(self.scatter, self.v1_A, self.rec_gsyn, stream=stream)
         if shape is not None:
             out["scatter_cgemm(tile,splits)"] = list(shape)
+        mode = tuner.tune_recv_pre(self.scatter, self.rec_gsyn, self.active, self.v1_A, stream=stream)
+        if mode is not None:
+            out["scatter_mode"] = mode
         shape = tuner.tune_recv_post(self.gather, self.res_A, self.v1_gsyn, stream=stream)
         if shape is not None:
             out["gather_cgemm(tile,splits)"] = list(shape)

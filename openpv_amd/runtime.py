@@ -344,7 +344,7 @@ class PresynapticPerspectiveGPUDelivery:
         route.refresh_weights(stream)
 
     def deliver(self, gsyn, active=None, pre_activity=None, accumulate=False, stream=None):
-        if self.transposed is not None:
+        if self.transposed is not None and (getattr(self, "use_route", True) or active is None):
             if pre_activity is None:
                 raise abi.PVHIPError("the %s scatter reads the dense pre activity"
                                      % ("BF16" if self.arith == abi.ARITH_BF16 else "FP32"))

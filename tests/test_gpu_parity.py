@@ -652,6 +652,24 @@ def test_autotuner_shapes_keep_results(tmp_path):
         d2 = R.PostsynapticPerspectiveGPUDelivery(hs, ht, p, p, dev(W), arith=arith)
         t2 = LaunchTuner(cache=cache)
         assert tuple(t2.tune_recv_post(d2, dev(A), g)) == tuple(shape) and t2.log == []
+    # a pre-perspective delivery: dense MFMA route or ordered sparse scatter, both to the FP32 bar
+    (hp, op), (hq, oq), p, dens = both(SCATTER_CASES[2][0]), both(SCATTER_CASES[2][1]), SCATTER_CASES[2][2], \
+        SCATTER_CASES[2][3]
+    kx, ky, kf = O.num_kernels(op, oq)
+    Ws = (rng.standard_normal((kx * ky * kf, p, p, oq.nf)) * 0.1).astype(np.float32)
+    As = ext_random(hp, rng, dens)
+    ent, cnt = O.active_indices(op, As)
+    sref = O.recv_pre(op, oq, p, p, Ws, entries=ent, num_active=cnt, num_threads=1).astype(np.float64)
+    sd = R.PresynapticPerspectiveGPUDelivery(hp, hq, p, p, dev(Ws), arith=abi.ARITH_FP32)
+    act = R.ActiveList(hp, DEV)
+    act.compact(dev(As))
+    gs = torch.zeros(hq.nbatch * hq.num_res, device=DEV)
+    mode = LaunchTuner(cache=None).tune_recv_pre(sd, gs, act, dev(As))
+    print("scatter mode", mode, flush=True)
+    assert mode in ("route", "ordered") and sd.use_route == (mode == "route")
+    gs.zero_()
+    sd.deliver(gs, active=act, pre_activity=dev(As))
+    assert np.abs(host(gs) - sref).max() <= 1e-5 * np.abs(sref).max()
     # Hebbian split-K rows
     (hp, op), (hq, oq), p = both((2, 16, 16, 96, 2)), both((2, 64, 64, 1, 6)), 16
     a = rng.standard_normal(hp.nbatch * hp.num_ext).astype(np.float32)
