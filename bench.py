@@ -71,7 +71,7 @@ def scatter_macs(col):
         abi.host().pvhost_patch_axis(C.byref(lp), C.byref(lq), NXP, NXP, axis,
                                      d.ctypes.data_as(C.c_void_p), None, None, None, None)
         dims.append(d)
-    e, n = col.active.host_entries()
+    e, n = col.active_list().host_entries()
     total = 0
     for b in range(lp.nbatch):
         idx = e[b, :n[b]]["index"].astype(np.int64) // lp.nf
@@ -109,7 +109,7 @@ def time_form(arith, images, W, steps, warmup, torch, tuner=None):
 
 def kernel_timing(col, s, n, torch):
     """Average duration of the fused gather + LCA-update kernel
-    (recv_post_mfma_bf16_kernel<4, 1> for the BF16 form: the step's launch
+    (recv_post_mfma_bf16_kernel<4, 1, GatherBF16> for the BF16 form: the step's launch
     without the active-list emission), eager launches on the column's stream
     bracketed by HIP events recorded on that stream."""
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -405,7 +405,7 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": profiled_traffic() if head == "bf16" else None,
-            "kernel": ("recv_post_mfma_bf16_kernel<4, 1> (gather + fused LCA update)"
+            "kernel": ("recv_post_mfma_bf16_kernel<4, 1, GatherBF16> (gather + fused LCA update)"
                        if head == "bf16" else "recv_post (gather + fused LCA update)"),
             "kernel_ms": round(kavg, 5), "kernel_ms_min": round(kmin, 5),
             "algorithmic_bytes": int(alg_bytes),

@@ -387,6 +387,29 @@ int pvhip_ann_activity(
       const pvhip_loc *loc, const pvhip_transfer *transfer, const float *V, float *A,
       pvhip_stream stream);
 
+/* A residual layer (ANNErrorLayer, ErrScaleInternalStateBuffer.cpp:41-53)
+ * fed by two IdentConn: channel 0 from `in`, channel 1 from `rec`
+ * (IdentDelivery.cpp:146-162 onto reset GSyn, GSynAccumulator.kpp:21-42,
+ * errScale, then the transfer).  One pass with the operations and order of
+ * pvhip_ident_deliver x2 + pvhip_layer_update (bit-identical), without
+ * materialising GSyn.  A fusion of the reference's components, no single
+ * reference counterpart. */
+typedef struct {
+   pvhip_loc loc;     /* the error layer (halo of its activity) */
+   pvhip_loc in_loc;  /* channel-0 source, same nbatch/nx/ny/nf */
+   const float *in;   /* extended activity of in_loc */
+   pvhip_loc rec_loc; /* channel-1 source */
+   const float *rec;
+   float channel_coeffs[2]; /* GSynAccumulator coefficients, {1, -1} */
+   float scale;
+   int apply_scale;   /* 0: V = G exactly (no multiply) */
+   pvhip_transfer transfer;
+   float *V;          /* restricted */
+   float *A;          /* extended */
+} pvhip_error_update_args;
+
+int pvhip_error_update(const pvhip_error_update_args *args, pvhip_stream stream);
+
 /* IdentDelivery::deliver (src/delivery/IdentDelivery.cpp:99-165), dense form:
  * GSyn[b][k] += A_ext[b][kex(k)]. */
 int pvhip_ident_deliver(

@@ -112,6 +112,12 @@ class LayerUpdateArgs(C.Structure):
                 ("transfer", Transfer)]
 
 
+class ErrorUpdateArgs(C.Structure):
+    _fields_ = [("loc", Loc), ("in_loc", Loc), ("in_", vp), ("rec_loc", Loc), ("rec", vp),
+                ("channel_coeffs", f32 * 2), ("scale", f32), ("apply_scale", i32),
+                ("transfer", Transfer), ("V", vp), ("A", vp)]
+
+
 class LCAAdjointArgs(C.Structure):
     _fields_ = [("loc", Loc), ("transfer", Transfer), ("exp_tau", vp), ("self_interact", i32),
                 ("V", vp), ("gA", vp), ("l1", f32), ("gV", vp), ("gG", vp), ("gA_prev", vp)]
@@ -167,6 +173,7 @@ PVHIP_SYMBOLS = {
     "pvhip_gsyn_accumulate": (i32, [i32, i32, C.POINTER(f32), vp, vp, vp]),
     "pvhip_layer_update": (i32, [C.POINTER(LayerUpdateArgs), vp]),
     "pvhip_ann_activity": (i32, [C.POINTER(Loc), C.POINTER(Transfer), vp, vp, vp]),
+    "pvhip_error_update": (i32, [C.POINTER(ErrorUpdateArgs), vp]),
     "pvhip_ident_deliver": (i32, [C.POINTER(Loc), vp, vp, i32, vp]),
     "pvhip_active_compact": (i32, [C.POINTER(Loc), vp, vp, vp, vp, vp, vp]),
     "pvhip_active_compact_scratch_bytes": (C.c_size_t, [C.POINTER(Loc)]),

@@ -128,6 +128,36 @@ __global__ void __launch_bounds__(256)
    gsyn[index]          = base + a;
 }
 
+// An error layer fed by two identity connections (channel 0 from `in`,
+// channel 1 from `rec`), the residual of an LCA column: the two
+// IdentDelivery passes onto reset GSyn, GSynAccumulator, ErrScale and the
+// transfer in one pass, with the same operations in the same order as the
+// separate kernels (so the same bits, signed zeros included).
+__global__ void __launch_bounds__(256) error_update_kernel(
+      Geo loc, Geo inLoc, const float *in, Geo recLoc, const float *rec, float c0, float c1,
+      int applyScale, float scale, Transfer t, float *V, float *A) {
+   size_t const n     = (size_t)loc.nbatch * loc.numRes();
+   size_t const index = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (index >= n) {
+      return;
+   }
+   int const numNeurons = loc.numRes();
+   int const b          = (int)(index / numNeurons);
+   int const k          = (int)(index % numNeurons);
+   float const g0 = 0.0f + in[(size_t)b * inLoc.numExt() + inLoc.kex(k)];    // IdentDelivery, reset GSyn
+   float const g1 = 0.0f + rec[(size_t)b * recLoc.numExt() + recLoc.kex(k)];
+   float value    = 0.0f;                                                     // GSynAccumulator
+   float prod     = c0 * g0;
+   value          = value + prod;
+   prod           = c1 * g1;
+   value          = value + prod;
+   if (applyScale) {
+      value = value * scale;
+   }
+   V[index]                               = value;
+   A[(size_t)b * loc.numExt() + loc.kex(k)] = apply_transfer(t, value);
+}
+
 // ---------------------------------------------------------------- compaction
 //
 // Ordered stream compaction of the extended buffer of each batch element
@@ -607,6 +637,27 @@ int pvhip_ann_activity(
    hipLaunchKernelGGL(ann_activity_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
          (hipStream_t)stream, g, t, V, A);
    PVHIP_LAUNCHED("ann_activity");
+   return PVHIP_OK;
+}
+
+int pvhip_error_update(const pvhip_error_update_args *a, pvhip_stream stream) {
+   PVHIP_REQUIRE(a && a->in && a->rec && a->V && a->A, "error_update: NULL argument");
+   PVHIP_REQUIRE(loc_valid(a->loc) && loc_valid(a->in_loc) && loc_valid(a->rec_loc), "error_update: invalid loc");
+   PVHIP_REQUIRE(a->in_loc.nbatch == a->loc.nbatch && a->in_loc.nx == a->loc.nx && a->in_loc.ny == a->loc.ny
+                       && a->in_loc.nf == a->loc.nf && a->rec_loc.nbatch == a->loc.nbatch
+                       && a->rec_loc.nx == a->loc.nx && a->rec_loc.ny == a->loc.ny && a->rec_loc.nf == a->loc.nf,
+         "error_update: the identity connections need equal sizes");
+   Transfer t;
+   int st = make_transfer(a->transfer, &t);
+   if (st) {
+      return st;
+   }
+   Geo g    = geo(a->loc);
+   size_t n = (size_t)g.nbatch * g.numRes();
+   hipLaunchKernelGGL(error_update_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, g,
+         geo(a->in_loc), a->in, geo(a->rec_loc), a->rec, a->channel_coeffs[0], a->channel_coeffs[1],
+         a->apply_scale, a->scale, t, a->V, a->A);
+   PVHIP_LAUNCHED("error_update");
    return PVHIP_OK;
 }
 

@@ -22,7 +22,7 @@ import torch
 
 from . import abi
 from .runtime import (ActiveList, HyPerLCAUpdate, PostsynapticPerspectiveGPUDelivery,
-                      PresynapticPerspectiveGPUDelivery, cost_sum_f64, fill_zero, ident_deliver,
+                      PresynapticPerspectiveGPUDelivery, cost_sum_f64, error_update, fill_zero,
                       layer_update, loc, make_transfer, _stream)
 
 FLT_MAX = float(np.finfo(np.float32).max)
@@ -61,7 +61,6 @@ class LCAColumn:
         d = self.device
         f = dict(dtype=torch.float32, device=d)
         self.inputA = images.reshape(-1).to(d).contiguous()
-        self.res_gsyn = torch.zeros(2 * nb * self.res_loc.num_res, **f)
         self.res_V = torch.zeros(nb * self.res_loc.num_res, **f)
         self.res_A = torch.zeros(nb * self.res_loc.num_ext, **f)
         self.v1_gsyn = torch.zeros(nb * self.v1_loc.num_res, **f)
@@ -112,12 +111,17 @@ class LCAColumn:
                                   self_interact, activity_from_v=True, device=d)
         self.energy = torch.zeros(2, nb, dtype=torch.float64, device=d)
         self._graph = None
-        self._fused_args = self._gather_args = None
+        self._fused_args = self._gather_args = self._gather_args_plain = None
+        # V1's active list is published when a consumer reads it in the step (the
+        # ordered scatter), else marked stale and rebuilt on request -- the
+        # reference's publisher also rebuilds it lazily (Publisher.cpp:85-134)
+        self._active_stale = False
         if fused:
             # argument blocks (and the delivery's scratch) prepared outside any graph capture
             self._fused_args = self.lca.args(None, self.v1_V, self.v1_A, num_channels=1)
             self._gather_args = self.gather.args(self.res_A, None, False, fused=self._fused_args,
                                                  active=self.active, halo_is_zero=True)
+            self._gather_args_plain = self.gather.args(self.res_A, None, False, fused=self._fused_args)
         # InitializeState: V1 V from V0, A = T(V); Residual and Recon start at 0.
         if V0 is not None:
             self.v1_V.copy_(torch.as_tensor(V0, dtype=torch.float32).reshape(-1).to(d))
@@ -150,7 +154,8 @@ class LCAColumn:
         shape = tuner.tune_recv_post(self.scatter, self.v1_A, self.rec_gsyn, stream=stream)
         if shape is not None:
             out["scatter_cgemm(tile,splits)"] = list(shape)
-        mode = tuner.tune_recv_pre(self.scatter, self.rec_gsyn, self.active, self.v1_A, stream=stream)
+        mode = tuner.tune_recv_pre(self.scatter, self.rec_gsyn, self.active_list(stream), self.v1_A,
+                                   stream=stream)
         if mode is not None:
             out["scatter_mode"] = mode
         shape = tuner.tune_recv_post(self.gather, self.res_A, self.v1_gsyn, stream=stream)
@@ -164,19 +169,18 @@ class LCAColumn:
         `stream` around the V1 gather (+ fused update) launch."""
         s = stream
         nb = self.in_loc.nbatch
-        # phase 1: Residual = err_scale * (Input - Recon(t-1))
-        n = nb * self.res_loc.num_res
-        g0, g1 = self.res_gsyn[:n], self.res_gsyn[n:]
-        ident_deliver(self.in_loc, self.inputA, g0, accumulate=False, stream=s)
-        ident_deliver(self.rec_loc, self.rec_A, g1, accumulate=False, stream=s)
-        layer_update(self.res_loc, self.res_gsyn, self.res_V, self.res_A, self.none_transfer,
-                     channel_coeffs=(1.0, -1.0), err_scale=self.err_scale, stream=s)
+        # phase 1: Residual = err_scale * (Input - Recon(t-1)): both identity
+        # deliveries and the layer update in one pass (GSyn not materialised)
+        error_update(self.res_loc, self.in_loc, self.inputA, self.rec_loc, self.rec_A, self.res_V,
+                     self.res_A, self.none_transfer, channel_coeffs=(1.0, -1.0),
+                     err_scale=self.err_scale, stream=s)
         # phase 2: V1 gather + LCA update (+ threshold), then its active list
         if gather_events is not None:
             gather_events[0].record(s)
+        needs_list = not self.dense_scatter()
         if self.fused:
-            # gather + LCA update + V1's active list (its halo is never written) in one call
-            self.gather.launch(self._gather_args, stream=s)
+            # gather + LCA update (+ V1's active list, its halo is never written) in one call
+            self.gather.launch(self._gather_args if needs_list else self._gather_args_plain, stream=s)
             if gather_events is not None:
                 gather_events[1].record(s)
         else:
@@ -184,7 +188,9 @@ class LCAColumn:
             self.lca.update(self.v1_gsyn, self.v1_V, self.v1_A, stream=s)
             if gather_events is not None:
                 gather_events[1].record(s)
-            self.active.compact(self.v1_A, stream=s)
+            if needs_list:
+                self.active.compact(self.v1_A, stream=s)
+        self._active_stale = not needs_list
         # phase 3: Recon = scatter of the sparse V1 activity
         self.scatter.deliver(self.rec_gsyn, active=self.active, pre_activity=self.v1_A,
                              accumulate=False, stream=s)
@@ -223,10 +229,12 @@ class LCAColumn:
                 st = H.pvhip_graph_end(_stream(stream), C.byref(g))
             abi.check(st, "pvhip_graph_end")
         self._graph = g
+        self._graph_leaves_stale = self._active_stale   # host flag set while capturing
         return g
 
     def replay(self, stream):
         abi.check(abi.hip().pvhip_graph_launch(self._graph, _stream(stream)), "pvhip_graph_launch")
+        self._active_stale = self._graph_leaves_stale
 
     def release_graph(self):
         if self._graph is not None:
@@ -237,8 +245,21 @@ class LCAColumn:
     def v1_state(self):
         return self.v1_V.cpu().numpy(), self.v1_A.cpu().numpy()
 
+    def dense_scatter(self):
+        """True when the Recon scatter reads V1's dense activity (the MFMA route)
+        rather than its active list."""
+        return self.scatter.transposed is not None and getattr(self.scatter, "use_route", True)
+
+    def active_list(self, stream=None):
+        """V1's active list after the last step (rebuilt here when the step did
+        not need it)."""
+        if self._active_stale:
+            self.active.compact(self.v1_A, stream=stream)
+            self._active_stale = False
+        return self.active
+
     def num_active(self):
-        return self.active.num_active.cpu().numpy()
+        return self.active_list().num_active.cpu().numpy()
 
 
 def run_column(col, steps, frames=None, display_period=0, adaptive=None, energy_cost="l1",

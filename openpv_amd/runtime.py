@@ -426,6 +426,22 @@ def layer_update(loc_, gsyn, V, A, transfer, channel_coeffs=(1.0,), err_scale=No
     abi.check(abi.hip().pvhip_layer_update(C.byref(a), _stream(stream)), "pvhip_layer_update")
 
 
+def error_update(loc_, in_loc, in_activity, rec_loc, rec_activity, V, A, transfer,
+                 channel_coeffs=(1.0, -1.0), err_scale=None, stream=None):
+    """Residual layer fed by two identity connections (channel 0 from in, channel
+    1 from rec) in one pass: the same result as two ident_deliver calls onto a
+    reset GSyn followed by layer_update, bit for bit."""
+    a = abi.ErrorUpdateArgs()
+    a.loc, a.in_loc, a.rec_loc = loc_, in_loc, rec_loc
+    a.in_, a.rec = _ptr(in_activity), _ptr(rec_activity)
+    a.channel_coeffs[0], a.channel_coeffs[1] = channel_coeffs
+    a.scale = 1.0 if err_scale is None else err_scale
+    a.apply_scale = 0 if err_scale is None else 1
+    a.transfer = transfer
+    a.V, a.A = _ptr(V), _ptr(A)
+    abi.check(abi.hip().pvhip_error_update(C.byref(a), _stream(stream)), "pvhip_error_update")
+
+
 def ident_deliver(loc_, pre_activity, gsyn, accumulate=False, stream=None):
     abi.check(abi.hip().pvhip_ident_deliver(C.byref(loc_), _ptr(pre_activity), _ptr(gsyn),
                                             int(accumulate), _stream(stream)), "pvhip_ident_deliver")

@@ -114,6 +114,32 @@ def test_lca_update_bit_exact(vertices, params, from_v):
     assert same(host(Vd), Vr) and same(host(Ad), Ar)
 
 
+def test_error_update_equals_ident_ident_layer_update():
+    """pvhip_error_update (the residual layer in one pass) against the separate
+    pvhip_ident_deliver x2 + pvhip_layer_update, bit for bit, signed zeros
+    included, with and without errScale and with a non-identity transfer."""
+    rng = np.random.default_rng(24)
+    hl = abi.Loc.make(2, 9, 7, 3, 3)
+    hi = abi.Loc.make(2, 9, 7, 3, 0)
+    hr = abi.Loc.make(2, 9, 7, 3, (1, 2, 0, 1))
+    n = hl.nbatch * hl.num_res
+    xin = (rng.standard_normal(hi.nbatch * hi.num_ext) * 0.5).astype(np.float32)
+    xrec = (rng.standard_normal(hr.nbatch * hr.num_ext) * 0.5).astype(np.float32)
+    xin[::7] = -0.0
+    xrec[::5] = -0.0
+    xrec[::11] = 0.0
+    for scale, tr in ((None, R.make_transfer()), (0.75, R.make_transfer()),
+                      (1.25, R.make_transfer(0.1, 0.0, FMAX, 0.05, 0.1))):
+        g = torch.zeros(2 * n, device=DEV)
+        V1, A1 = torch.zeros(n, device=DEV), torch.full((hl.nbatch * hl.num_ext,), 3.0, device=DEV)
+        R.ident_deliver(hi, dev(xin), g[:n], accumulate=False)
+        R.ident_deliver(hr, dev(xrec), g[n:], accumulate=False)
+        R.layer_update(hl, g, V1, A1, tr, channel_coeffs=(1.0, -1.0), err_scale=scale)
+        V2, A2 = torch.zeros(n, device=DEV), torch.full((hl.nbatch * hl.num_ext,), 3.0, device=DEV)
+        R.error_update(hl, hi, dev(xin), hr, dev(xrec), V2, A2, tr, err_scale=scale)
+        assert same(host(V1), host(V2)) and same(host(A1), host(A2))
+
+
 def test_layer_updates_and_ident_bit_exact():
     rng = np.random.default_rng(22)
     hl, ol = both((2, 9, 6, 3, 3))
