@@ -558,6 +558,30 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
          }
       }
    }
+   // Weight (A) fragments: KPRE k-steps per group, rows clamped to the last
+   // feature (rows past it give outputs the epilogue never stores) and k
+   // clamped inside Kpad, so every load is unconditional.  The first group is
+   // issued here, before the staging; later groups are prefetched during the
+   // MFMAs of the one before.
+   constexpr int KPRE = 4; // k-steps whose A fragments are fetched together
+   using RawA         = typename E::RawA;
+   const T *rowA[MFB];
+#pragma unroll
+   for (int fb = 0; fb < MFB; fb++) {
+      rowA[fb] = Wb + (size_t)min(fWave + fb * 16 + r, p.post.nf - 1) * Kpad;
+   }
+   auto loadA = [&](RawA (&dst)[KPRE][MFB], int kb) {
+#pragma unroll
+      for (int st = 0; st < KPRE; st++) {
+         int const kk = min(kb + 32 * st, Kpad - 32) + 8 * g;
+#pragma unroll
+         for (int fb = 0; fb < MFB; fb++) {
+            dst[st][fb] = *reinterpret_cast<const RawA *>(rowA[fb] + kk);
+         }
+      }
+   };
+   RawA aC[KPRE][MFB], aN[KPRE][MFB];
+   loadA(aC, 0);
    int const preRow  = p.pre.nxExt() * p.pre.nf;
    float const *preB = p.pre_data + (size_t)b * p.pre.numExt();
    int const rowElems = srcCols * p.nfp;
@@ -626,71 +650,103 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
          acc[fb][pb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
       }
    }
-   constexpr int KPRE = 4; // k-steps whose A fragments are fetched together
-   using RawA         = typename E::RawA;
-   for (int kb = 0; kb < Kpad; kb += 32 * KPRE) {
-      RawA apre[KPRE][MFB]; // fragments kept as raw registers
+   // B fragment of k-step k0 at position block pb, the k values masked by koff
+   // (per-element reads: patch rows not a multiple of 8, or zero padding)
+   auto frag_masked = [&](int pb, const int *kos) {
+      typename E::Frag bfrag;
+      if constexpr (E::SCALED) {
+         uint64_t v = 0;
 #pragma unroll
-      for (int st = 0; st < KPRE; st++) {
-         int const kk = kb + 32 * st + 8 * g;
-#pragma unroll
-         for (int fb = 0; fb < MFB; fb++) {
-            int f = fWave + fb * 16 + r;
-            apre[st][fb] = (f < p.post.nf && kk < Kpad)
-                                 ? *reinterpret_cast<const RawA *>(Wb + (size_t)f * Kpad + kk)
-                                 : RawA{};
+         for (int e = 0; e < 8; e++) {
+            v |= (uint64_t)(kos[e] >= 0 ? src[pbase[pb] + kos[e]] : (T)0) << (8 * e);
          }
+         bfrag = (long)v;
+      }
+      else {
+#pragma unroll
+         for (int e = 0; e < 8; e++) {
+            bfrag[e] = kos[e] >= 0 ? src[pbase[pb] + kos[e]] : (__bf16)0.0f;
+         }
+      }
+      return bfrag;
+   };
+   // Complete groups of KPRE k-steps whose 32 k are all real (no zero
+   // padding): one vector LDS read per fragment, no branch and no condition
+   // inside the group, so the accumulators stay in the MFMA registers.
+   int const Kfull  = ALIGN != 0 ? K / 32 * 32 : 0;
+   int const KfullG = Kfull / (32 * KPRE) * (32 * KPRE);
+   for (int kb = 0; kb < KfullG; kb += 32 * KPRE) {
+      if (kb + 32 * KPRE < KfullG) {
+         loadA(aN, kb + 32 * KPRE); // in flight during this group's MFMAs
       }
 #pragma unroll
       for (int st = 0; st < KPRE; st++) {
-         int const kk = kb + 32 * st + 8 * g;
-         if (kb + 32 * st < Kpad) {
-         int const ko = koff[kk];
-         int kos[8];
-         if (ALIGN == 0) {
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-               kos[e] = koff[kk + e];
-            }
-         }
+         int const ko = koff[kb + 32 * st + 8 * g];
 #pragma unroll
          for (int pb = 0; pb < MPB; pb++) {
             typename E::Frag bfrag;
             if constexpr (E::SCALED) {
-               if (ALIGN == 0) {
-                  uint64_t v = 0;
-#pragma unroll
-                  for (int e = 0; e < 8; e++) {
-                     v |= (uint64_t)(kos[e] >= 0 ? src[pbase[pb] + kos[e]] : (T)0) << (8 * e);
-                  }
-                  bfrag = (long)v;
-               }
-               else {
-                  bfrag = ko >= 0 ? lds_frag8<ALIGN>(src + pbase[pb] + ko) : 0L;
-               }
+               bfrag = lds_frag8<ALIGN>(src + pbase[pb] + ko);
             }
             else {
-               if (ALIGN == 0) {
-#pragma unroll
-                  for (int e = 0; e < 8; e++) {
-                     bfrag[e] = kos[e] >= 0 ? src[pbase[pb] + kos[e]] : (__bf16)0.0f;
-                  }
-               }
-               else {
-                  bfrag = ko >= 0 ? lds_frag<ALIGN>(src + pbase[pb] + ko) : bf16x8{};
-               }
+               bfrag = lds_frag<ALIGN>(src + pbase[pb] + ko);
             }
 #pragma unroll
             for (int fb = 0; fb < MFB; fb++) {
-               acc[fb][pb] = E::mfma(apre[st][fb], bfrag, acc[fb][pb]);
+               acc[fb][pb] = E::mfma(aC[st][fb], bfrag, acc[fb][pb]);
             }
          }
+      }
+#pragma unroll
+      for (int st = 0; st < KPRE; st++) {
+#pragma unroll
+         for (int fb = 0; fb < MFB; fb++) {
+            aC[st][fb] = aN[st][fb];
          }
       }
    }
-   // Lane holds D[4g+i][r]: features fWave+fb*16+4g+i at position pb*16+r.
-   // Transpose through LDS (the staged source is dead now) so that each
-   // position's MF features are written as one contiguous run.
+   // the remaining k-steps one at a time (all of them for ALIGN == 0): vector
+   // reads while the step is all real, masked reads for the padded one
+   for (int k0 = KfullG; k0 < Kpad; k0 += 32) {
+      int const kk = k0 + 8 * g;
+      RawA a1[MFB];
+#pragma unroll
+      for (int fb = 0; fb < MFB; fb++) {
+         a1[fb] = *reinterpret_cast<const RawA *>(rowA[fb] + kk);
+      }
+      if (ALIGN != 0 && k0 + 32 <= K) {
+         int const ko = koff[kk];
+#pragma unroll
+         for (int pb = 0; pb < MPB; pb++) {
+            typename E::Frag bfrag;
+            if constexpr (E::SCALED) {
+               bfrag = lds_frag8<ALIGN>(src + pbase[pb] + ko);
+            }
+            else {
+               bfrag = lds_frag<ALIGN>(src + pbase[pb] + ko);
+            }
+#pragma unroll
+            for (int fb = 0; fb < MFB; fb++) {
+               acc[fb][pb] = E::mfma(a1[fb], bfrag, acc[fb][pb]);
+            }
+         }
+      }
+      else {
+         int kos[8];
+#pragma unroll
+         for (int e = 0; e < 8; e++) {
+            kos[e] = koff[kk + e];
+         }
+#pragma unroll
+         for (int pb = 0; pb < MPB; pb++) {
+            typename E::Frag bfrag = frag_masked(pb, kos);
+#pragma unroll
+            for (int fb = 0; fb < MFB; fb++) {
+               acc[fb][pb] = E::mfma(a1[fb], bfrag, acc[fb][pb]);
+            }
+         }
+      }
+   }
    if constexpr (E::SCALED) {
       // back from the scaled operands: D[f][pos] / (weight scale of f * source scale)
 #pragma unroll
