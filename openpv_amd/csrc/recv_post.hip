@@ -436,6 +436,7 @@ __device__ __forceinline__ void lca_epilogue_vthresh(
    bool const thr     = t.VThresh > -FLT_MAX;
    bool const ramp    = t.VThresh + t.VWidth > t.VThresh;
    bool const clamp   = t.AMax < FLT_MAX;
+   bool const simple  = thr && !ramp && !clamp && q.aFromV && p.dtf == 1.0f; // uniform
    int const nf       = p.post.nf;
    int const rowV     = p.post.nx * nf;
    int const rowA     = q.loc.nxExt() * nf;
@@ -448,26 +449,43 @@ __device__ __forceinline__ void lca_epilogue_vthresh(
       int const xo    = it * MEPP % MTX * nf;
       f32x4 const acc = *reinterpret_cast<const f32x4 *>(out + it * MEPP * MOP);
       f32x4 const vo  = vpre[it];
-      f32x4 ao;
-      if (q.aFromV) {
+      f32x4 vn, an;
+      if (simple) {
+         // the plain threshold (no ramp, no AMax), dtFactor 1, A(t-1) from V:
+         // the same operations as the general path with those terms dropped
 #pragma unroll
          for (int i = 0; i < 4; i++) {
-            ao[i] = vthresh_select(vo[i], thr, ramp, clamp, t.VThresh, t.AMin, t.AShift, t.AMax, t.VWidth);
+            float const ao = vo[i] < t.VThresh ? t.AMin : vo[i] - t.AShift;
+            float G        = 0.0f + acc[i];
+            float t1       = e * vo[i];
+            float sa       = si * ao;
+            float t3       = G + sa;
+            float t4       = ome * t3;
+            vn[i]          = t1 + t4;
+            an[i]          = vn[i] < t.VThresh ? t.AMin : vn[i] - t.AShift;
          }
       }
       else {
-         ao = *reinterpret_cast<const f32x4 *>(Ap + yl * rowA + xo);
-      }
-      f32x4 vn, an;
+         f32x4 ao;
+         if (q.aFromV) {
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-         float G  = 0.0f + p.dtf * acc[i];
-         float t1 = e * vo[i];
-         float sa = si * ao[i];
-         float t3 = G + sa;
-         float t4 = ome * t3;
-         vn[i]    = t1 + t4;
-         an[i]    = vthresh_select(vn[i], thr, ramp, clamp, t.VThresh, t.AMin, t.AShift, t.AMax, t.VWidth);
+            for (int i = 0; i < 4; i++) {
+               ao[i] = vthresh_select(vo[i], thr, ramp, clamp, t.VThresh, t.AMin, t.AShift, t.AMax, t.VWidth);
+            }
+         }
+         else {
+            ao = *reinterpret_cast<const f32x4 *>(Ap + yl * rowA + xo);
+         }
+#pragma unroll
+         for (int i = 0; i < 4; i++) {
+            float G  = 0.0f + p.dtf * acc[i];
+            float t1 = e * vo[i];
+            float sa = si * ao[i];
+            float t3 = G + sa;
+            float t4 = ome * t3;
+            vn[i]    = t1 + t4;
+            an[i]    = vthresh_select(vn[i], thr, ramp, clamp, t.VThresh, t.AMin, t.AShift, t.AMax, t.VWidth);
+         }
       }
       __builtin_nontemporal_store(vn, reinterpret_cast<f32x4 *>(Vp + yl * rowV + xo));
       __builtin_nontemporal_store(an, reinterpret_cast<f32x4 *>(Ap + yl * rowA + xo));
