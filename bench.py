@@ -109,16 +109,17 @@ def time_form(arith, images, W, steps, warmup, torch, tuner=None):
 
 def kernel_timing(col, s, n, torch):
     """Average duration of the fused gather + LCA-update kernel
-    (recv_post_mfma_bf16_kernel<4, 1, GatherBF16> for the BF16 form: the step's launch
-    without the active-list emission), eager launches on the column's stream
-    bracketed by HIP events recorded on that stream."""
+    (recv_post_mfma_bf16_kernel<4, 1, GatherBF16> for the BF16 form), launched
+    eagerly with the argument block the step uses when its scatter is the dense
+    route (no active-list emission; V1's chunked BF16 activity written when the
+    column keeps one), bracketed by HIP events recorded on the column's stream."""
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(n)]
-    args = col.lca.args(None, col.v1_V, col.v1_A, num_channels=1)
+    args = col._gather_args_plain
     with torch.cuda.stream(s):
         for a, b in evs:
             a.record(s)
-            col.gather.deliver(col.res_A, None, accumulate=False, fused=args, stream=s)
+            col.gather.launch(args, stream=s)
             b.record(s)
     s.synchronize()
     torch.cuda.synchronize()
@@ -400,7 +401,9 @@ def main():
     n_v1 = NB * NXV * NYV * NFV
     src_bytes = NB * col.res_loc.num_ext * 4
     w_bytes = NFV * NXP * NXP * NF * (2 if head == "bf16" else 4)
-    alg_bytes = n_v1 * 12 + src_bytes + w_bytes   # V read+write, A write, source, weights
+    # V read + write, then A written in FP32 (4 B) or, when the scatter reads
+    # V1's chunked BF16 copy, only that copy (2 B); source; weights
+    alg_bytes = n_v1 * (10 if col.v1_A16 is not None else 12) + src_bytes + w_bytes
     achieved = alg_bytes / (kavg * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PVHIP_ABI_VERSION 5
+#define PVHIP_ABI_VERSION 6
 
 enum {
    PVHIP_OK              = 0,
@@ -195,7 +195,9 @@ typedef struct {
    float channel_coeffs[PVHIP_MAX_CHANNELS];
    const float *gsyn;    /* [num_channels][nbatch][ny][nx][nf] */
    float *V;             /* restricted */
-   float *A;             /* extended; halo untouched */
+   float *A;             /* extended; halo untouched.  May be NULL when A_bf16c is
+                          * set and activity_from_v holds: only the copy is written,
+                          * and A = transfer(V) is derived when a reader needs it. */
    const float *exp_tau; /* device [nbatch] */
    int self_interact;
    pvhip_transfer transfer;
@@ -205,7 +207,24 @@ typedef struct {
     * self-interaction term is then recomputed from V instead of read, which
     * saves one read of A per neuron with identical results. */
    int activity_from_v;
+   /* Optional (NULL: none): also write each updated activity, rounded to BF16
+    * (round to nearest even), into this copy of A in the chunked BF16 layout
+    * of pvhip_activity_to_bf16c (loc.nf a multiple of 32).  A BF16 channel GEMM
+    * reads it through pvhip_recv_post_args.pre_data_bf16c.  Only restricted
+    * neurons are written; the halo keeps what the caller put there. */
+   void *A_bf16c;
 } pvhip_lca_update_args;
+
+/* Chunked BF16 copy of a layer's extended activity, the source format of the
+ * BF16 channel GEMM: [nbatch][nf/32][nyExt][nxExt][32] values, feature f of
+ * extended position (xe, ye) of batch b at
+ *   ((b*(nf/32) + f/32)*nyExt + ye)*nxExt + xe)*32 + f%32,
+ * each the activity rounded to BF16 (round to nearest even).  A chunk of 32
+ * features of a row of positions is one contiguous run, which is what the GEMM
+ * stages.  Writes the whole buffer (halo included): nbatch*numExt*2 bytes.
+ * No counterpart in the reference (its GPU delivery reads the FP32 activity
+ * through cuDNN, CudaRecvPost.cpp:51-389). */
+int pvhip_activity_to_bf16c(const pvhip_loc *loc, const float *A, void *out, pvhip_stream stream);
 
 /* Host helper: exp_tau[b] = (float)exp(-dt[b]/(double)tau), the reference's
  * double-precision exp then cast (HyPerLCAInternalStateBuffer.kpp:51). */
@@ -269,6 +288,13 @@ typedef struct {
     * uses the format's range (0 = 1).  weights_lp then holds the FP8 shadow of
     * pvhip_recv_post_fp8_weights, which carries its own per-kernel scales. */
    float lp_source_scale;
+   /* Optional, BF16 channel GEMM only (NULL: convert pre_data while staging):
+    * pre_data in the chunked BF16 layout of pvhip_activity_to_bf16c (pre.nf a
+    * multiple of 32), e.g. written by the producing layer's update
+    * (pvhip_lca_update_args.A_bf16c).  The GEMM then reads half the bytes, in
+    * contiguous runs, with the same results: it rounds pre_data to BF16 the
+    * same way.  The other forms ignore it. */
+   const void *pre_data_bf16c;
 } pvhip_recv_post_args;
 
 /* Active-list outputs of a fused delivery (same layout and contents as

@@ -76,7 +76,7 @@ class Transfer(C.Structure):
 class LCAUpdateArgs(C.Structure):
     _fields_ = [("loc", Loc), ("num_channels", i32), ("channel_coeffs", f32 * MAX_CHANNELS),
                 ("gsyn", vp), ("V", vp), ("A", vp), ("exp_tau", vp), ("self_interact", i32),
-                ("transfer", Transfer), ("activity_from_v", i32)]
+                ("transfer", Transfer), ("activity_from_v", i32), ("A_bf16c", vp)]
 
 
 class ActiveOut(C.Structure):
@@ -94,7 +94,8 @@ class RecvPostArgs(C.Structure):
                 ("window_y0", i32), ("window_ny", i32),
                 ("fused_update", C.POINTER(LCAUpdateArgs)),
                 ("active_out", C.POINTER(ActiveOut)), ("scratch", vp), ("scratch_bytes", C.c_size_t),
-                ("cgemm_tile", i32), ("cgemm_splits", i32), ("lp_source_scale", f32)]
+                ("cgemm_tile", i32), ("cgemm_splits", i32), ("lp_source_scale", f32),
+                ("pre_data_bf16c", vp)]
 
 
 class RecvPreArgs(C.Structure):
@@ -174,6 +175,7 @@ PVHIP_SYMBOLS = {
     "pvhip_layer_update": (i32, [C.POINTER(LayerUpdateArgs), vp]),
     "pvhip_ann_activity": (i32, [C.POINTER(Loc), C.POINTER(Transfer), vp, vp, vp]),
     "pvhip_error_update": (i32, [C.POINTER(ErrorUpdateArgs), vp]),
+    "pvhip_activity_to_bf16c": (i32, [C.POINTER(Loc), vp, vp, vp]),
     "pvhip_ident_deliver": (i32, [C.POINTER(Loc), vp, vp, i32, vp]),
     "pvhip_active_compact": (i32, [C.POINTER(Loc), vp, vp, vp, vp, vp, vp]),
     "pvhip_active_compact_scratch_bytes": (C.c_size_t, [C.POINTER(Loc)]),

@@ -14,7 +14,9 @@ int make_lca_params(const pvhip_lca_update_args &a, LCAParams *p) {
    PVHIP_REQUIRE(
          a.num_channels >= 0 && a.num_channels <= PVHIP_MAX_CHANNELS,
          "lca_update: num_channels %d outside [0,%d]", a.num_channels, PVHIP_MAX_CHANNELS);
-   PVHIP_REQUIRE(a.V && a.A && a.exp_tau, "lca_update: NULL V, A or exp_tau");
+   PVHIP_REQUIRE(a.V && a.exp_tau, "lca_update: NULL V or exp_tau");
+   PVHIP_REQUIRE(a.A || (a.A_bf16c && a.activity_from_v),
+         "lca_update: NULL A (allowed only with A_bf16c and activity_from_v)");
    PVHIP_REQUIRE(a.num_channels == 0 || a.gsyn, "lca_update: NULL gsyn");
    p->loc         = geo(a.loc);
    p->numChannels = a.num_channels;
@@ -27,6 +29,10 @@ int make_lca_params(const pvhip_lca_update_args &a, LCAParams *p) {
    p->expTau       = a.exp_tau;
    p->selfInteract = a.self_interact ? 1.0f : 0.0f;
    p->aFromV       = a.activity_from_v ? 1 : 0;
+   p->Ab           = static_cast<__bf16 *>(a.A_bf16c);
+   PVHIP_REQUIRE(!a.A_bf16c || (a.loc.nf % 32 == 0 && reinterpret_cast<uintptr_t>(a.A_bf16c) % 16 == 0),
+         "lca_update: the chunked BF16 activity needs nf %% 32 == 0 (nf %d) and 16-byte alignment",
+         a.loc.nf);
    return make_transfer(a.transfer, &p->transfer);
 }
 
@@ -526,6 +532,30 @@ __global__ void __launch_bounds__(256) segment_pos_kernel(
    posStart[(size_t)b * (numPos + 1) + p] = v;
 }
 
+// Chunked BF16 copy of an extended activity buffer: one thread per 8 output
+// values (16 bytes), read as two float4 of the PV layout.
+__global__ void __launch_bounds__(256) activity_to_bf16c_kernel(Geo g, const float *A, __bf16 *out) {
+   size_t const plane = (size_t)g.nxExt() * g.nyExt();
+   int const chunks   = g.nf / 32;
+   size_t const n8    = (size_t)g.nbatch * chunks * plane * 4;
+   size_t const i     = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= n8) {
+      return;
+   }
+   int const q        = (int)(i % 4);
+   size_t const pos   = (i / 4) % plane;
+   size_t const bc    = i / (4 * plane); // b * chunks + chunk
+   int const chunk    = (int)(bc % chunks);
+   size_t const b     = bc / chunks;
+   float const *src   = A + b * (size_t)g.numExt() + pos * g.nf + chunk * 32 + q * 8;
+   float4 const lo    = *reinterpret_cast<const float4 *>(src);
+   float4 const hi    = *reinterpret_cast<const float4 *>(src + 4);
+   typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+   bf16x8v v{(__bf16)lo.x, (__bf16)lo.y, (__bf16)lo.z, (__bf16)lo.w,
+             (__bf16)hi.x, (__bf16)hi.y, (__bf16)hi.z, (__bf16)hi.w};
+   *reinterpret_cast<bf16x8v *>(out + i * 8) = v;
+}
+
 } // namespace
 
 size_t compact_scratch_bytes(const Geo &g) {
@@ -571,6 +601,20 @@ int pvhip_lca_update(const pvhip_lca_update_args *args, pvhip_stream stream) {
    hipLaunchKernelGGL(lca_update_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
          (hipStream_t)stream, p);
    PVHIP_LAUNCHED("lca_update");
+   return PVHIP_OK;
+}
+
+int pvhip_activity_to_bf16c(const pvhip_loc *loc, const float *A, void *out, pvhip_stream stream) {
+   PVHIP_REQUIRE(loc != nullptr && loc_valid(*loc), "activity_to_bf16c: invalid loc");
+   PVHIP_REQUIRE(loc->nf % 32 == 0, "activity_to_bf16c: nf %d is not a multiple of 32", loc->nf);
+   PVHIP_REQUIRE(A != nullptr && out != nullptr, "activity_to_bf16c: NULL buffer");
+   PVHIP_REQUIRE(((uintptr_t)A % 16) == 0 && ((uintptr_t)out % 16) == 0,
+         "activity_to_bf16c: buffers must be 16-byte aligned");
+   Geo const g    = geo(*loc);
+   size_t const n = (size_t)g.nbatch * g.numExt() / 8;
+   hipLaunchKernelGGL(activity_to_bf16c_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (hipStream_t)stream, g, A, static_cast<__bf16 *>(out));
+   PVHIP_LAUNCHED("activity_to_bf16c");
    return PVHIP_OK;
 }
 

@@ -192,7 +192,18 @@ struct LCAParams {
    float selfInteract;
    Transfer transfer;
    int aFromV; // A(t-1) = transfer(V(t-1)) is recomputed instead of read
+   __bf16 *Ab; // optional chunked BF16 copy of A (pvhip_activity_to_bf16c layout);
+               // A may then be NULL (with aFromV): only the copy is written
 };
+
+// Element of extended neuron kex (batch b) in the chunked BF16 layout:
+// [nbatch][nf/32][nyExt][nxExt][32].
+__device__ __forceinline__ size_t bf16c_index(const Geo &g, int b, int kex) {
+   int const f   = kex % g.nf;
+   int const pos = kex / g.nf;
+   size_t const plane = (size_t)g.nxExt() * g.nyExt();
+   return (((size_t)b * (g.nf / 32) + f / 32) * plane + pos) * 32 + f % 32;
+}
 
 int make_lca_params(const pvhip_lca_update_args &a, LCAParams *p);
 
@@ -218,8 +229,14 @@ __device__ __forceinline__ void lca_update_one(const LCAParams &p, int b, int k,
    float t3    = G + si;
    float t4    = t2 * t3;
    float Vnew  = t1 + t4;
+   float Anew  = transfer_as<MODE>(p.transfer, Vnew);
    p.V[vi]     = Vnew;
-   p.A[ai]     = transfer_as<MODE>(p.transfer, Vnew);
+   if (p.A) {
+      p.A[ai] = Anew;
+   }
+   if (p.Ab) {
+      p.Ab[bf16c_index(p.loc, b, p.loc.kex(k))] = (__bf16)Anew;
+   }
 }
 
 constexpr int kWave = 64;

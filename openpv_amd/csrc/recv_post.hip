@@ -15,6 +15,7 @@
 #include "pvhip_internal.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace pvhip {
 namespace {
@@ -40,6 +41,8 @@ struct PostParams {
    // FP8 form: per-kernel weight scales (after the FP8 shadow) and the source scale
    const float *lpWScale;
    float lpSrcScale;
+   // BF16 channel GEMM: pre_data in the chunked BF16 layout, or NULL
+   const __bf16 *preB16;
 };
 
 __device__ __forceinline__ void post_epilogue(const PostParams &p, int b, int k, float G) {
@@ -388,7 +391,14 @@ __device__ __forceinline__ void post_epilogue4(
       an[i]    = transfer_as<EPI == 1 ? 1 : 0>(q.transfer, vn[i]);
    }
    __builtin_nontemporal_store(vn, reinterpret_cast<f32x4 *>(q.V + gi));
-   __builtin_nontemporal_store(an, reinterpret_cast<f32x4 *>(q.A + ai));
+   if (q.A) {
+      __builtin_nontemporal_store(an, reinterpret_cast<f32x4 *>(q.A + ai));
+   }
+   if (q.Ab) {
+      // features k..k+3 sit in one 32-feature chunk (k % 4 == 0)
+      *reinterpret_cast<bf16x4 *>(q.Ab + bf16c_index(q.loc, b, (int)(ai - (size_t)b * q.loc.numExt()))) =
+            bf16x4{(__bf16)an[0], (__bf16)an[1], (__bf16)an[2], (__bf16)an[3]};
+   }
 }
 
 // Epilogue row layout: MEQ threads cover the MF features of one position
@@ -441,8 +451,13 @@ __device__ __forceinline__ void lca_epilogue_vthresh(
    int const rowV     = p.post.nx * nf;
    int const rowA     = q.loc.nxExt() * nf;
    float *Vp = q.V + (size_t)b * p.post.numRes() + (size_t)(y0 * p.post.nx + x0 + epr) * nf + ef;
-   float *Ap = q.A + (size_t)b * q.loc.numExt()
-               + (size_t)((y0 + q.loc.up) * q.loc.nxExt() + x0 + epr + q.loc.lt) * nf + ef;
+   float *Ap = q.A ? q.A + (size_t)b * q.loc.numExt()
+                           + (size_t)((y0 + q.loc.up) * q.loc.nxExt() + x0 + epr + q.loc.lt) * nf + ef
+                   : nullptr; // NULL: only the chunked BF16 copy is written (A(t-1) from V)
+   // chunked BF16 copy of A (optional): 4 features of one 32-feature chunk
+   __bf16 *Abp = q.Ab ? q.Ab + ((((size_t)b * (nf / 32) + ef / 32) * q.loc.nyExt() + y0 + q.loc.up)
+                                        * q.loc.nxExt() + x0 + epr + q.loc.lt) * 32 + ef % 32
+                      : nullptr;
 #pragma unroll
    for (int it = 0; it < MEP; it++) {
       int const yl    = it * MEPP / MTX;
@@ -488,7 +503,13 @@ __device__ __forceinline__ void lca_epilogue_vthresh(
          }
       }
       __builtin_nontemporal_store(vn, reinterpret_cast<f32x4 *>(Vp + yl * rowV + xo));
-      __builtin_nontemporal_store(an, reinterpret_cast<f32x4 *>(Ap + yl * rowA + xo));
+      if (Ap) {
+         __builtin_nontemporal_store(an, reinterpret_cast<f32x4 *>(Ap + yl * rowA + xo));
+      }
+      if (Abp) {
+         __builtin_nontemporal_store(bf16x4{(__bf16)an[0], (__bf16)an[1], (__bf16)an[2], (__bf16)an[3]},
+               reinterpret_cast<bf16x4 *>(Abp + (yl * q.loc.nxExt() + it * MEPP % MTX) * 32));
+      }
       if (EMIT) {
          // Rank of this thread's actives within its position's segment: the
          // actives of lower lanes of the same MEQ-lane group, from ballots.
@@ -887,9 +908,13 @@ __device__ __forceinline__ void store4(__bf16 *d, float4 v) {
 }
 __device__ __forceinline__ void store4(float *d, float4 v) { *reinterpret_cast<float4 *>(d) = v; }
 
-template <typename T, int MB, int CNBW>
+// SRCB (T = __bf16 only): the source is p.preB16, the chunked BF16 copy of
+// pre_data ([nbatch][C/32][nyExt][nxExt][32]), staged without conversion in
+// 16-byte items of 8 channels; otherwise FP32 pre_data converted to T.
+template <typename T, int MB, int CNBW, bool SRCB = false>
 __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
       PostParams p, const T *Wq, const int32_t *invX, const int32_t *invY, CStrip cs) {
+   static_assert(!SRCB || sizeof(T) == 2, "the chunked BF16 source feeds the BF16 GEMM only");
    constexpr int CNT = 4 * CNBW * 16;
    // Wq: weights in T, layout [M][nyp][nxp][C] (the post weights' own layout;
    // the BF16 shadow of pvhip_recv_post_bf16_weights or the FP32 weights).
@@ -931,7 +956,11 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
    int const nxExt = p.pre.nxExt();
    int const nyExt = p.pre.nyExt();
    float const *preB = p.pre_data + (size_t)b * p.pre.numExt();
-   int const srcItems = rows * tw * (CK / 4);
+   // chunked source: this batch element's planes of nyExt*nxExt*32 values
+   size_t const plane32 = (size_t)nyExt * nxExt * 32;
+   __bf16 const *preH   = SRCB ? p.preB16 + (size_t)b * (C / 32) * plane32 : nullptr;
+   constexpr int SI     = SRCB ? 8 : 4; // channels per source item
+   int const srcItems   = rows * tw * (CK / SI);
    int const wItems   = taps * Mp * (CK / EP);
    // the zero weight tap read by the tap loop past the last tap (ordered
    // before any read by the first chunk's barriers)
@@ -955,19 +984,40 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
    // Item offsets do not depend on the chunk: computed once.  A source item
    // is 4 channels of one staged position; a weight item 16 bytes of one
    // (tap, kernel) row of CK channels.
+   // (FP32 source: offset in the batch element's PV-layout activity, to which
+   // the chunk's first channel is added; chunked source: offset within a
+   // chunk plane.)
    auto src_goff = [&](int i) {
-      int q4 = i % (CK / 4);
-      int cc = (i / (CK / 4)) % tw;
-      int rr = i / ((CK / 4) * tw);
+      int q4 = i % (CK / SI);
+      int cc = (i / (CK / SI)) % tw;
+      int rr = i / ((CK / SI) * tw);
       int ye = min(cs.qy0 + qyA + rr, nyExt - 1);
       int xe = min(cs.qx0 + qxS + cc, nxExt - 1);
-      return (ye * nxExt + xe) * C + 4 * q4;
+      return SRCB ? (ye * nxExt + xe) * 32 + SI * q4 : (ye * nxExt + xe) * C + SI * q4;
    };
    auto src_loff = [&](int i) {
-      int q4 = i % (CK / 4);
-      int cc = (i / (CK / 4)) % tw;
-      int rr = i / ((CK / 4) * tw);
-      return (rr * cs.twMax + cc) * PP + 4 * q4;
+      int q4 = i % (CK / SI);
+      int cc = (i / (CK / SI)) % tw;
+      int rr = i / ((CK / SI) * tw);
+      return (rr * cs.twMax + cc) * PP + SI * q4;
+   };
+   // source item: 4 FP32 channels, or 8 BF16 channels of the chunked copy
+   using SV   = std::conditional_t<SRCB, uint4, float4>;
+   auto sload = [&](int goff, int c0) -> SV {
+      if constexpr (SRCB) {
+         return *reinterpret_cast<const uint4 *>(preH + (size_t)(c0 / 32) * plane32 + goff);
+      }
+      else {
+         return *reinterpret_cast<const float4 *>(preB + goff + c0);
+      }
+   };
+   auto sstore = [&](int loff, SV v) {
+      if constexpr (SRCB) {
+         *reinterpret_cast<uint4 *>(srcl + loff) = v;
+      }
+      else {
+         store4(srcl + loff, v);
+      }
    };
    auto w_goff = [&](int j) { // -1: padding kernel row (zeros)
       int pc = j % (CK / EP);
@@ -994,13 +1044,13 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
       wG[u] = w_goff(min(j, wItems - 1));
       wL[u] = j < wItems ? w_loff(j) : -1;
    }
-   float4 sv[CSV];
+   SV sv[CSV];
    uint4 wv[CAV];
    // global -> registers for chunk c0 (the first CSV / CAV items per thread)
    auto fetch = [&](int c0) {
 #pragma unroll
       for (int u = 0; u < CSV; u++) {
-         sv[u] = *reinterpret_cast<const float4 *>(preB + sG[u] + c0);
+         sv[u] = sload(sG[u], c0);
       }
 #pragma unroll
       for (int u = 0; u < CAV; u++) {
@@ -1013,11 +1063,11 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
 #pragma unroll
       for (int u = 0; u < CSV; u++) {
          if (sL[u] >= 0) {
-            store4(srcl + sL[u], sv[u]);
+            sstore(sL[u], sv[u]);
          }
       }
       for (int i = tid + CSV * 256; i < srcItems; i += 256) {
-         store4(srcl + src_loff(i), *reinterpret_cast<const float4 *>(preB + src_goff(i) + c0));
+         sstore(src_loff(i), sload(src_goff(i), c0));
       }
 #pragma unroll
       for (int u = 0; u < CAV; u++) {
@@ -1250,6 +1300,7 @@ int build_params(const pvhip_recv_post_args &a, PostParams *p) {
    p->W        = a.weights;
    p->gsyn     = a.post_gsyn;
    p->accumulate = a.accumulate;
+   p->preB16     = static_cast<const __bf16 *>(a.pre_data_bf16c);
    p->emit       = 0;
    p->stg        = nullptr;
    p->segCnt     = nullptr;
@@ -1463,6 +1514,7 @@ int pvhip_recv_post(const pvhip_recv_post_args *args, pvhip_stream stream) {
    const pvhip_active_out *aout = args->active_out;
    if (aout) {
       PVHIP_REQUIRE(p.fused, "recv_post: active_out needs fused_update");
+      PVHIP_REQUIRE(args->fused_update->A != nullptr, "recv_post: active_out needs the fused update's A");
       PVHIP_REQUIRE(aout->entries && aout->num_active && aout->scratch,
             "recv_post: active_out needs entries, num_active and scratch");
       PVHIP_REQUIRE(aout->scratch_bytes >= pvhip_recv_post_active_scratch_bytes(args),
@@ -1510,24 +1562,34 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
       pl.cs.nOut = (size_t)p.post.nbatch * p.post.numRes();
       const void *Wq = pl.bf ? args->weights_lp : (const void *)args->weights;
       dim3 grid((unsigned)pl.blocks, (unsigned)pl.cs.splits);
-#define PVHIP_CGEMM3(T, MBV, NB)                                                                  \
-   hipLaunchKernelGGL((recv_post_cgemm_kernel<T, MBV, NB>), grid, dim3(256), pl.lds, s, p,        \
+      bool const srcb = pl.bf && p.preB16 != nullptr;
+      if (srcb) {
+         PVHIP_REQUIRE(p.pre.nf % 32 == 0 && reinterpret_cast<uintptr_t>(p.preB16) % 16 == 0,
+               "recv_post: pre_data_bf16c needs pre nf %% 32 == 0 and a 16-byte aligned buffer");
+      }
+#define PVHIP_CGEMM3(T, MBV, NB, SB)                                                              \
+   hipLaunchKernelGGL((recv_post_cgemm_kernel<T, MBV, NB, SB>), grid, dim3(256), pl.lds, s, p,    \
          reinterpret_cast<const T *>(Wq), args->inverse_x, args->inverse_y, pl.cs)
-#define PVHIP_CGEMM(T, MBV)                                                                       \
+#define PVHIP_CGEMM(T, MBV, SB)                                                                   \
    do {                                                                                          \
-      if (pl.NBW == 1) PVHIP_CGEMM3(T, MBV, 1);                                                  \
-      else if (pl.NBW == 2) PVHIP_CGEMM3(T, MBV, 2);                                             \
-      else PVHIP_CGEMM3(T, MBV, 4);                                                              \
+      if (pl.NBW == 1) PVHIP_CGEMM3(T, MBV, 1, SB);                                              \
+      else if (pl.NBW == 2) PVHIP_CGEMM3(T, MBV, 2, SB);                                         \
+      else PVHIP_CGEMM3(T, MBV, 4, SB);                                                          \
    } while (0)
-      if (pl.bf) {
-         if (pl.MB == 1) PVHIP_CGEMM(__bf16, 1);
-         else if (pl.MB == 2) PVHIP_CGEMM(__bf16, 2);
-         else PVHIP_CGEMM(__bf16, 4);
+      if (srcb) {
+         if (pl.MB == 1) PVHIP_CGEMM(__bf16, 1, true);
+         else if (pl.MB == 2) PVHIP_CGEMM(__bf16, 2, true);
+         else PVHIP_CGEMM(__bf16, 4, true);
+      }
+      else if (pl.bf) {
+         if (pl.MB == 1) PVHIP_CGEMM(__bf16, 1, false);
+         else if (pl.MB == 2) PVHIP_CGEMM(__bf16, 2, false);
+         else PVHIP_CGEMM(__bf16, 4, false);
       }
       else {
-         if (pl.MB == 1) PVHIP_CGEMM(float, 1);
-         else if (pl.MB == 2) PVHIP_CGEMM(float, 2);
-         else PVHIP_CGEMM(float, 4);
+         if (pl.MB == 1) PVHIP_CGEMM(float, 1, false);
+         else if (pl.MB == 2) PVHIP_CGEMM(float, 2, false);
+         else PVHIP_CGEMM(float, 4, false);
       }
 #undef PVHIP_CGEMM3
 #undef PVHIP_CGEMM

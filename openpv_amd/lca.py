@@ -22,8 +22,9 @@ import torch
 
 from . import abi
 from .runtime import (ActiveList, HyPerLCAUpdate, PostsynapticPerspectiveGPUDelivery,
-                      PresynapticPerspectiveGPUDelivery, cost_sum_f64, error_update, fill_zero,
-                      layer_update, loc, make_transfer, _stream)
+                      PresynapticPerspectiveGPUDelivery, activity_to_bf16c, bf16c_buffer,
+                      cost_sum_f64, error_update, fill_zero, layer_update, loc, make_transfer,
+                      _stream)
 
 FLT_MAX = float(np.finfo(np.float32).max)
 
@@ -41,9 +42,12 @@ class LCAColumn:
                  AMax=FLT_MAX, AShift=None, VWidth=0.0, timeConstantTau=100.0, dt=1.0,
                  self_interact=True, err_scale=1.0, vertices=False, arith=abi.ARITH_EXACT,
                  fused=False, V0=None, res_halo=None, v1_halo=None, normalize=None,
-                 device="cuda"):
+                 bf16_copy=True, device="cuda"):
         """normalize: NormalizeL2 of the weights on initialisation (strength 1):
-        "exact" (the reference's float arithmetic), "fp64", or None."""
+        "exact" (the reference's float arithmetic), "fp64", or None.
+        bf16_copy: in the BF16/FP8 forms, let V1's update write the chunked BF16
+        copy of its activity that the scatter's channel GEMM reads (where the
+        geometry allows); False keeps the FP32 activity as the scatter's source."""
         images = torch.as_tensor(images, dtype=torch.float32)
         nb, ny, nx, nf = images.shape
         self.device = torch.device(device)
@@ -110,6 +114,18 @@ class LCAColumn:
         self.lca = HyPerLCAUpdate(self.v1_loc, self.v1_transfer, timeConstantTau, dt,
                                   self_interact, activity_from_v=True, device=d)
         self.energy = torch.zeros(2, nb, dtype=torch.float64, device=d)
+        # BF16 forms whose scatter route is a channel GEMM: V1's update writes
+        # its activity as chunked BF16, which the GEMM reads instead of
+        # converting the FP32 activity (the same BF16 values, half the bytes).
+        # Such a step writes no FP32 activity: nothing in it reads one (the next
+        # update recomputes A(t-1) = T(V) from V), so v1_A is derived from V, bit
+        # for bit, when a reader asks for it (v1_activity()).
+        self.v1_A16 = None
+        self._a_stale = False
+        if bf16_copy and arith in (abi.ARITH_BF16, abi.ARITH_FP8) and nf_v1 % 32 == 0:
+            buf = bf16c_buffer(self.v1_loc, d)
+            if self.scatter.bind_pre_bf16c(buf):
+                self.v1_A16 = buf
         self._graph = None
         self._fused_args = self._gather_args = self._gather_args_plain = None
         # V1's active list is published when a consumer reads it in the step (the
@@ -119,15 +135,26 @@ class LCAColumn:
         if fused:
             # argument blocks (and the delivery's scratch) prepared outside any graph capture
             self._fused_args = self.lca.args(None, self.v1_V, self.v1_A, num_channels=1)
+            self._fused_args_lp = self.lca.args(
+                None, self.v1_V, self.v1_A if self.v1_A16 is None else None, num_channels=1,
+                A_bf16c=self.v1_A16)
             self._gather_args = self.gather.args(self.res_A, None, False, fused=self._fused_args,
                                                  active=self.active, halo_is_zero=True)
-            self._gather_args_plain = self.gather.args(self.res_A, None, False, fused=self._fused_args)
+            self._gather_args_plain = self.gather.args(self.res_A, None, False,
+                                                       fused=self._fused_args_lp)
         # InitializeState: V1 V from V0, A = T(V); Residual and Recon start at 0.
         if V0 is not None:
             self.v1_V.copy_(torch.as_tensor(V0, dtype=torch.float32).reshape(-1).to(d))
         from .runtime import ann_activity
         ann_activity(self.v1_loc, self.v1_transfer, self.v1_V, self.v1_A)
         self.active.compact(self.v1_A)
+        self._sync_bf16c()
+
+    def _sync_bf16c(self, stream=None):
+        """Rebuild V1's chunked BF16 activity from its FP32 activity (after V1's
+        state was written other than by a step)."""
+        if self.v1_A16 is not None:
+            activity_to_bf16c(self.v1_loc, self.v1_activity(stream), self.v1_A16, stream)
 
     def autotune(self, tuner, stream=None, warm_steps=20):
         """Launch shapes of the column's tunable kernels (the FP32/BF16 scatter's
@@ -136,26 +163,30 @@ class LCAColumn:
         `warm_steps` steps (representative sparsity); the state those steps
         changed is restored afterwards.  Call before capture().  Returns
         {kernel: shape}."""
-        saved = [t.clone() for t in (self.v1_V, self.v1_A, self.rec_A)]
+        saved = [t.clone() for t in (self.v1_V, self.v1_activity(stream), self.rec_A)]
         for _ in range(warm_steps):
             self.step(stream)
         if stream is not None:
             stream.synchronize()
+        self._sync_bf16c(stream)
         out = self._tune(tuner, stream)
         torch.cuda.synchronize()
         for t, v in zip((self.v1_V, self.v1_A, self.rec_A), saved):
             t.copy_(v)
+        self._a_stale = False
         self.active.compact(self.v1_A)
+        self._sync_bf16c()
         torch.cuda.synchronize()
         return out
 
     def _tune(self, tuner, stream):
         out = {}
-        shape = tuner.tune_recv_post(self.scatter, self.v1_A, self.rec_gsyn, stream=stream)
+        shape = tuner.tune_recv_post(self.scatter, self.v1_activity(stream), self.rec_gsyn,
+                                     stream=stream)
         if shape is not None:
             out["scatter_cgemm(tile,splits)"] = list(shape)
-        mode = tuner.tune_recv_pre(self.scatter, self.rec_gsyn, self.active_list(stream), self.v1_A,
-                                   stream=stream)
+        mode = tuner.tune_recv_pre(self.scatter, self.rec_gsyn, self.active_list(stream),
+                                   self.v1_activity(stream), stream=stream)
         if mode is not None:
             out["scatter_mode"] = mode
         shape = tuner.tune_recv_post(self.gather, self.res_A, self.v1_gsyn, stream=stream)
@@ -185,12 +216,14 @@ class LCAColumn:
                 gather_events[1].record(s)
         else:
             self.gather.deliver(self.res_A, self.v1_gsyn, accumulate=False, stream=s)
-            self.lca.update(self.v1_gsyn, self.v1_V, self.v1_A, stream=s)
+            self.lca.update(self.v1_gsyn, self.v1_V, self.v1_A, stream=s,
+                            A_bf16c=None if needs_list else self.v1_A16)
             if gather_events is not None:
                 gather_events[1].record(s)
             if needs_list:
                 self.active.compact(self.v1_A, stream=s)
         self._active_stale = not needs_list
+        self._a_stale = self.fused and not needs_list and self.v1_A16 is not None
         # phase 3: Recon = scatter of the sparse V1 activity
         self.scatter.deliver(self.rec_gsyn, active=self.active, pre_activity=self.v1_A,
                              accumulate=False, stream=s)
@@ -203,10 +236,11 @@ class LCAColumn:
         (FirmThresholdCostFnLCAProbe with the layer's VThresh and VWidth)."""
         cost_sum_f64(self.res_loc, self.res_A, True, 0, out=self.energy[0], stream=stream)
         if cost == "firm":
-            cost_sum_f64(self.v1_loc, self.v1_A, True, 2, self.VThresh, self.VWidth,
+            cost_sum_f64(self.v1_loc, self.v1_activity(stream), True, 2, self.VThresh, self.VWidth,
                          out=self.energy[1], stream=stream)
         else:
-            cost_sum_f64(self.v1_loc, self.v1_A, True, 1, out=self.energy[1], stream=stream)
+            cost_sum_f64(self.v1_loc, self.v1_activity(stream), True, 1, out=self.energy[1],
+                         stream=stream)
         return 0.5 * self.energy[0] + self.VThresh * self.energy[1]
 
     def set_input(self, images):
@@ -229,12 +263,14 @@ class LCAColumn:
                 st = H.pvhip_graph_end(_stream(stream), C.byref(g))
             abi.check(st, "pvhip_graph_end")
         self._graph = g
-        self._graph_leaves_stale = self._active_stale   # host flag set while capturing
+        self._graph_leaves_stale = self._active_stale   # host flags set while capturing
+        self._graph_leaves_a_stale = self._a_stale
         return g
 
     def replay(self, stream):
         abi.check(abi.hip().pvhip_graph_launch(self._graph, _stream(stream)), "pvhip_graph_launch")
         self._active_stale = self._graph_leaves_stale
+        self._a_stale = self._graph_leaves_a_stale
 
     def release_graph(self):
         if self._graph is not None:
@@ -243,7 +279,16 @@ class LCAColumn:
 
     # host views -----------------------------------------------------------
     def v1_state(self):
-        return self.v1_V.cpu().numpy(), self.v1_A.cpu().numpy()
+        return self.v1_V.cpu().numpy(), self.v1_activity().cpu().numpy()
+
+    def v1_activity(self, stream=None):
+        """V1's FP32 activity after the last step (derived here as T(V) when the
+        step wrote only its chunked BF16 copy; the same bits the update gives)."""
+        if self._a_stale:
+            from .runtime import ann_activity
+            ann_activity(self.v1_loc, self.v1_transfer, self.v1_V, self.v1_A, stream=stream)
+            self._a_stale = False
+        return self.v1_A
 
     def dense_scatter(self):
         """True when the Recon scatter reads V1's dense activity (the MFMA route)
@@ -254,7 +299,7 @@ class LCAColumn:
         """V1's active list after the last step (rebuilt here when the step did
         not need it)."""
         if self._active_stale:
-            self.active.compact(self.v1_A, stream=stream)
+            self.active.compact(self.v1_activity(stream), stream=stream)
             self._active_stale = False
         return self.active
 

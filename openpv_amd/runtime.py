@@ -195,6 +195,7 @@ class PostsynapticPerspectiveGPUDelivery:
         a.cgemm_tile = getattr(self, "cgemm_tile", 0)
         a.cgemm_splits = getattr(self, "cgemm_splits", 0)
         a.lp_source_scale = getattr(self, "lp_source_scale", 0.0)
+        a.pre_data_bf16c = _ptr(getattr(self, "pre_bf16c", None))
         if fused is not None:
             a.fused_update = C.pointer(fused)
         if active is not None:
@@ -314,6 +315,15 @@ class PresynapticPerspectiveGPUDelivery:
         single = pkx.value * pky.value == 1  # target no larger than the source: a plain gather
         if nxq <= 0 or nyq <= 0 or (not single and (pre.nf % ck or m > 64)):
             return None
+        # The reference's post-perspective patch geometry offsets the source's x
+        # axis by its dn halo and the target's y axis by its dn halo
+        # (PatchGeometry.cpp:186-224), while the buffers are offset by lt and up:
+        # the transposed gather equals the scatter only where those halos agree,
+        # which holds when each layer's halo is the same on all four sides.
+        # Other halos keep the ordered scatter.
+        for l in (pre, post):
+            if len({l.halo.lt, l.halo.rt, l.halo.dn, l.halo.up}) > 1:
+                return None
         sx = np.zeros(post.nx, np.int32)
         sy = np.zeros(post.ny, np.int32)
         if H.pvhost_recv_post_tables(C.byref(pre), C.byref(post), nxq, nyq,
@@ -342,6 +352,18 @@ class PresynapticPerspectiveGPUDelivery:
             raise abi.PVHIPError("pvhost_transpose_weights failed (%d)" % r)
         route.weights.copy_(torch.as_tensor(out))
         route.refresh_weights(stream)
+
+    def bind_pre_bf16c(self, buf):
+        """Chunked BF16 copy of the pre layer's activity (pvhip_activity_to_bf16c
+        layout), kept current by the pre layer's update: the BF16 channel GEMM of
+        the dense route then reads it instead of converting the FP32 activity
+        (same results, half the bytes).  None unbinds.  Returns True when the
+        route can use it."""
+        t = self.transposed
+        if t is None or t.inverse is None or t.arith != abi.ARITH_BF16 or self.pre.nf % 32:
+            return False
+        t.pre_bf16c = buf
+        return True
 
     def deliver(self, gsyn, active=None, pre_activity=None, accumulate=False, stream=None):
         if self.transposed is not None and (getattr(self, "use_route", True) or active is None):
@@ -392,7 +414,8 @@ class HyPerLCAUpdate:
         """Per-batch dt (adaptive time scales): recompute exp(-dt/tau) on the host."""
         self.exp_tau.copy_(torch.as_tensor(exp_tau(dt, self.tau, self.loc.nbatch)))
 
-    def args(self, gsyn, V, A, num_channels=None):
+    def args(self, gsyn, V, A, num_channels=None, A_bf16c=None):
+        """A_bf16c: optional chunked BF16 copy of A that the update also writes."""
         a = abi.LCAUpdateArgs()
         a.loc = self.loc
         nch = len(self.coeffs) if num_channels is None else num_channels
@@ -405,11 +428,25 @@ class HyPerLCAUpdate:
         a.self_interact = int(self.self_interact)
         a.transfer = self.transfer
         a.activity_from_v = int(self.activity_from_v)
+        a.A_bf16c = _ptr(A_bf16c)
         return a
 
-    def update(self, gsyn, V, A, stream=None):
-        a = self.args(gsyn, V, A)
+    def update(self, gsyn, V, A, stream=None, A_bf16c=None):
+        a = self.args(gsyn, V, A, A_bf16c=A_bf16c)
         abi.check(abi.hip().pvhip_lca_update(C.byref(a), _stream(stream)), "pvhip_lca_update")
+
+
+def bf16c_buffer(loc_, device="cuda"):
+    """A zeroed chunked BF16 activity buffer for layer loc_ (nf % 32 == 0)."""
+    if loc_.nf % 32:
+        raise abi.PVHIPError("the chunked BF16 layout needs nf % 32 == 0 (nf %d)" % loc_.nf)
+    return torch.zeros(loc_.nbatch * loc_.num_ext, dtype=torch.bfloat16, device=device)
+
+
+def activity_to_bf16c(loc_, A, out, stream=None):
+    """out = the chunked BF16 copy of the extended activity A (halo included)."""
+    abi.check(abi.hip().pvhip_activity_to_bf16c(C.byref(loc_), _ptr(A), _ptr(out), _stream(stream)),
+              "pvhip_activity_to_bf16c")
 
 
 def layer_update(loc_, gsyn, V, A, transfer, channel_coeffs=(1.0,), err_scale=None, stream=None):

@@ -308,6 +308,35 @@ def test_recv_pre_exact_bit_exact(case):
     assert np.abs(host(g2) - ref).max() <= 1e-5 * max(1e-30, np.abs(ref).max())
 
 
+def test_recv_pre_exact_batches_with_unequal_side_halos():
+    """lt != rt with several batch elements: each batch element of the EXACT
+    scatter equals, bit for bit, the oracle run on that element alone.  The
+    library steps batches by the extended buffer's true size; the reference's
+    CPU scatter (and the oracle, which keeps its arithmetic) by nx + rt + rt
+    (PresynapticPerspectiveConvolveDelivery.cpp:73), which differs only for
+    lt != rt, a halo the reference never builds (its margins are equal on both
+    sides of an axis)."""
+    rng = np.random.default_rng(29)
+    pre, post, p = (2, 10, 14, 5, (2, 1, 3, 2)), (2, 20, 28, 3, 0), 6
+    (hp, op), (hq, oq) = both(pre), both(post)
+    kx, ky, kf = O.num_kernels(op, oq)
+    W = (rng.standard_normal((kx * ky * kf, p, p, oq.nf)) * 0.1).astype(np.float32)
+    A = ext_random(hp, rng, 0.2)
+    d = R.PresynapticPerspectiveGPUDelivery(hp, hq, p, p, dev(W), arith=abi.ARITH_EXACT)
+    al = R.ActiveList(hp, DEV)
+    al.compact(dev(A))
+    g = torch.zeros(hq.nbatch * hq.num_res, device=DEV)
+    d.deliver(g, active=al)
+    got = host(g).reshape(2, -1)
+    op1 = O.Loc.make(1, *pre[1:4], halo=pre[4])
+    oq1 = O.Loc.make(1, *post[1:4], halo=post[4])
+    for b in range(2):
+        Ab = A.reshape(2, -1)[b].copy()
+        ent, cnt = O.active_indices(op1, Ab)
+        ref = O.recv_pre(op1, oq1, p, p, W, entries=ent, num_active=cnt, num_threads=1)
+        assert same(got[b], ref), b
+
+
 @pytest.mark.parametrize("case", [
     ((2, 16, 16, 32, 2), (2, 32, 32, 3, 3), 8, True),     # V1 -> Residual (config[0] shape)
     ((1, 16, 16, 24, 2), (1, 64, 64, 1, 6), 16, False),   # config[2] geometry
