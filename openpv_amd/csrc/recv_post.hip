@@ -1031,9 +1031,13 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
       int t  = j / ((CK / EP) * Mp);
       return (t * Mp + m) * PP + pc * EP;
    };
-   int sG[CSV], sL[CSV], wG[CAV], wL[CAV];
+   // Channel chunks in flight in registers: the chunked BF16 source needs few
+   // registers per chunk, so two chunks are fetched ahead; the FP32 source one.
+   constexpr int DEPTH = SRCB ? 2 : 1;
+   constexpr int NSV   = SRCB ? 6 : CSV; // source items per thread and chunk held in registers
+   int sG[NSV], sL[NSV], wG[CAV], wL[CAV];
 #pragma unroll
-   for (int u = 0; u < CSV; u++) {
+   for (int u = 0; u < NSV; u++) {
       int i = tid + u * 256;
       sG[u] = src_goff(min(i, srcItems - 1));
       sL[u] = i < srcItems ? src_loff(i) : -1;
@@ -1044,35 +1048,35 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
       wG[u] = w_goff(min(j, wItems - 1));
       wL[u] = j < wItems ? w_loff(j) : -1;
    }
-   SV sv[CSV];
-   uint4 wv[CAV];
-   // global -> registers for chunk c0 (the first CSV / CAV items per thread)
-   auto fetch = [&](int c0) {
+   SV sv[DEPTH][NSV];
+   uint4 wv[DEPTH][CAV];
+   // global -> register set d for chunk c0 (the first NSV / CAV items per thread)
+   auto fetch = [&](int d, int c0) {
 #pragma unroll
-      for (int u = 0; u < CSV; u++) {
-         sv[u] = sload(sG[u], c0);
+      for (int u = 0; u < NSV; u++) {
+         sv[d][u] = sload(sG[u], c0);
       }
 #pragma unroll
       for (int u = 0; u < CAV; u++) {
-         wv[u] = wG[u] >= 0 ? *reinterpret_cast<const uint4 *>(Wq + (size_t)wG[u] + c0)
-                            : make_uint4(0u, 0u, 0u, 0u);
+         wv[d][u] = wG[u] >= 0 ? *reinterpret_cast<const uint4 *>(Wq + (size_t)wG[u] + c0)
+                               : make_uint4(0u, 0u, 0u, 0u);
       }
    };
-   // registers (and any items beyond them, loaded now) -> LDS for chunk c0
-   auto stage = [&](int c0) {
+   // register set d (and any items beyond it, loaded now) -> LDS for chunk c0
+   auto stage = [&](int d, int c0) {
 #pragma unroll
-      for (int u = 0; u < CSV; u++) {
+      for (int u = 0; u < NSV; u++) {
          if (sL[u] >= 0) {
-            sstore(sL[u], sv[u]);
+            sstore(sL[u], sv[d][u]);
          }
       }
-      for (int i = tid + CSV * 256; i < srcItems; i += 256) {
+      for (int i = tid + NSV * 256; i < srcItems; i += 256) {
          sstore(src_loff(i), sload(src_goff(i), c0));
       }
 #pragma unroll
       for (int u = 0; u < CAV; u++) {
          if (wL[u] >= 0) {
-            *reinterpret_cast<uint4 *>(wl + wL[u]) = wv[u];
+            *reinterpret_cast<uint4 *>(wl + wL[u]) = wv[d][u];
          }
       }
       for (int j = tid + CAV * 256; j < wItems; j += 256) {
@@ -1090,52 +1094,45 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
          acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
       }
    }
-   fetch(cBeg);
-   for (int c0 = cBeg; c0 < cEnd; c0 += CK) {
-      __syncthreads(); // every wave is done reading the previous chunk
-      stage(c0);
-      __syncthreads();
-      if (c0 + CK < cEnd) {
-         fetch(c0 + CK); // in flight during this chunk's MFMAs
+   // Software-pipelined over the taps, two register sets in turn: the
+   // fragments of the next tap are read from LDS while the MFMAs of the
+   // current one run.  Loads are issued unconditionally (no branch, so each
+   // MFMA waits only for its own set): a tap index past the last one reads
+   // the zero weight tap at index `taps` (adds nothing).  Tap offsets step
+   // (ky, kx) without a division.
+   using Frag = typename CGemm<T>::Frag;
+   auto load_frags = [&](Frag *af, Frag *bfr, int u, int ky, int kx) {
+      bool const real  = u < taps;
+      T const *wt      = wl + (real ? u : taps) * Mp * PP;
+      int const tapOff = real ? (ky * cs.twMax + kx) * PP : 0;
+#pragma unroll
+      for (int mb = 0; mb < MB; mb++) {
+         af[mb] = *reinterpret_cast<const Frag *>(wt + (mb * 16 + r) * PP + CGemm<T>::KL * g);
       }
-      // Software-pipelined over the taps, two register sets in turn: the
-      // fragments of the next tap are read from LDS while the MFMAs of the
-      // current one run.  Loads are issued unconditionally (no branch, so each
-      // MFMA waits only for its own set): a tap index past the last one reads
-      // the zero weight tap at index `taps` (adds nothing).  Tap offsets step
-      // (ky, kx) without a division.
-      using Frag = typename CGemm<T>::Frag;
-      Frag afC[MB], bfC[CNBW], afN[MB], bfN[CNBW];
-      auto load_frags = [&](Frag *af, Frag *bfr, int u, int ky, int kx) {
-         bool const real  = u < taps;
-         T const *wt      = wl + (real ? u : taps) * Mp * PP;
-         int const tapOff = real ? (ky * cs.twMax + kx) * PP : 0;
+#pragma unroll
+      for (int nb = 0; nb < CNBW; nb++) {
+         bfr[nb] = *reinterpret_cast<const Frag *>(srcl + nbase[nb] + tapOff + CGemm<T>::KL * g);
+      }
+   };
+   auto mfma_tap = [&](const Frag *af, const Frag *bfr) {
+#pragma unroll
+      for (int nb = 0; nb < CNBW; nb++) {
 #pragma unroll
          for (int mb = 0; mb < MB; mb++) {
-            af[mb] = *reinterpret_cast<const Frag *>(wt + (mb * 16 + r) * PP + CGemm<T>::KL * g);
-         }
-#pragma unroll
-         for (int nb = 0; nb < CNBW; nb++) {
-            bfr[nb] = *reinterpret_cast<const Frag *>(srcl + nbase[nb] + tapOff + CGemm<T>::KL * g);
-         }
-      };
-      auto mfma_tap = [&](const Frag *af, const Frag *bfr) {
-#pragma unroll
-         for (int nb = 0; nb < CNBW; nb++) {
-#pragma unroll
-            for (int mb = 0; mb < MB; mb++) {
-               if constexpr (CK == 32) {
-                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mb], bfr[nb], acc[mb][nb], 0, 0, 0);
-               }
-               else {
-                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].x, bfr[nb].x, acc[mb][nb], 0, 0, 0);
-                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].y, bfr[nb].y, acc[mb][nb], 0, 0, 0);
-                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].z, bfr[nb].z, acc[mb][nb], 0, 0, 0);
-                  acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].w, bfr[nb].w, acc[mb][nb], 0, 0, 0);
-               }
+            if constexpr (CK == 32) {
+               acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mb], bfr[nb], acc[mb][nb], 0, 0, 0);
+            }
+            else {
+               acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].x, bfr[nb].x, acc[mb][nb], 0, 0, 0);
+               acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].y, bfr[nb].y, acc[mb][nb], 0, 0, 0);
+               acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].z, bfr[nb].z, acc[mb][nb], 0, 0, 0);
+               acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mb].w, bfr[nb].w, acc[mb][nb], 0, 0, 0);
             }
          }
-      };
+      }
+   };
+   auto compute = [&]() { // every tap of the staged chunk
+      Frag afC[MB], bfC[CNBW], afN[MB], bfN[CNBW];
       int ky = 0, kx = 0; // tap of the next load
       auto advance = [&]() {
          if (++kx == p.nxp) {
@@ -1153,35 +1150,71 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
          advance();
          mfma_tap(afN, bfN);
       }
+   };
+   int const nCh = (cEnd - cBeg) / CK;
+#pragma unroll
+   for (int d = 0; d < DEPTH; d++) {
+      if (d < nCh) {
+         fetch(d, cBeg + d * CK);
+      }
+   }
+   for (int i0 = 0; i0 < nCh; i0 += DEPTH) {
+#pragma unroll
+      for (int d = 0; d < DEPTH; d++) {
+         int const i = i0 + d;
+         if (i < nCh) { // the same for the whole workgroup: no barrier is skipped by part of it
+            int const c0 = cBeg + i * CK;
+            __syncthreads(); // every wave is done reading the previous chunk
+            stage(d, c0);
+            __syncthreads();
+            if (i + DEPTH < nCh) {
+               fetch(d, c0 + DEPTH * CK); // in flight during the MFMAs of the next DEPTH chunks
+            }
+            compute();
+         }
+      }
    }
    // Lane holds D[4g+i][r]: kernel mb*16+4g+i at window start n of block nb.
+   // The target coordinates of every element are looked up first (clamped
+   // indices, so the loads are unconditional and in flight together), then
+   // the elements are stored.
    int const nf = p.post.nf;
+   int tx[CNBW][MB][4], ty[CNBW][MB][4];
 #pragma unroll
    for (int nb = 0; nb < CNBW; nb++) {
-      int const n       = n0 + (wave * CNBW + nb) * 16 + r;
-      bool const inside = n < nS;
-      int const qx      = qxS + (inside ? n % QWs : 0);
-      int const qy      = inside ? n / QWs : 0;
+      int const n  = min(n0 + (wave * CNBW + nb) * 16 + r, nS - 1);
+      int const qx = qxS + n % QWs;
+      int const qy = n / QWs;
 #pragma unroll
       for (int mb = 0; mb < MB; mb++) {
 #pragma unroll
          for (int i = 0; i < 4; i++) {
-            int m  = mb * 16 + 4 * g + i;
-            int F  = m % nf;
-            int kx = (m / nf) % p.nKx;
-            int ky = m / (nf * p.nKx);
-            if (inside && m < cs.M) {
-               int x = invX[kx * cs.nqx + qx];
-               int y = invY[ky * cs.nqy + qy];
-               if (x >= 0 && y >= 0) {
-                  size_t gi = (size_t)b * p.post.numRes() + ((size_t)y * p.post.nx + x) * nf + F;
-                  if (cs.part) {
-                     cs.part[(size_t)ks * cs.nOut + gi] = acc[mb][nb][i];
-                  }
-                  else {
-                     float base = p.accumulate ? p.gsyn[gi] : 0.0f;
-                     p.gsyn[gi] = base + p.dtf * acc[mb][nb][i];
-                  }
+            int const m  = min(mb * 16 + 4 * g + i, cs.M - 1);
+            int const kx = (m / nf) % p.nKx;
+            int const ky = m / (nf * p.nKx);
+            tx[nb][mb][i] = invX[kx * cs.nqx + qx];
+            ty[nb][mb][i] = invY[ky * cs.nqy + qy];
+         }
+      }
+   }
+#pragma unroll
+   for (int nb = 0; nb < CNBW; nb++) {
+      bool const inside = n0 + (wave * CNBW + nb) * 16 + r < nS;
+#pragma unroll
+      for (int mb = 0; mb < MB; mb++) {
+#pragma unroll
+         for (int i = 0; i < 4; i++) {
+            int const m = mb * 16 + 4 * g + i;
+            int const x = tx[nb][mb][i];
+            int const y = ty[nb][mb][i];
+            if (inside && m < cs.M && x >= 0 && y >= 0) {
+               size_t gi = (size_t)b * p.post.numRes() + ((size_t)y * p.post.nx + x) * nf + m % nf;
+               if (cs.part) {
+                  cs.part[(size_t)ks * cs.nOut + gi] = acc[mb][nb][i];
+               }
+               else {
+                  float base = p.accumulate ? p.gsyn[gi] : 0.0f;
+                  p.gsyn[gi] = base + p.dtf * acc[mb][nb][i];
                }
             }
          }
