@@ -52,17 +52,37 @@ __device__ __forceinline__ float accumulate_channels(
    return value;
 }
 
-__global__ void __launch_bounds__(256) lca_update_kernel(LCAParams p) {
-   size_t const n     = (size_t)p.loc.nbatch * p.loc.numRes();
-   size_t const index = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (index >= n) {
-      return;
+// Elementwise kernels: grid (ceil(nx*ny*nf / kRowSpan), nbatch); a workgroup
+// covers kRowSpan consecutive restricted neurons of one batch element, each
+// thread kRowIter of them kRowThreads apart (unit-stride accesses across a
+// wave; one 32-bit division per neuron for its row, no 64-bit division).
+constexpr int kRowThreads = 256;
+constexpr int kRowIter    = 4;
+constexpr int kRowSpan    = kRowThreads * kRowIter;
+
+inline dim3 row_grid(const Geo &g) {
+   return dim3((unsigned)((g.numRes() + kRowSpan - 1) / kRowSpan), (unsigned)g.nbatch);
+}
+
+inline bool row_grid_ok(const Geo &g) { return g.nbatch <= 65535; }
+
+// Offset of row y of batch b in an extended buffer, at the first restricted column.
+__device__ __forceinline__ size_t ext_row(const Geo &g, int b, int y) {
+   return (size_t)b * g.numExt() + ((size_t)(y + g.up) * g.nxExt() + g.lt) * g.nf;
+}
+
+__global__ void __launch_bounds__(kRowThreads) lca_update_kernel(LCAParams p) {
+   int const numRes = p.loc.numRes();
+   int const b      = blockIdx.y;
+   size_t const n   = (size_t)p.loc.nbatch * numRes;
+#pragma unroll
+   for (int u = 0; u < kRowIter; u++) {
+      int const k = blockIdx.x * kRowSpan + u * kRowThreads + threadIdx.x;
+      if (k < numRes) {
+         float G = accumulate_channels(p.numChannels, p.coeffs, p.gsyn, n, (size_t)b * numRes + k);
+         lca_update_one(p, b, k, G);
+      }
    }
-   int const numNeurons = p.loc.numRes();
-   int const b          = (int)(index / numNeurons);
-   int const k          = (int)(index % numNeurons);
-   float G              = accumulate_channels(p.numChannels, p.coeffs, p.gsyn, n, index);
-   lca_update_one(p, b, k, G);
 }
 
 struct LayerParams {
@@ -77,36 +97,42 @@ struct LayerParams {
    Transfer transfer;
 };
 
-__global__ void __launch_bounds__(256) layer_update_kernel(LayerParams p) {
-   size_t const n     = (size_t)p.loc.nbatch * p.loc.numRes();
-   size_t const index = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (index >= n) {
-      return;
-   }
-   int const numNeurons = p.loc.numRes();
-   int const b          = (int)(index / numNeurons);
-   int const k          = (int)(index % numNeurons);
-   float V              = accumulate_channels(p.numChannels, p.coeffs, p.gsyn, n, index);
-   if (p.applyScale) {
-      V = V * p.scale; // ErrScaleInternalStateBuffer.cpp:51
-   }
-   p.V[index] = V;
-   if (p.A) {
-      p.A[(size_t)b * p.loc.numExt() + p.loc.kex(k)] = apply_transfer(p.transfer, V);
+__global__ void __launch_bounds__(kRowThreads) layer_update_kernel(LayerParams p) {
+   int const numRes = p.loc.numRes();
+   int const rowLen = p.loc.nx * p.loc.nf;
+   int const b      = blockIdx.y;
+   size_t const n   = (size_t)p.loc.nbatch * numRes;
+#pragma unroll
+   for (int u = 0; u < kRowIter; u++) {
+      int const k = blockIdx.x * kRowSpan + u * kRowThreads + threadIdx.x;
+      if (k < numRes) {
+         size_t const index = (size_t)b * numRes + k;
+         float V            = accumulate_channels(p.numChannels, p.coeffs, p.gsyn, n, index);
+         if (p.applyScale) {
+            V = V * p.scale; // ErrScaleInternalStateBuffer.cpp:51
+         }
+         p.V[index] = V;
+         if (p.A) {
+            int const y = k / rowLen;
+            p.A[ext_row(p.loc, b, y) + (k - y * rowLen)] = apply_transfer(p.transfer, V);
+         }
+      }
    }
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRowThreads)
       ann_activity_kernel(Geo loc, Transfer t, const float *V, float *A) {
-   size_t const n     = (size_t)loc.nbatch * loc.numRes();
-   size_t const index = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (index >= n) {
-      return;
+   int const numRes = loc.numRes();
+   int const rowLen = loc.nx * loc.nf;
+   int const b      = blockIdx.y;
+#pragma unroll
+   for (int u = 0; u < kRowIter; u++) {
+      int const k = blockIdx.x * kRowSpan + u * kRowThreads + threadIdx.x;
+      if (k < numRes) {
+         int const y = k / rowLen;
+         A[ext_row(loc, b, y) + (k - y * rowLen)] = apply_transfer(t, V[(size_t)b * numRes + k]);
+      }
    }
-   int const numNeurons = loc.numRes();
-   int const b          = (int)(index / numNeurons);
-   int const k          = (int)(index % numNeurons);
-   A[(size_t)b * loc.numExt() + loc.kex(k)] = apply_transfer(t, V[index]);
 }
 
 __global__ void __launch_bounds__(256) gsyn_accumulate_kernel(
@@ -119,19 +145,22 @@ __global__ void __launch_bounds__(256) gsyn_accumulate_kernel(
 }
 
 // IdentDelivery dense branch (IdentDelivery.cpp:146-162): GSyn += A restricted.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRowThreads)
       ident_deliver_kernel(Geo loc, const float *pre, float *gsyn, int accumulate) {
-   size_t const n     = (size_t)loc.nbatch * loc.numRes();
-   size_t const index = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (index >= n) {
-      return;
+   int const numRes = loc.numRes();
+   int const rowLen = loc.nx * loc.nf;
+   int const b      = blockIdx.y;
+#pragma unroll
+   for (int u = 0; u < kRowIter; u++) {
+      int const k = blockIdx.x * kRowSpan + u * kRowThreads + threadIdx.x;
+      if (k < numRes) {
+         int const y        = k / rowLen;
+         size_t const index = (size_t)b * numRes + k;
+         float a            = pre[ext_row(loc, b, y) + (k - y * rowLen)];
+         float base         = accumulate ? gsyn[index] : 0.0f;
+         gsyn[index]        = base + a;
+      }
    }
-   int const numNeurons = loc.numRes();
-   int const b          = (int)(index / numNeurons);
-   int const k          = (int)(index % numNeurons);
-   float a              = pre[(size_t)b * loc.numExt() + loc.kex(k)];
-   float base           = accumulate ? gsyn[index] : 0.0f;
-   gsyn[index]          = base + a;
 }
 
 // An error layer fed by two identity connections (channel 0 from `in`,
@@ -139,29 +168,32 @@ __global__ void __launch_bounds__(256)
 // IdentDelivery passes onto reset GSyn, GSynAccumulator, ErrScale and the
 // transfer in one pass, with the same operations in the same order as the
 // separate kernels (so the same bits, signed zeros included).
-__global__ void __launch_bounds__(256) error_update_kernel(
+__global__ void __launch_bounds__(kRowThreads) error_update_kernel(
       Geo loc, Geo inLoc, const float *in, Geo recLoc, const float *rec, float c0, float c1,
       int applyScale, float scale, Transfer t, float *V, float *A) {
-   size_t const n     = (size_t)loc.nbatch * loc.numRes();
-   size_t const index = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (index >= n) {
-      return;
+   int const numRes = loc.numRes();
+   int const rowLen = loc.nx * loc.nf;
+   int const b      = blockIdx.y;
+#pragma unroll
+   for (int u = 0; u < kRowIter; u++) {
+      int const k = blockIdx.x * kRowSpan + u * kRowThreads + threadIdx.x;
+      if (k < numRes) {
+         int const y    = k / rowLen;
+         int const j    = k - y * rowLen;
+         float const g0 = 0.0f + in[ext_row(inLoc, b, y) + j]; // IdentDelivery, reset GSyn
+         float const g1 = 0.0f + rec[ext_row(recLoc, b, y) + j];
+         float value    = 0.0f;                                 // GSynAccumulator
+         float prod     = c0 * g0;
+         value          = value + prod;
+         prod           = c1 * g1;
+         value          = value + prod;
+         if (applyScale) {
+            value = value * scale;
+         }
+         V[(size_t)b * numRes + k]  = value;
+         A[ext_row(loc, b, y) + j] = apply_transfer(t, value);
+      }
    }
-   int const numNeurons = loc.numRes();
-   int const b          = (int)(index / numNeurons);
-   int const k          = (int)(index % numNeurons);
-   float const g0 = 0.0f + in[(size_t)b * inLoc.numExt() + inLoc.kex(k)];    // IdentDelivery, reset GSyn
-   float const g1 = 0.0f + rec[(size_t)b * recLoc.numExt() + recLoc.kex(k)];
-   float value    = 0.0f;                                                     // GSynAccumulator
-   float prod     = c0 * g0;
-   value          = value + prod;
-   prod           = c1 * g1;
-   value          = value + prod;
-   if (applyScale) {
-      value = value * scale;
-   }
-   V[index]                               = value;
-   A[(size_t)b * loc.numExt() + loc.kex(k)] = apply_transfer(t, value);
 }
 
 // ---------------------------------------------------------------- compaction
@@ -597,9 +629,8 @@ int pvhip_lca_update(const pvhip_lca_update_args *args, pvhip_stream stream) {
    if (st) {
       return st;
    }
-   size_t n = (size_t)p.loc.nbatch * p.loc.numRes();
-   hipLaunchKernelGGL(lca_update_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-         (hipStream_t)stream, p);
+   PVHIP_REQUIRE(row_grid_ok(p.loc), "lca_update: more than 65535 rows or batch elements");
+   hipLaunchKernelGGL(lca_update_kernel, row_grid(p.loc), dim3(kRowThreads), 0, (hipStream_t)stream, p);
    PVHIP_LAUNCHED("lca_update");
    return PVHIP_OK;
 }
@@ -659,9 +690,8 @@ int pvhip_layer_update(const pvhip_layer_update_args *a, pvhip_stream stream) {
    if (st) {
       return st;
    }
-   size_t n = (size_t)p.loc.nbatch * p.loc.numRes();
-   hipLaunchKernelGGL(layer_update_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-         (hipStream_t)stream, p);
+   PVHIP_REQUIRE(row_grid_ok(p.loc), "layer_update: more than 65535 rows or batch elements");
+   hipLaunchKernelGGL(layer_update_kernel, row_grid(p.loc), dim3(kRowThreads), 0, (hipStream_t)stream, p);
    PVHIP_LAUNCHED("layer_update");
    return PVHIP_OK;
 }
@@ -676,10 +706,9 @@ int pvhip_ann_activity(
    if (st) {
       return st;
    }
-   Geo g    = geo(*loc);
-   size_t n = (size_t)g.nbatch * g.numRes();
-   hipLaunchKernelGGL(ann_activity_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-         (hipStream_t)stream, g, t, V, A);
+   Geo g = geo(*loc);
+   PVHIP_REQUIRE(row_grid_ok(g), "ann_activity: more than 65535 rows or batch elements");
+   hipLaunchKernelGGL(ann_activity_kernel, row_grid(g), dim3(kRowThreads), 0, (hipStream_t)stream, g, t, V, A);
    PVHIP_LAUNCHED("ann_activity");
    return PVHIP_OK;
 }
@@ -696,9 +725,9 @@ int pvhip_error_update(const pvhip_error_update_args *a, pvhip_stream stream) {
    if (st) {
       return st;
    }
-   Geo g    = geo(a->loc);
-   size_t n = (size_t)g.nbatch * g.numRes();
-   hipLaunchKernelGGL(error_update_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, g,
+   Geo g = geo(a->loc);
+   PVHIP_REQUIRE(row_grid_ok(g), "error_update: more than 65535 rows or batch elements");
+   hipLaunchKernelGGL(error_update_kernel, row_grid(g), dim3(kRowThreads), 0, (hipStream_t)stream, g,
          geo(a->in_loc), a->in, geo(a->rec_loc), a->rec, a->channel_coeffs[0], a->channel_coeffs[1],
          a->apply_scale, a->scale, t, a->V, a->A);
    PVHIP_LAUNCHED("error_update");
@@ -710,10 +739,10 @@ int pvhip_ident_deliver(
       pvhip_stream stream) {
    PVHIP_REQUIRE(loc && pre_data && post_gsyn, "ident_deliver: NULL argument");
    PVHIP_REQUIRE(loc_valid(*loc), "ident_deliver: invalid loc");
-   Geo g    = geo(*loc);
-   size_t n = (size_t)g.nbatch * g.numRes();
-   hipLaunchKernelGGL(ident_deliver_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-         (hipStream_t)stream, g, pre_data, post_gsyn, accumulate);
+   Geo g = geo(*loc);
+   PVHIP_REQUIRE(row_grid_ok(g), "ident_deliver: more than 65535 rows or batch elements");
+   hipLaunchKernelGGL(ident_deliver_kernel, row_grid(g), dim3(kRowThreads), 0, (hipStream_t)stream, g,
+         pre_data, post_gsyn, accumulate);
    PVHIP_LAUNCHED("ident_deliver");
    return PVHIP_OK;
 }
