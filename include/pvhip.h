@@ -295,6 +295,13 @@ typedef struct {
     * contiguous runs, with the same results: it rounds pre_data to BF16 the
     * same way.  The other forms ignore it. */
    const void *pre_data_bf16c;
+   /* Optional, channel GEMM only (targets with several data patches per
+    * feature), with accumulate == 0 and no fused_update: instead of writing
+    * post_gsyn, apply the target layer's update to the delivered values, as
+    * pvhip_layer_update does with this delivery as its only channel
+    * (num_channels 1; its gsyn is ignored).  The same bits as the delivery
+    * followed by that update, without the GSyn round trip. */
+   const struct pvhip_layer_update_args *fused_layer;
 } pvhip_recv_post_args;
 
 /* Active-list outputs of a fused delivery (same layout and contents as
@@ -394,7 +401,7 @@ int pvhip_gsyn_accumulate(
  * ErrScaleInternalStateBuffer.cpp:41-53; scale = errScale, 1 for ANNLayer),
  * then A[kex] = transfer(V) (ANNActivityBuffer; HyPerActivityBuffer when the
  * transfer is disabled). */
-typedef struct {
+typedef struct pvhip_layer_update_args {
    pvhip_loc loc;
    int num_channels;
    float channel_coeffs[PVHIP_MAX_CHANNELS];

@@ -84,6 +84,12 @@ class ActiveOut(C.Structure):
                 ("scratch_bytes", C.c_size_t), ("halo_is_zero", i32)]
 
 
+class LayerUpdateArgs(C.Structure):
+    _fields_ = [("loc", Loc), ("num_channels", i32), ("channel_coeffs", f32 * MAX_CHANNELS),
+                ("gsyn", vp), ("V", vp), ("A", vp), ("scale", f32), ("apply_scale", i32),
+                ("transfer", Transfer)]
+
+
 class RecvPostArgs(C.Structure):
     _fields_ = [("pre", Loc), ("post", Loc), ("nxp", i32), ("nyp", i32), ("nfp", i32),
                 ("num_kernels_x", i32), ("num_kernels_y", i32),
@@ -95,7 +101,7 @@ class RecvPostArgs(C.Structure):
                 ("fused_update", C.POINTER(LCAUpdateArgs)),
                 ("active_out", C.POINTER(ActiveOut)), ("scratch", vp), ("scratch_bytes", C.c_size_t),
                 ("cgemm_tile", i32), ("cgemm_splits", i32), ("lp_source_scale", f32),
-                ("pre_data_bf16c", vp)]
+                ("pre_data_bf16c", vp), ("fused_layer", C.POINTER(LayerUpdateArgs))]
 
 
 class RecvPreArgs(C.Structure):
@@ -105,12 +111,6 @@ class RecvPreArgs(C.Structure):
                 ("row_ptr", vp), ("row_ye", vp), ("row_wy", vp), ("row_ky", vp),
                 ("dt_factor", f32), ("active", vp), ("pos_start", vp), ("pre_data", vp),
                 ("weights", vp), ("post_gsyn", vp), ("accumulate", i32), ("arith", i32)]
-
-
-class LayerUpdateArgs(C.Structure):
-    _fields_ = [("loc", Loc), ("num_channels", i32), ("channel_coeffs", f32 * MAX_CHANNELS),
-                ("gsyn", vp), ("V", vp), ("A", vp), ("scale", f32), ("apply_scale", i32),
-                ("transfer", Transfer)]
 
 
 class ErrorUpdateArgs(C.Structure):

@@ -43,7 +43,30 @@ struct PostParams {
    float lpSrcScale;
    // BF16 channel GEMM: pre_data in the chunked BF16 layout, or NULL
    const __bf16 *preB16;
+   // Channel GEMM with the target layer's update fused (pvhip_layer_update
+   // with this delivery as channel 0): V and A written instead of GSyn
+   int layer;
+   Geo layerLoc; // the target layer (restricted size of post; its own halo)
+   float layerCoeff, layerScale;
+   int layerApplyScale;
+   float *layerV, *layerA;
+   Transfer layerT;
 };
+
+// Target-layer update of one delivered value G (restricted index gi, extended
+// index ai), the operations of layer_update_kernel with one channel.
+__device__ __forceinline__ void fused_layer_store(const PostParams &p, size_t gi, size_t ai, float G) {
+   float V    = 0.0f;
+   float prod = p.layerCoeff * G;
+   V          = V + prod;
+   if (p.layerApplyScale) {
+      V = V * p.layerScale;
+   }
+   p.layerV[gi] = V;
+   if (p.layerA) {
+      p.layerA[ai] = apply_transfer(p.layerT, V);
+   }
+}
 
 __device__ __forceinline__ void post_epilogue(const PostParams &p, int b, int k, float G) {
    if (p.fused) {
@@ -1212,12 +1235,41 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
                if (cs.part) {
                   cs.part[(size_t)ks * cs.nOut + gi] = acc[mb][nb][i];
                }
+               else if (p.layer) {
+                  Geo const &L = p.layerLoc;
+                  size_t ai    = (size_t)b * L.numExt() + ((size_t)(y + L.up) * L.nxExt() + x + L.lt) * nf + m % nf;
+                  fused_layer_store(p, gi, ai, 0.0f + p.dtf * acc[mb][nb][i]);
+               }
                else {
                   float base = p.accumulate ? p.gsyn[gi] : 0.0f;
                   p.gsyn[gi] = base + p.dtf * acc[mb][nb][i];
                }
             }
          }
+      }
+   }
+}
+
+// Split channel GEMM with the target layer's update fused: the partials of
+// each target neuron added in split order, then fused_layer_store.  Grid
+// (ceil(numRes / 1024), nbatch), four neurons per thread, unit stride.
+__global__ void __launch_bounds__(256) cgemm_reduce_layer_kernel(PostParams p, const float *part, int splits, size_t n) {
+   int const numRes = p.post.numRes();
+   int const rowLen = p.post.nx * p.post.nf;
+   int const b      = blockIdx.y;
+#pragma unroll
+   for (int u = 0; u < 4; u++) {
+      int const k = blockIdx.x * 1024 + u * 256 + threadIdx.x;
+      if (k < numRes) {
+         size_t const i = (size_t)b * numRes + k;
+         float acc      = part[i];
+         for (int s = 1; s < splits; s++) {
+            acc += part[(size_t)s * n + i];
+         }
+         int const y     = k / rowLen;
+         Geo const &L    = p.layerLoc;
+         size_t const ai = (size_t)b * L.numExt() + ((size_t)(y + L.up) * L.nxExt() + L.lt) * L.nf + (k - y * rowLen);
+         fused_layer_store(p, i, ai, 0.0f + p.dtf * acc);
       }
    }
 }
@@ -1337,6 +1389,26 @@ int build_params(const pvhip_recv_post_args &a, PostParams *p) {
    p->emit       = 0;
    p->stg        = nullptr;
    p->segCnt     = nullptr;
+   p->layer      = a.fused_layer != nullptr;
+   if (p->layer) {
+      const pvhip_layer_update_args &l = *a.fused_layer;
+      PVHIP_REQUIRE(!a.fused_update && !a.accumulate,
+            "recv_post: fused_layer needs accumulate == 0 and no fused_update");
+      PVHIP_REQUIRE(loc_valid(l.loc) && l.loc.nbatch == a.post.nbatch && l.loc.nx == a.post.nx
+                          && l.loc.ny == a.post.ny && l.loc.nf == a.post.nf,
+            "recv_post: fused_layer loc differs from post loc");
+      PVHIP_REQUIRE(l.V != nullptr, "recv_post: fused_layer needs V");
+      p->layerLoc        = geo(l.loc);
+      p->layerCoeff      = l.channel_coeffs[0];
+      p->layerScale      = l.scale;
+      p->layerApplyScale = l.apply_scale;
+      p->layerV          = l.V;
+      p->layerA          = l.A;
+      int st             = make_transfer(l.transfer, &p->layerT);
+      if (st) {
+         return st;
+      }
+   }
    p->fused      = a.fused_update != nullptr;
    if (p->fused) {
       PVHIP_REQUIRE(!a.accumulate, "recv_post: fused update requires accumulate == 0");
@@ -1357,7 +1429,7 @@ int build_params(const pvhip_recv_post_args &a, PostParams *p) {
       }
    }
    else {
-      PVHIP_REQUIRE(a.post_gsyn != nullptr, "recv_post: NULL post_gsyn");
+      PVHIP_REQUIRE(a.post_gsyn != nullptr || p->layer, "recv_post: NULL post_gsyn");
    }
    return PVHIP_OK;
 }
@@ -1485,6 +1557,7 @@ size_t pvhip_recv_post_scratch_bytes(const pvhip_recv_post_args *args) {
    a.weights              = a.weights ? a.weights : &unused;
    a.weights_lp           = a.weights_lp ? a.weights_lp : &unused;
    a.post_gsyn            = a.post_gsyn ? a.post_gsyn : &unused;
+   a.fused_layer          = nullptr; // writes V/A instead of GSyn: the same scratch
    PostParams p;
    if (build_params(a, &p) != PVHIP_OK || (p.nKx == 1 && p.nKy == 1)) {
       return 0;
@@ -1627,7 +1700,12 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
 #undef PVHIP_CGEMM3
 #undef PVHIP_CGEMM
       PVHIP_LAUNCHED("recv_post_cgemm");
-      if (pl.cs.splits > 1) {
+      if (pl.cs.splits > 1 && p.layer) {
+         dim3 rg((unsigned)((p.post.numRes() + 1023) / 1024), (unsigned)p.post.nbatch);
+         hipLaunchKernelGGL(cgemm_reduce_layer_kernel, rg, dim3(256), 0, s, p, pl.cs.part, pl.cs.splits, pl.cs.nOut);
+         PVHIP_LAUNCHED("recv_post_cgemm_reduce_layer");
+      }
+      else if (pl.cs.splits > 1) {
          size_t n     = pl.cs.nOut;
          bool aligned = (reinterpret_cast<uintptr_t>(p.gsyn) % 16 == 0)
                         && (reinterpret_cast<uintptr_t>(pl.cs.part) % 16 == 0);
@@ -1646,6 +1724,8 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
 
    PVHIP_REQUIRE(arith != PVHIP_ARITH_FP8 || single,
          "recv_post: the FP8 form needs one data patch per feature (the channel GEMM has BF16 and FP32 forms)");
+   PVHIP_REQUIRE(!p.layer, "recv_post: fused_layer is supported by the channel GEMM (BF16 and FP32 forms with "
+                           "several data patches per feature) only");
    if (arith == PVHIP_ARITH_BF16 || arith == PVHIP_ARITH_FP8) {
       bool const fp8 = arith == PVHIP_ARITH_FP8;
       PVHIP_REQUIRE(args->weights_lp != nullptr,

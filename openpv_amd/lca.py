@@ -23,8 +23,8 @@ import torch
 from . import abi
 from .runtime import (ActiveList, HyPerLCAUpdate, PostsynapticPerspectiveGPUDelivery,
                       PresynapticPerspectiveGPUDelivery, activity_to_bf16c, bf16c_buffer,
-                      cost_sum_f64, error_update, fill_zero, layer_update, loc, make_transfer,
-                      _stream)
+                      cost_sum_f64, error_update, fill_zero, layer_update_args, loc,
+                      make_transfer, _stream)
 
 FLT_MAX = float(np.finfo(np.float32).max)
 
@@ -126,6 +126,10 @@ class LCAColumn:
             buf = bf16c_buffer(self.v1_loc, d)
             if self.scatter.bind_pre_bf16c(buf):
                 self.v1_A16 = buf
+        # Recon = ANNLayer with the scatter as its only channel: the dense
+        # route's channel GEMM applies this update in its epilogue
+        self._rec_update = layer_update_args(self.rec_loc, self.rec_gsyn, self.rec_V, self.rec_A,
+                                             self.none_transfer, channel_coeffs=(1.0,))
         self._graph = None
         self._fused_args = self._gather_args = self._gather_args_plain = None
         # V1's active list is published when a consumer reads it in the step (the
@@ -226,9 +230,7 @@ class LCAColumn:
         self._a_stale = self.fused and not needs_list and self.v1_A16 is not None
         # phase 3: Recon = scatter of the sparse V1 activity
         self.scatter.deliver(self.rec_gsyn, active=self.active, pre_activity=self.v1_A,
-                             accumulate=False, stream=s)
-        layer_update(self.rec_loc, self.rec_gsyn, self.rec_V, self.rec_A, self.none_transfer,
-                     channel_coeffs=(1.0,), err_scale=None, stream=s)
+                             accumulate=False, stream=s, fused_layer=self._rec_update)
 
     def probe_energy(self, stream=None, cost="l1"):
         """ColumnEnergyProbe in FP64: E_b = 0.5*sum r^2 (L2NormProbe, exponent 2) +

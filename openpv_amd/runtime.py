@@ -174,7 +174,7 @@ class PostsynapticPerspectiveGPUDelivery:
         self.lp_source_scale = float(scale)
 
     def args(self, pre_activity, gsyn, accumulate=False, fused=None, active=None,
-             halo_is_zero=False):
+             halo_is_zero=False, fused_layer=None):
         a = abi.RecvPostArgs()
         a.pre, a.post = self.source, self.target
         a.nxp, a.nyp, a.nfp = self.nxp, self.nyp, self.nfp
@@ -198,6 +198,8 @@ class PostsynapticPerspectiveGPUDelivery:
         a.pre_data_bf16c = _ptr(getattr(self, "pre_bf16c", None))
         if fused is not None:
             a.fused_update = C.pointer(fused)
+        if fused_layer is not None:
+            a.fused_layer = C.pointer(fused_layer)
         if active is not None:
             if fused is None:
                 raise abi.PVHIPError("an active list needs the fused update")
@@ -213,12 +215,15 @@ class PostsynapticPerspectiveGPUDelivery:
         return a
 
     def deliver(self, pre_activity, gsyn, accumulate=False, fused=None, active=None,
-                halo_is_zero=False, stream=None):
+                halo_is_zero=False, stream=None, fused_layer=None):
         """GSyn (+)= gather; with `fused` the LCA update is applied instead of
         writing GSyn, and with `active` (an ActiveList of the updated layer) the
         active list is published in the same call (halo_is_zero: the layer's
-        activity halo holds only zeros, which lets the epilogue emit it)."""
-        self.launch(self.args(pre_activity, gsyn, accumulate, fused, active, halo_is_zero), stream)
+        activity halo holds only zeros, which lets the epilogue emit it).
+        fused_layer (channel GEMM only): the target layer's update
+        (layer_update_args) applied instead of writing GSyn."""
+        self.launch(self.args(pre_activity, gsyn, accumulate, fused, active, halo_is_zero,
+                              fused_layer), stream)
 
     @staticmethod
     def launch(args, stream=None):
@@ -365,12 +370,22 @@ class PresynapticPerspectiveGPUDelivery:
         t.pre_bf16c = buf
         return True
 
-    def deliver(self, gsyn, active=None, pre_activity=None, accumulate=False, stream=None):
+    def deliver(self, gsyn, active=None, pre_activity=None, accumulate=False, stream=None,
+                fused_layer=None):
+        """GSyn (+)= scatter.  fused_layer: the target layer's update
+        (layer_update_args, reading `gsyn` as its only channel) follows the
+        delivery; the dense route's channel GEMM applies it in its own epilogue
+        (no GSyn round trip), the other paths write GSyn and then update."""
         if self.transposed is not None and (getattr(self, "use_route", True) or active is None):
             if pre_activity is None:
                 raise abi.PVHIPError("the %s scatter reads the dense pre activity"
                                      % ("BF16" if self.arith == abi.ARITH_BF16 else "FP32"))
+            if fused_layer is not None and self.transposed.inverse is not None and not accumulate:
+                self.transposed.deliver(pre_activity, gsyn, stream=stream, fused_layer=fused_layer)
+                return
             self.transposed.deliver(pre_activity, gsyn, accumulate=accumulate, stream=stream)
+            if fused_layer is not None:
+                launch_layer_update(fused_layer, stream)
             return
         a = abi.RecvPreArgs()
         a.pre, a.post = self.pre, self.post
@@ -391,6 +406,8 @@ class PresynapticPerspectiveGPUDelivery:
         # geometry has no transposed MFMA route runs it in FP32
         a.arith = abi.ARITH_EXACT if self.arith == abi.ARITH_EXACT else abi.ARITH_FP32
         abi.check(abi.hip().pvhip_recv_pre(C.byref(a), _stream(stream)), "pvhip_recv_pre")
+        if fused_layer is not None:
+            launch_layer_update(fused_layer, stream)
 
 
 class HyPerLCAUpdate:
@@ -449,8 +466,8 @@ def activity_to_bf16c(loc_, A, out, stream=None):
               "pvhip_activity_to_bf16c")
 
 
-def layer_update(loc_, gsyn, V, A, transfer, channel_coeffs=(1.0,), err_scale=None, stream=None):
-    """ANNLayer / ANNErrorLayer / HyPerLayer update: V = [errScale*] sum coeff*GSyn; A = T(V)."""
+def layer_update_args(loc_, gsyn, V, A, transfer, channel_coeffs=(1.0,), err_scale=None):
+    """Argument block of pvhip_layer_update (also the fused_layer of a delivery)."""
     a = abi.LayerUpdateArgs()
     a.loc = loc_
     a.num_channels = len(channel_coeffs)
@@ -460,7 +477,17 @@ def layer_update(loc_, gsyn, V, A, transfer, channel_coeffs=(1.0,), err_scale=No
     a.scale = 1.0 if err_scale is None else err_scale
     a.apply_scale = 0 if err_scale is None else 1
     a.transfer = transfer
-    abi.check(abi.hip().pvhip_layer_update(C.byref(a), _stream(stream)), "pvhip_layer_update")
+    return a
+
+
+def launch_layer_update(args, stream=None):
+    abi.check(abi.hip().pvhip_layer_update(C.byref(args), _stream(stream)), "pvhip_layer_update")
+
+
+def layer_update(loc_, gsyn, V, A, transfer, channel_coeffs=(1.0,), err_scale=None, stream=None):
+    """ANNLayer / ANNErrorLayer / HyPerLayer update: V = [errScale*] sum coeff*GSyn; A = T(V)."""
+    launch_layer_update(layer_update_args(loc_, gsyn, V, A, transfer, channel_coeffs, err_scale),
+                        stream)
 
 
 def error_update(loc_, in_loc, in_activity, rec_loc, rec_activity, V, A, transfer,

@@ -296,3 +296,52 @@ def test_lca_column_with_bf16c_is_bit_identical(arith):
     assert np.array_equal(a.num_active(), b.num_active()) and a.num_active().sum() > 0
     ea, eb = host(a.probe_energy()), host(b.probe_energy())
     assert np.array_equal(ea, eb)
+
+
+@pytest.mark.parametrize("arith", [abi.ARITH_FP32, abi.ARITH_BF16])
+@pytest.mark.parametrize("shape", [(0, 0), (128, 1), (128, 4)])
+def test_channel_gemm_fused_layer_update(arith, shape):
+    """The target layer's update applied in the channel GEMM (fused_layer) gives
+    V and A bit-identical to the delivery into GSyn followed by
+    pvhip_layer_update, split or not, with an error scale, a threshold transfer
+    and a target halo."""
+    rng = np.random.default_rng(67)
+    pre, post, p = (2, 16, 16, 128, 2), (2, 32, 32, 3, 1), 8
+    hp, hq = abi.Loc.make(*pre[:4], halo=pre[4]), abi.Loc.make(*post[:4], halo=post[4])
+    op, oq = O.Loc.make(*pre[:4], halo=pre[4]), O.Loc.make(*post[:4], halo=post[4])
+    kx, ky, kf = O.num_kernels(op, oq)
+    W = (rng.standard_normal((kx * ky * kf, p, p, hq.nf)) * 0.1).astype(np.float32)
+    A = dev(ext_random(hp, rng, 0.1))
+    tr = R.make_transfer(0.05, 0.0, FMAX, 0.02, 0.0)
+    d = R.PresynapticPerspectiveGPUDelivery(hp, hq, p, p, dev(W), arith=arith)
+    assert d.transposed is not None and d.transposed.inverse is not None
+    d.transposed.set_launch(*shape)
+    n, ne = hq.nbatch * hq.num_res, hq.nbatch * hq.num_ext
+    out = []
+    for fused in (False, True):
+        g = torch.full((n,), 5.0, device=DEV)
+        V = torch.zeros(n, device=DEV)
+        Ae = torch.zeros(ne, device=DEV)
+        la = R.layer_update_args(hq, g, V, Ae, tr, channel_coeffs=(1.0,), err_scale=0.5)
+        if fused:
+            d.deliver(g, pre_activity=A, fused_layer=la)
+            assert np.all(host(g) == 5.0)   # GSyn not written
+        else:
+            d.deliver(g, pre_activity=A)
+            R.launch_layer_update(la)
+        out.append((host(V), host(Ae)))
+    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    assert np.array_equal(out[0][1].view(np.uint32), out[1][1].view(np.uint32))
+    assert np.abs(out[1][0]).max() > 0
+
+
+def test_fused_layer_refused_outside_the_channel_gemm():
+    """fused_layer on a one-patch gather (no channel GEMM) is an error."""
+    rng = np.random.default_rng(68)
+    hs, ht = abi.Loc.make(2, 32, 32, 3, halo=3), abi.Loc.make(2, 16, 16, 64, halo=0)
+    W = (rng.standard_normal((64, 8, 8, 3)) * 0.05).astype(np.float32)
+    d = R.PostsynapticPerspectiveGPUDelivery(hs, ht, 8, 8, dev(W), arith=abi.ARITH_BF16)
+    V = torch.zeros(ht.nbatch * ht.num_res, device=DEV)
+    la = R.layer_update_args(ht, None, V, None, R.make_transfer(-FMAX, -FMAX, FMAX, 0.0, 0.0))
+    with pytest.raises(abi.PVHIPError, match="fused_layer"):
+        d.deliver(dev(ext_random(hs, rng)), None, fused_layer=la)
