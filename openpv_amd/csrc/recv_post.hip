@@ -255,6 +255,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x8 __attribute__((ext_vector_type(8)));
 
 // Small-channel implicit GEMM (sources whose patch rows hold nxp*nfp values,
 // a multiple of 8): M = features (one data patch per feature), N = target
@@ -298,6 +299,50 @@ struct GatherFP8 {
       return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(__builtin_bit_cast(long, a), b, c, 0, 0, 0);
    }
 };
+
+// The FP32 form on the matrix cores: a fragment is the same 8 consecutive k
+// in FP32, consumed by eight v_mfma_f32_16x16x4f32 (MFMA j sums k = 8g + j
+// of lane group g, the same permutation of k for A and B).
+struct GatherF32 {
+   using T    = float;
+   using Frag = f32x8;
+   using RawA = f32x8;
+   static constexpr bool SCALED = false;
+   __device__ static T cvt(float x) { return x; }
+   __device__ static f32x4 mfma(RawA a, Frag b, f32x4 c) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+         c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j], b[j], c, 0, 0, 0);
+      }
+      return c;
+   }
+};
+
+// 8 consecutive FP32 values at p, read with the widest load ALIGN allows.
+template <int ALIGN>
+__device__ __forceinline__ f32x8 lds_frag(const float *p) {
+   f32x8 v;
+   if (ALIGN >= 4) {
+      f32x4 a = *reinterpret_cast<const f32x4 *>(p);
+      f32x4 b = *reinterpret_cast<const f32x4 *>(p + 4);
+      v       = f32x8{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+   }
+   else if (ALIGN >= 2) {
+#pragma unroll
+      for (int e = 0; e < 8; e += 2) {
+         float2 a = *reinterpret_cast<const float2 *>(p + e);
+         v[e]     = a.x;
+         v[e + 1] = a.y;
+      }
+   }
+   else {
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+         v[e] = p[e];
+      }
+   }
+   return v;
+}
 
 // 8 consecutive FP8 values at p, read with the widest load ALIGN allows.
 template <int ALIGN>
@@ -727,7 +772,7 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
       else {
 #pragma unroll
          for (int e = 0; e < 8; e++) {
-            bfrag[e] = kos[e] >= 0 ? src[pbase[pb] + kos[e]] : (__bf16)0.0f;
+            bfrag[e] = kos[e] >= 0 ? src[pbase[pb] + kos[e]] : (T)0.0f;
          }
       }
       return bfrag;
@@ -1726,9 +1771,22 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
          "recv_post: the FP8 form needs one data patch per feature (the channel GEMM has BF16 and FP32 forms)");
    PVHIP_REQUIRE(!p.layer, "recv_post: fused_layer is supported by the channel GEMM (BF16 and FP32 forms with "
                            "several data patches per feature) only");
-   if (arith == PVHIP_ARITH_BF16 || arith == PVHIP_ARITH_FP8) {
+   // FP32 on the matrix cores: the gather kernel of the low-precision forms
+   // with FP32 operands, for patches of a multiple of 32 values (the FP32
+   // weights are then their own padded copy) whose tile fits the LDS.
+   bool mfma32 = false;
+   if (arith == PVHIP_ARITH_FP32) {
+      int const K          = p.nxp * p.nfp * p.nyp;
+      int const srcColsMax = (MTX - 1) * sx + p.nxp;
+      int const srcRowsMax = (MTY - 1) * sy + p.nyp;
+      int const pitch      = (srcColsMax * p.nfp + 7) / 8 * 8;
+      size_t const lds     = std::max(((size_t)srcRowsMax * pitch * 4 + 3) / 4 * 4 + (size_t)K * sizeof(int),
+                                      (size_t)MTX * MTY * MOP * sizeof(float));
+      mfma32 = K % 32 == 0 && lds <= 64 * 1024;
+   }
+   if (arith == PVHIP_ARITH_BF16 || arith == PVHIP_ARITH_FP8 || mfma32) {
       bool const fp8 = arith == PVHIP_ARITH_FP8;
-      PVHIP_REQUIRE(args->weights_lp != nullptr,
+      PVHIP_REQUIRE(mfma32 || args->weights_lp != nullptr,
             "recv_post: %s needs weights_lp (pvhip_recv_post_%s_weights)", fp8 ? "FP8" : "BF16",
             fp8 ? "fp8" : "bf16");
       int const rowK = p.nxp * p.nfp;
@@ -1737,10 +1795,10 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
       int const srcColsMax = (MTX - 1) * sx + p.nxp;
       int const srcRowsMax = (MTY - 1) * sy + p.nyp;
       int const pitch      = (srcColsMax * p.nfp + 7) / 8 * 8;
-      size_t const esz = fp8 ? 1 : 2;
+      size_t const esz = fp8 ? 1 : (mfma32 ? 4 : 2);
       size_t lds = ((size_t)srcRowsMax * pitch * esz + 3) / 4 * 4 + (size_t)Kpad * sizeof(int);
       lds        = std::max(lds, (size_t)MTX * MTY * MOP * sizeof(float)); // epilogue tile
-      PVHIP_REQUIRE(lds <= 160 * 1024, "recv_post: BF16 tile needs %zu bytes of LDS", lds);
+      PVHIP_REQUIRE(lds <= 160 * 1024, "recv_post: MFMA tile needs %zu bytes of LDS", lds);
       // alignment class of a fragment's LDS address: rows are 8-aligned, window
       // starts differ by multiples of sx*nfp, fragments start at multiples of 8
       int step  = sx * p.nfp;
@@ -1771,7 +1829,11 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
       }
 #define PVHIP_MFMA(AL, EP)                                                                       \
    do {                                                                                          \
-      if (fp8) {                                                                                 \
+      if (mfma32) {                                                                              \
+         hipLaunchKernelGGL((recv_post_mfma_bf16_kernel<AL, EP, GatherF32>), grid, dim3(256), lds, \
+               s, p, args->weights, srcRowsMax, srcColsMax, Kpad);                               \
+      }                                                                                          \
+      else if (fp8) {                                                                            \
          hipLaunchKernelGGL((recv_post_mfma_bf16_kernel<AL, EP, GatherFP8>), grid, dim3(256), lds, \
                s, p, static_cast<const uint8_t *>(args->weights_lp), srcRowsMax, srcColsMax, Kpad); \
       }                                                                                          \
@@ -1794,7 +1856,7 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
       else PVHIP_MFMA_EPI(0);
 #undef PVHIP_MFMA_EPI
 #undef PVHIP_MFMA
-      PVHIP_LAUNCHED(fp8 ? "recv_post_mfma_fp8" : "recv_post_mfma_bf16");
+      PVHIP_LAUNCHED(fp8 ? "recv_post_mfma_fp8" : (mfma32 ? "recv_post_mfma_f32" : "recv_post_mfma_bf16"));
       return PVHIP_OK;
    }
    if ((arith == PVHIP_ARITH_EXACT || arith == PVHIP_ARITH_FP32) && single) {
