@@ -1782,7 +1782,7 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
       int const pitch      = (srcColsMax * p.nfp + 7) / 8 * 8;
       size_t const lds     = std::max(((size_t)srcRowsMax * pitch * 4 + 3) / 4 * 4 + (size_t)K * sizeof(int),
                                       (size_t)MTX * MTY * MOP * sizeof(float));
-      mfma32 = K % 32 == 0 && lds <= 64 * 1024;
+      mfma32 = K % 32 == 0 && lds <= 80 * 1024; // two workgroups per CU at least
    }
    if (arith == PVHIP_ARITH_BF16 || arith == PVHIP_ARITH_FP8 || mfma32) {
       bool const fp8 = arith == PVHIP_ARITH_FP8;
