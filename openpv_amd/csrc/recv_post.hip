@@ -917,6 +917,229 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
    }
 }
 
+// Persistent form of the gather + fused LCA update for the common case (full
+// 16x8x64 tiles, VTHRESH transfer, patches of KS*32 values read with vector
+// LDS loads): a workgroup keeps its 64 features' weight fragments in
+// registers and walks `ipw` tiles (over positions and batch), fetching the
+// next tile's old V and source window into registers while the current tile
+// runs its MFMAs and epilogue, so the loads of one tile overlap the stores of
+// the one before.  Launched for the FP32 form, where it measured faster.  Per neuron the arithmetic is that of
+// recv_post_mfma_bf16_kernel<ALIGN, 1, E> (same MFMA order, same epilogue):
+// the same bits.
+constexpr int PSREG = 16; // source window values per thread held in registers (window <= 4096 values)
+
+template <int ALIGN, typename E, int KS>
+__global__ void __launch_bounds__(256) recv_post_mfma_persist_kernel(
+      PostParams p, const typename E::T *Wb, int srcRowsMax, int srcColsMax, int items, int ipw) {
+   using T    = typename E::T;
+   using RawA = typename E::RawA;
+   extern __shared__ float smem[];
+   constexpr int K  = KS * 32;
+   int const rowK   = p.nxp * p.nfp;
+   int const pitch  = (srcColsMax * p.nfp + 7) / 8 * 8;
+   T *src           = reinterpret_cast<T *>(smem);
+   float *outl      = smem; // [MTX*MTY][MOP], after the MFMAs of a tile
+   int const tilesX = p.post.nx / MTX;
+   int const tilesXY = tilesX * (p.post.ny / MTY);
+   int const fBase  = blockIdx.y * MF;
+   int const tid    = threadIdx.x;
+   int const lane   = tid % 64;
+   int const wave   = tid / 64;
+   int const r      = lane % 16;
+   int const g      = lane / 16;
+   int const fWave  = fBase + wave * (MF / 4);
+   int const eq     = tid % MEQ;
+   int const epr    = tid / MEQ;
+   int const ef     = fBase + 4 * eq;
+   int const preRow = p.pre.nxExt() * p.pre.nf;
+   // weight fragments of every k-step, once
+   RawA aAll[KS][MFB];
+#pragma unroll
+   for (int fb = 0; fb < MFB; fb++) {
+      const T *rowA = Wb + (size_t)(fWave + fb * 16 + r) * K;
+#pragma unroll
+      for (int st = 0; st < KS; st++) {
+         aAll[st][fb] = *reinterpret_cast<const RawA *>(rowA + 32 * st + 8 * g);
+      }
+   }
+   int koffR[KS]; // LDS offset of this lane group's 8 k of each k-step (no fragment straddles a patch row)
+#pragma unroll
+   for (int st = 0; st < KS; st++) {
+      int const kk = 32 * st + 8 * g;
+      koffR[st]    = (kk / rowK) * pitch + (kk % rowK);
+   }
+   int const first = blockIdx.x * ipw;
+   int const last  = min(first + ipw, items);
+   if (first >= last) {
+      return; // whole workgroup
+   }
+   // geometry of one tile: batch, corner, window origin, window size
+   struct Tile {
+      int b, x0, y0, srcX0, srcY0, rowElems, total;
+   };
+   auto tile_of = [&](int item) {
+      Tile t;
+      t.b        = item / tilesXY;
+      int const tl = item % tilesXY;
+      t.x0       = (tl % tilesX) * MTX;
+      t.y0       = (tl / tilesX) * MTY;
+      t.srcX0    = p.startX[t.x0];
+      t.srcY0    = p.startY[t.y0];
+      // the host's tables keep every window inside the extended source and
+      // within the staged size; the clamps only keep the loads in bounds
+      int const srcCols = min(p.startX[t.x0 + MTX - 1] - t.srcX0 + p.nxp, srcColsMax);
+      int const srcRows = min(p.startY[t.y0 + MTY - 1] - t.srcY0 + p.nyp, srcRowsMax);
+      t.srcX0    = max(0, min(t.srcX0, p.pre.nxExt() - srcCols));
+      t.srcY0    = max(0, min(t.srcY0, p.pre.nyExt() - srcRows));
+      t.rowElems = srcCols * p.nfp;
+      t.total    = srcRows * t.rowElems;
+      return t;
+   };
+   // source window of a tile -> registers (element i = tid + 256*u, clamped)
+   auto fetch_src = [&](const Tile &t, float (&sv)[PSREG]) {
+      float const *winB = p.pre_data + (size_t)t.b * p.pre.numExt() + (size_t)t.srcY0 * preRow
+                          + (size_t)t.srcX0 * p.nfp;
+      int sr = tid / t.rowElems;
+      int sc = tid - sr * t.rowElems;
+      int const dR = 256 / t.rowElems;
+      int const dC = 256 - dR * t.rowElems;
+      int const rows = t.total / t.rowElems;
+#pragma unroll
+      for (int u = 0; u < PSREG; u++) {
+         bool const ok = sr < rows;
+         sv[u] = winB[(size_t)(ok ? sr : rows - 1) * preRow + (ok ? sc : t.rowElems - 1)];
+         sc += dC;
+         sr += dR;
+         if (sc >= t.rowElems) {
+            sc -= t.rowElems;
+            sr++;
+         }
+      }
+   };
+   auto store_src = [&](const Tile &t, const float (&sv)[PSREG]) {
+      int sr = tid / t.rowElems;
+      int sc = tid - sr * t.rowElems;
+      int const dR = 256 / t.rowElems;
+      int const dC = 256 - dR * t.rowElems;
+      int const rows = t.total / t.rowElems;
+#pragma unroll
+      for (int u = 0; u < PSREG; u++) {
+         if (sr < rows) {
+            src[sr * pitch + sc] = E::cvt(E::SCALED ? sv[u] * p.lpSrcScale : sv[u]);
+         }
+         sc += dC;
+         sr += dR;
+         if (sc >= t.rowElems) {
+            sc -= t.rowElems;
+            sr++;
+         }
+      }
+   };
+   auto fetch_v = [&](const Tile &t, f32x4 (&v)[MEP]) {
+#pragma unroll
+      for (int it = 0; it < MEP; it++) {
+         int const pj = it * MEPP + epr;
+         int const x  = t.x0 + pj % MTX;
+         int const y  = t.y0 + pj / MTX;
+         v[it] = *reinterpret_cast<const f32x4 *>(
+               p.lca.V + (size_t)t.b * p.post.numRes() + (y * p.post.nx + x) * p.post.nf + ef);
+      }
+   };
+   auto bases = [&](const Tile &t, int (&pbase)[MPB]) {
+#pragma unroll
+      for (int pb = 0; pb < MPB; pb++) {
+         int const pj = pb * 16 + r;
+         int const x  = t.x0 + pj % MTX;
+         int const y  = t.y0 + pj / MTX;
+         pbase[pb]    = (p.startY[y] - t.srcY0) * pitch + (p.startX[x] - t.srcX0) * p.nfp;
+      }
+   };
+
+   Tile cur = tile_of(first);
+   f32x4 vpre[MEP], vpreN[MEP];
+   float sv[PSREG];
+   int pbase[MPB], pbaseN[MPB];
+   fetch_v(cur, vpre);
+   fetch_src(cur, sv);
+   bases(cur, pbase);
+   store_src(cur, sv);
+   __syncthreads();
+   for (int item = first; item < last; item++) {
+      f32x4 acc[MFB][MPB];
+#pragma unroll
+      for (int fb = 0; fb < MFB; fb++) {
+#pragma unroll
+         for (int pb = 0; pb < MPB; pb++) {
+            acc[fb][pb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+         }
+      }
+#pragma unroll
+      for (int st = 0; st < KS; st++) {
+#pragma unroll
+         for (int pb = 0; pb < MPB; pb++) {
+            typename E::Frag bfrag;
+            if constexpr (E::SCALED) {
+               bfrag = lds_frag8<ALIGN>(src + pbase[pb] + koffR[st]);
+            }
+            else {
+               bfrag = lds_frag<ALIGN>(src + pbase[pb] + koffR[st]);
+            }
+#pragma unroll
+            for (int fb = 0; fb < MFB; fb++) {
+               acc[fb][pb] = E::mfma(aAll[st][fb], bfrag, acc[fb][pb]);
+            }
+         }
+      }
+      // the next tile's loads, in flight during this tile's epilogue
+      bool const more = item + 1 < last; // the same for the whole workgroup
+      Tile nxt        = cur;
+      if (more) {
+         nxt = tile_of(item + 1);
+         fetch_v(nxt, vpreN);
+         fetch_src(nxt, sv);
+         bases(nxt, pbaseN);
+      }
+      if constexpr (E::SCALED) {
+#pragma unroll
+         for (int fb = 0; fb < MFB; fb++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+               float const unscl = 1.0f / (p.lpWScale[fWave + fb * 16 + 4 * g + i] * p.lpSrcScale);
+#pragma unroll
+               for (int pb = 0; pb < MPB; pb++) {
+                  acc[fb][pb][i] *= unscl;
+               }
+            }
+         }
+      }
+      __syncthreads(); // every wave is done reading the source tile
+#pragma unroll
+      for (int pb = 0; pb < MPB; pb++) {
+#pragma unroll
+         for (int fb = 0; fb < MFB; fb++) {
+            *reinterpret_cast<f32x4 *>(outl + (pb * 16 + r) * MOP + wave * (MF / 4) + fb * 16 + 4 * g) =
+                  acc[fb][pb];
+         }
+      }
+      __syncthreads();
+      lca_epilogue_vthresh<false>(p, cur.b, cur.x0, cur.y0, epr, eq, ef, outl + epr * MOP + 4 * eq, vpre);
+      if (more) {
+         __syncthreads(); // every wave is done reading the output tile
+         store_src(nxt, sv);
+#pragma unroll
+         for (int it = 0; it < MEP; it++) {
+            vpre[it] = vpreN[it];
+         }
+#pragma unroll
+         for (int pb = 0; pb < MPB; pb++) {
+            pbase[pb] = pbaseN[pb];
+         }
+         cur = nxt;
+         __syncthreads();
+      }
+   }
+}
+
 // ------------------------------------------------------------------ mfma (channel GEMM)
 // Implicit GEMM for sources with nfp % CK == 0 channels and any number of
 // data patches (targets larger than the source, e.g. the transposed form of
@@ -1826,6 +2049,38 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
          p.segCnt = reinterpret_cast<uint32_t *>(
                static_cast<char *>(aout->scratch) + segment_stage_bytes(args->fused_update->loc));
          epi      = 3;
+      }
+      // the persistent kernel for full tiles of the fused VTHRESH update in
+      // FP32 (measured: faster for FP32 operands, whose weight fragments it
+      // keeps in registers across tiles; no faster for BF16/FP8, which keep the
+      // one-tile kernel)
+      int const KS       = K / 32;
+      bool const persist = mfma32 && epi == 1 && align >= 2 && K % 32 == 0 && (KS == 6 || KS == 8)
+                           && p.post.nx % MTX == 0 && p.post.ny % MTY == 0 && p.post.nf % MF == 0
+                           && (long long)srcRowsMax * srcColsMax * p.nfp <= 256LL * PSREG;
+      if (persist) {
+         int const items = tilesX * tilesY * p.post.nbatch;
+         int const fblk  = p.post.nf / MF;
+         // about 512 workgroups (two per CU), each walking ipw tiles
+         int const ipw  = std::max(1, (int)(((long long)items * fblk + 511) / 512));
+         dim3 pgrid((unsigned)((items + ipw - 1) / ipw), (unsigned)fblk);
+         size_t const plds = std::max(((size_t)srcRowsMax * pitch * esz + 3) / 4 * 4,
+                                      (size_t)MTX * MTY * MOP * sizeof(float));
+#define PVHIP_PERSIST(AL, KSV)                                                                   \
+   hipLaunchKernelGGL((recv_post_mfma_persist_kernel<AL, GatherF32, KSV>), pgrid, dim3(256), plds, s, \
+         p, args->weights, srcRowsMax, srcColsMax, items, ipw)
+#define PVHIP_PERSIST_KS(AL)                                                                     \
+   do {                                                                                          \
+      if (KS == 8) PVHIP_PERSIST(AL, 8);                                                         \
+      else PVHIP_PERSIST(AL, 6);                                                                 \
+   } while (0)
+         if (align == 8) PVHIP_PERSIST_KS(8);
+         else if (align == 4) PVHIP_PERSIST_KS(4);
+         else PVHIP_PERSIST_KS(2);
+#undef PVHIP_PERSIST_KS
+#undef PVHIP_PERSIST
+         PVHIP_LAUNCHED("recv_post_mfma_persist");
+         return PVHIP_OK;
       }
 #define PVHIP_MFMA(AL, EP)                                                                       \
    do {                                                                                          \
