@@ -72,16 +72,47 @@ __global__ void __launch_bounds__(256) recv_pre_gather_kernel(PreParams p) {
             int const kerBase = (p.colKx[ci] + kyK * p.nKx) * nfPre;
             int const pos  = ye * nxExt + xe;
             if (ps) {
-               uint32_t const s = ps[pos];
-               uint32_t const e = ps[pos + 1];
-               for (uint32_t i = s; i < e; i++) {
+               uint32_t const s    = ps[pos];
+               uint32_t const e    = ps[pos + 1];
+               uint32_t const kex0 = (uint32_t)pos * (uint32_t)nfPre; // kex = pos*nf + f
+               // four entries (and their weights) loaded together, then added
+               // in entry order
+               constexpr int U = 8;
+               uint32_t i      = s;
+               for (; i + U <= e; i += U) {
+                  pvhip_sparse_entry en[U];
+                  float w[U];
+#pragma unroll
+                  for (int u = 0; u < U; u++) {
+                     en[u] = ent[i + u];
+                  }
+#pragma unroll
+                  for (int u = 0; u < U; u++) {
+                     w[u] = p.W[(size_t)(kerBase + (int)(en[u].index - kex0)) * patchSize + wrow];
+                  }
+#pragma unroll
+                  for (int u = 0; u < U; u++) {
+                     float a = en[u].value;
+                     if (a != 0.0f) {
+                        a = a * p.dtf;
+                        if (EXACT) {
+                           float prod = a * w[u];
+                           g          = g + prod;
+                        }
+                        else {
+                           g = __builtin_fmaf(a, w[u], g);
+                        }
+                     }
+                  }
+               }
+               for (; i < e; i++) {
                   pvhip_sparse_entry const en = ent[i];
                   float a = en.value;
                   if (a == 0.0f) {
                      continue;
                   }
                   a           = a * p.dtf;
-                  int const f = (int)(en.index % (uint32_t)nfPre);
+                  int const f = (int)(en.index - kex0);
                   float w     = p.W[(size_t)(kerBase + f) * patchSize + wrow];
                   if (EXACT) {
                      float prod = a * w;
