@@ -155,11 +155,19 @@ __device__ __forceinline__ f32x8v hload_b<float>(const float *bl, int, int e) {
    return v;
 }
 
+// Staging of a pre row, software-pipelined: the next row's activity values
+// are loaded into registers (HAREG per thread, any rest loaded when stored)
+// while the MFMAs of the current row run.  The window's post rows sit in a
+// ring of nyp rows (slot = post row mod nyp), so a pre row following the one
+// before in the same image stages only the post rows the window advanced by
+// (sy of them), not all nyp.
+constexpr int HAREG = 40;
+
 template <typename T, bool COUNT>
 __global__ void __launch_bounds__(256) hebbian_mfma_kernel(HebbMfma h) {
    extern __shared__ float smem[];
    int const copies = sizeof(T) == 2 ? 2 : 1;
-   int const nbrow  = h.nyp * h.nfp * h.sx; // B rows per pre row
+   int const nbrow  = h.nyp * h.nfp * h.sx; // B rows (ring slot, feature, phase)
    int const cstr   = (nbrow * h.lq + 7) / 8 * 8; // elements per B copy (16-byte multiple)
    T *bl            = reinterpret_cast<T *>(smem);
    T *al            = bl + (size_t)copies * cstr; // [HMT][apitch]
@@ -177,17 +185,20 @@ __global__ void __launch_bounds__(256) hebbian_mfma_kernel(HebbMfma h) {
    int const r      = lane % 16;
    int const g      = lane / 16;
    int const nxExtP = h.post.nxExt();
+   int const rowB   = h.nfp * h.sx * h.lq; // B elements of one post row
+   int const aItems = HMT * h.kxPad;
 
-   // this lane's B row and offset for each of its item blocks
-   int bbase[HNB];
+   // this lane's item of each item block: window row ky, feature fp, and the
+   // B element offset within a ring slot
+   int nky[HNB], nsub[HNB];
 #pragma unroll
    for (int nb = 0; nb < HNB; nb++) {
       int n     = min(n0 + (wave * HNB + nb) * 16 + r, h.N - 1);
       int fp    = n % h.nfp;
       int kx    = (n / h.nfp) % h.nxp;
-      int ky    = n / (h.nfp * h.nxp);
+      nky[nb]   = n / (h.nfp * h.nxp);
       int ph    = (h.cx0 + kx) % h.sx;
-      bbase[nb] = ((ky * h.nfp + fp) * h.sx + ph) * h.lq + (h.cx0 + kx) / h.sx + 8 * g;
+      nsub[nb]  = (fp * h.sx + ph) * h.lq + (h.cx0 + kx) / h.sx + 8 * g;
    }
    f32x4 acc[4][HNB];
 #pragma unroll
@@ -197,50 +208,116 @@ __global__ void __launch_bounds__(256) hebbian_mfma_kernel(HebbMfma h) {
          acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
       }
    }
-   for (int row = r0; row < r1; row++) {
+   auto ring = [&](int py) { int m = py % h.nyp; return m < 0 ? m + h.nyp : m; };
+   // activity element i (feature m = i % HMT, pre column xe = i / HMT) of pre row `row`
+   auto a_val = [&](int row, int i) -> float {
       int const b  = row / nyExt;
       int const ye = row % nyExt;
-      float const *preRow = h.preData + ((size_t)b * nyExt + ye) * nxExt * h.pre.nf;
-      float const *postB  = h.postData + (size_t)b * h.post.numExt();
-      __syncthreads(); // previous row's fragments are consumed
-      // A: features f0.. of every pre column of this row, transposed; zero
-      // columns past the row and features past nK
-      for (int i = tid; i < HMT * h.kxPad; i += 256) {
-         int const m  = i % HMT;
-         int const xe = i / HMT;
+      int const m  = i % HMT;
+      int const xe = i / HMT;
+      float v      = 0.0f;
+      if (xe < nxExt && f0 + m < h.nK) {
+         v = h.preData[(((size_t)b * nyExt + ye) * nxExt + xe) * h.pre.nf + f0 + m];
+         if (COUNT) {
+            v = v != 0.0f ? 1.0f : 0.0f;
+         }
+      }
+      return v;
+   };
+   // post rows a pre row stages: [pyA, pyB) (all nyp rows, or the sy new ones
+   // when the row before was the previous pre row of the same image)
+   auto new_rows = [&](int row, int &pyA, int &pyB) {
+      int const ye = row % nyExt;
+      int const wy = h.sy * ye + h.cy;
+      pyA          = wy;
+      pyB          = wy + h.nyp;
+      if (row > r0 && ye > 0 && h.sy < h.nyp) {
+         pyA = wy + h.nyp - h.sy;
+      }
+   };
+   // window element i of the staged post rows (post row pyA + i / rowB)
+   auto b_val = [&](int row, int pyA, int i) -> float {
+      int const b  = row / nyExt;
+      int const py = pyA + i / rowB;
+      int const e  = i % rowB;
+      int const xq = e % h.lq;
+      int const fp = e / (h.sx * h.lq);
+      int const ph = (e / h.lq) % h.sx;
+      int const px = h.sx * (xq + h.qx0) + ph;
+      bool const in = py >= h.post.up && py < h.post.up + h.post.ny && px >= h.post.lt
+                      && px < h.post.lt + h.post.nx;
+      float v = 0.0f;
+      if (in) {
+         v = COUNT ? 1.0f : h.postData[(size_t)b * h.post.numExt() + ((size_t)py * nxExtP + px) * h.post.nf + fp];
+      }
+      return v;
+   };
+   auto b_store = [&](int pyA, int i, float v) {
+      int const py   = pyA + i / rowB;
+      int const e    = i % rowB;
+      int const idx  = ring(py) * rowB + e;
+      T const t      = hcvt<T>(v);
+      bl[idx]        = t;
+      if (copies == 2 && (e % h.lq) > 0) {
+         bl[cstr + idx - 1] = t; // copy 1: row shifted left by one element
+      }
+   };
+   // element tid + 256*u of the activity slab is feature tid % HMT of pre
+   // column tid / HMT + 4*u (256 is a multiple of HMT)
+   static_assert(256 % HMT == 0, "activity slab walk assumes HMT divides 256");
+   int const am  = tid % HMT;
+   int const ax0 = tid / HMT;
+   float va[HAREG];
+   auto fetch = [&](int row) {
+      float const *base = h.preData + ((size_t)row * nxExt + ax0) * h.pre.nf + f0 + am; // row = b*nyExt + ye
+      bool const fok    = f0 + am < h.nK;
+#pragma unroll
+      for (int u = 0; u < HAREG; u++) {
+         int const xe = ax0 + (256 / HMT) * u;
          float v      = 0.0f;
-         if (xe < nxExt && f0 + m < h.nK) {
-            v = preRow[(size_t)xe * h.pre.nf + f0 + m];
+         if (fok && xe < nxExt) {
+            v = base[(size_t)(256 / HMT) * u * h.pre.nf];
             if (COUNT) {
                v = v != 0.0f ? 1.0f : 0.0f;
             }
          }
-         al[m * h.apitch + xe] = hcvt<T>(v);
+         va[u] = v;
       }
-      // B: the window rows of this pre row, phase-split, zero outside the
-      // post restricted region
-      int const wy = h.sy * ye + h.cy;
-      for (int i = tid; i < nbrow * h.lq; i += 256) {
-         int const xq  = i % h.lq;
-         int const br  = i / h.lq;
-         int const ph  = br % h.sx;
-         int const fp  = (br / h.sx) % h.nfp;
-         int const ky  = br / (h.sx * h.nfp);
-         int const py  = wy + ky;
-         int const px  = h.sx * (xq + h.qx0) + ph;
-         bool const in = py >= h.post.up && py < h.post.up + h.post.ny && px >= h.post.lt
-                         && px < h.post.lt + h.post.nx;
-         float v = 0.0f;
-         if (in) {
-            v = COUNT ? 1.0f : postB[((size_t)py * nxExtP + px) * h.post.nf + fp];
-         }
-         T const t = hcvt<T>(v);
-         bl[i]     = t;
-         if (copies == 2 && xq > 0) {
-            bl[cstr + i - 1] = t; // copy 1: row shifted left by one element
+   };
+   auto stage = [&](int row) {
+#pragma unroll
+      for (int u = 0; u < HAREG; u++) {
+         int const i = tid + 256 * u;
+         if (i < aItems) {
+            al[(i % HMT) * h.apitch + i / HMT] = hcvt<T>(va[u]);
          }
       }
+      for (int i = tid + 256 * HAREG; i < aItems; i += 256) {
+         al[(i % HMT) * h.apitch + i / HMT] = hcvt<T>(a_val(row, i));
+      }
+      int pyA, pyB;
+      new_rows(row, pyA, pyB);
+      int const bItems = (pyB - pyA) * rowB;
+      for (int i = tid; i < bItems; i += 256) {
+         b_store(pyA, i, b_val(row, pyA, i));
+      }
+   };
+   if (r0 < r1) {
+      fetch(r0);
+   }
+   for (int row = r0; row < r1; row++) {
+      __syncthreads(); // the previous row's fragments are consumed
+      stage(row);
       __syncthreads();
+      if (row + 1 < r1) {
+         fetch(row + 1); // in flight during this row's MFMAs
+      }
+      int const wy = h.sy * (row % nyExt) + h.cy;
+      int bbase[HNB];
+#pragma unroll
+      for (int nb = 0; nb < HNB; nb++) {
+         bbase[nb] = ring(wy + nky[nb]) * rowB + nsub[nb];
+      }
       for (int x0 = 0; x0 < h.kxPad; x0 += 32) {
          typename HebbFrag<T>::type af[4], bf[HNB];
 #pragma unroll
