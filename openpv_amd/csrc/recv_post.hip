@@ -1322,10 +1322,11 @@ __global__ void __launch_bounds__(256) recv_post_cgemm_kernel(
       int t  = j / ((CK / EP) * Mp);
       return (t * Mp + m) * PP + pc * EP;
    };
-   // Channel chunks in flight in registers: the chunked BF16 source needs few
-   // registers per chunk, so two chunks are fetched ahead; the FP32 source one.
-   constexpr int DEPTH = SRCB ? 2 : 1;
-   constexpr int NSV   = SRCB ? 6 : CSV; // source items per thread and chunk held in registers
+   // Channel chunks in flight in registers: the chunked BF16 source and the
+   // FP32 GEMM (16-channel chunks) need few registers per chunk, so two chunks
+   // are fetched ahead; the BF16 GEMM converting an FP32 source one.
+   constexpr int DEPTH = (SRCB || sizeof(T) == 4) ? 2 : 1;
+   constexpr int NSV   = DEPTH == 2 ? 6 : CSV; // source items per thread and chunk held in registers
    int sG[NSV], sL[NSV], wG[CAV], wL[CAV];
 #pragma unroll
    for (int u = 0; u < NSV; u++) {
