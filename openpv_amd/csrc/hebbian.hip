@@ -156,12 +156,16 @@ __device__ __forceinline__ f32x8v hload_b<float>(const float *bl, int, int e) {
 }
 
 // Staging of a pre row, software-pipelined: the next row's activity values
-// are loaded into registers (HAREG per thread, any rest loaded when stored)
-// while the MFMAs of the current row run.  The window's post rows sit in a
+// and window values are loaded into registers (HAREG and HBR x HBX per
+// thread, any rest loaded when stored) while the MFMAs of the current row run.  The window's post rows sit in a
 // ring of nyp rows (slot = post row mod nyp), so a pre row following the one
 // before in the same image stages only the post rows the window advanced by
 // (sy of them), not all nyp.
-constexpr int HAREG = 40;
+constexpr int HAREG = 32;
+// window values held per lane: HBR B rows per wave (B row = post row, feature,
+// phase) x HBX values 64 apart along the row
+constexpr int HBR = 4;
+constexpr int HBX = 3;
 
 template <typename T, bool COUNT>
 __global__ void __launch_bounds__(256) hebbian_mfma_kernel(HebbMfma h) {
@@ -235,40 +239,52 @@ __global__ void __launch_bounds__(256) hebbian_mfma_kernel(HebbMfma h) {
          pyA = wy + h.nyp - h.sy;
       }
    };
-   // window element i of the staged post rows (post row pyA + i / rowB)
-   auto b_val = [&](int row, int pyA, int i) -> float {
-      int const b  = row / nyExt;
-      int const py = pyA + i / rowB;
-      int const e  = i % rowB;
-      int const xq = e % h.lq;
-      int const fp = e / (h.sx * h.lq);
-      int const ph = (e / h.lq) % h.sx;
-      int const px = h.sx * (xq + h.qx0) + ph;
-      bool const in = py >= h.post.up && py < h.post.up + h.post.ny && px >= h.post.lt
-                      && px < h.post.lt + h.post.nx;
-      float v = 0.0f;
-      if (in) {
-         v = COUNT ? 1.0f : h.postData[(size_t)b * h.post.numExt() + ((size_t)py * nxExtP + px) * h.post.nf + fp];
-      }
-      return v;
-   };
-   auto b_store = [&](int pyA, int i, float v) {
-      int const py   = pyA + i / rowB;
-      int const e    = i % rowB;
-      int const idx  = ring(py) * rowB + e;
-      T const t      = hcvt<T>(v);
-      bl[idx]        = t;
-      if (copies == 2 && (e % h.lq) > 0) {
-         bl[cstr + idx - 1] = t; // copy 1: row shifted left by one element
-      }
-   };
    // element tid + 256*u of the activity slab is feature tid % HMT of pre
    // column tid / HMT + 4*u (256 is a multiple of HMT)
    static_assert(256 % HMT == 0, "activity slab walk assumes HMT divides 256");
    int const am  = tid % HMT;
    int const ax0 = tid / HMT;
-   float va[HAREG];
+   // window value (B row br of the staged post rows [pyA, ..), position xq),
+   // and its ring slot; B rows are walked one per wave, lanes along xq, so the
+   // row's decomposition is wave-uniform
+   auto b_row = [&](int row, int pyA, int br, int &py, int &fp, int &ph) {
+      int const pr = h.nfp * h.sx;
+      py           = pyA + br / pr;
+      fp           = (br % pr) / h.sx;
+      ph           = br % h.sx;
+   };
+   auto b_elem = [&](int b, int py, int fp, int ph, int xq) -> float {
+      int const px = h.sx * (xq + h.qx0) + ph;
+      bool const in = py >= h.post.up && py < h.post.up + h.post.ny && px >= h.post.lt
+                      && px < h.post.lt + h.post.nx;
+      return in ? (COUNT ? 1.0f : h.postData[(size_t)b * h.post.numExt() + ((size_t)py * nxExtP + px) * h.post.nf + fp])
+                : 0.0f;
+   };
+   auto b_put = [&](int py, int fp, int ph, int xq, float v) {
+      int const idx = ring(py) * rowB + (fp * h.sx + ph) * h.lq + xq;
+      T const t     = hcvt<T>(v);
+      bl[idx]       = t;
+      if (copies == 2 && xq > 0) {
+         bl[cstr + idx - 1] = t; // copy 1: row shifted left by one element
+      }
+   };
+   float va[HAREG], vb[HBR][HBX];
    auto fetch = [&](int row) {
+      int pyA, pyB;
+      new_rows(row, pyA, pyB);
+      int const nbr = (pyB - pyA) * h.nfp * h.sx; // B rows to stage
+      int const b   = row / nyExt;
+#pragma unroll
+      for (int j = 0; j < HBR; j++) {
+         int const br = wave + 4 * j;
+         int py = 0, fp = 0, ph = 0;
+         b_row(row, pyA, min(br, max(nbr - 1, 0)), py, fp, ph);
+#pragma unroll
+         for (int x = 0; x < HBX; x++) {
+            int const xq = lane + 64 * x;
+            vb[j][x]     = (br < nbr && xq < h.lq) ? b_elem(b, py, fp, ph, xq) : 0.0f;
+         }
+      }
       float const *base = h.preData + ((size_t)row * nxExt + ax0) * h.pre.nf + f0 + am; // row = b*nyExt + ye
       bool const fok    = f0 + am < h.nK;
 #pragma unroll
@@ -297,9 +313,32 @@ __global__ void __launch_bounds__(256) hebbian_mfma_kernel(HebbMfma h) {
       }
       int pyA, pyB;
       new_rows(row, pyA, pyB);
-      int const bItems = (pyB - pyA) * rowB;
-      for (int i = tid; i < bItems; i += 256) {
-         b_store(pyA, i, b_val(row, pyA, i));
+      int const nbr = (pyB - pyA) * h.nfp * h.sx;
+      int const b   = row / nyExt;
+      for (int br = wave; br < nbr; br += 4) {
+         int py, fp, ph;
+         b_row(row, pyA, br, py, fp, ph);
+         int const j = br / 4; // register slot of this B row, if any
+         for (int xq = lane; xq < h.lq; xq += 64) {
+            int const x = xq / 64;
+            float v;
+            if (j < HBR && x < HBX) {
+               // the prefetched value (the register index is a compile-time
+               // choice: select among the slots)
+               v = 0.0f;
+#pragma unroll
+               for (int jj = 0; jj < HBR; jj++) {
+#pragma unroll
+                  for (int xx = 0; xx < HBX; xx++) {
+                     v = (jj == j && xx == x) ? vb[jj][xx] : v;
+                  }
+               }
+            }
+            else {
+               v = b_elem(b, py, fp, ph, xq);
+            }
+            b_put(py, fp, ph, xq, v);
+         }
       }
    };
    if (r0 < r1) {
