@@ -198,7 +198,7 @@ def _time_events(fn, s, reps, torch):
 
 
 def c1_delivery(torch, reps=10):
-    """Delivery microbench: pre 256x256x16 halo 6 -> post 128x128x256,
+    """Full-size delivery case: pre 256x256x16 halo 6 -> post 128x128x256,
     pre-perspective patch 6x6x256 (post-perspective 12x12x16), W ~ N(0, 0.05^2).
     Dense gather (pre ~N(0,1)) and sparse scatter (2 % of pre neurons, Exp(1))."""
     import ctypes as C
@@ -261,7 +261,7 @@ def c1_delivery(torch, reps=10):
 
 
 def c4_hebbian(torch, reps=5, tuner=None):
-    """HebbianUpdater dW (SURVEY C4 shape): V1 8 x 128x128x1024 (5 % active, Exp(1))
+    """HebbianUpdater dW at full size: V1 8 x 128x128x1024 (5 % active, Exp(1))
     x Residual 8 x 512x512x1 (~N(0,1)), 16x16 patches at stride 4, MFMA forms."""
     import ctypes as C
     from openpv_amd import abi, runtime as R
@@ -302,7 +302,7 @@ def c4_hebbian(torch, reps=5, tuner=None):
 
 
 def c3_adjoint(torch, T=100, checkpoint=10):
-    """Adjoint check workload: 16 images 64x64x3 ~N(0,1), 256 features
+    """Adjoint workload: 16 images 64x64x3 ~N(0,1), 256 features
     8x8x3 stride 2 (V1 32x32x256), 100 unrolled steps, checkpoint every 10;
     E = 0.5||x - W^T a||^2 + 0.1||a||_1.  FP32 forms; forward and backward timed."""
     from openpv_amd import abi
