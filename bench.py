@@ -344,7 +344,7 @@ def main():
                     help="also run a plain device copy of MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (the batch would "
+        print("bench.py runs on one GPU: the library has no multi-GPU path (the batch would "
               "shard over GPUs with one dW all-reduce per update, not built here).",
               file=sys.stderr)
         sys.exit(2)

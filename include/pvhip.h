@@ -39,7 +39,7 @@ enum {
    PVHIP_OK              = 0,
    PVHIP_ERR_INVALID     = 1, /* bad argument or shape */
    PVHIP_ERR_RUNTIME     = 2, /* HIP runtime call failed */
-   PVHIP_ERR_UNSUPPORTED = 3, /* valid request this build does not implement */
+   PVHIP_ERR_UNSUPPORTED = 3, /* valid request the library does not implement */
    PVHIP_ERR_NOMEM       = 4
 };
 
