@@ -282,6 +282,11 @@ struct GatherBF16 {
    using Frag = bf16x8;
    using RawA = uint4; // 8 elements as raw 128-bit registers
    static constexpr bool SCALED = false;
+   // four waves per SIMD (128 registers, the fused update spills only on the
+   // padded tail's path) with two k-steps of weights in flight: measured
+   // faster than three waves with four k-steps
+   static constexpr int WAVES = 4; // waves per SIMD the gather is compiled for
+   static constexpr int KPRE  = 2; // k-steps whose weight fragments are fetched together
    __device__ static T cvt(float x) { return (__bf16)x; }
    __device__ static f32x4 mfma(RawA a, Frag b, f32x4 c) {
       return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), b, c, 0, 0, 0);
@@ -292,6 +297,8 @@ struct GatherFP8 {
    using Frag = long;
    using RawA = uint2;
    static constexpr bool SCALED = true;
+   static constexpr int WAVES   = 1; // no bound (four waves with two k-steps measured slower)
+   static constexpr int KPRE    = 4;
    __device__ static T cvt(float x) {
       return (T)(__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false) & 0xff);
    }
@@ -308,6 +315,8 @@ struct GatherF32 {
    using Frag = f32x8;
    using RawA = f32x8;
    static constexpr bool SCALED = false;
+   static constexpr int WAVES   = 1; // no bound: four waves would spill the FP32 fragments
+   static constexpr int KPRE    = 4;
    __device__ static T cvt(float x) { return x; }
    __device__ static f32x4 mfma(RawA a, Frag b, f32x4 c) {
 #pragma unroll
@@ -612,7 +621,7 @@ __device__ __forceinline__ void lca_epilogue_vthresh(
 // EPI: 0 GSyn only, 1 fused VTHRESH update, 2 fused update with either
 // transfer, 3 as 1 plus the active-list emission of full tiles.
 template <int ALIGN, int EPI, typename E = GatherBF16>
-__global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
+__global__ void __launch_bounds__(256, E::WAVES) recv_post_mfma_bf16_kernel(
       PostParams p, const typename E::T *Wb, int srcRowsMax, int srcColsMax, int Kpad) {
    using T = typename E::T;
    extern __shared__ float smem[];
@@ -670,7 +679,7 @@ __global__ void __launch_bounds__(256) recv_post_mfma_bf16_kernel(
    // clamped inside Kpad, so every load is unconditional.  The first group is
    // issued here, before the staging; later groups are prefetched during the
    // MFMAs of the one before.
-   constexpr int KPRE = 4; // k-steps whose A fragments are fetched together
+   constexpr int KPRE = E::KPRE; // k-steps whose A fragments are fetched together
    using RawA         = typename E::RawA;
    const T *rowA[MFB];
 #pragma unroll
