@@ -297,7 +297,7 @@ struct GatherFP8 {
    using Frag = long;
    using RawA = uint2;
    static constexpr bool SCALED = true;
-   static constexpr int WAVES   = 1; // no bound (four waves with two k-steps measured slower)
+   static constexpr int WAVES   = 1; // no bound (four waves measured slower, with two or four k-steps)
    static constexpr int KPRE    = 4;
    __device__ static T cvt(float x) {
       return (T)(__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false) & 0xff);
