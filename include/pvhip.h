@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PVHIP_ABI_VERSION 6
+#define PVHIP_ABI_VERSION 7
 
 enum {
    PVHIP_OK              = 0,
@@ -387,6 +387,19 @@ int pvhip_recv_pre(const pvhip_recv_pre_args *args, pvhip_stream stream);
 /* Fused K8+K6+K7 (see pvhip_lca_update_args). */
 int pvhip_lca_update(const pvhip_lca_update_args *args, pvhip_stream stream);
 
+/* MomentumLCALayer V update (MomentumLCAInternalStateBuffer.kpp:44-63, the
+ * reference's K10 updateMomentumLCAOnGPU) with the GSyn accumulation and the
+ * transfer fused as in pvhip_lca_update:
+ *   drive = (1-e)*(G + selfInteract*A[kex])
+ *   V     = e*V + drive + momentum_rate*prev_drive
+ *   prev_drive = drive;  A[kex] = transfer(V)
+ * prev_drive: restricted [nbatch][ny][nx][nf], read and rewritten
+ * (MomentumLCALayer's "prevDrive" buffer, zero at start).  momentum_rate is
+ * the layer's LCAMomentumRate. */
+int pvhip_momentum_lca_update(
+      const pvhip_lca_update_args *args, float momentum_rate, float *prev_drive,
+      pvhip_stream stream);
+
 /* GSynAccumulator (src/components/GSynAccumulator.kpp:21-42):
  * out[i] = sum_ch coeff[ch]*gsyn[ch*n + i], starting from 0.0f. */
 int pvhip_gsyn_accumulate(
@@ -516,6 +529,30 @@ size_t pvhip_hebbian_dw_scratch_bytes(const pvhip_hebbian_args *args);
  * where count == 0; then updateWeights (:822-833): w += dw. */
 int pvhip_hebbian_apply(
       float *weights, float *dw, const int64_t *counts, size_t n, int normalize_dw,
+      pvhip_stream stream);
+
+/* MomentumUpdater (src/weightupdaters/MomentumUpdater.cpp).  Method codes of
+ * momentumMethod: */
+enum { PVHIP_MOMENTUM_VISCOSITY = 0, PVHIP_MOMENTUM_SIMPLE = 1, PVHIP_MOMENTUM_ALEX = 2 };
+
+/* Host only: the factors of applyMomentum (MomentumUpdater.cpp:296-307) from
+ * timeConstantTau, momentumDecay and dWMax -- viscosity: exp(-1/tau) (0 for
+ * tau == 0) and decay; simple: tau and decay; alex: tau and decay*dWMax --
+ * after the reference's parameter checks (:82-112, :138-145): a rejected
+ * value returns PVHIP_ERR_INVALID. */
+int pvhip_momentum_factors(
+      int method, float tau, float decay, float dw_max, float *dw_factor, float *w_factor);
+
+/* MomentumUpdater::updateWeights in one pass (MomentumUpdater.cpp:276-294):
+ * normalize_dW as pvhip_hebbian_apply (normalize_dw, counts), then
+ *   dw = dw*(1-dw_factor) + dw_factor*prev_dw - w_factor*w   (:309-329), or,
+ *   with deprecated_tau (the momentumTau parameter, :346-360),
+ *   dw = dw + (dw_factor*prev_dw - w_factor*w);
+ * then prev_dw = dw and w += dw (HebbianUpdater::updateWeights).  Every
+ * array holds n values; prev_dw starts at zero (or initPrev_dWFile). */
+int pvhip_momentum_apply(
+      float *weights, float *dw, float *prev_dw, const int64_t *counts, size_t n,
+      int normalize_dw, float dw_factor, float w_factor, int deprecated_tau,
       pvhip_stream stream);
 
 /* NormalizeL2 per data patch in the reference's float arithmetic and order

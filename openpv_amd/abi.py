@@ -12,6 +12,7 @@ LIBDIR = os.path.join(HERE, "lib")
 
 PVHIP_OK = 0
 ARITH_EXACT, ARITH_FP32, ARITH_BF16, ARITH_FP8 = 0, 1, 2, 3
+MOMENTUM_VISCOSITY, MOMENTUM_SIMPLE, MOMENTUM_ALEX = 0, 1, 2  # momentumMethod codes
 TRANSFER_VTHRESH, TRANSFER_VERTICES = 0, 1
 MAX_VERTICES = 8
 MAX_CHANNELS = 4
@@ -171,6 +172,7 @@ PVHIP_SYMBOLS = {
     "pvhip_recv_post_bf16_weights_bytes": (C.c_size_t, [i32, i32]),
     "pvhip_recv_pre": (i32, [C.POINTER(RecvPreArgs), vp]),
     "pvhip_lca_update": (i32, [C.POINTER(LCAUpdateArgs), vp]),
+    "pvhip_momentum_lca_update": (i32, [C.POINTER(LCAUpdateArgs), C.c_float, vp, vp]),
     "pvhip_gsyn_accumulate": (i32, [i32, i32, C.POINTER(f32), vp, vp, vp]),
     "pvhip_layer_update": (i32, [C.POINTER(LayerUpdateArgs), vp]),
     "pvhip_ann_activity": (i32, [C.POINTER(Loc), C.POINTER(Transfer), vp, vp, vp]),
@@ -182,6 +184,9 @@ PVHIP_SYMBOLS = {
     "pvhip_hebbian_dw": (i32, [C.POINTER(HebbianArgs), vp]),
     "pvhip_hebbian_dw_scratch_bytes": (C.c_size_t, [C.POINTER(HebbianArgs)]),
     "pvhip_hebbian_apply": (i32, [vp, vp, vp, C.c_size_t, i32, vp]),
+    "pvhip_momentum_factors": (i32, [i32, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
+                                     C.POINTER(C.c_float)]),
+    "pvhip_momentum_apply": (i32, [vp, vp, vp, vp, C.c_size_t, i32, C.c_float, C.c_float, i32, vp]),
     "pvhip_normalize_l2_f64": (i32, [vp, i32, i32, C.c_double, C.c_double, vp]),
     "pvhip_normalize_l2": (i32, [vp, i32, i32, f32, f32, vp]),
     "pvhip_cost_sum_f64": (i32, [C.POINTER(Loc), vp, i32, i32, C.c_double, C.c_double, vp, vp]),

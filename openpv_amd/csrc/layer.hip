@@ -85,6 +85,45 @@ __global__ void __launch_bounds__(kRowThreads) lca_update_kernel(LCAParams p) {
    }
 }
 
+// MomentumLCA V update (MomentumLCAInternalStateBuffer.kpp:44-63) with the
+// GSyn accumulation and the transfer fused: the HyPerLCA operations up to the
+// drive, then V = (e*V + drive) + rate*prevDrive in the reference's order.
+__global__ void __launch_bounds__(kRowThreads) momentum_lca_update_kernel(
+      LCAParams p, float rate, float *prevDrive) {
+   int const numRes = p.loc.numRes();
+   int const b      = blockIdx.y;
+   size_t const n   = (size_t)p.loc.nbatch * numRes;
+   float const e    = p.expTau[b];
+#pragma unroll
+   for (int u = 0; u < kRowIter; u++) {
+      int const k = blockIdx.x * kRowSpan + u * kRowThreads + threadIdx.x;
+      if (k < numRes) {
+         size_t const vi = (size_t)b * numRes + k;
+         size_t const ai = (size_t)b * p.loc.numExt() + p.loc.kex(k);
+         float const G    = accumulate_channels(p.numChannels, p.coeffs, p.gsyn, n, vi);
+         float const Vold = p.V[vi];
+         float const Aold = p.aFromV ? apply_transfer(p.transfer, Vold) : p.A[ai];
+         float t2         = 1.0f - e;
+         float si         = p.selfInteract * Aold;
+         float t3         = G + si;
+         float drive      = t2 * t3;
+         float t1         = e * Vold;
+         float t5         = t1 + drive;
+         float mom        = rate * prevDrive[vi];
+         float Vnew       = t5 + mom;
+         float Anew       = apply_transfer(p.transfer, Vnew);
+         prevDrive[vi]    = drive;
+         p.V[vi]          = Vnew;
+         if (p.A) {
+            p.A[ai] = Anew;
+         }
+         if (p.Ab) {
+            p.Ab[bf16c_index(p.loc, b, p.loc.kex(k))] = (__bf16)Anew;
+         }
+      }
+   }
+}
+
 struct LayerParams {
    Geo loc;
    int numChannels;
@@ -504,6 +543,38 @@ __global__ void __launch_bounds__(256) hebbian_apply_kernel(
    w[i] = w[i] + d; // HebbianUpdater.cpp:828-830
 }
 
+// MomentumUpdater::updateWeights per weight (MomentumUpdater.cpp:276-294):
+// normalize_dW as hebbian_apply_kernel, applyMomentum (:309-329, or the
+// deprecated momentumTau form :346-360), prev_dW = dW, w += dW.
+__global__ void __launch_bounds__(256) momentum_apply_kernel(
+      float *w, float *dw, float *prev, const int64_t *counts, size_t n, int normalizeDw,
+      float dwFactor, float wFactor, int deprecatedTau) {
+   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= n) {
+      return;
+   }
+   float d = dw[i];
+   if (normalizeDw) {
+      long divisor = (long)counts[i];
+      d            = divisor != 0 ? d / (float)divisor : 0.0f;
+   }
+   float const wo = w[i];
+   float const pd = prev[i];
+   if (deprecatedTau) {
+      float a = dwFactor * pd;
+      float c = wFactor * wo;
+      d       = d + (a - c);
+   }
+   else {
+      d = d * (1.0f - dwFactor);
+      d = d + dwFactor * pd;
+      d = d - wFactor * wo;
+   }
+   dw[i]   = d;
+   prev[i] = d;
+   w[i]    = wo + d;
+}
+
 inline unsigned blocks_for(size_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
 
 // ------------------------------------------- active list from epilogue segments
@@ -632,6 +703,23 @@ int pvhip_lca_update(const pvhip_lca_update_args *args, pvhip_stream stream) {
    PVHIP_REQUIRE(row_grid_ok(p.loc), "lca_update: more than 65535 rows or batch elements");
    hipLaunchKernelGGL(lca_update_kernel, row_grid(p.loc), dim3(kRowThreads), 0, (hipStream_t)stream, p);
    PVHIP_LAUNCHED("lca_update");
+   return PVHIP_OK;
+}
+
+int pvhip_momentum_lca_update(
+      const pvhip_lca_update_args *args, float momentum_rate, float *prev_drive,
+      pvhip_stream stream) {
+   PVHIP_REQUIRE(args != nullptr, "pvhip_momentum_lca_update: NULL args");
+   PVHIP_REQUIRE(prev_drive != nullptr, "pvhip_momentum_lca_update: NULL prev_drive");
+   LCAParams p;
+   int st = make_lca_params(*args, &p);
+   if (st) {
+      return st;
+   }
+   PVHIP_REQUIRE(row_grid_ok(p.loc), "momentum_lca_update: more than 65535 batch elements");
+   hipLaunchKernelGGL(momentum_lca_update_kernel, row_grid(p.loc), dim3(kRowThreads), 0,
+         (hipStream_t)stream, p, momentum_rate, prev_drive);
+   PVHIP_LAUNCHED("momentum_lca_update");
    return PVHIP_OK;
 }
 
@@ -829,6 +917,45 @@ int pvhip_hebbian_apply(
    hipLaunchKernelGGL(hebbian_apply_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
          (hipStream_t)stream, weights, dw, counts, n, normalize_dw);
    PVHIP_LAUNCHED("hebbian_apply");
+   return PVHIP_OK;
+}
+
+int pvhip_momentum_factors(
+      int method, float tau, float decay, float dw_max, float *dw_factor, float *w_factor) {
+   PVHIP_REQUIRE(dw_factor && w_factor, "momentum_factors: NULL output");
+   PVHIP_REQUIRE(decay >= 0.0f && decay <= 1.0f,
+         "momentum_factors: momentumDecay must be between 0 and 1 inclusive (%g)", (double)decay);
+   switch (method) {
+      case PVHIP_MOMENTUM_VISCOSITY:
+         PVHIP_REQUIRE(tau >= 0.0f, "momentum_factors: viscosity needs timeConstantTau >= 0 (%g)",
+               (double)tau);
+         *dw_factor = tau != 0.0f ? std::exp(-1.0f / tau) : 0.0f;
+         *w_factor  = decay;
+         return PVHIP_OK;
+      case PVHIP_MOMENTUM_SIMPLE:
+      case PVHIP_MOMENTUM_ALEX:
+         PVHIP_REQUIRE(tau >= 0.0f && tau < 1.0f,
+               "momentum_factors: simple and alex need 0 <= timeConstantTau < 1 (%g)", (double)tau);
+         *dw_factor = tau;
+         *w_factor  = method == PVHIP_MOMENTUM_ALEX ? decay * dw_max : decay;
+         return PVHIP_OK;
+      default: PVHIP_REQUIRE(false, "momentum_factors: unknown momentumMethod %d", method);
+   }
+}
+
+int pvhip_momentum_apply(
+      float *weights, float *dw, float *prev_dw, const int64_t *counts, size_t n,
+      int normalize_dw, float dw_factor, float w_factor, int deprecated_tau,
+      pvhip_stream stream) {
+   PVHIP_REQUIRE(weights && dw && prev_dw, "momentum_apply: NULL buffer");
+   PVHIP_REQUIRE(!normalize_dw || counts, "momentum_apply: normalize_dw needs counts");
+   if (n == 0) {
+      return PVHIP_OK;
+   }
+   hipLaunchKernelGGL(momentum_apply_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (hipStream_t)stream, weights, dw, prev_dw, counts, n, normalize_dw, dw_factor, w_factor,
+         deprecated_tau);
+   PVHIP_LAUNCHED("momentum_apply");
    return PVHIP_OK;
 }
 

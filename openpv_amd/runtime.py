@@ -453,6 +453,25 @@ class HyPerLCAUpdate:
         abi.check(abi.hip().pvhip_lca_update(C.byref(a), _stream(stream)), "pvhip_lca_update")
 
 
+class MomentumLCAUpdate(HyPerLCAUpdate):
+    """MomentumLCALayer update (MomentumLCAInternalStateBuffer.kpp:44-63): the
+    HyPerLCA update plus LCAMomentumRate times the previous step's drive, which
+    the layer keeps in its "prevDrive" buffer (restricted, zero at start)."""
+
+    def __init__(self, loc_, transfer, timeConstantTau=1.0, dt=1.0, self_interact=True,
+                 LCAMomentumRate=0.0, channel_coeffs=(1.0,), activity_from_v=False, device="cuda"):
+        super().__init__(loc_, transfer, timeConstantTau, dt, self_interact, channel_coeffs,
+                         activity_from_v, device)
+        self.momentum_rate = float(LCAMomentumRate)
+        self.prev_drive = torch.zeros(loc_.nbatch * loc_.num_res, dtype=torch.float32, device=device)
+
+    def update(self, gsyn, V, A, stream=None, A_bf16c=None):
+        a = self.args(gsyn, V, A, A_bf16c=A_bf16c)
+        abi.check(abi.hip().pvhip_momentum_lca_update(C.byref(a), self.momentum_rate,
+                                                      _ptr(self.prev_drive), _stream(stream)),
+                  "pvhip_momentum_lca_update")
+
+
 def bf16c_buffer(loc_, device="cuda"):
     """A zeroed chunked BF16 activity buffer for layer loc_ (nf % 32 == 0)."""
     if loc_.nf % 32:
@@ -599,3 +618,49 @@ class HebbianUpdater:
         abi.check(abi.hip().pvhip_hebbian_apply(_ptr(weights), _ptr(self.dw), _ptr(self.counts),
                                                 self.size, int(self.normalize_dw), _stream(stream)),
                   "pvhip_hebbian_apply")
+
+
+_MOMENTUM_METHODS = {"viscosity": abi.MOMENTUM_VISCOSITY, "simple": abi.MOMENTUM_SIMPLE,
+                     "alex": abi.MOMENTUM_ALEX}
+
+
+def momentum_factors(method, tau, decay=0.0, dw_max=1.0):
+    """(dwFactor, wFactor) of MomentumUpdater::applyMomentum for momentumMethod
+    `method` ("viscosity", "simple", "alex" or its code); PVHIPError where the
+    reference's parameter checks reject the values."""
+    code = _MOMENTUM_METHODS[method] if isinstance(method, str) else int(method)
+    f, wf = C.c_float(), C.c_float()
+    abi.check(abi.hip().pvhip_momentum_factors(code, tau, decay, dw_max, C.byref(f), C.byref(wf)),
+              "pvhip_momentum_factors")
+    return f.value, wf.value
+
+
+class MomentumUpdater(HebbianUpdater):
+    """MomentumUpdater (src/weightupdaters/MomentumUpdater.cpp): the Hebbian dW,
+    then, at each weight update, dW blended with the previous update's dW and
+    decayed by the weights, prev_dW = dW, w += dW.  Parameters as the reference's
+    .params keys: momentumMethod, timeConstantTau (its default per method:
+    viscosity 100, simple 0.25, alex 0.9), momentumDecay, and momentumTau, the
+    deprecated spelling that selects the older blend (MomentumUpdater.cpp:346-360)."""
+
+    _TAU_DEFAULT = {"viscosity": 100.0, "simple": 0.25, "alex": 0.9}
+
+    def __init__(self, pre, post, nxp, nyp, dw_max, momentumMethod="viscosity",
+                 timeConstantTau=None, momentumDecay=0.0, momentumTau=None, normalize_dw=True,
+                 device="cuda"):
+        super().__init__(pre, post, nxp, nyp, dw_max, normalize_dw=normalize_dw, device=device)
+        self.method = momentumMethod
+        self.deprecated_tau = timeConstantTau is None and momentumTau is not None
+        if self.deprecated_tau:
+            tau = float(momentumTau)
+        else:
+            tau = self._TAU_DEFAULT[momentumMethod] if timeConstantTau is None else float(timeConstantTau)
+        self.tau, self.decay = tau, float(momentumDecay)
+        self.dw_factor, self.w_factor = momentum_factors(momentumMethod, tau, self.decay, dw_max)
+        self.prev_dw = torch.zeros(self.size, dtype=torch.float32, device=self.dw.device)
+
+    def apply(self, weights, stream=None):
+        abi.check(abi.hip().pvhip_momentum_apply(
+            _ptr(weights), _ptr(self.dw), _ptr(self.prev_dw), _ptr(self.counts), self.size,
+            int(self.normalize_dw), self.dw_factor, self.w_factor, int(self.deprecated_tau),
+            _stream(stream)), "pvhip_momentum_apply")

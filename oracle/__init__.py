@@ -118,6 +118,13 @@ def lib():
         L.or_hebbian_update_dw.argtypes = [LP, LP, C.c_int, C.c_int, C.c_int, C.c_float,
                                            f32p, f32p, f32p, C.c_void_p]
         L.or_hebbian_normalize_apply.argtypes = [f32p, f32p, C.c_void_p, C.c_size_t, C.c_int]
+        L.or_momentum_factors.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float,
+                                          C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.or_momentum_factors.restype = C.c_int
+        L.or_momentum_apply.argtypes = [f32p, f32p, f32p, C.c_void_p, C.c_size_t, C.c_int,
+                                        C.c_float, C.c_float, C.c_int]
+        L.or_update_momentum_lca.argtypes = [LP, C.c_int, C.c_float, f64p, C.c_float, f32p, f32p,
+                                             f32p, f32p]
         L.or_normalize_l2.argtypes = [f32p, C.c_int, C.c_int, C.c_float, C.c_float]
         L.or_normalize_l2_f64.argtypes = [f32p, C.c_int, C.c_int, C.c_double, C.c_double]
         L.or_cost_sum.argtypes = [LP, f32p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
@@ -141,6 +148,8 @@ def ref():
                                          u64p, u64p, u64p, C.c_void_p]
         R.ref_patch_geometry.restype = C.c_int
         R.ref_update_lca.argtypes = [i32p, C.c_int, f64p, C.c_float, f32p, f32p, f32p]
+        R.ref_update_momentum_lca.argtypes = [i32p, C.c_int, C.c_float, f64p, C.c_float, f32p,
+                                              f32p, f32p, f32p]
         R.ref_apply_vertices.argtypes = [i32p, C.c_int, f32p, f32p, f32p, f32p, f32p]
         R.ref_gsyn_accumulate.argtypes = [C.c_int, C.c_int, f32p, f32p, f32p]
         R.ref_cost.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
@@ -222,6 +231,35 @@ def update_lca(loc, self_interact, dt, tau, gsyn, A, V):
     lib().or_update_lca(C.byref(loc), int(self_interact), np.asarray(dt, np.float64), tau,
                         np.ascontiguousarray(gsyn, np.float32), np.ascontiguousarray(A, np.float32), V)
     return V
+
+
+def update_momentum_lca(loc, self_interact, momentum_rate, dt, tau, gsyn, A, prev_drive, V):
+    """MomentumLCA V update; returns (V, prev_drive), both new arrays."""
+    V, prev = V.copy(), np.array(prev_drive, np.float32)
+    lib().or_update_momentum_lca(C.byref(loc), int(self_interact), momentum_rate,
+                                 np.asarray(dt, np.float64), tau,
+                                 np.ascontiguousarray(gsyn, np.float32),
+                                 np.ascontiguousarray(A, np.float32), prev, V)
+    return V, prev
+
+
+def momentum_factors(method, tau, decay=0.0, dw_max=1.0):
+    """(dwFactor, wFactor) of MomentumUpdater::applyMomentum; method 0 viscosity,
+    1 simple, 2 alex.  Raises ValueError where the reference rejects the parameters."""
+    f, wf = C.c_float(), C.c_float()
+    if lib().or_momentum_factors(method, tau, decay, dw_max, C.byref(f), C.byref(wf)):
+        raise ValueError("momentum parameters rejected")
+    return f.value, wf.value
+
+
+def momentum_apply(w, dw, prev_dw, counts, dw_factor, w_factor, deprecated_tau=False):
+    """normalize_dW (counts given) + applyMomentum + prev_dW = dW + w += dW;
+    returns new (w, dw, prev_dw)."""
+    w, dw, prev = (np.array(x, np.float32) for x in (w, dw, prev_dw))
+    cnt = None if counts is None else np.ascontiguousarray(counts, np.int64)
+    lib().or_momentum_apply(w, dw, prev, None if cnt is None else cnt.ctypes.data, w.size,
+                            int(cnt is not None), dw_factor, w_factor, int(deprecated_tau))
+    return w, dw, prev
 
 
 def transfer(loc, V, VThresh, AMin, AMax, AShift, VWidth, vertices=False, A=None):

@@ -788,6 +788,85 @@ void or_hebbian_normalize_apply(float *w, float *dw, const int64_t *counts, size
    for (size_t i = 0; i < n; i++) w[i] += dw[i];
 }
 
+int or_momentum_factors(int method, float tau, float decay, float dwMax, float *dwFactor,
+      float *wFactor) {
+   /* MomentumUpdater::applyMomentum (MomentumUpdater.cpp:296-307) after the
+    * parameter checks of checkTimeConstantTau (:82-112) and
+    * ioParam_momentumDecay (:138-145).  method 0 viscosity, 1 simple, 2 alex.
+    * Returns nonzero for a parameter the reference rejects. */
+   if (decay < 0.0f || decay > 1.0f) return 1;
+   switch (method) {
+      case 0:
+         if (tau < 0) return 1;
+         *dwFactor = tau ? expf(-1.0f / tau) : 0.0f;
+         *wFactor  = decay;
+         return 0;
+      case 1:
+         if (tau < 0 || tau >= 1) return 1;
+         *dwFactor = tau;
+         *wFactor  = decay;
+         return 0;
+      case 2:
+         if (tau < 0 || tau >= 1) return 1;
+         *dwFactor = tau;
+         *wFactor  = decay * dwMax;
+         return 0;
+      default: return 1;
+   }
+}
+
+void or_momentum_apply(float *w, float *dw, float *prevDw, const int64_t *counts, size_t n,
+      int normalizeDw, float dwFactor, float wFactor, int deprecatedTau) {
+   /* HebbianUpdater::normalize_dW (:773-807), then MomentumUpdater::updateWeights
+    * (MomentumUpdater.cpp:276-294): applyMomentum (:309-329, or the deprecated
+    * momentumTau form :346-360), prev_dW = dW, then HebbianUpdater::updateWeights
+    * (w += dW, HebbianUpdater.cpp:822-833). */
+   if (normalizeDw) {
+      for (size_t i = 0; i < n; i++) {
+         long divisor = (long)counts[i];
+         if (divisor != 0) {
+            dw[i] /= divisor;
+         }
+         else {
+            dw[i] = 0;
+         }
+      }
+   }
+   for (size_t k = 0; k < n; k++) {
+      if (deprecatedTau) {
+         dw[k] += dwFactor * prevDw[k] - wFactor * w[k];
+      }
+      else {
+         dw[k] *= 1 - dwFactor;
+         dw[k] += dwFactor * prevDw[k];
+         dw[k] -= wFactor * w[k];
+      }
+   }
+   memcpy(prevDw, dw, sizeof(float) * n);
+   for (size_t i = 0; i < n; i++) w[i] += dw[i];
+}
+
+void or_update_momentum_lca(const or_loc *loc, int selfInteract, float momentumRate,
+      const double *dtAdapt, float tau, const float *accumulatedGSyn, const float *A,
+      float *prevDrive, float *V) {
+   /* MomentumLCAInternalStateBuffer.kpp:44-63 */
+   int const numNeurons = locNumRes(loc);
+   int const numExt     = locNumExt(loc);
+   for (int index = 0; index < numNeurons * loc->nbatch; index++) {
+      int b = index / numNeurons;
+      int k = index % numNeurons;
+      float exp_tau         = (float)exp(-dtAdapt[b] / (double)tau);
+      float *VBatch         = V + (size_t)b * numNeurons;
+      float const gSyn      = accumulatedGSyn[(size_t)b * numNeurons + k];
+      float *prevDriveBatch = prevDrive + (size_t)b * numNeurons;
+      float const *ABatch   = A + (size_t)b * numExt;
+      int kex = or_kIndexExtended(k, loc->nx, loc->ny, loc->nf, loc->lt, loc->rt, loc->dn, loc->up);
+      float currentDrive = (1.0f - exp_tau) * (gSyn + (float)selfInteract * ABatch[kex]);
+      VBatch[k]          = exp_tau * VBatch[k] + currentDrive + momentumRate * prevDriveBatch[k];
+      prevDriveBatch[k]  = currentDrive;
+   }
+}
+
 void or_normalize_l2(float *w, int numPatches, int patchSize, float strength, float minL2) {
    /* NormalizeL2::normalizeWeights non-individual-arbor branch, NormalizeL2.cpp:93-118 */
    float scaleFactor = 1.0f * strength;

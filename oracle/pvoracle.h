@@ -137,6 +137,13 @@ void or_hebbian_update_dw(const or_loc *pre, const or_loc *post, int nxp, int ny
       float dWMax, const float *preActivity, const float *postActivity, float *dw, int64_t *counts);
 /* normalize_dW (:773-807) and updateWeights (:822-833). */
 void or_hebbian_normalize_apply(float *w, float *dw, const int64_t *counts, size_t n, int normalizeDw);
+int or_momentum_factors(int method, float tau, float decay, float dwMax, float *dwFactor,
+      float *wFactor);
+void or_momentum_apply(float *w, float *dw, float *prevDw, const int64_t *counts, size_t n,
+      int normalizeDw, float dwFactor, float wFactor, int deprecatedTau);
+void or_update_momentum_lca(const or_loc *loc, int selfInteract, float momentumRate,
+      const double *dtAdapt, float tau, const float *accumulatedGSyn, const float *A,
+      float *prevDrive, float *V);
 /* NormalizeL2::normalizeWeights, one arbor, one connection, not from post perspective
  * (src/normalizers/NormalizeL2.cpp:36-120; NormalizeBase.cpp:184-193). FP32 like the reference. */
 void or_normalize_l2(float *w, int numPatches, int patchSize, float strength, float minL2);

@@ -6,6 +6,7 @@
  * `make ref` from the reference checkout where it lies, never copied):
  *   - PV::PatchGeometry        (src/components/PatchGeometry.cpp)
  *   - updateHyPerLCAOnCPU      (src/components/HyPerLCAInternalStateBuffer.kpp)
+ *   - updateMomentumLCAOnCPU   (src/components/MomentumLCAInternalStateBuffer.kpp)
  *   - applyVerticesANNActivityBufferOnCPU (src/components/ANNActivityBuffer.kpp)
  *   - updateGSynAccumulatorOnCPU (src/components/GSynAccumulator.kpp)
  *   - requiredConvolveMargin   (src/utils/requiredConvolveMargin.hpp)
@@ -23,6 +24,7 @@
 #include "components/ANNActivityBuffer.kpp"
 #include "components/GSynAccumulator.kpp"
 #include "components/HyPerLCAInternalStateBuffer.kpp"
+#include "components/MomentumLCAInternalStateBuffer.kpp"
 
 #include <cstdint>
 #include <cstring>
@@ -94,6 +96,14 @@ void ref_update_lca(const int *loc, int selfInteract, const double *dtAdapt, flo
       const float *gsyn, const float *A, float *V) {
    PV::updateHyPerLCAOnCPU(loc[0], loc[1] * loc[2] * loc[3], loc[1], loc[2], loc[3], loc[4],
          loc[5], loc[6], loc[7], selfInteract != 0, dtAdapt, tau, gsyn, A, V);
+}
+
+void ref_update_momentum_lca(const int *loc, int selfInteract, float momentumRate,
+      const double *dtAdapt, float tau, const float *gsyn, const float *A, float *prevDrive,
+      float *V) {
+   PV::updateMomentumLCAOnCPU(loc[0], loc[1] * loc[2] * loc[3], loc[1], loc[2], loc[3], loc[4],
+         loc[5], loc[6], loc[7], selfInteract != 0, momentumRate, dtAdapt, tau, gsyn, A,
+         prevDrive, V);
 }
 
 void ref_apply_vertices(const int *loc, int numVertices, const float *vV, const float *vA,

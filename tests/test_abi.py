@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol(header, loader, table):
 
 
 def test_abi_version_and_error_string():
-    assert abi.hip().pvhip_abi_version() == 6
+    assert abi.hip().pvhip_abi_version() == 7
     assert isinstance(abi.hip().pvhip_last_error(), bytes)
 
 
@@ -242,3 +242,17 @@ def test_hebbian_windows_refuse_unclipped_geometry():
     post = abi.Loc.make(1, 24, 20, 5, (3, 4, 4, 3))
     w = [C.c_int() for _ in range(4)]
     assert abi.host().pvhost_hebbian_windows(C.byref(pre), C.byref(post), 6, 6, *[C.byref(v) for v in w]) < 0
+
+
+def test_momentum_factors_host_only():
+    """pvhip_momentum_factors runs on the host: the oracle's factors (the
+    reference's applyMomentum) and the reference's parameter checks."""
+    import oracle as O
+    from openpv_amd import runtime as R
+    for method, code, tau, decay, dwmax in [("viscosity", 0, 100.0, 0.0, 1.0), ("viscosity", 0, 0.0, 0.1, 1.0),
+                                            ("simple", 1, 0.75, 0.01, 1.0), ("alex", 2, 0.9, 0.5, 0.25)]:
+        assert R.momentum_factors(method, tau, decay, dwmax) == O.momentum_factors(code, tau, decay, dwmax)
+    for bad in [("simple", 1.0, 0.0), ("alex", -0.5, 0.0), ("viscosity", -1.0, 0.0),
+                ("viscosity", 100.0, 2.0), (7, 0.5, 0.0)]:
+        with pytest.raises(abi.PVHIPError):
+            R.momentum_factors(*bad)

@@ -544,3 +544,94 @@ def test_firm_threshold_cost_matches_reference_fixture():
     for b in range(2):
         total = O.cost_sum(loc, x[:2].ravel(), False, b, 2, 0.5, 0.3)
         assert abs(total - y[b].astype(np.float64).sum()) <= 1e-6 * total
+
+
+@needs_ref
+@pytest.mark.parametrize("self_interact", [True, False])
+def test_momentum_lca_matches_reference(self_interact):
+    """The oracle's MomentumLCA V update against the reference's own
+    updateMomentumLCAOnCPU (MomentumLCAInternalStateBuffer.kpp), bit for bit over
+    three steps that carry prevDrive."""
+    rng = np.random.default_rng(41)
+    loc = O.Loc.make(3, 7, 5, 4, (1, 2, 2, 1))
+    n = loc.nbatch * loc.num_res
+    V = rng.standard_normal(n).astype(np.float32)
+    prev = np.zeros(n, np.float32)
+    Vr, prevr = V.copy(), prev.copy()
+    dt = np.array([1.0, 0.5, 0.25])
+    for _ in range(3):
+        G = rng.standard_normal(n).astype(np.float32)
+        A = rng.standard_normal(loc.nbatch * loc.num_ext).astype(np.float32)
+        V, prev = O.update_momentum_lca(loc, self_interact, 0.4, dt, 100.0, G, A, prev, V)
+        O.ref().ref_update_momentum_lca(np.array(loc.as_list(), np.int32), int(self_interact), 0.4,
+                                        dt, 100.0, G, A, prevr, Vr)
+        assert np.array_equal(V.view(np.uint32), Vr.view(np.uint32))
+        assert np.array_equal(prev.view(np.uint32), prevr.view(np.uint32))
+
+
+def test_momentum_lca_reduces_to_lca():
+    """Rate 0 is the HyPerLCA update bit for bit; the drive it stores is that
+    update from V = 0; a rate r adds r*prevDrive as one rounded term."""
+    rng = np.random.default_rng(42)
+    loc = O.Loc.make(2, 6, 5, 3, 1)
+    n = loc.nbatch * loc.num_res
+    G = rng.standard_normal(n).astype(np.float32)
+    A = rng.standard_normal(loc.nbatch * loc.num_ext).astype(np.float32)
+    V = rng.standard_normal(n).astype(np.float32)
+    dt = np.array([1.0, 0.5])
+    prev0 = rng.standard_normal(n).astype(np.float32)
+    plain = O.update_lca(loc, True, dt, 100.0, G, A, V)
+    V0, drive = O.update_momentum_lca(loc, True, 0.0, dt, 100.0, G, A, prev0, V)
+    assert np.array_equal(V0.view(np.uint32), plain.view(np.uint32))
+    from_zero = O.update_lca(loc, True, dt, 100.0, G, A, np.zeros(n, np.float32))
+    assert np.array_equal(drive.view(np.uint32), from_zero.view(np.uint32))
+    V1, _ = O.update_momentum_lca(loc, True, 0.4, dt, 100.0, G, A, prev0, V)
+    expect = plain + np.float32(0.4) * prev0
+    assert np.array_equal(V1.view(np.uint32), expect.astype(np.float32).view(np.uint32))
+
+
+def test_momentum_simple_matches_reference_correct_state():
+    """MomentumConnSimpleCheckpointerTest's expected state (its CorrectState.cpp):
+    a 1x1 identity weight, input ceil(t/4), output input*w, momentum "simple" with
+    timeConstantTau 0.75 and no decay, an update every fourth step with the
+    Hebbian dW input*output (one count each, normalizeDw).  The oracle's
+    momentum step gives the test's weights and dW bit for bit."""
+    tau = np.float32(0.75)
+    f, wf = O.momentum_factors(1, 0.75, 0.0, 1.0)
+    assert (f, wf) == (0.75, 0.0)
+    cw, cdw = np.float32(1.0), np.float32(0.0)     # CorrectState
+    w, prev = np.array([1.0], np.float32), np.zeros(1, np.float32)
+    updates = 0
+    for t in range(1, 41):
+        inp = np.float32(np.ceil(np.float32(t) / np.float32(4.0)))
+        out = inp * cw
+        if abs(np.fmod(t + 1.0, 4.0) - 1.0) < 0.5:
+            base = inp * out
+            cdw = (np.float32(1) - tau) * base + tau * cdw
+            cw = cw + cdw
+            w, dw, prev = O.momentum_apply(w, [base], prev, [1], f, wf)
+            updates += 1
+            assert dw[0].view(np.uint32) == np.float32(cdw).view(np.uint32)
+            assert w[0].view(np.uint32) == np.float32(cw).view(np.uint32)
+    assert updates == 10
+
+
+def test_momentum_factors_and_forms():
+    """applyMomentum's factors per method and the reference's parameter checks;
+    both blends on values whose every product and sum is exact."""
+    f, wf = O.momentum_factors(0, 100.0, 0.25)
+    assert f == pytest.approx(np.exp(-0.01), rel=1e-7) and wf == 0.25
+    assert O.momentum_factors(0, 0.0) == (0.0, 0.0)
+    assert O.momentum_factors(2, 0.5, 0.25, 2.0) == (0.5, 0.5)
+    for bad in [(0, -1.0, 0.0), (1, 1.0, 0.0), (2, -0.1, 0.0), (1, 0.5, 1.5), (3, 0.5, 0.0)]:
+        with pytest.raises(ValueError):
+            O.momentum_factors(*bad)
+    # dw 1, prev 2, w 4, f 0.5, wf 0.25: 1*0.5 + 0.5*2 - 0.25*4 = 0.5
+    w, dw, prev = O.momentum_apply([4.0], [1.0], [2.0], None, 0.5, 0.25)
+    assert (w[0], dw[0], prev[0]) == (4.5, 0.5, 0.5)
+    # deprecated momentumTau blend: 1 + (0.5*2 - 0.25*4) = 1
+    w, dw, prev = O.momentum_apply([4.0], [1.0], [2.0], None, 0.5, 0.25, deprecated_tau=True)
+    assert (w[0], dw[0], prev[0]) == (5.0, 1.0, 1.0)
+    # normalize_dW first: counts 4 and 0
+    w, dw, prev = O.momentum_apply([0.0, 0.0], [8.0, 3.0], [0.0, 0.0], [4, 0], 0.5, 0.0)
+    assert dw.tolist() == [1.0, 0.0] and w.tolist() == [1.0, 0.0]
