@@ -301,6 +301,51 @@ def c4_hebbian(torch, reps=5, tuner=None):
     return out
 
 
+def c2_momentum(torch, reps=20):
+    """Momentum forms on the LCA workload's V1 (64 x 32x32x512): the
+    MomentumLCA V update against the HyPerLCA one (both reading GSyn, reading
+    and writing V, writing A, A(t-1) from V; momentum also reads and writes
+    prevDrive: 16 vs 24 bytes per neuron), and the MomentumUpdater weight step
+    at the Hebbian workload's 1024 x 16x16 weights (w, dW and prev_dW read and
+    written, counts read: 32 bytes per weight)."""
+    from openpv_amd import abi, runtime as R
+    dev = "cuda"
+    loc = R.loc(64, 32, 32, 512, 2)
+    n = loc.nbatch * loc.num_res
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    G = torch.randn(n, device=dev, generator=g) * 0.1
+    V = torch.randn(n, device=dev, generator=g) * 0.1
+    A = torch.zeros(loc.nbatch * loc.num_ext, device=dev)
+    t = R.make_transfer(VTHRESH, 0.0, float(np.finfo(np.float32).max), VTHRESH, 0.0)
+    R.ann_activity(loc, t, V, A)
+    s = torch.cuda.Stream()
+    out = {"workload": "V1 64x32x32x512 (halo 2); weights 1024x16x16"}
+    plain = R.HyPerLCAUpdate(loc, t, TAU, DT, True, activity_from_v=True, device=dev)
+    mom = R.MomentumLCAUpdate(loc, t, TAU, DT, True, LCAMomentumRate=0.4, activity_from_v=True,
+                              device=dev)
+    for name, upd, nbytes in (("lca", plain, 16), ("momentum_lca", mom, 24)):
+        ms, _ = _time_events(lambda: upd.update(G, V, A, stream=s), s, reps, torch)
+        out[name + "_us"] = round(1000 * ms, 2)
+        out[name + "_gbs"] = round(nbytes * n / (ms * 1e-3) / 1e9, 1)
+    nw = 1024 * 256
+    w = torch.randn(nw, device=dev, generator=g)
+    dw = torch.randn(nw, device=dev, generator=g)
+    prev = torch.zeros(nw, device=dev)
+    cnt = torch.ones(nw, dtype=torch.int64, device=dev)
+    f, wf = R.momentum_factors("viscosity", 100.0, 0.0, 1.0)
+    import ctypes as C
+    ptr = lambda x: C.c_void_p(x.data_ptr())
+
+    def step():
+        abi.check(abi.hip().pvhip_momentum_apply(ptr(w), ptr(dw), ptr(prev), ptr(cnt), nw, 1, f, wf,
+                                                 0, C.c_void_p(s.cuda_stream)), "momentum_apply")
+    ms, _ = _time_events(step, s, reps, torch)
+    out["momentum_apply_us"] = round(1000 * ms, 2)
+    out["momentum_apply_gbs"] = round(32 * nw / (ms * 1e-3) / 1e9, 1)
+    return out
+
+
 def c3_adjoint(torch, T=100, checkpoint=10):
     """Adjoint workload: 16 images 64x64x3 ~N(0,1), 256 features
     8x8x3 stride 2 (V1 32x32x256), 100 unrolled steps, checkpoint every 10;
@@ -436,6 +481,7 @@ def main():
     if not args.no_extras:
         line["delivery_c1"] = c1_delivery(torch)
         line["hebbian_c4"] = c4_hebbian(torch, tuner=tuner)
+        line["momentum_c2"] = c2_momentum(torch)
         line["adjoint_c3"] = c3_adjoint(torch)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(images, W)
