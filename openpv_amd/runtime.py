@@ -641,7 +641,9 @@ class MomentumUpdater(HebbianUpdater):
     decayed by the weights, prev_dW = dW, w += dW.  Parameters as the reference's
     .params keys: momentumMethod, timeConstantTau (its default per method:
     viscosity 100, simple 0.25, alex 0.9), momentumDecay, and momentumTau, the
-    deprecated spelling that selects the older blend (MomentumUpdater.cpp:346-360)."""
+    deprecated spelling that selects the older blend (MomentumUpdater.cpp:346-360).
+    A momentumTau outside the range timeConstantTau allows for the method is
+    rejected here; the reference checks only timeConstantTau and accepts it."""
 
     _TAU_DEFAULT = {"viscosity": 100.0, "simple": 0.25, "alex": 0.9}
 
