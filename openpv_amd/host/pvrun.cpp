@@ -5,7 +5,7 @@
 //   pvrun -p file.params [-o outputPath] [--steps N] [--autotune]
 //
 // Supported objects (the LCA column of tests/GPULCATest and LCATest):
-//   HyPerCol; PvpLayer, ANNErrorLayer, HyPerLCALayer, ANNLayer, HyPerLayer;
+//   HyPerCol; PvpLayer, ANNErrorLayer, HyPerLCALayer, MomentumLCALayer, ANNLayer, HyPerLayer;
 //   IdentConn, HyPerConn (FileWeight, normalizeL2), TransposeConn, CloneConn;
 //   AdaptiveTimeScaleProbe, ColumnEnergyProbe, L2NormProbe, L1NormProbe,
 //   L1NormLCAProbe, FirmThresholdCostFnLCAProbe, RequireAllZeroActivityProbe.
@@ -231,6 +231,9 @@ struct Layer {
    // LCA
    float tau = 1.0f;
    bool selfInteract = true;
+   bool momentum = false;          // MomentumLCALayer
+   float momentumRate = 0.0f;      // LCAMomentumRate
+   float *prevDrive = nullptr;     // restricted, zero at start
    std::string adaptiveProbe;
    float *expTau = nullptr;
    std::vector<double> dt;
@@ -399,12 +402,14 @@ void Net::build() {
    for (int g = 0; g < pvparams_num_groups(P); g++) {
       std::string kw = pvparams_group_keyword(P, g);
       Params q{P, g};
-      if (kw == "PvpLayer" || kw == "ANNErrorLayer" || kw == "HyPerLCALayer" || kw == "ANNLayer"
-          || kw == "HyPerLayer") {
+      if (kw == "PvpLayer" || kw == "ANNErrorLayer" || kw == "HyPerLCALayer"
+          || kw == "MomentumLCALayer" || kw == "ANNLayer" || kw == "HyPerLayer") {
          Layer L;
          L.name  = pvparams_group_name(P, g);
          L.prm   = q;
-         L.kind  = kw == "PvpLayer" ? L_INPUT : kw == "ANNErrorLayer" ? L_ERROR : kw == "HyPerLCALayer" ? L_LCA : L_ANN;
+         L.kind  = kw == "PvpLayer" ? L_INPUT : kw == "ANNErrorLayer" ? L_ERROR
+                 : kw == "HyPerLCALayer" || kw == "MomentumLCALayer" ? L_LCA : L_ANN;
+         L.momentum = kw == "MomentumLCALayer";
          L.phase = (int)q.num("phase", 0);
          L.loc.nbatch = nbatch;
          L.loc.nx     = (int)std::lround(nxGlobal * q.num("nxScale", 1.0));
@@ -552,6 +557,10 @@ void Net::build() {
       if (L.kind == L_LCA) {
          L.tau          = (float)(L.prm.num("timeConstantTau", 1.0) / dtCol);
          L.selfInteract = L.prm.num("selfInteract", 1) != 0;
+         if (L.momentum) { // MomentumLCAInternalStateBuffer.cpp: LCAMomentumRate, the prevDrive buffer
+            L.momentumRate = (float)L.prm.num("LCAMomentumRate", 0.0);
+            L.prevDrive    = static_cast<float *>(D.alloc(nres * 4));
+         }
          L.adaptiveProbe = L.prm.str("adaptiveTimeScaleProbe");
          L.expTau        = static_cast<float *>(D.alloc(nbatch * 4));
          L.dt.assign(nbatch, dtCol);
@@ -1041,7 +1050,13 @@ void Net::updateLayer(Layer &L, double t) {
       u.self_interact = L.selfInteract;
       u.transfer = L.transfer;
       u.activity_from_v = 1;
-      check(pvhip_lca_update(&u, D.stream), "pvhip_lca_update");
+      if (L.momentum) {
+         check(pvhip_momentum_lca_update(&u, L.momentumRate, L.prevDrive, D.stream),
+               "pvhip_momentum_lca_update");
+      }
+      else {
+         check(pvhip_lca_update(&u, D.stream), "pvhip_lca_update");
+      }
    }
    else {
       pvhip_layer_update_args u{};

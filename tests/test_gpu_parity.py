@@ -929,16 +929,17 @@ def _gpulca_run(arith, fused=True, steps=40):
     return np.stack(recon).reshape(steps, 64, 128, 1)
 
 
-def _gpulca_oracle(steps):
+def _gpulca_oracle(steps, momentum=None):
     """Reconstruction of each step of the column on the oracle."""
-    return _gpulca_oracle_run(steps)[0]
+    return _gpulca_oracle_run(steps, momentum)[0]
 
 
 @functools.lru_cache(maxsize=None)
-def _gpulca_oracle_run(steps):
+def _gpulca_oracle_run(steps, momentum=None):
     """The same column on the oracle (one-thread CPU restatement), reusing the
     host timescale controller: (Reconstruction of each step, total energy of
-    each step).  Cached: callers must not modify the arrays."""
+    each step).  momentum: V1 as a MomentumLCALayer with that LCAMomentumRate.
+    Cached: callers must not modify the arrays."""
     from openpv_amd import pvp, abi as A
     from openpv_amd.timescale import AdaptiveTimeScaleController, PeriodicUpdate
     _, _, frames = pvp.read_activity(os.path.join(GOLDEN, "inputImages.pvp"))
@@ -968,7 +969,11 @@ def _gpulca_oracle_run(steps):
         Rext = np.zeros((res.ny_ext, res.nx_ext), np.float32)
         Rext[mres:mres + 64, mres:mres + 128] = (Rres * np.float32(1.0)).reshape(64, 128)
         G = O.recv_post(res, v1, 18, 18, Rext.ravel(), W)
-        V = O.update_lca(v1, True, ts, 100.0, G, Av, V)
+        if momentum is None:
+            V = O.update_lca(v1, True, ts, 100.0, G, Av, V)
+        else:
+            V, prev = O.update_momentum_lca(v1, True, momentum, ts, 100.0, G, Av,
+                                            prev if t > 1 else np.zeros(v1.num_res, np.float32), V)
         Av = O.transfer(v1, V, *tr)
         ent, cnt = O.active_indices(v1, Av)
         rc = O.recv_pre(v1, rec, 18, 18, W, entries=ent, num_active=cnt, num_threads=1)
@@ -1041,6 +1046,27 @@ def test_pvrun_two_columns_from_params(tmp_path):
     assert err <= 1e-4 * np.abs(ref).max()
     # RequireAllZeroActivityProbe at nnzThreshold 2e-7 on Difference = ReconDevice - ReconHost
     assert (r.returncode == 0) == (diff <= np.float32(2e-7)), r.stderr[-500:]
+
+
+def test_pvrun_momentum_lca_column(tmp_path):
+    """The host column's V1 as a MomentumLCALayer (LCAMomentumRate 0.4, the
+    rate of the reference's MomentumLCATest): pvrun's reconstructions equal the
+    oracle's momentum column bit for bit over 25 steps, and differ from the
+    plain LCA column's."""
+    from openpv_amd import pvp
+    steps = 25
+    text = open(os.path.join(GOLDEN, "LCAColumn.params")).read()
+    old = 'HyPerLCALayer "SparseCodeHost" = {'
+    assert text.count(old) == 1
+    params = tmp_path / "momentum.params"
+    params.write_text(text.replace(old, 'MomentumLCALayer "SparseCodeHost" = {\n    LCAMomentumRate = 0.4;'))
+    r = _pvrun(params, tmp_path, steps)
+    assert r.returncode in (0, 1), r.stderr[-500:]   # 1: the host and device columns now differ
+    ref = _gpulca_oracle(steps, 0.4)
+    _, _, host = pvp.read_activity(str(tmp_path / "ReconHost.pvp"))
+    for t in range(steps):
+        assert same(host[t], ref[t]), "step %d (max %g)" % (t + 1, np.abs(host[t] - ref[t]).max())
+    assert not same(host, _gpulca_oracle(steps))
 
 
 @pytest.mark.parametrize("form,tol", [("bf16", 3e-2), ("fp8", 6e-2)])
