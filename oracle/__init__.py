@@ -131,6 +131,11 @@ def lib():
         L.or_cost_sum.restype = C.c_double
         L.or_lca_run.argtypes = [C.POINTER(LCAConfig), f32p, f32p, C.c_void_p, C.c_int,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_timescale_init.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+        L.or_timescale_calc.argtypes = [C.c_void_p, C.c_int, f64p, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, f64p]
+        L.or_need_update.argtypes = [C.c_double] * 4
+        L.or_need_update.restype = C.c_int
         _lib = L
     return _lib
 
@@ -307,3 +312,24 @@ def lca_run(cfg, weights, inp, V0, nsteps):
                      np.ascontiguousarray(inp, np.float32).ravel(), _ptr(v0), nsteps,
                      _ptr(V), _ptr(A), _ptr(rec), _ptr(en), _ptr(na))
     return dict(V=V, A=A, recon=rec, energy=en, num_active=na)
+
+
+class AdaptiveTimeScales:
+    """The oracle's AdaptiveTimeScaleController (or_timescale_*): calc(values)
+    returns the per-batch time scales for the coming step."""
+
+    def __init__(self, nbatch, base_max, base_min, tau_factor, growth_factor):
+        self.nb = nbatch
+        self.p = (float(base_max), float(base_min), float(tau_factor), float(growth_factor))
+        self.state = np.zeros(3 * nbatch, np.float64)
+        lib().or_timescale_init(self.state.ctypes.data, nbatch, self.p[0], self.p[1])
+
+    def calc(self, values):
+        out = np.zeros(self.nb, np.float64)
+        lib().or_timescale_calc(self.state.ctypes.data, self.nb,
+                                np.ascontiguousarray(values, np.float64), *self.p, out)
+        return out
+
+
+def need_update(sim_time, dt, last_update_time, period):
+    return bool(lib().or_need_update(sim_time, dt, last_update_time, period))

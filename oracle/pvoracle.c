@@ -925,6 +925,51 @@ double or_cost_sum(const or_loc *loc, const float *buf, int extended, int batch,
    return sum;
 }
 
+/* ------------------------------------------------- host-side loop controllers */
+
+void or_timescale_init(or_timescale *ts, int nbatch, double baseMax, double baseMin) {
+   /* AdaptiveTimeScaleController.cpp:17-47 */
+   for (int b = 0; b < nbatch; b++) {
+      ts[b].timeScale     = baseMin;
+      ts[b].timeScaleMax  = baseMax;
+      ts[b].timeScaleTrue = -1.0;
+   }
+}
+
+void or_timescale_calc(or_timescale *ts, int nbatch, const double *values, double baseMax,
+      double baseMin, double tauFactor, double growthFactor, double *dt_out) {
+   /* AdaptiveTimeScaleController::calcTimesteps, AdaptiveTimeScaleController.cpp:60-88 */
+   for (int b = 0; b < nbatch; b++) {
+      double const E_0  = ts[b].timeScaleTrue; /* the copy taken at :67 */
+      ts[b].timeScaleTrue = values[b];
+      double const E_dt = values[b];
+      if (E_dt != E_0) {
+         double const dE_dt_scaled = (E_0 - E_dt) / ts[b].timeScale;
+         if (dE_dt_scaled < 0.0 || E_0 <= 0.0 || E_dt <= 0.0) {
+            ts[b].timeScale    = baseMin;
+            ts[b].timeScaleMax = baseMax;
+         }
+         else {
+            double const tau_eff_scaled = E_0 / dE_dt_scaled;
+            ts[b].timeScale = tauFactor * tau_eff_scaled;
+            if (ts[b].timeScale >= ts[b].timeScaleMax) {
+               ts[b].timeScale = ts[b].timeScaleMax;
+               ts[b].timeScaleMax *= (1 + growthFactor);
+            }
+         }
+      }
+      dt_out[b] = ts[b].timeScale;
+   }
+}
+
+int or_need_update(double simTime, double deltaTime, double lastUpdateTime, double period) {
+   /* LayerUpdateController::needUpdate, LayerUpdateController.cpp:318-335 (no trigger) */
+   if (period <= 0) return 0;
+   double const numUpdates = (simTime - lastUpdateTime) / period;
+   double const closest    = fabs(numUpdates - nearbyint(numUpdates)) * period;
+   return closest < 0.5 * deltaTime;
+}
+
 /* ---------------------------------------------------------------- LCA loop */
 
 void or_lca_run(const or_lca_config *cfg, const float *weights, const float *input,

@@ -154,6 +154,23 @@ void or_normalize_l2_f64(float *w, int numPatches, int patchSize, double strengt
 double or_cost_sum(const or_loc *loc, const float *buf, int extended, int batch, int kind,
       double p0, double p1);
 
+/* --- host-side controllers of the LCA loop --- */
+/* AdaptiveTimeScaleController state of one batch element
+ * (src/components/AdaptiveTimeScaleController.hpp TimeScaleData). */
+typedef struct {
+   double timeScale, timeScaleMax, timeScaleTrue;
+} or_timescale;
+/* AdaptiveTimeScaleController ctor (AdaptiveTimeScaleController.cpp:17-47). */
+void or_timescale_init(or_timescale *ts, int nbatch, double baseMax, double baseMin);
+/* AdaptiveTimeScaleController::calcTimesteps (AdaptiveTimeScaleController.cpp:60-88):
+ * values[b] is the target energy probe's value (or -1 on a trigger step);
+ * writes the new time scale of each batch element to dt_out[b]. */
+void or_timescale_calc(or_timescale *ts, int nbatch, const double *values, double baseMax,
+      double baseMin, double tauFactor, double growthFactor, double *dt_out);
+/* LayerUpdateController::needUpdate without a trigger layer
+ * (src/components/LayerUpdateController.cpp:318-335). */
+int or_need_update(double simTime, double deltaTime, double lastUpdateTime, double period);
+
 /* --- whole LCA loop --- */
 /* The reference's 4-phase LCA column (tests/GPULCATest/input/GPULCATest.params,
  * tutorials/LCA-CIFAR): Input -IdentConn ch0-> Residual (ANNErrorLayer, errScale)

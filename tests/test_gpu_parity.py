@@ -5,7 +5,6 @@ lists, counts); FP32 (FMA) forms within 1e-5 relative to the largest
 magnitude; BF16 forms within the tolerance written at each test.
 """
 import ctypes as C
-import functools
 import os
 import subprocess
 
@@ -934,55 +933,14 @@ def _gpulca_oracle(steps, momentum=None):
     return _gpulca_oracle_run(steps, momentum)[0]
 
 
-@functools.lru_cache(maxsize=None)
 def _gpulca_oracle_run(steps, momentum=None):
-    """The same column on the oracle (one-thread CPU restatement), reusing the
-    host timescale controller: (Reconstruction of each step, total energy of
-    each step).  momentum: V1 as a MomentumLCALayer with that LCAMomentumRate.
-    Cached: callers must not modify the arrays."""
-    from openpv_amd import pvp, abi as A
-    from openpv_amd.timescale import AdaptiveTimeScaleController, PeriodicUpdate
-    _, _, frames = pvp.read_activity(os.path.join(GOLDEN, "inputImages.pvp"))
-    _, _, V0 = pvp.read_activity(os.path.join(GOLDEN, "initialV1.pvp"))
-    _, _, W = pvp.read_weights(os.path.join(GOLDEN, "weights.pvp"))
-    W = W[0].copy()
-    O.lib().or_normalize_l2(W.ravel(), 24, 18 * 18, 1.0, 0.0)  # in place (contiguous)
-    H = A.host()
-    mres = H.pvhost_required_margin(128, 64, H.pvhost_post_patch_size(18, 64, 128))
-    mv1 = H.pvhost_required_margin(64, 128, 18)
-    res, v1, rec = O.Loc.make(1, 128, 64, 1, mres), O.Loc.make(1, 64, 32, 24, mv1), O.Loc.make(1, 128, 64, 1, 0)
-    tr = (0.025, 0.0, FMAX, 0.0, 0.025)
-    V = V0[0].ravel().copy()
-    Av = O.transfer(v1, V, *tr)
-    rc = np.zeros(rec.num_res, np.float32)
-    ctrl, trig = AdaptiveTimeScaleController(1, 0.55, 0.5, 0.1, 0.01), PeriodicUpdate(20)
-    x, idx, e_prev, out, energies = frames[0], 0, np.zeros(1), [], []
-    for t in range(1, steps + 1):
-        up = trig.need_update(t)
-        ts = ctrl.calc_timesteps(np.full(1, -1.0) if up else e_prev)
-        if up:
-            x, idx = frames[idx], (idx + 1) % 2
-            trig.updated(t)
-        Rres = np.zeros(res.num_res, np.float32)
-        O.lib().or_gsyn_accumulate(res.num_res, 2, np.array([1.0, -1.0], np.float32),
-                                   np.concatenate([x.ravel(), rc]), Rres)
-        Rext = np.zeros((res.ny_ext, res.nx_ext), np.float32)
-        Rext[mres:mres + 64, mres:mres + 128] = (Rres * np.float32(1.0)).reshape(64, 128)
-        G = O.recv_post(res, v1, 18, 18, Rext.ravel(), W)
-        if momentum is None:
-            V = O.update_lca(v1, True, ts, 100.0, G, Av, V)
-        else:
-            V, prev = O.update_momentum_lca(v1, True, momentum, ts, 100.0, G, Av,
-                                            prev if t > 1 else np.zeros(v1.num_res, np.float32), V)
-        Av = O.transfer(v1, V, *tr)
-        ent, cnt = O.active_indices(v1, Av)
-        rc = O.recv_pre(v1, rec, 18, 18, W, entries=ent, num_active=cnt, num_threads=1)
-        vt = float(np.float32(0.025))
-        e_prev = np.array([0.5 * O.cost_sum(res, Rext.ravel(), True, 0, 0, 0, 0)
-                           + vt * O.cost_sum(v1, Av, True, 0, 2, vt, vt)])
-        out.append(rc.copy())
-        energies.append(e_prev[0])
-    return np.stack(out).reshape(steps, 64, 128, 1), np.array(energies)
+    """The same column on the oracle alone (its deliveries, updates, cost sums,
+    time-scale controller and input trigger; lcatest_column.oracle_run):
+    (Reconstruction of each step, total energy of each step).  momentum: V1 as
+    a MomentumLCALayer with that LCAMomentumRate.  Cached: callers must not
+    modify the arrays."""
+    import lcatest_column
+    return lcatest_column.oracle_run(steps, momentum)
 
 
 def test_gpulca_inputs_adaptive_timescales_exact():
