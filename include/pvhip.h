@@ -113,7 +113,8 @@ int pvhip_abi_version(void);
 int pvhip_device_create(int ordinal, pvhip_device **out);
 /* ~CudaDevice (src/arch/cuda/CudaDevice.cpp:96-109). */
 int pvhip_device_destroy(pvhip_device *dev);
-/* CudaDevice::syncDevice (src/arch/cuda/CudaDevice.cpp:96). */
+/* CudaDevice::syncDevice (src/arch/cuda/CudaDevice.cpp:96), then the check of
+ * pvhip_check_faults. */
 int pvhip_device_sync(pvhip_device *dev);
 /* CudaDevice getters (src/arch/cuda/CudaDevice.cpp:167-183). */
 int pvhip_device_props(pvhip_device *dev, pvhip_props *out);
@@ -137,7 +138,15 @@ size_t pvhip_buffer_size(pvhip_buffer *buf);
 
 /* Zero-fill with a kernel (graph-capturable; no hipMemsetAsync node). */
 int pvhip_fill_zero(void *dptr, size_t bytes, pvhip_stream stream);
+/* hipStreamSynchronize, then the check of pvhip_check_faults. */
 int pvhip_stream_sync(pvhip_stream stream);
+/* Waits for the device, then reports (and clears) tiles a delivery kernel
+ * refused since the last check: geometry tables that do not belong to the
+ * locs of their call leave those outputs unwritten, and the next
+ * pvhip_stream_sync / pvhip_device_sync / pvhip_check_faults returns
+ * PVHIP_ERR_INVALID with a message.  The reference has no such case: its
+ * tables are built inside the delivery (CudaRecvPost.cpp:51-389). */
+int pvhip_check_faults(void);
 
 /* CudaTimer (src/arch/cuda/CudaTimer.cpp:41-72): events on a stream. */
 int pvhip_event_create(pvhip_event **out);

@@ -154,6 +154,7 @@ PVHIP_SYMBOLS = {
     "pvhip_buffer_size": (C.c_size_t, [vp]),
     "pvhip_fill_zero": (i32, [vp, C.c_size_t, vp]),
     "pvhip_stream_sync": (i32, [vp]),
+    "pvhip_check_faults": (i32, []),
     "pvhip_event_create": (i32, [C.POINTER(vp)]),
     "pvhip_event_destroy": (i32, [vp]),
     "pvhip_event_record": (i32, [vp, vp]),

@@ -44,12 +44,23 @@ def _run(cmd, verbose):
 
 
 def build_hip(verbose=False):
+    """Each source compiled to an object in parallel (device code is not
+    relocatable: every object carries its own code object), then linked."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIBDIR, exist_ok=True)
+    objdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    hipcc = _hipcc()
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    objs = [os.path.join(objdir, os.path.splitext(src)[0] + ".o") for src in HIP_SOURCES]
+    cmds = [[hipcc] + compile_flags + ["-I" + INCLUDE, "-c", "-o", o, os.path.join(CSRC, src)]
+            for src, o in zip(HIP_SOURCES, objs)]
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
     out = os.path.join(LIBDIR, "libpvhip.so")
     tmp = out + ".tmp"
-    cmd = [_hipcc()] + HIPCC_FLAGS + ["-I" + INCLUDE, "-o", tmp] + \
-        [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    _run(cmd, verbose)
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, out)
     return out
 

@@ -24,6 +24,24 @@ int set_error(int code, const char *fmt, ...) {
 
 void clear_error() { g_error = "no error"; }
 
+int check_faults() {
+   unsigned mask = 0;
+   int st        = gather_faults(&mask);
+   if (st) {
+      return st;
+   }
+   if (mask & 1u) {
+      return set_error(PVHIP_ERR_INVALID, "recv_post: a kernel refused tiles whose source window lies outside the "
+                                          "bounds of the call (geometry tables that do not belong to its locs); "
+                                          "those outputs were not written");
+   }
+   if (mask & 2u) {
+      return set_error(PVHIP_ERR_INVALID, "recv_post: a kernel refused tiles whose window starts are off the "
+                                          "alignment class of the call's geometry; those outputs were not written");
+   }
+   return PVHIP_OK;
+}
+
 const char *last_error() { return g_error.c_str(); }
 
 } // namespace pvhip
@@ -122,7 +140,7 @@ int pvhip_device_destroy(pvhip_device *dev) {
 int pvhip_device_sync(pvhip_device *dev) {
    PVHIP_REQUIRE(dev != nullptr, "pvhip_device_sync: NULL device");
    PVHIP_TRY(hipStreamSynchronize(dev->stream));
-   return PVHIP_OK;
+   return pvhip::check_faults();
 }
 
 int pvhip_device_props(pvhip_device *dev, pvhip_props *out) {
@@ -238,7 +256,12 @@ int pvhip_fill_zero(void *dptr, size_t bytes, pvhip_stream stream) {
 
 int pvhip_stream_sync(pvhip_stream stream) {
    PVHIP_TRY(hipStreamSynchronize((hipStream_t)stream));
-   return PVHIP_OK;
+   return pvhip::check_faults();
+}
+
+int pvhip_check_faults(void) {
+   PVHIP_TRY(hipDeviceSynchronize());
+   return pvhip::check_faults();
 }
 
 int pvhip_event_create(pvhip_event **out) {

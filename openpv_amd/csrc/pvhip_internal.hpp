@@ -16,6 +16,11 @@ namespace pvhip {
 // Records the message for pvhip_last_error() and returns `code`.
 int set_error(int code, const char *fmt, ...);
 void clear_error();
+// Reads and clears the tiles the gather kernels refused since the last call
+// (bit 1: a source window outside its bounds, bit 2: off the alignment class).
+int gather_faults(unsigned *mask);
+// Turns refused tiles into an error (after the caller has synchronised).
+int check_faults();
 
 #define PVHIP_TRY(call)                                                                     \
    do {                                                                                    \

@@ -53,6 +53,17 @@ struct PostParams {
    Transfer layerT;
 };
 
+// Set by a gather kernel that refuses a tile whose source window breaks the
+// bounds the host sized the launch for (geometry tables that do not belong to
+// the locs of the call, or window starts off the alignment class the host
+// assumed); the tile is then left unwritten.  Read and cleared by
+// gather_faults(), which pvhip_stream_sync / pvhip_device_sync /
+// pvhip_check_faults turn into an error.
+enum : unsigned { FAULT_WINDOW = 1u, FAULT_ALIGN = 2u };
+__device__ unsigned int g_gather_fault;
+
+__device__ __forceinline__ void report_fault(unsigned bit) { atomicOr(&g_gather_fault, bit); }
+
 // Target-layer update of one delivered value G (restricted index gi, extended
 // index ai), the operations of layer_update_kernel with one channel.
 __device__ __forceinline__ void fused_layer_store(const PostParams &p, size_t gi, size_t ai, float G) {
@@ -139,9 +150,12 @@ __global__ void __launch_bounds__(256) recv_post_tiled_kernel(PostParams p, int 
    int const srcRows     = p.startY[yl] - srcY0 + p.nyp;
    int const tid         = threadIdx.x;
    // Tables outside the bound the host sized LDS for, or patches reaching
-   // outside the extended source buffer, leave this tile unwritten.
+   // outside the extended source buffer: the tile is refused and reported.
    if (srcCols > srcColsMax || srcRows > srcRowsMax || srcX0 < 0 || srcY0 < 0
        || srcX0 + srcCols > p.pre.nxExt() || srcY0 + srcRows > p.pre.nyExt()) {
+      if (tid == 0) {
+         report_fault(FAULT_WINDOW);
+      }
       return;
    }
    int const preRow      = p.pre.nxExt() * p.pre.nf;
@@ -645,6 +659,9 @@ __global__ void __launch_bounds__(256, E::WAVES) recv_post_mfma_bf16_kernel(
    int const tid     = threadIdx.x;
    if (srcCols > srcColsMax || srcRows > srcRowsMax || srcX0 < 0 || srcY0 < 0
        || srcX0 + srcCols > p.pre.nxExt() || srcY0 + srcRows > p.pre.nyExt()) {
+      if (tid == 0) {
+         report_fault(FAULT_WINDOW); // tile refused: tables do not match the locs
+      }
       return;
    }
    int const lane = tid % 64;
@@ -755,6 +772,9 @@ __global__ void __launch_bounds__(256, E::WAVES) recv_post_mfma_bf16_kernel(
          bad = bad || (pbase[pb] % ALIGN) != 0;
       }
       if (__syncthreads_or(bad)) {
+         if (tid == 0) {
+            report_fault(FAULT_ALIGN); // tile refused: a window start off the alignment class
+         }
          return;
       }
    }
@@ -995,9 +1015,18 @@ __global__ void __launch_bounds__(256) recv_post_mfma_persist_kernel(
       t.srcX0    = p.startX[t.x0];
       t.srcY0    = p.startY[t.y0];
       // the host's tables keep every window inside the extended source and
-      // within the staged size; the clamps only keep the loads in bounds
-      int const srcCols = min(p.startX[t.x0 + MTX - 1] - t.srcX0 + p.nxp, srcColsMax);
-      int const srcRows = min(p.startY[t.y0 + MTY - 1] - t.srcY0 + p.nyp, srcRowsMax);
+      // within the staged size; tables that do not are reported, and the
+      // clamps keep the loads in bounds
+      int const cols = p.startX[t.x0 + MTX - 1] - t.srcX0 + p.nxp;
+      int const rows = p.startY[t.y0 + MTY - 1] - t.srcY0 + p.nyp;
+      if (cols > srcColsMax || rows > srcRowsMax || t.srcX0 < 0 || t.srcY0 < 0
+          || t.srcX0 + cols > p.pre.nxExt() || t.srcY0 + rows > p.pre.nyExt()) {
+         if (tid == 0) {
+            report_fault(FAULT_WINDOW);
+         }
+      }
+      int const srcCols = min(cols, srcColsMax);
+      int const srcRows = min(rows, srcRowsMax);
       t.srcX0    = max(0, min(t.srcX0, p.pre.nxExt() - srcCols));
       t.srcY0    = max(0, min(t.srcY0, p.pre.nyExt() - srcRows));
       t.rowElems = srcCols * p.nfp;
@@ -1640,6 +1669,30 @@ __global__ void weights_to_bf16_kernel(
    Wb[i]  = kk < K ? (__bf16)W[(size_t)f * K + kk] : (__bf16)0.0f;
 }
 
+// Halo a source of n_pre needs on each side for a post-perspective patch of
+// `patch` values into a target of n_post: requiredConvolveMargin
+// (src/utils/requiredConvolveMargin.hpp:14-40) of the connection's own
+// pre-perspective patch, rewritten in terms of the post-perspective patch of
+// PatchSize::calcPostPatchSize (src/components/PatchSize.cpp:98-130).
+// -1: the patch does not fit the scale.
+int gather_margin(int n_pre, int n_post, int patch) {
+   if (n_pre <= 0 || n_post <= 0 || patch <= 0) {
+      return -1;
+   }
+   if (n_pre == n_post) {
+      return patch % 2 == 1 ? (patch - 1) / 2 : -1; // one to one: odd patches
+   }
+   if (n_pre > n_post) { // many to one, ratio r: pre-perspective patch patch/r
+      int const r = n_pre / n_post;
+      if (n_pre % n_post || patch % r) {
+         return -1;
+      }
+      return (patch / r - 1) * r / 2;
+   }
+   // one to many: the pre-perspective patch is patch * (n_post / n_pre)
+   return n_post % n_pre ? -1 : patch / 2;
+}
+
 int build_params(const pvhip_recv_post_args &a, PostParams *p) {
    PVHIP_REQUIRE(loc_valid(a.pre) && loc_valid(a.post), "recv_post: invalid loc");
    PVHIP_REQUIRE(a.pre.nbatch == a.post.nbatch, "recv_post: batch sizes differ");
@@ -1647,6 +1700,19 @@ int build_params(const pvhip_recv_post_args &a, PostParams *p) {
    PVHIP_REQUIRE(a.num_kernels_x > 0 && a.num_kernels_y > 0, "recv_post: bad kernel counts");
    PVHIP_REQUIRE(a.start_x && a.start_y && a.kernel_x && a.kernel_y, "recv_post: NULL geometry table");
    PVHIP_REQUIRE(a.pre_data && a.weights, "recv_post: NULL pre_data or weights");
+   {
+      // every side of the source halo must cover the reach of the patches
+      // (the margin the connection requires of its pre layer,
+      // requiredConvolveMargin in terms of the post-perspective patch)
+      int const mx = gather_margin(a.pre.nx, a.post.nx, a.nxp);
+      int const my = gather_margin(a.pre.ny, a.post.ny, a.nyp);
+      PVHIP_REQUIRE(mx >= 0 && my >= 0, "recv_post: patch %dx%d does not fit the scale %d->%d x %d->%d", a.nxp,
+            a.nyp, a.pre.nx, a.post.nx, a.pre.ny, a.post.ny);
+      PVHIP_REQUIRE(a.pre.halo.lt >= mx && a.pre.halo.rt >= mx && a.pre.halo.dn >= my && a.pre.halo.up >= my,
+            "recv_post: source halo (lt %d, rt %d, dn %d, up %d) smaller than the margin %d x %d the %dx%d "
+            "post-perspective patch needs",
+            a.pre.halo.lt, a.pre.halo.rt, a.pre.halo.dn, a.pre.halo.up, mx, my, a.nxp, a.nyp);
+   }
    p->pre      = geo(a.pre);
    p->post     = geo(a.post);
    p->nxp      = a.nxp;
@@ -1817,6 +1883,17 @@ size_t cgemm_scratch_bytes(const PostParams &p, const CGemmPlan &pl) {
 }
 
 } // namespace
+
+int gather_faults(unsigned *mask) {
+   unsigned v = 0, zero = 0;
+   PVHIP_TRY(hipMemcpyFromSymbol(&v, HIP_SYMBOL(g_gather_fault), sizeof v, 0, hipMemcpyDeviceToHost));
+   if (v != 0) {
+      PVHIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_gather_fault), &zero, sizeof zero, 0, hipMemcpyHostToDevice));
+   }
+   *mask = v;
+   return PVHIP_OK;
+}
+
 } // namespace pvhip
 
 using namespace pvhip;

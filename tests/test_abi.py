@@ -47,6 +47,64 @@ def test_invalid_arguments_are_rejected_without_a_device():
     assert H.pvhip_active_compact_scratch_bytes(C.byref(bad)) == 0
 
 
+@pytest.mark.parametrize("arith", [abi.ARITH_EXACT, abi.ARITH_FP32, abi.ARITH_BF16, abi.ARITH_FP8])
+@pytest.mark.parametrize("case", [
+    # source (nb, nx, ny, nf), target, post-perspective patch, required margin
+    ((1, 128, 128, 1), (1, 32, 32, 512), 16, 6),   # the LCA gather: many to one, ratio 4
+    ((1, 256, 256, 16), (1, 128, 128, 256), 12, 5),
+    ((1, 32, 32, 3), (1, 32, 32, 8), 5, 2),         # one to one
+    ((1, 16, 16, 64), (1, 64, 64, 2), 4, 2),        # one to many (the transposed scatter)
+])
+def test_recv_post_refuses_a_source_halo_below_the_margin(case, arith):
+    """pvhip_recv_post checks on the host that every side of the source halo
+    covers the patch's reach (the margin the connection requires of its pre
+    layer) and returns PVHIP_ERR_INVALID before any launch when it does not;
+    the margin itself is accepted up to the launch (which needs a device).
+    The margins are pvhost_required_margin's for the connection's own
+    pre-perspective patch, and pvhost_recv_post_tables refuses the same
+    halos."""
+    src, tgt, P, m = case
+    H, Ho = abi.hip(), abi.host()
+    fake = C.c_void_p(4096)   # never dereferenced: the call must stop before a launch
+
+    def call(halo):
+        a = abi.RecvPostArgs()
+        a.pre = abi.Loc.make(*src, halo=halo)
+        a.post = abi.Loc.make(*tgt, halo=0)
+        a.nxp = a.nyp = P
+        a.nfp = src[3]
+        a.num_kernels_x = a.num_kernels_y = 1
+        for f in ("start_x", "start_y", "kernel_x", "kernel_y", "pre_data", "weights", "post_gsyn"):
+            setattr(a, f, fake)
+        a.arith = arith
+        return H.pvhip_recv_post(C.byref(a), None), a
+
+    for halo in ((m - 1,) * 4, (m - 1, m, m, m), (m, m - 1, m, m), (m, m, m - 1, m), (m, m, m, m - 1)):
+        st, a = call(halo)
+        assert st == 1, halo
+        assert b"halo" in H.pvhip_last_error()
+        n = np.zeros(max(tgt[1], tgt[2]), np.int32)
+        ptr = lambda x: x.ctypes.data_as(C.c_void_p)
+        nk = C.c_int()
+        assert Ho.pvhost_recv_post_tables(C.byref(a.pre), C.byref(a.post), P, P, ptr(n), ptr(n), ptr(n),
+                                          ptr(n), C.byref(nk), C.byref(nk)) < 0, halo
+    # the required margin passes the halo check: a BF16 / FP8 call without its
+    # weight shadow (or, for several data patches per feature, without inverse
+    # tables) then stops at that check, still before any launch
+    if arith in (abi.ARITH_BF16, abi.ARITH_FP8):
+        st, _ = call((m,) * 4)
+        msg = H.pvhip_last_error()
+        assert st == 1 and b"halo" not in msg and (b"weights_lp" in msg or b"inverse" in msg
+                                                   or b"FP8" in msg), msg
+
+
+def test_check_faults_is_exported_and_needs_a_device():
+    """pvhip_check_faults reports refused tiles after a device synchronisation;
+    without a device the synchronisation itself fails (a runtime error, not a
+    silent success)."""
+    assert abi.hip().pvhip_check_faults() in (0, 2)
+
+
 def test_channel_gemm_scratch_query_follows_the_launch_shape():
     """pvhip_recv_post_scratch_bytes (host logic only): the stride-4 transposed
     gather (32x32x512 -> 128x128x1) needs splits x GSyn floats of scratch:
