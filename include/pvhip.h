@@ -552,6 +552,14 @@ enum { PVHIP_MOMENTUM_VISCOSITY = 0, PVHIP_MOMENTUM_SIMPLE = 1, PVHIP_MOMENTUM_A
 int pvhip_momentum_factors(
       int method, float tau, float decay, float dw_max, float *dw_factor, float *w_factor);
 
+/* The factors of applyMomentumDeprecated (MomentumUpdater.cpp:333-343) for the
+ * deprecated momentumTau parameter (used when timeConstantTau is absent):
+ * viscosity exp(-1/momentumTau), simple momentumTau, alex momentumTau with
+ * decay*dWMax.  The reference applies no range check to momentumTau (only
+ * momentumDecay's, :138-145), and neither does this. */
+int pvhip_momentum_factors_deprecated(
+      int method, float momentum_tau, float decay, float dw_max, float *dw_factor, float *w_factor);
+
 /* MomentumUpdater::updateWeights in one pass (MomentumUpdater.cpp:276-294):
  * normalize_dW as pvhip_hebbian_apply (normalize_dw, counts), then
  *   dw = dw*(1-dw_factor) + dw_factor*prev_dw - w_factor*w   (:309-329), or,
@@ -563,6 +571,14 @@ int pvhip_momentum_apply(
       float *weights, float *dw, float *prev_dw, const int64_t *counts, size_t n,
       int normalize_dw, float dw_factor, float w_factor, int deprecated_tau,
       pvhip_stream stream);
+
+/* TransposeWeights::transposeShared on the device (src/utils/TransposeWeights.cpp:56-113,
+ * called by WeightsPair::finalizeUpdate after each weight update,
+ * src/components/WeightsPair.cpp:261-280): dst[i] = src[index[i]], 0 where
+ * index[i] < 0, with index from pvhost_transpose_weights_index.  n = the size
+ * of dst. */
+int pvhip_permute_weights(
+      const float *src, const int32_t *index, float *dst, size_t n, pvhip_stream stream);
 
 /* NormalizeL2 per data patch in the reference's float arithmetic and order
  * (src/normalizers/NormalizeL2.cpp:93-125): sumsq += w*w, w *= strength /

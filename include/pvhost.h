@@ -77,6 +77,12 @@ int pvhost_make_transfer(float VThresh, float AMin, float AMax, float AShift, fl
  * pre-perspective patch nxp x nyp x nfp. */
 int pvhost_transpose_weights(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int nyp,
       int nfp, const float *pre_weights, float *post_weights);
+/* The same transposition as a table for the device (pvhip_permute_weights):
+ * post_weights[i] = pre_weights[index[i]], index[i] = -1 where no pre weight
+ * lands (the post weight stays 0).  index holds n entries, the size of the
+ * post weights. */
+int pvhost_transpose_weights_index(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int nyp,
+      int nfp, int32_t *index, size_t n);
 
 #ifdef __cplusplus
 }

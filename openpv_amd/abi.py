@@ -187,7 +187,10 @@ PVHIP_SYMBOLS = {
     "pvhip_hebbian_apply": (i32, [vp, vp, vp, C.c_size_t, i32, vp]),
     "pvhip_momentum_factors": (i32, [i32, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),
                                      C.POINTER(C.c_float)]),
+    "pvhip_momentum_factors_deprecated": (i32, [i32, C.c_float, C.c_float, C.c_float,
+                                                C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "pvhip_momentum_apply": (i32, [vp, vp, vp, vp, C.c_size_t, i32, C.c_float, C.c_float, i32, vp]),
+    "pvhip_permute_weights": (i32, [vp, vp, vp, C.c_size_t, vp]),
     "pvhip_normalize_l2_f64": (i32, [vp, i32, i32, C.c_double, C.c_double, vp]),
     "pvhip_normalize_l2": (i32, [vp, i32, i32, f32, f32, vp]),
     "pvhip_cost_sum_f64": (i32, [C.POINTER(Loc), vp, i32, i32, C.c_double, C.c_double, vp, vp]),
@@ -211,6 +214,8 @@ PVHOST_SYMBOLS = {
                                      C.POINTER(i32), vp, vp, vp, vp, C.POINTER(i32)]),
     "pvhost_make_transfer": (i32, [f32, f32, f32, f32, f32, i32, C.POINTER(Transfer)]),
     "pvhost_transpose_weights": (i32, [C.POINTER(Loc), C.POINTER(Loc), i32, i32, i32, vp, vp]),
+    "pvhost_transpose_weights_index": (i32, [C.POINTER(Loc), C.POINTER(Loc), i32, i32, i32, vp,
+                                             C.c_size_t]),
     "pvhost_hebbian_windows": (i32, [C.POINTER(Loc), C.POINTER(Loc), i32, i32, C.POINTER(i32),
                                      C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
 }

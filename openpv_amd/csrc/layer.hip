@@ -575,6 +575,19 @@ __global__ void __launch_bounds__(256) momentum_apply_kernel(
    w[i]    = wo + d;
 }
 
+// dst[i] = src[index[i]] (0 where index[i] < 0): the shared-weight
+// transposition of TransposeWeights::transposeShared as a table
+// (pvhost_transpose_weights_index), so a weight refresh stays on the device.
+__global__ void __launch_bounds__(256) permute_weights_kernel(
+      const float *src, const int32_t *index, float *dst, size_t n) {
+   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= n) {
+      return;
+   }
+   int32_t const j = index[i];
+   dst[i]          = j >= 0 ? src[j] : 0.0f;
+}
+
 inline unsigned blocks_for(size_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
 
 // ------------------------------------------- active list from epilogue segments
@@ -941,6 +954,38 @@ int pvhip_momentum_factors(
          return PVHIP_OK;
       default: PVHIP_REQUIRE(false, "momentum_factors: unknown momentumMethod %d", method);
    }
+}
+
+int pvhip_momentum_factors_deprecated(
+      int method, float momentum_tau, float decay, float dw_max, float *dw_factor, float *w_factor) {
+   PVHIP_REQUIRE(dw_factor && w_factor, "momentum_factors: NULL output");
+   PVHIP_REQUIRE(decay >= 0.0f && decay <= 1.0f,
+         "momentum_factors: momentumDecay must be between 0 and 1 inclusive (%g)", (double)decay);
+   switch (method) { // applyMomentumDeprecated (MomentumUpdater.cpp:333-343): no range check
+      case PVHIP_MOMENTUM_VISCOSITY:
+         *dw_factor = std::exp(-1.0f / momentum_tau);
+         *w_factor  = decay;
+         return PVHIP_OK;
+      case PVHIP_MOMENTUM_SIMPLE:
+      case PVHIP_MOMENTUM_ALEX:
+         *dw_factor = momentum_tau;
+         *w_factor  = method == PVHIP_MOMENTUM_ALEX ? decay * dw_max : decay;
+         return PVHIP_OK;
+      default: PVHIP_REQUIRE(false, "momentum_factors: unknown momentumMethod %d", method);
+   }
+}
+
+int pvhip_permute_weights(
+      const float *src, const int32_t *index, float *dst, size_t n, pvhip_stream stream) {
+   PVHIP_REQUIRE(src && index && dst, "permute_weights: NULL buffer");
+   PVHIP_REQUIRE(src != dst, "permute_weights: src and dst must differ");
+   if (n == 0) {
+      return PVHIP_OK;
+   }
+   hipLaunchKernelGGL(permute_weights_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, src,
+         index, dst, n);
+   PVHIP_LAUNCHED("permute_weights");
+   return PVHIP_OK;
 }
 
 int pvhip_momentum_apply(

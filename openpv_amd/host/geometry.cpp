@@ -115,6 +115,51 @@ int posMod(int a, int m) {
 
 } // namespace
 
+// TransposeWeights::transposeShared (src/utils/TransposeWeights.cpp:56-113)
+// as a visit of (post-weight index, pre-weight index) pairs: the post
+// (post-perspective) weights of a shared-weight connection pre -> post with
+// pre-perspective patch nxp x nyp x nfp are a permutation of its pre weights.
+template <typename Visit>
+int transposeItems(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int nyp, int nfp, Visit visit) {
+   if (!locOk(pre) || !locOk(post) || nfp != post->nf) {
+      return -1;
+   }
+   // Strides between the two layers (PatchGeometry::setTransposeItemIndices :281-332).
+   int sx  = pre->nx > post->nx ? pre->nx / post->nx : 1;  // many-to-one stride
+   int sy  = pre->ny > post->ny ? pre->ny / post->ny : 1;
+   int tx  = post->nx > pre->nx ? post->nx / pre->nx : 1;  // one-to-many stride
+   int ty  = post->ny > pre->ny ? post->ny / pre->ny : 1;
+   int nKx = sx, nKy = sy, nKf = pre->nf;
+   int pxPost = nxp * sx / tx;
+   int pyPost = nyp * sy / ty;
+   int pKx    = tx; // data patches of the post weights along x (kernelsAlong(post, pre))
+   int pKy    = ty;
+   int patchPre  = nxp * nyp * nfp;
+   int patchPost = pxPost * pyPost * pre->nf;
+   for (int kPre = 0; kPre < nKx * nKy * nKf; kPre++) {
+      int kf  = kPre % nKf;
+      int kxp = (kPre / nKf) % nKx;
+      int kyp = kPre / (nKf * nKx);
+      for (int item = 0; item < patchPre; item++) {
+         int f  = item % nfp;
+         int ix = (item / nfp) % nxp;
+         int iy = item / (nfp * nxp);
+         // mirrored item position and the sub-cell phase of even many-to-one patches
+         int phaseX = posMod(kxp - (nxp - 1) * sx / 2, sx);
+         int phaseY = posMod(kyp - (nyp - 1) * sy / 2, sy);
+         int ixPost = (sx * (nxp - 1 - ix) + phaseX) / tx;
+         int iyPost = (sy * (nyp - 1 - iy) + phaseY) / ty;
+         int itemPost = kf + (ixPost + iyPost * pxPost) * pre->nf;
+         // data patch of the post weights (TransposeWeights.cpp:73-99)
+         int dpx = pKx > 1 ? posMod(-(nxp - pKx) / 2 + ix, pKx) : 0;
+         int dpy = pKy > 1 ? posMod(-(nyp - pKy) / 2 + iy, pKy) : 0;
+         int dataPost = f + (dpx + dpy * pKx) * nfp;
+         visit((size_t)dataPost * patchPost + itemPost, (size_t)kPre * patchPre + item);
+      }
+   }
+   return 0;
+}
+
 extern "C" {
 
 int pvhost_required_margin(int nPre, int nPost, int patch) {
@@ -448,43 +493,30 @@ int pvhost_make_transfer(float VThresh, float AMin, float AMax, float AShift, fl
 
 int pvhost_transpose_weights(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int nyp,
       int nfp, const float *preW, float *postW) {
-   if (!locOk(pre) || !locOk(post) || !preW || !postW || nfp != post->nf) {
+   if (!preW || !postW) {
       return -1;
    }
-   // Strides between the two layers (PatchGeometry::setTransposeItemIndices :281-332).
-   int sx  = pre->nx > post->nx ? pre->nx / post->nx : 1;  // many-to-one stride
-   int sy  = pre->ny > post->ny ? pre->ny / post->ny : 1;
-   int tx  = post->nx > pre->nx ? post->nx / pre->nx : 1;  // one-to-many stride
-   int ty  = post->ny > pre->ny ? post->ny / pre->ny : 1;
-   int nKx = sx, nKy = sy, nKf = pre->nf;
-   int pxPost = nxp * sx / tx;
-   int pyPost = nyp * sy / ty;
-   int pKx    = tx; // data patches of the post weights along x (kernelsAlong(post, pre))
-   int pKy    = ty;
-   int patchPre  = nxp * nyp * nfp;
-   int patchPost = pxPost * pyPost * pre->nf;
-   for (int kPre = 0; kPre < nKx * nKy * nKf; kPre++) {
-      int kf  = kPre % nKf;
-      int kxp = (kPre / nKf) % nKx;
-      int kyp = kPre / (nKf * nKx);
-      for (int item = 0; item < patchPre; item++) {
-         int f  = item % nfp;
-         int ix = (item / nfp) % nxp;
-         int iy = item / (nfp * nxp);
-         // mirrored item position and the sub-cell phase of even many-to-one patches
-         int phaseX = posMod(kxp - (nxp - 1) * sx / 2, sx);
-         int phaseY = posMod(kyp - (nyp - 1) * sy / 2, sy);
-         int ixPost = (sx * (nxp - 1 - ix) + phaseX) / tx;
-         int iyPost = (sy * (nyp - 1 - iy) + phaseY) / ty;
-         int itemPost = kf + (ixPost + iyPost * pxPost) * pre->nf;
-         // data patch of the post weights (TransposeWeights.cpp:73-99)
-         int dpx = pKx > 1 ? posMod(-(nxp - pKx) / 2 + ix, pKx) : 0;
-         int dpy = pKy > 1 ? posMod(-(nyp - pKy) / 2 + iy, pKy) : 0;
-         int dataPost = f + (dpx + dpy * pKx) * nfp;
-         postW[(size_t)dataPost * patchPost + itemPost] = preW[(size_t)kPre * patchPre + item];
-      }
+   return transposeItems(pre, post, nxp, nyp, nfp, [&](size_t dst, size_t src) { postW[dst] = preW[src]; });
+}
+
+int pvhost_transpose_weights_index(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int nyp,
+      int nfp, int32_t *index, size_t n) {
+   if (!index) {
+      return -1;
    }
-   return 0;
+   for (size_t i = 0; i < n; i++) {
+      index[i] = -1;
+   }
+   bool ok = true;
+   int st  = transposeItems(pre, post, nxp, nyp, nfp, [&](size_t dst, size_t src) {
+      if (dst < n && src <= (size_t)INT32_MAX) {
+         index[dst] = (int32_t)src;
+      }
+      else {
+         ok = false;
+      }
+   });
+   return st != 0 || !ok ? -1 : 0;
 }
 
 } // extern "C"

@@ -232,6 +232,92 @@ class LCAColumn:
         self.scatter.deliver(self.rec_gsyn, active=self.active, pre_activity=self.v1_A,
                              accumulate=False, stream=s, fused_layer=self._rec_update)
 
+    # dictionary learning -----------------------------------------------------
+    def enable_learning(self, dw_max, normalize_dw=False, arith=None, renormalize="fp64",
+                        strength=1.0, momentum=None):
+        """Plasticity of the dictionary (the V1 -> Residual HyPerConn that the
+        gather's TransposeConn and the scatter's CloneConn share): at each
+        update_weights(), HebbianUpdater's dW = dw_max * sum a_V1 (x) r over the
+        batch (pre: V1's activity, post: the Residual's; -dw_max * dE/dW of
+        E = 0.5|r|^2 with the activity held), W += dW, NormalizeL2 per feature
+        (normalizeOnWeightUpdate; "fp64": the sum of squares and scale in FP64,
+        "exact": the reference's float arithmetic, None: off), then the
+        delivery copies of the weights are re-derived (WeightsPair::finalizeUpdate).
+        arith: the dW correlation's form (EXACT: the reference's order; FP32 /
+        BF16: the MFMA correlation); default EXACT for the EXACT column, else
+        the column's own MFMA form (BF16 for FP8).  momentum: dict of
+        MomentumUpdater parameters (momentumMethod, timeConstantTau,
+        momentumDecay, momentumTau) or None."""
+        from .runtime import HebbianUpdater, MomentumUpdater
+        if arith is None:
+            arith = {abi.ARITH_EXACT: abi.ARITH_EXACT, abi.ARITH_FP32: abi.ARITH_FP32}.get(
+                self.arith, abi.ARITH_BF16)
+        if renormalize not in ("fp64", "exact", None):
+            raise ValueError("renormalize must be 'fp64', 'exact' or None")
+        nxp, nyp = self.gather.nxp, self.gather.nyp
+        if momentum is None:
+            self.learner = HebbianUpdater(self.v1_loc, self.res_loc, nxp, nyp, dw_max,
+                                          normalize_dw=normalize_dw, device=self.device)
+        else:
+            self.learner = MomentumUpdater(self.v1_loc, self.res_loc, nxp, nyp, dw_max,
+                                           normalize_dw=normalize_dw, device=self.device, **momentum)
+        if self.learner.size != self.weights.numel():
+            raise abi.PVHIPError("the dictionary has %d weights, its updater %d"
+                                 % (self.weights.numel(), self.learner.size))
+        self.learn_arith = arith
+        self.renormalize = renormalize
+        self.strength = float(strength)
+        self._learn_graph = None
+
+    def update_weights(self, stream=None):
+        """One weight update of the dictionary (HebbianUpdater::updateState with
+        immediateWeightUpdate at a trigger step, HebbianUpdater.cpp:343-382,
+        822-833; NormalizeL2, NormalizeL2.cpp:36-120; the copies the deliveries
+        read), from the activities of the last step.  Device work only, so it
+        can be captured (capture_learning)."""
+        L = self.learner
+        L.update_dw(self.v1_activity(stream), self.res_A, clear=True, arith=self.learn_arith,
+                    stream=stream)
+        L.apply(self.weights, stream=stream)
+        npatch = self.v1_loc.nf
+        psize = self.weights.numel() // npatch
+        H = abi.hip()
+        if self.renormalize == "fp64":
+            abi.check(H.pvhip_normalize_l2_f64(C.c_void_p(self.weights.data_ptr()), npatch, psize,
+                                               self.strength, 0.0, _stream(stream)),
+                      "pvhip_normalize_l2_f64")
+        elif self.renormalize == "exact":
+            abi.check(H.pvhip_normalize_l2(C.c_void_p(self.weights.data_ptr()), npatch, psize,
+                                           self.strength, 0.0, _stream(stream)), "pvhip_normalize_l2")
+        self.refresh_weights(stream)
+
+    def refresh_weights(self, stream=None):
+        """Re-derive what the deliveries read from the dictionary: the gather's
+        BF16 / FP8 shadow, the scatter route's transposed weights and their
+        shadow (all on the device)."""
+        self.gather.refresh_weights(stream)
+        self.scatter.refresh_weights(stream)
+
+    def capture_learning(self, stream):
+        """Capture update_weights() as a hipGraph on `stream` (call it once
+        eagerly first, so the updater's scratch exists)."""
+        H = abi.hip()
+        with torch.cuda.stream(stream):
+            abi.check(H.pvhip_graph_begin(_stream(stream)), "pvhip_graph_begin")
+            try:
+                self.update_weights(stream)
+            finally:
+                g = C.c_void_p()
+                st = H.pvhip_graph_end(_stream(stream), C.byref(g))
+            abi.check(st, "pvhip_graph_end")
+        self._learn_graph = g
+        return g
+
+    def replay_learning(self, stream):
+        abi.check(abi.hip().pvhip_graph_launch(self._learn_graph, _stream(stream)),
+                  "pvhip_graph_launch")
+        self._a_stale = False
+
     def probe_energy(self, stream=None, cost="l1"):
         """ColumnEnergyProbe in FP64: E_b = 0.5*sum r^2 (L2NormProbe, exponent 2) +
         VThresh*cost(A) with cost "l1" (L1NormLCAProbe: sum |a|) or "firm"
@@ -278,6 +364,9 @@ class LCAColumn:
         if self._graph is not None:
             abi.hip().pvhip_graph_destroy(self._graph)
             self._graph = None
+        if getattr(self, "_learn_graph", None) is not None:
+            abi.hip().pvhip_graph_destroy(self._learn_graph)
+            self._learn_graph = None
 
     # host views -----------------------------------------------------------
     def v1_state(self):

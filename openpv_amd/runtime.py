@@ -344,18 +344,24 @@ class PresynapticPerspectiveGPUDelivery:
         return route
 
     def refresh_weights(self, stream=None):
-        """Re-derive the transposed weights after a weight change."""
+        """Re-derive the transposed weights (and their low-precision shadow)
+        after a weight change, on the device: WeightsPair::finalizeUpdate's
+        transposition as a permutation table (graph-capturable, no host copy)."""
         if self.transposed is None and getattr(self, "_route", None) is None:
             return
         route = self._route
-        w = self.weights.detach().float().cpu().numpy().ravel()
-        out = np.zeros(route.weights.numel(), np.float32)
-        r = abi.host().pvhost_transpose_weights(C.byref(self.pre), C.byref(self.post), self.nxp,
-                                                self.nyp, self.nfp, w.ctypes.data_as(C.c_void_p),
-                                                out.ctypes.data_as(C.c_void_p))
-        if r != 0:
-            raise abi.PVHIPError("pvhost_transpose_weights failed (%d)" % r)
-        route.weights.copy_(torch.as_tensor(out))
+        if getattr(self, "_tindex", None) is None:
+            n = route.weights.numel()
+            idx = np.zeros(n, np.int32)
+            r = abi.host().pvhost_transpose_weights_index(
+                C.byref(self.pre), C.byref(self.post), self.nxp, self.nyp, self.nfp,
+                idx.ctypes.data_as(C.c_void_p), n)
+            if r != 0:
+                raise abi.PVHIPError("pvhost_transpose_weights_index failed (%d)" % r)
+            self._tindex = _dev_i32(idx, route.weights.device)
+        abi.check(abi.hip().pvhip_permute_weights(_ptr(self.weights), _ptr(self._tindex),
+                                                  _ptr(route.weights), route.weights.numel(),
+                                                  _stream(stream)), "pvhip_permute_weights")
         route.refresh_weights(stream)
 
     def bind_pre_bf16c(self, buf):
@@ -624,14 +630,16 @@ _MOMENTUM_METHODS = {"viscosity": abi.MOMENTUM_VISCOSITY, "simple": abi.MOMENTUM
                      "alex": abi.MOMENTUM_ALEX}
 
 
-def momentum_factors(method, tau, decay=0.0, dw_max=1.0):
+def momentum_factors(method, tau, decay=0.0, dw_max=1.0, deprecated_tau=False):
     """(dwFactor, wFactor) of MomentumUpdater::applyMomentum for momentumMethod
     `method` ("viscosity", "simple", "alex" or its code); PVHIPError where the
-    reference's parameter checks reject the values."""
+    reference's parameter checks reject the values.  deprecated_tau: `tau` is
+    the deprecated momentumTau (applyMomentumDeprecated's factors, which the
+    reference does not range-check)."""
     code = _MOMENTUM_METHODS[method] if isinstance(method, str) else int(method)
     f, wf = C.c_float(), C.c_float()
-    abi.check(abi.hip().pvhip_momentum_factors(code, tau, decay, dw_max, C.byref(f), C.byref(wf)),
-              "pvhip_momentum_factors")
+    fn = "pvhip_momentum_factors_deprecated" if deprecated_tau else "pvhip_momentum_factors"
+    abi.check(getattr(abi.hip(), fn)(code, tau, decay, dw_max, C.byref(f), C.byref(wf)), fn)
     return f.value, wf.value
 
 
@@ -641,9 +649,9 @@ class MomentumUpdater(HebbianUpdater):
     decayed by the weights, prev_dW = dW, w += dW.  Parameters as the reference's
     .params keys: momentumMethod, timeConstantTau (its default per method:
     viscosity 100, simple 0.25, alex 0.9), momentumDecay, and momentumTau, the
-    deprecated spelling that selects the older blend (MomentumUpdater.cpp:346-360).
-    A momentumTau outside the range timeConstantTau allows for the method is
-    rejected here; the reference checks only timeConstantTau and accepts it."""
+    deprecated spelling that selects the older blend (MomentumUpdater.cpp:346-360),
+    whose factors the reference computes without a range check
+    (applyMomentumDeprecated, :333-343); only momentumDecay is checked then."""
 
     _TAU_DEFAULT = {"viscosity": 100.0, "simple": 0.25, "alex": 0.9}
 
@@ -658,7 +666,8 @@ class MomentumUpdater(HebbianUpdater):
         else:
             tau = self._TAU_DEFAULT[momentumMethod] if timeConstantTau is None else float(timeConstantTau)
         self.tau, self.decay = tau, float(momentumDecay)
-        self.dw_factor, self.w_factor = momentum_factors(momentumMethod, tau, self.decay, dw_max)
+        self.dw_factor, self.w_factor = momentum_factors(momentumMethod, tau, self.decay, dw_max,
+                                                         deprecated_tau=self.deprecated_tau)
         self.prev_dw = torch.zeros(self.size, dtype=torch.float32, device=self.dw.device)
 
     def apply(self, weights, stream=None):

@@ -121,6 +121,8 @@ def lib():
         L.or_momentum_factors.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float,
                                           C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.or_momentum_factors.restype = C.c_int
+        L.or_momentum_factors_deprecated.argtypes = L.or_momentum_factors.argtypes
+        L.or_momentum_factors_deprecated.restype = C.c_int
         L.or_momentum_apply.argtypes = [f32p, f32p, f32p, C.c_void_p, C.c_size_t, C.c_int,
                                         C.c_float, C.c_float, C.c_int]
         L.or_update_momentum_lca.argtypes = [LP, C.c_int, C.c_float, f64p, C.c_float, f32p, f32p,
@@ -248,11 +250,13 @@ def update_momentum_lca(loc, self_interact, momentum_rate, dt, tau, gsyn, A, pre
     return V, prev
 
 
-def momentum_factors(method, tau, decay=0.0, dw_max=1.0):
-    """(dwFactor, wFactor) of MomentumUpdater::applyMomentum; method 0 viscosity,
-    1 simple, 2 alex.  Raises ValueError where the reference rejects the parameters."""
+def momentum_factors(method, tau, decay=0.0, dw_max=1.0, deprecated_tau=False):
+    """(dwFactor, wFactor) of MomentumUpdater::applyMomentum (or, with
+    deprecated_tau, applyMomentumDeprecated); method 0 viscosity, 1 simple,
+    2 alex.  Raises ValueError where the reference rejects the parameters."""
     f, wf = C.c_float(), C.c_float()
-    if lib().or_momentum_factors(method, tau, decay, dw_max, C.byref(f), C.byref(wf)):
+    fn = lib().or_momentum_factors_deprecated if deprecated_tau else lib().or_momentum_factors
+    if fn(method, tau, decay, dw_max, C.byref(f), C.byref(wf)):
         raise ValueError("momentum parameters rejected")
     return f.value, wf.value
 

@@ -815,6 +815,20 @@ int or_momentum_factors(int method, float tau, float decay, float dwMax, float *
    }
 }
 
+int or_momentum_factors_deprecated(int method, float momentumTau, float decay, float dwMax,
+      float *dwFactor, float *wFactor) {
+   /* MomentumUpdater::applyMomentumDeprecated (MomentumUpdater.cpp:333-343):
+    * momentumTau as given, no range check; momentumDecay checked by
+    * ioParam_momentumDecay (:138-145). */
+   if (decay < 0.0f || decay > 1.0f) return 1;
+   switch (method) {
+      case 0: *dwFactor = expf(-1.0f / momentumTau); *wFactor = decay; return 0;
+      case 1: *dwFactor = momentumTau; *wFactor = decay; return 0;
+      case 2: *dwFactor = momentumTau; *wFactor = decay * dwMax; return 0;
+      default: return 1;
+   }
+}
+
 void or_momentum_apply(float *w, float *dw, float *prevDw, const int64_t *counts, size_t n,
       int normalizeDw, float dwFactor, float wFactor, int deprecatedTau) {
    /* HebbianUpdater::normalize_dW (:773-807), then MomentumUpdater::updateWeights

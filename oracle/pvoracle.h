@@ -135,6 +135,10 @@ void or_errscale(int n, float errScale, float *V);
  * (caller clears, as clear_dW / clearNumActivations do); counts may be NULL. */
 void or_hebbian_update_dw(const or_loc *pre, const or_loc *post, int nxp, int nyp, int nfp,
       float dWMax, const float *preActivity, const float *postActivity, float *dw, int64_t *counts);
+/* The factors of MomentumUpdater::applyMomentumDeprecated (MomentumUpdater.cpp:333-343)
+ * for the deprecated momentumTau: no range check on it. */
+int or_momentum_factors_deprecated(int method, float momentumTau, float decay, float dwMax,
+      float *dwFactor, float *wFactor);
 /* normalize_dW (:773-807) and updateWeights (:822-833). */
 void or_hebbian_normalize_apply(float *w, float *dw, const int64_t *counts, size_t n, int normalizeDw);
 int or_momentum_factors(int method, float tau, float decay, float dwMax, float *dwFactor,

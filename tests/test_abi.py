@@ -266,6 +266,15 @@ def test_transpose_weights_matches_oracle(case):
                                                W.ctypes.data_as(C.c_void_p),
                                                b.ctypes.data_as(C.c_void_p)) == 0
     assert np.array_equal(a, b, equal_nan=True)
+    # the permutation table of the device refresh gives the same weights
+    # (entries no pre weight reaches are 0 there, as in a zero-initialised copy)
+    idx = np.zeros(n, np.int32)
+    assert abi.host().pvhost_transpose_weights_index(C.byref(hp), C.byref(hq), nxp, nyp, oq.nf,
+                                                     idx.ctypes.data_as(C.c_void_p), n) == 0
+    perm = np.where(idx >= 0, W[np.maximum(idx, 0)], np.float32(0.0))
+    assert np.array_equal(perm, np.where(np.isnan(a), np.float32(0.0), a))
+    hit = idx[idx >= 0]
+    assert len(np.unique(hit)) == len(hit)      # a permutation: no pre weight used twice
 
 
 @pytest.mark.parametrize("case", [
@@ -314,3 +323,25 @@ def test_momentum_factors_host_only():
                 ("viscosity", 100.0, 2.0), (7, 0.5, 0.0)]:
         with pytest.raises(abi.PVHIPError):
             R.momentum_factors(*bad)
+
+
+def test_momentum_factors_of_the_deprecated_momentum_tau():
+    """momentumTau (deprecated, used when timeConstantTau is absent) takes the
+    factors of applyMomentumDeprecated without timeConstantTau's range check:
+    viscosity with a negative momentumTau, simple and alex with momentumTau >= 1
+    are accepted, as the reference accepts them; the closed forms
+    exp(-1/momentumTau), momentumTau and decay*dWMax hold; momentumDecay is still
+    checked.  The runtime's MomentumUpdater builds from such values."""
+    import oracle as O
+    from openpv_amd import runtime as R
+    for method, code, tau, decay, dwmax in [("viscosity", 0, -2.0, 0.0, 1.0), ("viscosity", 0, 4.0, 0.25, 1.0),
+                                            ("simple", 1, 1.5, 0.0, 1.0), ("alex", 2, 2.0, 0.5, 0.25)]:
+        got = R.momentum_factors(method, tau, decay, dwmax, deprecated_tau=True)
+        assert got == O.momentum_factors(code, tau, decay, dwmax, deprecated_tau=True)
+        want = np.float32(np.exp(np.float32(-1.0) / np.float32(tau))) if code == 0 else np.float32(tau)
+        assert got[0] == pytest.approx(float(want), rel=1e-6)
+        assert got[1] == pytest.approx(decay * (dwmax if code == 2 else 1.0))
+    with pytest.raises(abi.PVHIPError):
+        R.momentum_factors("simple", 1.5, 1.5, 1.0, deprecated_tau=True)
+    with pytest.raises(abi.PVHIPError):     # the same value as timeConstantTau is rejected
+        R.momentum_factors("simple", 1.5, 0.0, 1.0)
