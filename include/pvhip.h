@@ -488,6 +488,23 @@ int pvhip_active_compact(
       pvhip_stream stream);
 size_t pvhip_active_compact_scratch_bytes(const pvhip_loc *loc);
 
+/* pvhip_active_compact without a caller workspace: the per-chunk counts live
+ * in the library's own device memory (a fixed 256 KiB array per GPU, held
+ * until the process ends), so the call allocates nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under hipGraph capture, serving loops, harnesses that forbid hipMalloc).
+ * Limit: pvhip_active_compact_scratch_bytes(loc) <= 262144, i.e. nbatch times
+ * ceil(numExtended / 16384) at most 65536; larger requests return
+ * PVHIP_ERR_INVALID.  Every call on one GPU shares the workspace: calls that
+ * overlap on one GPU (different streams, concurrent host threads) race. */
+int pvhip_active_compact_noalloc(
+      const pvhip_loc *loc,
+      const float *A,
+      pvhip_sparse_entry *entries,
+      int64_t *num_active,
+      uint32_t *pos_start,
+      pvhip_stream stream);
+
 /* ------------------------------------------------------------------------ */
 /* Plasticity, normalisation, energy                                         */
 /* ------------------------------------------------------------------------ */

@@ -181,6 +181,7 @@ PVHIP_SYMBOLS = {
     "pvhip_activity_to_bf16c": (i32, [C.POINTER(Loc), vp, vp, vp]),
     "pvhip_ident_deliver": (i32, [C.POINTER(Loc), vp, vp, i32, vp]),
     "pvhip_active_compact": (i32, [C.POINTER(Loc), vp, vp, vp, vp, vp, vp]),
+    "pvhip_active_compact_noalloc": (i32, [C.POINTER(Loc), vp, vp, vp, vp, vp]),
     "pvhip_active_compact_scratch_bytes": (C.c_size_t, [C.POINTER(Loc)]),
     "pvhip_hebbian_dw": (i32, [C.POINTER(HebbianArgs), vp]),
     "pvhip_hebbian_dw_scratch_bytes": (C.c_size_t, [C.POINTER(HebbianArgs)]),

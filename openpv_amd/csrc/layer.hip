@@ -7,6 +7,8 @@
 // operation (the library is built with -ffp-contract=off).
 #include "pvhip_internal.hpp"
 
+#include <mutex>
+
 namespace pvhip {
 
 int make_lca_params(const pvhip_lca_update_args &a, LCAParams *p) {
@@ -700,6 +702,43 @@ int active_from_segments(
    return PVHIP_OK;
 }
 
+int compact_launch(const pvhip_loc &l, const float *A, pvhip_sparse_entry *entries, int64_t *num_active,
+      uint32_t *pos_start, uint32_t *chunk_scratch, pvhip_stream stream);
+
+// Workspace of pvhip_active_compact_noalloc (the compaction's per-chunk
+// counts) in the library's own device memory.  In host code `scratch` names a
+// host-side stand-in, not a device address: the address of each GPU's copy
+// comes from hipGetSymbolAddress with that GPU current, once per device, and
+// only that address is passed to kernels.
+constexpr size_t kNoAllocScratchBytes = 256 * 1024;
+constexpr int kMaxDevices             = 64;
+
+} // namespace pvhip
+
+static __device__ __align__(256) uint32_t scratch[pvhip::kNoAllocScratchBytes / sizeof(uint32_t)];
+
+namespace pvhip {
+namespace {
+std::mutex g_workspace_mu;
+uint32_t *g_workspace[kMaxDevices] = {};
+} // namespace
+
+// The current device's copy of `scratch` (looked up on its first use).
+int noalloc_workspace(uint32_t **out) {
+   int dev = -1;
+   PVHIP_TRY(hipGetDevice(&dev));
+   PVHIP_REQUIRE(dev >= 0 && dev < kMaxDevices, "noalloc workspace: device %d beyond %d", dev, kMaxDevices);
+   std::lock_guard<std::mutex> lock(g_workspace_mu);
+   if (g_workspace[dev] == nullptr) {
+      void *ptr = nullptr;
+      PVHIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      PVHIP_REQUIRE(ptr != nullptr, "noalloc workspace: no device address for device %d", dev);
+      g_workspace[dev] = static_cast<uint32_t *>(ptr);
+   }
+   *out = g_workspace[dev];
+   return PVHIP_OK;
+}
+
 } // namespace pvhip
 
 using namespace pvhip;
@@ -853,6 +892,36 @@ int pvhip_active_compact(
       uint32_t *pos_start, uint32_t *chunk_scratch, pvhip_stream stream) {
    PVHIP_REQUIRE(loc && A && entries && num_active, "active_compact: NULL argument");
    PVHIP_REQUIRE(loc_valid(*loc), "active_compact: invalid loc");
+   return compact_launch(*loc, A, entries, num_active, pos_start, chunk_scratch, stream);
+}
+
+int pvhip_active_compact_noalloc(
+      const pvhip_loc *loc, const float *A, pvhip_sparse_entry *entries, int64_t *num_active,
+      uint32_t *pos_start, pvhip_stream stream) {
+   PVHIP_REQUIRE(loc && A && entries && num_active, "active_compact_noalloc: NULL argument");
+   PVHIP_REQUIRE(loc_valid(*loc), "active_compact_noalloc: invalid loc");
+   size_t const need = compact_scratch_bytes(geo(*loc));
+   PVHIP_REQUIRE(need <= kNoAllocScratchBytes,
+         "active_compact_noalloc: %d batch elements of %d extended neurons need %zu bytes of chunk counts, "
+         "beyond the library's %zu-byte workspace (use pvhip_active_compact with a caller workspace)",
+         loc->nbatch, geo(*loc).numExt(), need, kNoAllocScratchBytes);
+   uint32_t *ws = nullptr;
+   int st       = noalloc_workspace(&ws);
+   if (st) {
+      return st;
+   }
+   // every chunk count is written by compact_count_kernel before the scan
+   // reads it, so nothing left by an earlier call needs clearing
+   return compact_launch(*loc, A, entries, num_active, pos_start, ws, stream);
+}
+
+} // extern "C"
+
+namespace pvhip {
+
+int compact_launch(const pvhip_loc &l, const float *A, pvhip_sparse_entry *entries, int64_t *num_active,
+      uint32_t *pos_start, uint32_t *chunk_scratch, pvhip_stream stream) {
+   const pvhip_loc *loc = &l;
    Geo g              = geo(*loc);
    int const numExt   = g.numExt();
    int const chunks   = (numExt + kChunk - 1) / kChunk;
@@ -885,6 +954,10 @@ int pvhip_active_compact(
    PVHIP_LAUNCHED("compact_write");
    return PVHIP_OK;
 }
+
+} // namespace pvhip
+
+extern "C" {
 
 int pvhip_cost_sum_f64(
       const pvhip_loc *loc, const float *buffer, int extended, int kind, double p0, double p1,
