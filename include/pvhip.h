@@ -409,6 +409,17 @@ int pvhip_momentum_lca_update(
       const pvhip_lca_update_args *args, float momentum_rate, float *prev_drive,
       pvhip_stream stream);
 
+/* ISTALayer V update (ISTAInternalStateBuffer.kpp:34-57, the reference's K9
+ * updateISTAInternalStateBufferOnGPU) with the GSyn accumulation and the
+ * transfer fused as in pvhip_lca_update:
+ *   V += rate[b]*(G - vthresh*sign(A[kex])),  A[kex] = transfer(V)
+ * where args->exp_tau holds the per-batch rates (float)dt[b]/tau of
+ * pvhip_ista_rate (tau = timeConstantTau/dt) and vthresh is the layer's
+ * VThresh (ANNActivityBuffer::getVThresh).  self_interact is not used. */
+int pvhip_ista_update(const pvhip_lca_update_args *args, float vthresh, pvhip_stream stream);
+/* Host helper: rate_out[b] = (float)dt[b] / tau (ISTAInternalStateBuffer.kpp:56). */
+void pvhip_ista_rate(const double *dt, int nbatch, float tau, float *rate_out);
+
 /* GSynAccumulator (src/components/GSynAccumulator.kpp:21-42):
  * out[i] = sum_ch coeff[ch]*gsyn[ch*n + i], starting from 0.0f. */
 int pvhip_gsyn_accumulate(

@@ -174,6 +174,8 @@ PVHIP_SYMBOLS = {
     "pvhip_recv_pre": (i32, [C.POINTER(RecvPreArgs), vp]),
     "pvhip_lca_update": (i32, [C.POINTER(LCAUpdateArgs), vp]),
     "pvhip_momentum_lca_update": (i32, [C.POINTER(LCAUpdateArgs), C.c_float, vp, vp]),
+    "pvhip_ista_update": (i32, [C.POINTER(LCAUpdateArgs), C.c_float, vp]),
+    "pvhip_ista_rate": (None, [C.POINTER(C.c_double), i32, f32, C.POINTER(f32)]),
     "pvhip_gsyn_accumulate": (i32, [i32, i32, C.POINTER(f32), vp, vp, vp]),
     "pvhip_layer_update": (i32, [C.POINTER(LayerUpdateArgs), vp]),
     "pvhip_ann_activity": (i32, [C.POINTER(Loc), C.POINTER(Transfer), vp, vp, vp]),

@@ -126,6 +126,44 @@ __global__ void __launch_bounds__(kRowThreads) momentum_lca_update_kernel(
    }
 }
 
+// ISTA V update (ISTAInternalStateBuffer.kpp:34-57, the reference's K9) with
+// the GSyn accumulation and the transfer fused as in lca_update_kernel:
+//   V += rate[b]*(G - VThresh*sign(A(t-1)[kex])),  rate[b] = (float)dt[b]/tau,
+// in the reference's operation order; A(t-1) recomputed from V when aFromV.
+__global__ void __launch_bounds__(kRowThreads) ista_update_kernel(LCAParams p, float vthresh) {
+   int const numRes = p.loc.numRes();
+   int const b      = blockIdx.y;
+   size_t const n   = (size_t)p.loc.nbatch * numRes;
+   float const rate = p.expTau[b];
+#pragma unroll
+   for (int u = 0; u < kRowIter; u++) {
+      int const k = blockIdx.x * kRowSpan + u * kRowThreads + threadIdx.x;
+      if (k < numRes) {
+         size_t const vi = (size_t)b * numRes + k;
+         size_t const ai = (size_t)b * p.loc.numExt() + p.loc.kex(k);
+         float const G    = accumulate_channels(p.numChannels, p.coeffs, p.gsyn, n, vi);
+         float const Vold = p.V[vi];
+         float const Aold = p.aFromV ? apply_transfer(p.transfer, Vold) : p.A[ai];
+         float sign       = 0.0f;
+         if (Aold != 0.0f) {
+            sign = Aold / fabsf(Aold);
+         }
+         float thr   = vthresh * sign;
+         float drive = G - thr;
+         float step  = rate * drive;
+         float Vnew  = Vold + step;
+         float Anew  = apply_transfer(p.transfer, Vnew);
+         p.V[vi]     = Vnew;
+         if (p.A) {
+            p.A[ai] = Anew;
+         }
+         if (p.Ab) {
+            p.Ab[bf16c_index(p.loc, b, p.loc.kex(k))] = (__bf16)Anew;
+         }
+      }
+   }
+}
+
 struct LayerParams {
    Geo loc;
    int numChannels;
@@ -773,6 +811,26 @@ int pvhip_momentum_lca_update(
          (hipStream_t)stream, p, momentum_rate, prev_drive);
    PVHIP_LAUNCHED("momentum_lca_update");
    return PVHIP_OK;
+}
+
+int pvhip_ista_update(const pvhip_lca_update_args *args, float vthresh, pvhip_stream stream) {
+   PVHIP_REQUIRE(args != nullptr, "ista_update: NULL args");
+   LCAParams p;
+   int st = make_lca_params(*args, &p);
+   if (st) {
+      return st;
+   }
+   PVHIP_REQUIRE(row_grid_ok(p.loc), "ista_update: nbatch too large");
+   hipLaunchKernelGGL(ista_update_kernel, row_grid(p.loc), dim3(kRowThreads), 0, (hipStream_t)stream, p,
+         vthresh);
+   PVHIP_LAUNCHED("ista_update");
+   return PVHIP_OK;
+}
+
+void pvhip_ista_rate(const double *dt, int nbatch, float tau, float *rate_out) {
+   for (int b = 0; b < nbatch; b++) {
+      rate_out[b] = (float)dt[b] / tau; // ISTAInternalStateBuffer.kpp:56
+   }
 }
 
 int pvhip_activity_to_bf16c(const pvhip_loc *loc, const float *A, void *out, pvhip_stream stream) {

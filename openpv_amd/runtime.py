@@ -478,6 +478,31 @@ class MomentumLCAUpdate(HyPerLCAUpdate):
                   "pvhip_momentum_lca_update")
 
 
+class ISTAUpdate(HyPerLCAUpdate):
+    """ISTALayer update (ISTAInternalStateBuffer.kpp:34-57):
+    V += (dt/tau)*(G - VThresh*sign(A)), then the transfer; VThresh is the
+    layer's (its ANNActivityBuffer's)."""
+
+    def __init__(self, loc_, transfer, VThresh, timeConstantTau=1.0, dt=1.0, channel_coeffs=(1.0,),
+                 activity_from_v=False, device="cuda"):
+        super().__init__(loc_, transfer, timeConstantTau, dt, True, channel_coeffs, activity_from_v,
+                         device)
+        self.vthresh = float(VThresh)
+        self.set_dt(np.full(loc_.nbatch, dt))
+
+    def set_dt(self, dt):
+        """Per-batch rates (float)dt/tau (pvhip_ista_rate)."""
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float64), (self.loc.nbatch,)))
+        out = (C.c_float * self.loc.nbatch)()
+        abi.hip().pvhip_ista_rate(dt.ctypes.data_as(C.POINTER(C.c_double)), self.loc.nbatch, self.tau, out)
+        self.exp_tau.copy_(torch.as_tensor(np.ctypeslib.as_array(out).copy()))
+
+    def update(self, gsyn, V, A, stream=None, A_bf16c=None):
+        a = self.args(gsyn, V, A, A_bf16c=A_bf16c)
+        abi.check(abi.hip().pvhip_ista_update(C.byref(a), self.vthresh, _stream(stream)),
+                  "pvhip_ista_update")
+
+
 def bf16c_buffer(loc_, device="cuda"):
     """A zeroed chunked BF16 activity buffer for layer loc_ (nf % 32 == 0)."""
     if loc_.nf % 32:

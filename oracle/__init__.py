@@ -128,6 +128,7 @@ def lib():
         L.or_update_momentum_lca.argtypes = [LP, C.c_int, C.c_float, f64p, C.c_float, f32p, f32p,
                                              f32p, f32p]
         L.or_normalize_l2.argtypes = [f32p, C.c_int, C.c_int, C.c_float, C.c_float]
+        L.or_update_ista.argtypes = [LP, C.c_float, f64p, C.c_float, f32p, f32p, f32p]
         L.or_normalize_l2_f64.argtypes = [f32p, C.c_int, C.c_int, C.c_double, C.c_double]
         L.or_cost_sum.argtypes = [LP, f32p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
         L.or_cost_sum.restype = C.c_double
@@ -158,6 +159,7 @@ def ref():
         R.ref_update_momentum_lca.argtypes = [i32p, C.c_int, C.c_float, f64p, C.c_float, f32p,
                                               f32p, f32p, f32p]
         R.ref_apply_vertices.argtypes = [i32p, C.c_int, f32p, f32p, f32p, f32p, f32p]
+        R.ref_update_ista.argtypes = [i32p, C.c_float, f64p, C.c_float, f32p, f32p, f32p]
         R.ref_gsyn_accumulate.argtypes = [C.c_int, C.c_int, f32p, f32p, f32p]
         R.ref_cost.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
         R.ref_cost.restype = C.c_double
@@ -237,6 +239,14 @@ def update_lca(loc, self_interact, dt, tau, gsyn, A, V):
     V = V.copy()
     lib().or_update_lca(C.byref(loc), int(self_interact), np.asarray(dt, np.float64), tau,
                         np.ascontiguousarray(gsyn, np.float32), np.ascontiguousarray(A, np.float32), V)
+    return V
+
+
+def update_ista(loc, VThresh, dt, tau, gsyn, A, V):
+    """ISTA V update (ISTAInternalStateBuffer.kpp); returns the new V."""
+    V = np.array(V, np.float32)
+    lib().or_update_ista(C.byref(loc), VThresh, np.asarray(dt, np.float64), tau,
+                         np.ascontiguousarray(gsyn, np.float32), np.ascontiguousarray(A, np.float32), V)
     return V
 
 

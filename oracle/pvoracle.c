@@ -881,6 +881,28 @@ void or_update_momentum_lca(const or_loc *loc, int selfInteract, float momentumR
    }
 }
 
+void or_update_ista(const or_loc *loc, float VThresh, const double *dtAdapt, float tau,
+      const float *accumulatedGSyn, const float *A, float *V) {
+   /* ISTAInternalStateBuffer.kpp:34-57: V += (dt/tau)*(G - VThresh*sign(A)) */
+   int const numNeurons = locNumRes(loc);
+   int const numExt     = locNumExt(loc);
+   for (int index = 0; index < numNeurons * loc->nbatch; index++) {
+      int b = index / numNeurons;
+      int k = index % numNeurons;
+      float *VBatch       = V + (size_t)b * numNeurons;
+      float const gSyn    = accumulatedGSyn[(size_t)b * numNeurons + k];
+      float const *ABatch = A + (size_t)b * numExt;
+      int kex = or_kIndexExtended(k, loc->nx, loc->ny, loc->nf, loc->lt, loc->rt, loc->dn, loc->up);
+      float sign = 0.0f;
+      if (ABatch[kex] != 0.0f) {
+         sign = ABatch[kex] / fabsf(ABatch[kex]);
+      }
+      float const rate = (float)dtAdapt[b] / tau;
+      float const drive = gSyn - VThresh * sign;
+      VBatch[k] += rate * drive;
+   }
+}
+
 void or_normalize_l2(float *w, int numPatches, int patchSize, float strength, float minL2) {
    /* NormalizeL2::normalizeWeights non-individual-arbor branch, NormalizeL2.cpp:93-118 */
    float scaleFactor = 1.0f * strength;

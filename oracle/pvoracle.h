@@ -148,6 +148,10 @@ void or_momentum_apply(float *w, float *dw, float *prevDw, const int64_t *counts
 void or_update_momentum_lca(const or_loc *loc, int selfInteract, float momentumRate,
       const double *dtAdapt, float tau, const float *accumulatedGSyn, const float *A,
       float *prevDrive, float *V);
+/* ISTAInternalStateBuffer.kpp:34-57 (the reference's K9 updateISTAInternalStateBufferOnGPU):
+ * V += ((float)dt[b]/tau)*(G - VThresh*sign(A[kex])), tau = timeConstantTau/dt. */
+void or_update_ista(const or_loc *loc, float VThresh, const double *dtAdapt, float tau,
+      const float *accumulatedGSyn, const float *A, float *V);
 /* NormalizeL2::normalizeWeights, one arbor, one connection, not from post perspective
  * (src/normalizers/NormalizeL2.cpp:36-120; NormalizeBase.cpp:184-193). FP32 like the reference. */
 void or_normalize_l2(float *w, int numPatches, int patchSize, float strength, float minL2);

@@ -7,6 +7,7 @@
  *   - PV::PatchGeometry        (src/components/PatchGeometry.cpp)
  *   - updateHyPerLCAOnCPU      (src/components/HyPerLCAInternalStateBuffer.kpp)
  *   - updateMomentumLCAOnCPU   (src/components/MomentumLCAInternalStateBuffer.kpp)
+ *   - updateISTAInternalStateBufferOnCPU (src/components/ISTAInternalStateBuffer.kpp)
  *   - applyVerticesANNActivityBufferOnCPU (src/components/ANNActivityBuffer.kpp)
  *   - updateGSynAccumulatorOnCPU (src/components/GSynAccumulator.kpp)
  *   - requiredConvolveMargin   (src/utils/requiredConvolveMargin.hpp)
@@ -25,6 +26,7 @@
 #include "components/GSynAccumulator.kpp"
 #include "components/HyPerLCAInternalStateBuffer.kpp"
 #include "components/MomentumLCAInternalStateBuffer.kpp"
+#include "components/ISTAInternalStateBuffer.kpp"
 
 #include <cstdint>
 #include <cstring>
@@ -104,6 +106,12 @@ void ref_update_momentum_lca(const int *loc, int selfInteract, float momentumRat
    PV::updateMomentumLCAOnCPU(loc[0], loc[1] * loc[2] * loc[3], loc[1], loc[2], loc[3], loc[4],
          loc[5], loc[6], loc[7], selfInteract != 0, momentumRate, dtAdapt, tau, gsyn, A,
          prevDrive, V);
+}
+
+void ref_update_ista(const int *loc, float VThresh, const double *dtAdapt, float tau,
+      const float *gsyn, const float *A, float *V) {
+   PV::updateISTAInternalStateBufferOnCPU(loc[0], loc[1] * loc[2] * loc[3], loc[1], loc[2], loc[3],
+         loc[4], loc[5], loc[6], loc[7], VThresh, dtAdapt, tau, gsyn, A, V);
 }
 
 void ref_apply_vertices(const int *loc, int numVertices, const float *vV, const float *vA,

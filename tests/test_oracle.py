@@ -569,6 +569,49 @@ def test_momentum_lca_matches_reference(self_interact):
         assert np.array_equal(prev.view(np.uint32), prevr.view(np.uint32))
 
 
+@needs_ref
+def test_ista_update_matches_reference():
+    """The oracle's ISTA V update against the reference's own
+    updateISTAInternalStateBufferOnCPU (ISTAInternalStateBuffer.kpp), bit for
+    bit over three steps, with activity of both signs and zeros, and per-batch
+    time scales."""
+    rng = np.random.default_rng(43)
+    loc = O.Loc.make(3, 7, 5, 4, (1, 2, 2, 1))
+    n = loc.nbatch * loc.num_res
+    V = rng.standard_normal(n).astype(np.float32)
+    Vr = V.copy()
+    dt = np.array([1.0, 0.5, 0.3])
+    for _ in range(3):
+        G = rng.standard_normal(n).astype(np.float32)
+        A = rng.standard_normal(loc.nbatch * loc.num_ext).astype(np.float32)
+        A[rng.random(A.size) < 0.4] = 0.0
+        V = O.update_ista(loc, 0.25, dt, 100.0, G, A, V)
+        O.ref().ref_update_ista(np.array(loc.as_list(), np.int32), 0.25, dt, 100.0, G, A, Vr)
+        assert np.array_equal(V.view(np.uint32), Vr.view(np.uint32))
+
+
+def test_ista_update_closed_forms():
+    """Where A = 0 the step is V + (dt/tau)*G; where G = VThresh*sign(A) it is a
+    fixed point; a sign error (+VThresh*sign) would move those points."""
+    loc = O.Loc.make(1, 4, 3, 2, 1)
+    n = loc.num_res
+    rng = np.random.default_rng(44)
+    V = rng.standard_normal(n).astype(np.float32)
+    A = np.zeros(loc.num_ext, np.float32)
+    G = rng.standard_normal(n).astype(np.float32)
+    dt = np.array([2.0])
+    got = O.update_ista(loc, 0.5, dt, 10.0, G, A, V)
+    rate = np.float32(np.float32(2.0) / np.float32(10.0))
+    assert np.array_equal(got.view(np.uint32), (V + rate * G).astype(np.float32).view(np.uint32))
+    # activity of both signs at every restricted neuron, G = VThresh*sign(A)
+    Aext = np.zeros((loc.ny_ext, loc.nx_ext, loc.nf), np.float32)
+    signs = np.where(rng.random((loc.ny, loc.nx, loc.nf)) < 0.5, -1.0, 1.0).astype(np.float32)
+    Aext[1:1 + loc.ny, 1:1 + loc.nx] = signs * rng.uniform(0.1, 2.0, signs.shape).astype(np.float32)
+    G = (np.float32(0.5) * signs).ravel()
+    got = O.update_ista(loc, 0.5, dt, 10.0, G, Aext.ravel(), V)
+    assert np.array_equal(got.view(np.uint32), V.view(np.uint32))
+
+
 def test_momentum_lca_reduces_to_lca():
     """Rate 0 is the HyPerLCA update bit for bit; the drive it stores is that
     update from V = 0; a rate r adds r*prevDrive as one rounded term."""
