@@ -138,6 +138,9 @@ size_t pvhip_buffer_size(pvhip_buffer *buf);
 
 /* Zero-fill with a kernel (graph-capturable; no hipMemsetAsync node). */
 int pvhip_fill_zero(void *dptr, size_t bytes, pvhip_stream stream);
+/* Device-to-device copy on a stream: the Publisher's copy of a layer's
+ * activity into a DataStore level (src/columns/Publisher.cpp:104-135). */
+int pvhip_copy_device(void *dst, const void *src, size_t bytes, pvhip_stream stream);
 /* hipStreamSynchronize, then the check of pvhip_check_faults. */
 int pvhip_stream_sync(pvhip_stream stream);
 /* Waits for the device, then reports (and clears) tiles a delivery kernel

@@ -155,6 +155,7 @@ PVHIP_SYMBOLS = {
     "pvhip_fill_zero": (i32, [vp, C.c_size_t, vp]),
     "pvhip_stream_sync": (i32, [vp]),
     "pvhip_check_faults": (i32, []),
+    "pvhip_copy_device": (i32, [vp, vp, C.c_size_t, vp]),
     "pvhip_event_create": (i32, [C.POINTER(vp)]),
     "pvhip_event_destroy": (i32, [vp]),
     "pvhip_event_record": (i32, [vp, vp]),

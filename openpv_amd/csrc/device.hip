@@ -254,6 +254,15 @@ int pvhip_fill_zero(void *dptr, size_t bytes, pvhip_stream stream) {
    return PVHIP_OK;
 }
 
+int pvhip_copy_device(void *dst, const void *src, size_t bytes, pvhip_stream stream) {
+   PVHIP_REQUIRE(dst && src, "copy_device: NULL buffer");
+   if (bytes == 0) {
+      return PVHIP_OK;
+   }
+   PVHIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+   return PVHIP_OK;
+}
+
 int pvhip_stream_sync(pvhip_stream stream) {
    PVHIP_TRY(hipStreamSynchronize((hipStream_t)stream));
    return pvhip::check_faults();

@@ -5,8 +5,9 @@
 //   pvrun -p file.params [-o outputPath] [--steps N] [--autotune]
 //
 // Supported objects (the LCA column of tests/GPULCATest and LCATest):
-//   HyPerCol; PvpLayer, ANNErrorLayer, HyPerLCALayer, MomentumLCALayer, ANNLayer, HyPerLayer;
-//   IdentConn, HyPerConn (FileWeight, normalizeL2), TransposeConn, CloneConn;
+//   HyPerCol; PvpLayer, ANNErrorLayer, HyPerLCALayer, MomentumLCALayer, ISTALayer, ANNLayer,
+//   HyPerLayer; IdentConn, HyPerConn and MomentumConn (FileWeight or IdentWeight,
+//   normalizeL2, plasticity), TransposeConn, CloneConn; connection delays;
 //   AdaptiveTimeScaleProbe, ColumnEnergyProbe, L2NormProbe, L1NormProbe,
 //   L1NormLCAProbe, FirmThresholdCostFnLCAProbe, RequireAllZeroActivityProbe.
 // Delivery selection follows HyPerDeliveryCreator (HyPerDeliveryCreator.cpp:41-126):
@@ -137,6 +138,39 @@ void writeActivity(const std::string &path, int nx, int ny, int nf,
    }
 }
 
+// Shared weights (type 5), one arbor, one frame per timestamp: each frame is a
+// weight header (BufferUtils::buildSharedWeightHeader, BufferUtilsPvp.cpp:12-98:
+// nx, ny, nf = the data patches along x, y, f; minVal / maxVal over the
+// weights) followed by every patch as {u16 nxp, u16 nyp, u32 offset 0}
+// (unshrunken) and its nxp*nyp*nfp floats (WeightsFileIO::writeSharedWeights,
+// WeightsFileIO.cpp:268-298).
+void writeWeights(const std::string &path, int nkx, int nky, int nkf, int nxp, int nyp, int nfp,
+      const std::vector<std::vector<float>> &frames, const std::vector<double> &times) {
+   std::ofstream f(path, std::ios::binary);
+   size_t psize = (size_t)nxp * nyp * nfp;
+   int32_t npatch = nkx * nky * nkf;
+   for (size_t i = 0; i < frames.size(); i++) {
+      const std::vector<float> &w = frames[i];
+      float mn = w.empty() ? 0.0f : *std::min_element(w.begin(), w.end());
+      float mx = w.empty() ? 0.0f : *std::max_element(w.begin(), w.end());
+      int32_t h[18] = {104, 26, 5, nkx, nky, nkf, 1, 0, 4, 3, 1, 1, nkx, nky, 0, 0, 1, 1};
+      f.write(reinterpret_cast<const char *>(h), sizeof h);
+      f.write(reinterpret_cast<const char *>(&times[i]), sizeof(double));
+      int32_t pq[3] = {nxp, nyp, nfp};
+      f.write(reinterpret_cast<const char *>(pq), sizeof pq);
+      f.write(reinterpret_cast<const char *>(&mn), 4);
+      f.write(reinterpret_cast<const char *>(&mx), 4);
+      f.write(reinterpret_cast<const char *>(&npatch), 4);
+      for (int32_t p = 0; p < npatch; p++) {
+         uint16_t dims[2] = {(uint16_t)nxp, (uint16_t)nyp};
+         uint32_t off     = 0;
+         f.write(reinterpret_cast<const char *>(dims), sizeof dims);
+         f.write(reinterpret_cast<const char *>(&off), sizeof off);
+         f.write(reinterpret_cast<const char *>(w.data() + p * psize), psize * 4);
+      }
+   }
+}
+
 // ------------------------------------------------------------------ device memory
 struct Device {
    pvhip_device *dev = nullptr;
@@ -232,6 +266,7 @@ struct Layer {
    float tau = 1.0f;
    bool selfInteract = true;
    bool momentum = false;          // MomentumLCALayer
+   bool ista     = false;          // ISTALayer (ISTAInternalStateBuffer)
    float momentumRate = 0.0f;      // LCAMomentumRate
    float *prevDrive = nullptr;     // restricted, zero at start
    std::string adaptiveProbe;
@@ -241,6 +276,12 @@ struct Layer {
    PvpFrames input;
    int displayPeriod = 0, frameIndex = 0;
    double lastUpdate = 1.0;
+   bool mirrorBC = false;
+   // DataStore levels beyond 0 (src/columns/DataStore.cpp:18-38): ring[(head + d - 1) % D]
+   // holds the activity published d steps before the live buffer A; allocated
+   // when a connection reads this layer with a delay
+   int levels = 1, head = 0;
+   std::vector<float *> ring;
    // sparse publish
    bool sparse = false, needActive = false;
    pvhip_sparse_entry *entries = nullptr;
@@ -276,6 +317,38 @@ struct Conn {
    size_t scratchBytes = 0;
    int tile = 0, splits = 0; // channel-GEMM launch shape (0: the library's default)
    float srcScale = 0.0f;    // FP8 gather: power-of-two source scale
+   // a delivery whose weights derive from a plastic original: dev weights =
+   // permutation of the original's (index table; NULL when it reads them directly)
+   int32_t *perm = nullptr;
+   size_t permN  = 0;
+   // plasticity of an original (HebbianUpdater / MomentumUpdater)
+   bool plastic = false, immediate = true, normalizeDw = true, momentum = false, deprecatedTau = false;
+   float dWMax = 0.0f, dWMaxDecayFactor = 0.0f, dwFactor = 0.0f, wFactor = 0.0f;
+   int dWMaxDecayInterval = 0, dWMaxDecayTimer = 0;
+   std::string triggerLayer;
+   double triggerOffset = 0.0, updatePeriod = 0.0, lastUpdateTime = 0.0, weightUpdateTime = 0.0;
+   int dwArith = PVHIP_ARITH_EXACT;
+   int delay = 0; // delay in timesteps (ArborList::convertDelay)
+   pvhip_sparse_entry *dEntries = nullptr; // active list of the delayed level (sparse scatter)
+   int64_t *dNumActive = nullptr;
+   uint32_t *dPosStart = nullptr, *dScratch = nullptr;
+   float *dW = nullptr, *prevDw = nullptr;
+   int64_t *counts = nullptr;
+   int32_t *hebbTab[6] = {};
+   int hebbWin[4] = {};
+   void *hebbScratch = nullptr;
+   size_t hebbScratchBytes = 0;
+   bool updatedNow = false; // weights changed this timestep (NormalizeBase::weightsHaveUpdated)
+   int lpKernels = 0, lpPatch = 0; // shape of the low-precision shadow of a gather's weights
+   int momentumCode = 0;
+   float momentumTau = 0.0f, momentumDecay = 0.0f;
+   // NormalizeL2 of an original
+   bool normalizeL2 = false, normalizeOnWeightUpdate = true;
+   float strength = 1.0f, minL2 = 0.0f;
+   // weight output of an original (WeightsPair::outputState): writeStep >= 0
+   double writeStep = -1.0, writeTime = 0.0;
+   std::vector<std::vector<float>> outFrames;
+   std::vector<double> outTimes;
 };
 
 struct Probe {
@@ -302,7 +375,14 @@ struct Net {
    Device D;
    int nxGlobal = 0, nyGlobal = 0, nbatch = 1;
    double stopTime = 1, dtCol = 1;
+   // Checkpointer (src/checkpointing/Checkpointer.cpp): checkpointWrite with
+   // checkpointWriteTriggerMode "step", initializeFromCheckpointDir
+   bool checkpointWrite = false;
+   std::string checkpointWriteDir, initFromCheckpointDir;
+   long checkpointInterval = 1;
+   int checkpointIndexWidth = -1;
    std::string outputPath, gpuArith = "fp32"; // outputPath: -o, else the HyPerCol's
+   std::string gpuDwArith = "exact";          // dW correlation form of plastic connections
    std::vector<Layer> layers;
    std::vector<Conn> conns;
    std::vector<Probe> probes;
@@ -344,9 +424,26 @@ struct Net {
    void receive(Layer &L);
    void updateLayer(Layer &L, double t);
    void checkAllZero(Probe &p, double t);
+   void loadInputFrames(Layer &L);
+   float *levelData(Layer &L, int delay);
+   void advanceDataStore(Layer &L);
+   void readPlasticity(Conn &C);
+   void momentumFactors(Conn &C);
+   pvhip_hebbian_args hebbianArgs(Conn &C);
+   bool needWeightUpdate(Conn &C, double t);
+   void computeDw(Conn &C);
+   void applyDw(Conn &C);
+   void updateConnections(double t);
+   void refreshDerived(Conn &O);
    std::vector<double> probeEnergy(Probe &p);
    void output(double t);
    int finish();
+   std::string checkpointDir(long step, long finalStep) const;
+   void writeCheckpoint(const std::string &dir, double simTime, long step);
+   double readCheckpoint(const std::string &dir);
+   std::vector<std::vector<float>> restrictedFrames(Layer &L, const float *ext, bool extended);
+   void setRestrictedFrames(Layer &L, float *buf, bool extended, const std::vector<std::vector<float>> &frames,
+         size_t first);
 };
 
 // ANNActivityBuffer::ioParam_VThresh .. ioParam_VWidth (ANNActivityBuffer.cpp:136-164):
@@ -396,20 +493,41 @@ void Net::build() {
    if (c.has("gpuArithmetic")) {
       gpuArith = c.str("gpuArithmetic");
    }
+   if (c.has("dwArithmetic")) {
+      gpuDwArith = c.str("dwArithmetic");
+   }
+   checkpointWrite = c.num("checkpointWrite", 0) != 0;
+   if (checkpointWrite) {
+      checkpointWriteDir = c.str("checkpointWriteDir");
+      if (checkpointWriteDir.empty()) {
+         fatal("checkpointWrite is set, so checkpointWriteDir is required");
+      }
+      std::string mode = c.str("checkpointWriteTriggerMode", "step");
+      if (mode != "step" && mode != "Step" && mode != "STEP") {
+         fatal("checkpointWriteTriggerMode \"" + mode + "\" is not supported (step)");
+      }
+      checkpointInterval = (long)c.num("checkpointWriteStepInterval", 1);
+      if (checkpointInterval <= 0) {
+         fatal("checkpointWriteStepInterval must be positive");
+      }
+      checkpointIndexWidth = (int)c.num("checkpointIndexWidth", -1);
+   }
+   initFromCheckpointDir = c.str("initializeFromCheckpointDir");
    if (dtCol != 1.0) {
       fatal("only dt = 1 is supported");
    }
    for (int g = 0; g < pvparams_num_groups(P); g++) {
       std::string kw = pvparams_group_keyword(P, g);
       Params q{P, g};
-      if (kw == "PvpLayer" || kw == "ANNErrorLayer" || kw == "HyPerLCALayer"
-          || kw == "MomentumLCALayer" || kw == "ANNLayer" || kw == "HyPerLayer") {
+      if (kw == "PvpLayer" || kw == "ANNErrorLayer" || kw == "HyPerLCALayer" || kw == "MomentumLCALayer"
+          || kw == "ISTALayer" || kw == "ANNLayer" || kw == "HyPerLayer") {
          Layer L;
          L.name  = pvparams_group_name(P, g);
          L.prm   = q;
          L.kind  = kw == "PvpLayer" ? L_INPUT : kw == "ANNErrorLayer" ? L_ERROR
-                 : kw == "HyPerLCALayer" || kw == "MomentumLCALayer" ? L_LCA : L_ANN;
+                 : kw == "HyPerLCALayer" || kw == "MomentumLCALayer" || kw == "ISTALayer" ? L_LCA : L_ANN;
          L.momentum = kw == "MomentumLCALayer";
+         L.ista     = kw == "ISTALayer";
          L.phase = (int)q.num("phase", 0);
          L.loc.nbatch = nbatch;
          L.loc.nx     = (int)std::lround(nxGlobal * q.num("nxScale", 1.0));
@@ -418,15 +536,18 @@ void Net::build() {
          L.sparse     = q.num("sparseLayer", 0) != 0;
          L.writeStep  = (int)q.num("writeStep", -1);
          L.initialWriteTime = q.num("initialWriteTime", 0);
-         if (q.num("mirrorBCflag", 0) != 0 || q.num("valueBC", 0) != 0) {
-            fatal(L.name + ": mirrorBCflag and a nonzero valueBC are not supported");
+         L.mirrorBC = q.num("mirrorBCflag", 0) != 0;
+         if (q.num("valueBC", 0) != 0) {
+            fatal(L.name + ": a nonzero valueBC is not supported");
          }
          layers.push_back(std::move(L));
       }
-      else if (kw == "IdentConn" || kw == "HyPerConn" || kw == "TransposeConn" || kw == "CloneConn") {
+      else if (kw == "IdentConn" || kw == "HyPerConn" || kw == "MomentumConn" || kw == "TransposeConn"
+               || kw == "CloneConn") {
          Conn C;
-         C.name    = pvparams_group_name(P, g);
-         C.keyword = kw;
+         C.name     = pvparams_group_name(P, g);
+         C.keyword  = kw == "MomentumConn" ? "HyPerConn" : kw; // a HyPerConn with a MomentumUpdater
+         C.momentum = kw == "MomentumConn";
          C.prm     = q;
          conns.push_back(std::move(C));
       }
@@ -456,10 +577,19 @@ void Net::build() {
       }
       Layer &post      = layers[C.post];
       post.numChannels = std::max(post.numChannels, C.channel + 1);
+      // ArborList::initializeDelays / convertDelay (ArborList.cpp:102-124): one
+      // delay for the one arbor, in timesteps
       double d[4] = {0};
-      if (pvparams_array(P, C.prm.g, "delay", d, 4) > 0 && d[0] != 0.0) {
-         fatal(C.name + ": delays are not supported");
+      int nd      = pvparams_array(P, C.prm.g, "delay", d, 4);
+      if (nd > 1) {
+         fatal(C.name + ": one delay per connection (one arbor)");
       }
+      C.delay = nd > 0 ? (int)std::nearbyint(d[0] / dtCol) : 0;
+      if (C.delay < 0) {
+         fatal(C.name + ": negative delay");
+      }
+      Layer &pl = layers[C.pre];
+      pl.levels = std::max(pl.levels, C.delay + 1);
    }
    for (auto &C : conns) {
       Layer &pre = layers[C.pre], &post = layers[C.post];
@@ -536,12 +666,18 @@ void Net::build() {
    check(pvhip_device_create(0, &D.dev), "pvhip_device_create");
    D.stream = pvhip_device_stream(D.dev);
    for (auto &L : layers) {
+      if (L.mirrorBC && L.halo > 0) { // a layer without a halo has no border to mirror into
+         fatal(L.name + ": mirrorBCflag is supported for layers without a halo only");
+      }
       L.loc.halo = {L.halo, L.halo, L.halo, L.halo};
       size_t nres = (size_t)nbatch * L.loc.nx * L.loc.ny * L.loc.nf;
       size_t next = (size_t)nbatch * (L.loc.nx + 2 * L.halo) * (L.loc.ny + 2 * L.halo) * L.loc.nf;
       L.V    = static_cast<float *>(D.alloc(nres * 4));
       L.A    = static_cast<float *>(D.alloc(next * 4));
       L.gsyn = static_cast<float *>(D.alloc((size_t)L.numChannels * nres * 4));
+      for (int lv = 1; lv < L.levels; lv++) { // older DataStore levels start at zero (RingBuffer)
+         L.ring.push_back(static_cast<float *>(D.alloc(next * 4)));
+      }
       bool updGpu = L.prm.num("updateGpu", 0) != 0;
       if (L.kind == L_INPUT) {
          L.input         = readActivity(L.prm.str("inputPath"));
@@ -549,6 +685,8 @@ void Net::build() {
          if (L.input.nx != L.loc.nx || L.input.ny != L.loc.ny || L.input.nf != L.loc.nf) {
             fatal(L.name + ": input file size differs from the layer (autoResizeFlag is not supported)");
          }
+         // InputActivityBuffer::initializeState: the first frames without advancing the index
+         loadInputFrames(L);
       }
       else {
          L.transfer = makeTransfer(L.prm, updGpu);
@@ -575,9 +713,16 @@ void Net::build() {
                D.h2d(L.V + (size_t)b * fr.size(), fr.data(), fr.size() * 4);
             }
          }
-         else if (init != "ZeroV") {
-            fatal(L.name + ": InitVType " + init + " is not supported (ZeroV, InitVFromFile)");
+         else if (init != "ZeroV" && init != "ConstantV") {
+            fatal(L.name + ": InitVType " + init + " is not supported (ZeroV, ConstantV, InitVFromFile)");
          }
+      }
+      if (L.kind != L_INPUT && L.prm.str("InitVType", "ZeroV") == "ConstantV") { // InitV.cpp: valueV
+         std::vector<float> v(nres, (float)L.prm.num("valueV", 0.0));
+         D.h2d(L.V, v.data(), v.size() * 4);
+      }
+      else if (L.kind != L_INPUT && L.kind != L_LCA && L.prm.str("InitVType", "ZeroV") != "ZeroV") {
+         fatal(L.name + ": InitVType " + L.prm.str("InitVType") + " is not supported (ZeroV, ConstantV)");
       }
       if (L.kind != L_INPUT) {
          check(pvhip_ann_activity(&L.loc, &L.transfer, L.V, L.A, D.stream), "initial activity");
@@ -593,26 +738,51 @@ void Net::build() {
       pvhost_num_kernels(&pre.loc, &post.loc, &nkx, &nky);
       int nK = nkx * nky * pre.loc.nf;
       std::string init = C.prm.str("weightInitType");
-      if (init != "FileWeight") {
-         fatal(C.name + ": weightInitType " + init + " is not supported (FileWeight)");
+      if (init == "IdentWeight") {
+         // InitIdentWeights: a one-to-one connection's weight 1 from feature k to
+         // feature k (1x1 patches)
+         if (C.nxp != 1 || C.nyp != 1 || C.nfp != pre.loc.nf || nK != pre.loc.nf) {
+            fatal(C.name + ": IdentWeight is supported for 1x1 patches between layers of equal nf");
+         }
+         C.hostW.assign((size_t)nK * C.nfp, 0.0f);
+         for (int k = 0; k < nK; k++) {
+            C.hostW[(size_t)k * C.nfp + k] = 1.0f;
+         }
       }
-      C.hostW = readWeights(C.prm.str("initWeightsFile"), C.nxp, C.nyp, C.nfp, nK);
+      else if (init == "FileWeight") {
+         C.hostW = readWeights(C.prm.str("initWeightsFile"), C.nxp, C.nyp, C.nfp, nK);
+      }
+      else {
+         fatal(C.name + ": weightInitType " + init + " is not supported (FileWeight, IdentWeight)");
+      }
       C.W     = static_cast<float *>(D.alloc(C.hostW.size() * 4));
       D.h2d(C.W, C.hostW.data(), C.hostW.size() * 4);
       std::string norm = C.prm.str("normalizeMethod", "none");
-      if (norm == "normalizeL2" && C.prm.num("normalizeOnInitialize", 1) != 0) {
+      if (norm == "normalizeL2") {
          if (C.prm.num("normalizeArborsIndividually", 0) != 0 || C.prm.num("normalizeFromPostPerspective", 0) != 0
              || C.prm.num("rMinX", 0) != 0 || C.prm.num("rMinY", 0) != 0 || C.prm.num("normalize_cutoff", 0) != 0
              || C.prm.num("nonnegativeConstraintFlag", 0) != 0) {
             fatal(C.name + ": only plain normalizeL2 is supported");
          }
-         check(pvhip_normalize_l2(C.W, nK, C.nxp * C.nyp * C.nfp, (float)C.prm.num("strength", 1.0),
-                     (float)C.prm.num("minL2NormTolerated", 0.0), D.stream),
-               "pvhip_normalize_l2");
-         D.d2h(C.hostW.data(), C.W, C.hostW.size() * 4);
+         C.normalizeL2             = true;
+         C.strength                = (float)C.prm.num("strength", 1.0);
+         C.minL2                   = (float)C.prm.num("minL2NormTolerated", 0.0);
+         C.normalizeOnWeightUpdate = C.prm.num("normalizeOnWeightUpdate", 1) != 0;
+         if (C.prm.num("normalizeOnInitialize", 1) != 0) {
+            check(pvhip_normalize_l2(C.W, nK, C.nxp * C.nyp * C.nfp, C.strength, C.minL2, D.stream),
+                  "pvhip_normalize_l2");
+            D.d2h(C.hostW.data(), C.W, C.hostW.size() * 4);
+         }
       }
       else if (norm != "none" && norm != "") {
          fatal(C.name + ": normalizeMethod " + norm + " is not supported");
+      }
+      readPlasticity(C);
+      // weights are written only when writeStep is given (WeightsPair.cpp:53-95)
+      C.writeStep = C.prm.has("writeStep") ? C.prm.num("writeStep", -1.0) : -1.0;
+      C.writeTime = C.prm.num("initialWriteTime", 0.0);
+      while (C.writeStep > 0 && C.writeTime < 0.0) {
+         C.writeTime += C.writeStep;
       }
    }
    for (auto &C : conns) {
@@ -622,8 +792,17 @@ void Net::build() {
       }
    }
    for (auto &C : conns) {
-      if (C.mode == C_SCATTER && layers[C.pre].sparse) {
-         layers[C.pre].needActive = true;
+      Layer &L = layers[C.pre];
+      if (C.mode == C_SCATTER && L.sparse && C.delay == 0) {
+         L.needActive = true;
+      }
+      else if (C.mode == C_SCATTER && L.sparse) {
+         size_t next = (size_t)nbatch * (L.loc.nx + 2 * L.halo) * (L.loc.ny + 2 * L.halo) * L.loc.nf;
+         size_t npos = (size_t)(L.loc.nx + 2 * L.halo) * (L.loc.ny + 2 * L.halo) + 1;
+         C.dEntries   = static_cast<pvhip_sparse_entry *>(D.alloc(next * 8));
+         C.dNumActive = static_cast<int64_t *>(D.alloc(nbatch * 8));
+         C.dPosStart  = static_cast<uint32_t *>(D.alloc(nbatch * npos * 4));
+         C.dScratch   = static_cast<uint32_t *>(D.alloc(pvhip_active_compact_scratch_bytes(&L.loc)));
       }
    }
    for (auto &L : layers) {
@@ -716,18 +895,63 @@ double Net::sourceBound(const Layer &pre) {
 // Pre-perspective weights of C (its own pre -> post) on the host: a HyPerConn's
 // or CloneConn's are the original's; a TransposeConn's are the transpose of
 // its original's (TransposeWeights::transposeShared).
-std::vector<float> prePerspectiveWeights(Net &net, Conn &C, Conn &O) {
+// `src` holds the original's pre-perspective weights.
+std::vector<float> prePerspectiveWeights(Net &net, Conn &C, Conn &O, const std::vector<float> &src) {
    if (C.keyword != "TransposeConn") {
-      return O.hostW;
+      return src;
    }
    Layer &opre = net.layers[O.pre], &opost = net.layers[O.post];
    int pkx, pky;
    pvhost_num_kernels(&opost.loc, &opre.loc, &pkx, &pky);
    std::vector<float> wt((size_t)pkx * pky * opost.loc.nf * C.nxp * C.nyp * C.nfp);
-   if (pvhost_transpose_weights(&opre.loc, &opost.loc, O.nxp, O.nyp, O.nfp, O.hostW.data(), wt.data()) != 0) {
+   if (pvhost_transpose_weights(&opre.loc, &opost.loc, O.nxp, O.nyp, O.nfp, src.data(), wt.data()) != 0) {
       fatal(C.name + ": weight transpose failed");
    }
    return wt;
+}
+
+// The weights a delivery reads when they are not its original's own buffer:
+// its pre-perspective weights, transposed again for a gather
+// (TransposeWeights::transposeShared), from the original's weights `src`.
+std::vector<float> deliveryWeights(Net &net, Conn &C, Conn &O, const std::vector<float> &src) {
+   std::vector<float> w = prePerspectiveWeights(net, C, O, src);
+   if (C.mode != C_GATHER) {
+      return w;
+   }
+   Layer &pre = net.layers[C.pre], &post = net.layers[C.post];
+   int pkx, pky;
+   pvhost_num_kernels(&post.loc, &pre.loc, &pkx, &pky);
+   std::vector<float> wt((size_t)pkx * pky * post.loc.nf * C.nxq * C.nyq * pre.loc.nf);
+   if (pvhost_transpose_weights(&pre.loc, &post.loc, C.nxp, C.nyp, C.nfp, w.data(), wt.data()) != 0) {
+      fatal(C.name + ": weight transpose failed");
+   }
+   return wt;
+}
+
+// For a delivery of a plastic original: the same derivation as a table of
+// indices into the original's weights (pvhip_permute_weights), so that each
+// weight update re-derives the delivery's copy on the device
+// (WeightsPair::finalizeUpdate, WeightsPair.cpp:261-280).  Each weight is
+// carried through the derivation as its index + 1 (exact in float below
+// 2^24); 0 marks a weight no original weight reaches.
+void derivationTable(Net &net, Conn &C, Conn &O, size_t n) {
+   if (O.hostW.size() >= (1u << 24)) {
+      fatal(C.name + ": too many weights for the derivation table");
+   }
+   std::vector<float> code(O.hostW.size());
+   for (size_t i = 0; i < code.size(); i++) {
+      code[i] = (float)(i + 1);
+   }
+   std::vector<float> d = deliveryWeights(net, C, O, code);
+   if (d.size() != n) {
+      fatal(C.name + ": derivation table size differs from the delivery's weights");
+   }
+   std::vector<int32_t> idx(n);
+   for (size_t j = 0; j < n; j++) {
+      idx[j] = (int32_t)d[j] - 1;
+   }
+   C.perm  = net.D.table(idx);
+   C.permN = n;
 }
 
 // The MFMA route of a GPU scatter: a post-perspective gather with the
@@ -770,17 +994,14 @@ void Net::buildConn(Conn &C) {
          C.nyq = O.nyp;
       }
       else {
-         std::vector<float> w = prePerspectiveWeights(*this, C, O);
-         C.nxq                = pvhost_post_patch_size(C.nxp, pre.loc.nx, post.loc.nx);
-         C.nyq                = pvhost_post_patch_size(C.nyp, pre.loc.ny, post.loc.ny);
-         int pkx, pky;
-         pvhost_num_kernels(&post.loc, &pre.loc, &pkx, &pky);
-         std::vector<float> wt((size_t)pkx * pky * post.loc.nf * C.nxq * C.nyq * pre.loc.nf);
-         if (pvhost_transpose_weights(&pre.loc, &post.loc, C.nxp, C.nyp, C.nfp, w.data(), wt.data()) != 0) {
-            fatal(C.name + ": weight transpose failed");
-         }
-         C.W = static_cast<float *>(D.alloc(wt.size() * 4));
+         C.nxq                 = pvhost_post_patch_size(C.nxp, pre.loc.nx, post.loc.nx);
+         C.nyq                 = pvhost_post_patch_size(C.nyp, pre.loc.ny, post.loc.ny);
+         std::vector<float> wt = deliveryWeights(*this, C, O, O.hostW);
+         C.W                   = static_cast<float *>(D.alloc(wt.size() * 4));
          D.h2d(C.W, wt.data(), wt.size() * 4);
+         if (O.plastic) {
+            derivationTable(*this, C, O, wt.size());
+         }
       }
       std::vector<int32_t> sx(post.loc.nx), sy(post.loc.ny), kx(post.loc.nx), ky(post.loc.ny);
       if (pvhost_recv_post_tables(&pre.loc, &post.loc, C.nxq, C.nyq, sx.data(), sy.data(), kx.data(), ky.data(),
@@ -811,6 +1032,8 @@ void Net::buildConn(Conn &C) {
       if (C.arith == PVHIP_ARITH_FP8 && C.nkx * C.nky > 1) {
          C.arith = PVHIP_ARITH_BF16; // the channel GEMM has BF16 and FP32 forms
       }
+      C.lpKernels = nK;
+      C.lpPatch   = psize;
       if (C.arith == PVHIP_ARITH_BF16) {
          C.Wlp = D.alloc(pvhip_recv_post_bf16_weights_bytes(nK, psize));
          check(pvhip_recv_post_bf16_weights(C.W, C.Wlp, nK, psize, D.stream), "bf16 weights");
@@ -828,9 +1051,12 @@ void Net::buildConn(Conn &C) {
    }
    else {
       if (C.keyword == "TransposeConn") {
-         std::vector<float> w = prePerspectiveWeights(*this, C, O);
+         std::vector<float> w = deliveryWeights(*this, C, O, O.hostW);
          C.W                  = static_cast<float *>(D.alloc(w.size() * 4));
          D.h2d(C.W, w.data(), w.size() * 4);
+         if (O.plastic) {
+            derivationTable(*this, C, O, w.size());
+         }
       }
       else {
          C.W = O.W;
@@ -876,7 +1102,7 @@ pvhip_recv_post_args Net::gatherArgs(Conn &C) {
    a.kernel_x      = C.tab[2];
    a.kernel_y      = C.tab[3];
    a.dt_factor     = 1.0f;
-   a.pre_data      = pre.A;
+   a.pre_data      = levelData(pre, C.delay);
    a.weights       = C.W;
    a.post_gsyn     = post.gsyn ? post.gsyn + (size_t)C.channel * nres : nullptr;
    a.accumulate    = 1;
@@ -960,7 +1186,7 @@ void Net::deliver(Conn &C) {
    size_t nres = (size_t)nbatch * post.loc.nx * post.loc.ny * post.loc.nf;
    float *g    = post.gsyn + (size_t)C.channel * nres;
    if (C.mode == C_IDENT) {
-      check(pvhip_ident_deliver(&pre.loc, pre.A, g, 1, D.stream), "pvhip_ident_deliver");
+      check(pvhip_ident_deliver(&pre.loc, levelData(pre, C.delay), g, 1, D.stream), "pvhip_ident_deliver");
    }
    else if (C.mode == C_GATHER) {
       pvhip_recv_post_args a = gatherArgs(C);
@@ -984,12 +1210,19 @@ void Net::deliver(Conn &C) {
       a.row_wy = C.tab[6];
       a.row_ky = C.tab[7];
       a.dt_factor = 1.0f;
-      if (pre.needActive) {
+      if (pre.needActive && C.delay == 0) {
          a.active = pre.entries;
          a.pos_start = pre.posStart;
       }
+      else if (C.dEntries) { // the delayed level's active list (DataStore::updateActiveIndices)
+         check(pvhip_active_compact(&pre.loc, levelData(pre, C.delay), C.dEntries, C.dNumActive, C.dPosStart,
+                     C.dScratch, D.stream),
+               "pvhip_active_compact");
+         a.active    = C.dEntries;
+         a.pos_start = C.dPosStart;
+      }
       else {
-         a.pre_data = pre.A;
+         a.pre_data = levelData(pre, C.delay);
       }
       a.weights = C.W;
       a.post_gsyn = g;
@@ -997,6 +1230,44 @@ void Net::deliver(Conn &C) {
       a.arith = C.arith;
       check(pvhip_recv_pre(&a, D.stream), "pvhip_recv_pre");
    }
+}
+
+// The input layer's activity from the frames at its index (one per batch
+// element), halo zero (InputActivityBuffer::retrieveInput, valueBC 0).
+void Net::loadInputFrames(Layer &L) {
+   size_t nxe = L.loc.nx + 2 * L.halo, nye = L.loc.ny + 2 * L.halo;
+   for (int b = 0; b < nbatch; b++) {
+      const auto &fr = L.input.frames[(L.frameIndex + b) % L.input.frames.size()];
+      std::vector<float> ext(nxe * nye * L.loc.nf, 0.0f);
+      for (int y = 0; y < L.loc.ny; y++) {
+         std::memcpy(&ext[((size_t)(y + L.halo) * nxe + L.halo) * L.loc.nf], &fr[(size_t)y * L.loc.nx * L.loc.nf],
+               (size_t)L.loc.nx * L.loc.nf * 4);
+      }
+      D.h2d(L.A + (size_t)b * ext.size(), ext.data(), ext.size() * 4);
+   }
+}
+
+// DataStore level `delay` of L: the live activity for 0, else the ring slot
+// holding the activity published `delay` steps before it.
+float *Net::levelData(Layer &L, int delay) {
+   if (delay == 0) {
+      return L.A;
+   }
+   return L.ring[(L.head + delay - 1) % (int)L.ring.size()];
+}
+
+// Publisher::increaseTimeLevel + publish / copyForward (Publisher.cpp:104-135,
+// BasePublisherComponent.cpp:182-197): before the live buffer takes this
+// step's activity, the level it held becomes level 1 (every step, whether or
+// not the layer updates).
+void Net::advanceDataStore(Layer &L) {
+   if (L.ring.empty()) {
+      return;
+   }
+   int nlev = (int)L.ring.size();
+   L.head   = (L.head + nlev - 1) % nlev;
+   size_t next = (size_t)nbatch * (L.loc.nx + 2 * L.halo) * (L.loc.ny + 2 * L.halo) * L.loc.nf;
+   check(pvhip_copy_device(L.ring[L.head], L.A, next * 4, D.stream), "DataStore level copy");
 }
 
 // resetGSynBuffers, then every connection into L adds its channel
@@ -1019,15 +1290,7 @@ void Net::updateLayer(Layer &L, double t) {
       if (L.displayPeriod > 0) {
          double n = (t - L.lastUpdate) / L.displayPeriod;
          if (std::fabs(n - std::nearbyint(n)) * L.displayPeriod < 0.5 * dtCol) {
-            for (int b = 0; b < nbatch; b++) {
-               const auto &fr = L.input.frames[(L.frameIndex + b) % L.input.frames.size()];
-               std::vector<float> ext((size_t)(L.loc.nx + 2 * L.halo) * (L.loc.ny + 2 * L.halo) * L.loc.nf, 0.0f);
-               for (int y = 0; y < L.loc.ny; y++) {
-                  std::memcpy(&ext[((size_t)(y + L.halo) * (L.loc.nx + 2 * L.halo) + L.halo) * L.loc.nf],
-                        &fr[(size_t)y * L.loc.nx * L.loc.nf], (size_t)L.loc.nx * L.loc.nf * 4);
-               }
-               D.h2d(L.A + (size_t)b * ext.size(), ext.data(), ext.size() * 4);
-            }
+            loadInputFrames(L);
             L.frameIndex = (L.frameIndex + nbatch) % (int)L.input.frames.size();
             L.lastUpdate = t;
          }
@@ -1036,7 +1299,12 @@ void Net::updateLayer(Layer &L, double t) {
    }
    if (L.kind == L_LCA) {
       std::vector<float> e(nbatch);
-      pvhip_lca_exp_tau(L.dt.data(), nbatch, L.tau, e.data());
+      if (L.ista) { // (float)dt/tau per batch element (ISTAInternalStateBuffer.kpp:56)
+         pvhip_ista_rate(L.dt.data(), nbatch, L.tau, e.data());
+      }
+      else {
+         pvhip_lca_exp_tau(L.dt.data(), nbatch, L.tau, e.data());
+      }
       D.h2d(L.expTau, e.data(), nbatch * 4);
       pvhip_lca_update_args u{};
       u.loc = L.loc;
@@ -1050,7 +1318,10 @@ void Net::updateLayer(Layer &L, double t) {
       u.self_interact = L.selfInteract;
       u.transfer = L.transfer;
       u.activity_from_v = 1;
-      if (L.momentum) {
+      if (L.ista) {
+         check(pvhip_ista_update(&u, (float)L.prm.num("VThresh", -FLT_MAX), D.stream), "pvhip_ista_update");
+      }
+      else if (L.momentum) {
          check(pvhip_momentum_lca_update(&u, L.momentumRate, L.prevDrive, D.stream),
                "pvhip_momentum_lca_update");
       }
@@ -1115,6 +1386,292 @@ std::vector<double> Net::probeEnergy(Probe &p) {
       }
    }
    return v;
+}
+
+// ------------------------------------------------------------------ plasticity
+// HebbianUpdater (src/weightupdaters/HebbianUpdater.cpp) and MomentumUpdater
+// (src/weightupdaters/MomentumUpdater.cpp) of a HyPerConn / MomentumConn:
+// the parameters of ioParamsFillGroup (:32-170; MomentumUpdater.cpp:30-145),
+// the device buffers of dW, the activation counts and prev_dW.
+void Net::readPlasticity(Conn &C) {
+   C.plastic = C.prm.num("plasticityFlag", 0) != 0;
+   if (!C.plastic) {
+      return;
+   }
+   Layer &pre = layers[C.pre], &post = layers[C.post];
+   if (!C.prm.has("dWMax")) {
+      fatal(C.name + ": plasticityFlag is set, so dWMax is required");
+   }
+   C.dWMax              = (float)C.prm.num("dWMax", 0.0);
+   C.dWMaxDecayInterval = (int)C.prm.num("dWMaxDecayInterval", 0);
+   C.dWMaxDecayFactor   = (float)C.prm.num("dWMaxDecayFactor", 0.0);
+   if (C.dWMaxDecayFactor < 0.0f || C.dWMaxDecayFactor >= 1.0f) {
+      fatal(C.name + ": dWMaxDecayFactor must be in the interval [0.0, 1.0)");
+   }
+   C.normalizeDw = C.prm.num("normalizeDw", 1) != 0;
+   C.immediate   = C.prm.num("immediateWeightUpdate", 1) != 0;
+   if (C.prm.num("useMask", 0) != 0) {
+      fatal(C.name + " has useMask set to true. This parameter is obsolete.");
+   }
+   if (C.prm.num("combine_dW_with_W_flag", 0) != 0) {
+      fatal(C.name + ": combine_dW_with_W_flag is not supported");
+   }
+   C.triggerLayer = C.prm.str("triggerLayerName");
+   if (!C.triggerLayer.empty()) {
+      layerIndex(C.triggerLayer);
+      C.triggerOffset = C.prm.num("triggerOffset", 0.0);
+      if (C.triggerOffset < 0) {
+         fatal(C.name + ": triggerOffset must be positive");
+      }
+      if (C.prm.has("weightUpdatePeriod")) {
+         fatal(C.name + " sets both triggerLayerName and weightUpdatePeriod; only one of these can be set.");
+      }
+   }
+   else {
+      if (!C.prm.has("weightUpdatePeriod")) {
+         fatal(C.name + ": weightUpdatePeriod is required without a triggerLayerName");
+      }
+      C.updatePeriod = C.prm.num("weightUpdatePeriod", 1.0);
+      double t0      = C.prm.num("initialWeightUpdateTime", 0.0);
+      C.weightUpdateTime = t0;
+      if (C.weightUpdateTime < 0.0) {
+         while (C.weightUpdateTime <= 0.0) {
+            C.weightUpdateTime += C.updatePeriod;
+         }
+      }
+      C.lastUpdateTime = t0;
+   }
+   if (C.momentum) {
+      std::string m = C.prm.str("momentumMethod");
+      int code      = m == "viscosity" ? PVHIP_MOMENTUM_VISCOSITY
+                      : m == "simple"  ? PVHIP_MOMENTUM_SIMPLE
+                      : m == "alex"    ? PVHIP_MOMENTUM_ALEX
+                                       : -1;
+      if (code < 0) {
+         fatal("MomentumUpdater " + C.name + ": momentumMethod of " + m
+               + " is not known. Options are \"viscosity\", \"simple\", and \"alex\".");
+      }
+      C.momentumCode = code;
+      // timeConstantTau wins over the deprecated momentumTau (MomentumUpdater.cpp:114-133)
+      C.deprecatedTau = !C.prm.has("timeConstantTau") && C.prm.has("momentumTau");
+      double dflt     = code == PVHIP_MOMENTUM_VISCOSITY ? 100.0 : code == PVHIP_MOMENTUM_SIMPLE ? 0.25 : 0.9;
+      C.momentumTau   = (float)(C.deprecatedTau ? C.prm.num("momentumTau", 0.0) : C.prm.num("timeConstantTau", dflt));
+      C.momentumDecay = (float)C.prm.num("momentumDecay", 0.0);
+      if (!C.prm.str("initPrev_dWFile").empty()) {
+         fatal(C.name + ": initPrev_dWFile is not supported");
+      }
+      momentumFactors(C); // the parameter checks, before any device work of the update
+   }
+   size_t n = C.hostW.size();
+   C.dW     = static_cast<float *>(D.alloc(n * 4));
+   if (C.normalizeDw) {
+      C.counts = static_cast<int64_t *>(D.alloc(n * 8));
+   }
+   if (C.momentum) {
+      C.prevDw = static_cast<float *>(D.alloc(n * 4));
+   }
+   // PatchGeometry over the pre extended axes: patch sizes, first kept item,
+   // post extended start of each patch (aPostOffset)
+   std::vector<int32_t> t[6];
+   for (int axis = 0; axis < 2; axis++) {
+      int ne = axis == 0 ? pre.loc.nx + pre.loc.halo.lt + pre.loc.halo.rt : pre.loc.ny + pre.loc.halo.dn + pre.loc.halo.up;
+      std::vector<int32_t> dim(ne), ps(ne), pr(ne), pe(ne), us(ne);
+      if (pvhost_patch_axis(&pre.loc, &post.loc, C.nxp, C.nyp, axis, dim.data(), ps.data(), pr.data(), pe.data(),
+                us.data())
+          != 0) {
+         fatal(C.name + ": patch geometry failed");
+      }
+      t[axis]     = dim;
+      t[2 + axis] = ps;
+      t[4 + axis] = pe;
+   }
+   for (int i = 0; i < 6; i++) {
+      C.hebbTab[i] = D.table(t[i]);
+   }
+   std::string ar = gpuDwArith;
+   C.dwArith      = ar == "fp32" ? PVHIP_ARITH_FP32 : ar == "bf16" ? PVHIP_ARITH_BF16 : PVHIP_ARITH_EXACT;
+   if (ar != "exact" && ar != "fp32" && ar != "bf16") {
+      fatal("dwArithmetic \"" + ar + "\" is not exact, fp32 or bf16");
+   }
+   if (C.dwArith != PVHIP_ARITH_EXACT) {
+      int nkx, nky;
+      pvhost_num_kernels(&pre.loc, &post.loc, &nkx, &nky);
+      if (nkx * nky != 1
+          || pvhost_hebbian_windows(&pre.loc, &post.loc, C.nxp, C.nyp, &C.hebbWin[0], &C.hebbWin[1], &C.hebbWin[2],
+                   &C.hebbWin[3])
+                   != 0) {
+         std::printf("pvrun: %s: the dW correlation runs in the EXACT form (the MFMA form needs one data patch per "
+                     "feature and clipped windows)\n",
+               C.name.c_str());
+         C.dwArith = PVHIP_ARITH_EXACT;
+      }
+      else {
+         pvhip_hebbian_args a = hebbianArgs(C);
+         C.hebbScratchBytes   = pvhip_hebbian_dw_scratch_bytes(&a);
+         if (C.hebbScratchBytes > 0) {
+            C.hebbScratch = D.alloc(C.hebbScratchBytes);
+         }
+      }
+   }
+   std::printf("pvrun: %s: plastic (%s, dWMax %g, %s, %s)\n", C.name.c_str(),
+         C.momentum ? "MomentumUpdater" : "HebbianUpdater", C.dWMax,
+         C.triggerLayer.empty() ? "weightUpdatePeriod" : "triggered", C.immediate ? "immediate" : "delayed");
+}
+
+// The factors of MomentumUpdater::applyMomentum (or applyMomentumDeprecated)
+// for the current dWMax (alex scales the decay by it).
+void Net::momentumFactors(Conn &C) {
+   int st = C.deprecatedTau
+                  ? pvhip_momentum_factors_deprecated(C.momentumCode, C.momentumTau, C.momentumDecay, C.dWMax,
+                          &C.dwFactor, &C.wFactor)
+                  : pvhip_momentum_factors(C.momentumCode, C.momentumTau, C.momentumDecay, C.dWMax, &C.dwFactor,
+                          &C.wFactor);
+   if (st != PVHIP_OK) {
+      fatal("MomentumUpdater " + C.name + ": " + pvhip_last_error());
+   }
+}
+
+pvhip_hebbian_args Net::hebbianArgs(Conn &C) {
+   Layer &pre = layers[C.pre], &post = layers[C.post];
+   pvhip_hebbian_args a{};
+   a.pre  = pre.loc;
+   a.post = post.loc;
+   a.nxp  = C.nxp;
+   a.nyp  = C.nyp;
+   a.nfp  = C.nfp;
+   pvhost_num_kernels(&pre.loc, &post.loc, &a.num_kernels_x, &a.num_kernels_y);
+   a.dim_x           = C.hebbTab[0];
+   a.dim_y           = C.hebbTab[1];
+   a.pstart_x        = C.hebbTab[2];
+   a.pstart_y        = C.hebbTab[3];
+   a.apost_x         = C.hebbTab[4];
+   a.apost_y         = C.hebbTab[5];
+   a.dw_max          = C.dWMax;
+   a.pre_data        = levelData(pre, C.delay); // HebbianUpdater::update_dW: getLayerData(delay)
+   a.post_data       = post.A;
+   a.dw              = C.dW;
+   a.counts          = C.counts;
+   a.arith           = C.dwArith;
+   a.window_stride_x = C.hebbWin[0];
+   a.window_start_x  = C.hebbWin[1];
+   a.window_stride_y = C.hebbWin[2];
+   a.window_start_y  = C.hebbWin[3];
+   a.scratch         = C.hebbScratch;
+   a.scratch_bytes   = C.hebbScratchBytes;
+   return a;
+}
+
+// HebbianUpdater::needUpdate (HebbianUpdater.cpp:361-374): the trigger layer's
+// LayerUpdateController::needUpdate at t + triggerOffset, or every
+// weightUpdatePeriod from the last update.
+bool Net::needWeightUpdate(Conn &C, double t) {
+   if (!C.triggerLayer.empty()) {
+      Layer &T  = layers[layerIndex(C.triggerLayer)];
+      double tt = t + C.triggerOffset;
+      if (T.kind != L_INPUT) {
+         return true; // a layer without a trigger of its own updates every timestep
+      }
+      if (T.displayPeriod <= 0) {
+         return false;
+      }
+      double n = (tt - T.lastUpdate) / T.displayPeriod;
+      return std::fabs(n - std::nearbyint(n)) * T.displayPeriod < 0.5 * dtCol;
+   }
+   double n = (t - C.lastUpdateTime) / C.updatePeriod;
+   return std::fabs(n - std::nearbyint(n)) * C.updatePeriod < 0.5 * dtCol;
+}
+
+// updateLocal_dW: clear dW (and the activation counts), then update_dW
+// (HebbianUpdater.cpp:384-560); reduce_dW is the identity on one rank.
+void Net::computeDw(Conn &C) {
+   size_t n = C.hostW.size();
+   check(pvhip_fill_zero(C.dW, n * 4, D.stream), "clear dW");
+   if (C.counts) {
+      check(pvhip_fill_zero(C.counts, n * 8, D.stream), "clear activation counts");
+   }
+   pvhip_hebbian_args a = hebbianArgs(C);
+   check(pvhip_hebbian_dw(&a, D.stream), "pvhip_hebbian_dw");
+}
+
+// blockingNormalize_dW + updateArbors (HebbianUpdater.cpp:773-833), or
+// MomentumUpdater::updateWeights (MomentumUpdater.cpp:276-294).
+void Net::applyDw(Conn &C) {
+   size_t n = C.hostW.size();
+   if (C.momentum) {
+      momentumFactors(C);
+      check(pvhip_momentum_apply(C.W, C.dW, C.prevDw, C.counts, n, C.normalizeDw, C.dwFactor, C.wFactor,
+                  C.deprecatedTau, D.stream),
+            "pvhip_momentum_apply");
+   }
+   else {
+      check(pvhip_hebbian_apply(C.W, C.dW, C.counts, n, C.normalizeDw, D.stream), "pvhip_hebbian_apply");
+   }
+}
+
+// WeightsPair::finalizeUpdate for the deliveries that read O's weights: their
+// own copies re-derived on the device, and the low-precision shadows.
+void Net::refreshDerived(Conn &O) {
+   for (auto &X : conns) {
+      if (X.mode != C_GATHER && X.mode != C_SCATTER) {
+         continue;
+      }
+      if (&X != &O && X.original != O.name) {
+         continue;
+      }
+      if (X.perm) {
+         check(pvhip_permute_weights(O.W, X.perm, X.W, X.permN, D.stream), "pvhip_permute_weights");
+      }
+      if (X.Wlp && X.arith == PVHIP_ARITH_BF16) {
+         check(pvhip_recv_post_bf16_weights(X.W, X.Wlp, X.lpKernels, X.lpPatch, D.stream), "bf16 weights");
+      }
+      else if (X.Wlp && X.arith == PVHIP_ARITH_FP8) {
+         check(pvhip_recv_post_fp8_weights(X.W, X.Wlp, X.lpKernels, X.lpPatch, D.stream), "fp8 weights");
+      }
+   }
+}
+
+// HyPerCol::advanceTime after the layer phases (HyPerCol.cpp:776-779):
+// ConnectionUpdate (HebbianUpdater::updateState, :343-359), ConnectionNormalize
+// (NormalizeBase::respondConnectionNormalize with normalizeOnWeightUpdate),
+// ConnectionFinalizeUpdate.
+void Net::updateConnections(double t) {
+   for (auto &C : conns) {
+      C.updatedNow = false;
+      if (!C.plastic || !needWeightUpdate(C, t)) {
+         continue;
+      }
+      if (C.immediate) {
+         computeDw(C);
+         applyDw(C);
+      }
+      else {
+         applyDw(C);
+         computeDw(C);
+      }
+      if (C.dWMaxDecayInterval > 0 && --C.dWMaxDecayTimer < 0) { // decay_dWMax (:835-844)
+         C.dWMaxDecayTimer = C.dWMaxDecayInterval;
+         C.dWMax *= 1.0f - C.dWMaxDecayFactor;
+      }
+      C.lastUpdateTime = t;
+      if (C.triggerLayer.empty()) { // computeNewWeightUpdateTime (:846-853)
+         while (t >= C.weightUpdateTime) {
+            C.weightUpdateTime += C.updatePeriod;
+         }
+      }
+      C.updatedNow = true;
+   }
+   for (auto &C : conns) {
+      if (C.updatedNow && C.normalizeL2 && C.normalizeOnWeightUpdate) {
+         int nK = (int)(C.hostW.size() / ((size_t)C.nxp * C.nyp * C.nfp));
+         check(pvhip_normalize_l2(C.W, nK, C.nxp * C.nyp * C.nfp, C.strength, C.minL2, D.stream),
+               "pvhip_normalize_l2");
+      }
+   }
+   for (auto &C : conns) {
+      if (C.updatedNow) {
+         refreshDerived(C);
+      }
+   }
 }
 
 // RequireAllZeroActivityProbe (RequireAllZeroActivityProbe.cpp, CheckStatsAllZeros.cpp):
@@ -1208,10 +1765,12 @@ void Net::step(double t) {
       }
       for (auto &L : layers) {
          if (L.phase == ph) {
+            advanceDataStore(L);
             updateLayer(L, t);
          }
       }
    }
+   updateConnections(t);
    for (auto &p : probes) {
       if (p.keyword == "RequireAllZeroActivityProbe") {
          checkAllZero(p, t);
@@ -1233,6 +1792,15 @@ void Net::step(double t) {
 }
 
 void Net::output(double t) {
+   for (auto &C : conns) { // WeightsPair::outputState (WeightsPair.cpp:327-337)
+      if (C.keyword == "HyPerConn" && C.writeStep >= 0 && t >= C.writeTime) {
+         C.writeTime += C.writeStep;
+         std::vector<float> w(C.hostW.size());
+         D.d2h(w.data(), C.W, w.size() * 4);
+         C.outFrames.push_back(std::move(w));
+         C.outTimes.push_back(t);
+      }
+   }
    for (auto &L : layers) {
       if (L.writeStep <= 0 || t < L.initialWriteTime) {
          continue;
@@ -1256,10 +1824,270 @@ void Net::output(double t) {
    }
 }
 
+// ------------------------------------------------------------------ checkpoints
+// Checkpointer::makeCheckpointDirectoryFromCurrentStep (Checkpointer.cpp:785-795):
+// <checkpointWriteDir>/Checkpoint<step>, zero-padded to checkpointIndexWidth
+// or to the width of the final step number.
+std::string Net::checkpointDir(long step, long finalStep) const {
+   int width = checkpointIndexWidth >= 0 ? checkpointIndexWidth
+                                          : (int)std::floor(std::log10((double)std::max(finalStep, 1L))) + 1;
+   char num[64];
+   std::snprintf(num, sizeof num, "%0*ld", width, step);
+   return checkpointWriteDir + "/Checkpoint" + num;
+}
+
+// The restricted part of a layer buffer, one frame per batch element (the
+// LayerFile of a checkpoint entry writes restricted data, fileExtended false).
+std::vector<std::vector<float>> Net::restrictedFrames(Layer &L, const float *buf, bool extended) {
+   size_t nxe = L.loc.nx + (extended ? 2 * L.halo : 0), nye = L.loc.ny + (extended ? 2 * L.halo : 0);
+   int h      = extended ? L.halo : 0;
+   std::vector<float> all((size_t)nbatch * nxe * nye * L.loc.nf);
+   D.d2h(all.data(), buf, all.size() * 4);
+   std::vector<std::vector<float>> out;
+   for (int b = 0; b < nbatch; b++) {
+      std::vector<float> fr((size_t)L.loc.nx * L.loc.ny * L.loc.nf);
+      for (int y = 0; y < L.loc.ny; y++) {
+         std::memcpy(&fr[(size_t)y * L.loc.nx * L.loc.nf], &all[(((size_t)b * nye + y + h) * nxe + h) * L.loc.nf],
+               (size_t)L.loc.nx * L.loc.nf * 4);
+      }
+      out.push_back(std::move(fr));
+   }
+   return out;
+}
+
+// Inverse of restrictedFrames for frames[first .. first + nbatch); the halo
+// of an extended buffer is left zero (valueBC 0, no mirror boundary).
+void Net::setRestrictedFrames(Layer &L, float *buf, bool extended, const std::vector<std::vector<float>> &frames,
+      size_t first) {
+   size_t nxe = L.loc.nx + (extended ? 2 * L.halo : 0), nye = L.loc.ny + (extended ? 2 * L.halo : 0);
+   int h      = extended ? L.halo : 0;
+   if (frames.size() < first + nbatch) {
+      fatal(L.name + ": checkpoint holds too few frames");
+   }
+   std::vector<float> all((size_t)nbatch * nxe * nye * L.loc.nf, 0.0f);
+   for (int b = 0; b < nbatch; b++) {
+      const auto &fr = frames[first + b];
+      if (fr.size() != (size_t)L.loc.nx * L.loc.ny * L.loc.nf) {
+         fatal(L.name + ": checkpoint frame size differs from the layer");
+      }
+      for (int y = 0; y < L.loc.ny; y++) {
+         std::memcpy(&all[(((size_t)b * nye + y + h) * nxe + h) * L.loc.nf], &fr[(size_t)y * L.loc.nx * L.loc.nf],
+               (size_t)L.loc.nx * L.loc.nf * 4);
+      }
+   }
+   D.h2d(buf, all.data(), all.size() * 4);
+}
+
+namespace {
+template <typename T>
+void writeBin(const std::string &dir, const std::string &name, const T *v, size_t n) {
+   // CheckpointEntryData::write (CheckpointEntryData.tpp:14-27): raw values and a text copy
+   std::ofstream b(dir + "/" + name + ".bin", std::ios::binary);
+   b.write(reinterpret_cast<const char *>(v), sizeof(T) * n);
+   std::ofstream t(dir + "/" + name + ".txt");
+   for (size_t i = 0; i < n; i++) {
+      t << v[i] << "\n";
+   }
+}
+template <typename T>
+std::vector<T> readBin(const std::string &dir, const std::string &name, size_t n) {
+   std::ifstream f(dir + "/" + name + ".bin", std::ios::binary);
+   std::vector<T> v(n);
+   if (!f || !f.read(reinterpret_cast<char *>(v.data()), sizeof(T) * n)) {
+      fatal("checkpoint " + dir + ": cannot read " + name + ".bin");
+   }
+   return v;
+}
+} // namespace
+
+// Checkpointer::checkpointToDirectory (Checkpointer.cpp:817-852) for the
+// objects pvrun builds, in the reference's file names and formats: per layer
+// <name>_V.pvp, <name>_A.pvp (restricted, one frame per batch element),
+// <name>_Delays.pvp (every DataStore level), <name>_prevDrive.pvp
+// (MomentumLCALayer), <name>_lastUpdateTime.bin and <name>_FrameNumbers.bin
+// (input layers); per original connection <name>_W.pvp, and when plastic
+// <name>_lastUpdateTime.bin, <name>_weightUpdateTime.bin (no trigger),
+// <name>_dW.pvp (delayed updates), <name>_prev_dW.pvp (MomentumUpdater); per
+// adaptive time-scale probe <name>_timescaleinfo.bin; timeinfo.bin last.
+void Net::writeCheckpoint(const std::string &dir, double simTime, long step) {
+   std::error_code ec;
+   std::filesystem::create_directories(dir, ec);
+   if (ec) {
+      fatal("cannot create the checkpoint directory " + dir);
+   }
+   for (auto &L : layers) {
+      auto put = [&](const std::string &label, const std::vector<std::vector<float>> &frames) {
+         std::vector<double> times(frames.size(), simTime);
+         writeActivity(dir + "/" + L.name + "_" + label + ".pvp", L.loc.nx, L.loc.ny, L.loc.nf, frames, times);
+      };
+      if (L.kind != L_INPUT) {
+         put("V", restrictedFrames(L, L.V, false));
+      }
+      put("A", restrictedFrames(L, L.A, true));
+      std::vector<std::vector<float>> levels;
+      for (int lv = 0; lv < L.levels; lv++) {
+         auto f = restrictedFrames(L, levelData(L, lv), true);
+         levels.insert(levels.end(), f.begin(), f.end());
+      }
+      put("Delays", levels);
+      if (L.momentum) {
+         put("prevDrive", restrictedFrames(L, L.prevDrive, false));
+      }
+      if (L.kind == L_INPUT) {
+         std::vector<int> idx(nbatch);
+         for (int b = 0; b < nbatch; b++) {
+            idx[b] = (L.frameIndex + b) % (int)L.input.frames.size();
+         }
+         writeBin(dir, L.name + "_FrameNumbers", idx.data(), idx.size());
+         writeBin(dir, L.name + "_lastUpdateTime", &L.lastUpdate, 1);
+      }
+   }
+   for (auto &C : conns) {
+      if (C.keyword != "HyPerConn") {
+         continue;
+      }
+      int nkx, nky;
+      pvhost_num_kernels(&layers[C.pre].loc, &layers[C.post].loc, &nkx, &nky);
+      auto putW = [&](const std::string &label, const float *d) {
+         std::vector<float> w(C.hostW.size());
+         D.d2h(w.data(), d, w.size() * 4);
+         writeWeights(dir + "/" + C.name + "_" + label + ".pvp", nkx, nky, layers[C.pre].loc.nf, C.nxp, C.nyp,
+               C.nfp, {w}, {simTime});
+      };
+      putW("W", C.W);
+      if (C.plastic) {
+         writeBin(dir, C.name + "_lastUpdateTime", &C.lastUpdateTime, 1);
+         if (C.triggerLayer.empty()) {
+            writeBin(dir, C.name + "_weightUpdateTime", &C.weightUpdateTime, 1);
+         }
+         if (!C.immediate) {
+            putW("dW", C.dW);
+         }
+         if (C.momentum) {
+            putW("prev_dW", C.prevDw);
+         }
+      }
+   }
+   for (auto &p : probes) {
+      if (p.keyword == "AdaptiveTimeScaleProbe") { // CheckpointEntryTimeScaleInfo::write
+         std::vector<double> v;
+         for (int b = 0; b < nbatch; b++) {
+            v.push_back(p.ts[b]);
+            v.push_back(p.tsTrue[b]);
+            v.push_back(p.tsMax[b]);
+         }
+         writeBin(dir, p.name + "_timescaleinfo", v.data(), v.size());
+      }
+   }
+   struct {
+      double simTime;
+      long step;
+   } timeinfo{simTime, step};
+   std::ofstream b(dir + "/timeinfo.bin", std::ios::binary);
+   b.write(reinterpret_cast<const char *>(&timeinfo), sizeof timeinfo);
+   std::ofstream t(dir + "/timeinfo.txt");
+   t << "time = " << simTime << "\nnextCheckpointStep = " << step << "\n";
+}
+
+// initializeFromCheckpointDir: the state writeCheckpoint saved; returns its
+// simulation time.  Derived weights and low-precision shadows are re-derived
+// and every active list rebuilt from the restored activity.
+double Net::readCheckpoint(const std::string &dir) {
+   struct {
+      double simTime;
+      long step;
+   } timeinfo{};
+   {
+      std::ifstream b(dir + "/timeinfo.bin", std::ios::binary);
+      if (!b || !b.read(reinterpret_cast<char *>(&timeinfo), sizeof timeinfo)) {
+         fatal("checkpoint " + dir + ": cannot read timeinfo.bin");
+      }
+   }
+   for (auto &L : layers) {
+      auto get = [&](const std::string &label) {
+         PvpFrames f = readActivity(dir + "/" + L.name + "_" + label + ".pvp");
+         if (f.nx != L.loc.nx || f.ny != L.loc.ny || f.nf != L.loc.nf) {
+            fatal("checkpoint " + dir + ": " + L.name + "_" + label + ".pvp has another size");
+         }
+         return f.frames;
+      };
+      if (L.kind != L_INPUT) {
+         setRestrictedFrames(L, L.V, false, get("V"), 0);
+      }
+      setRestrictedFrames(L, L.A, true, get("A"), 0);
+      auto levels = get("Delays");
+      for (int lv = 1; lv < L.levels; lv++) {
+         setRestrictedFrames(L, levelData(L, lv), true, levels, (size_t)lv * nbatch);
+      }
+      if (L.momentum) {
+         setRestrictedFrames(L, L.prevDrive, false, get("prevDrive"), 0);
+      }
+      if (L.kind == L_INPUT) {
+         L.frameIndex = readBin<int>(dir, L.name + "_FrameNumbers", nbatch)[0];
+         L.lastUpdate = readBin<double>(dir, L.name + "_lastUpdateTime", 1)[0];
+      }
+   }
+   for (auto &C : conns) {
+      if (C.keyword != "HyPerConn") {
+         continue;
+      }
+      int nK = (int)(C.hostW.size() / ((size_t)C.nxp * C.nyp * C.nfp));
+      auto getW = [&](const std::string &label, float *d) {
+         std::vector<float> w = readWeights(dir + "/" + C.name + "_" + label + ".pvp", C.nxp, C.nyp, C.nfp, nK);
+         D.h2d(d, w.data(), w.size() * 4);
+      };
+      getW("W", C.W);
+      if (C.plastic) {
+         C.lastUpdateTime = readBin<double>(dir, C.name + "_lastUpdateTime", 1)[0];
+         if (C.triggerLayer.empty()) {
+            C.weightUpdateTime = readBin<double>(dir, C.name + "_weightUpdateTime", 1)[0];
+         }
+         if (!C.immediate) {
+            getW("dW", C.dW);
+         }
+         if (C.momentum) {
+            getW("prev_dW", C.prevDw);
+         }
+      }
+      refreshDerived(C);
+   }
+   for (auto &p : probes) {
+      if (p.keyword == "AdaptiveTimeScaleProbe") {
+         auto v = readBin<double>(dir, p.name + "_timescaleinfo", (size_t)3 * nbatch);
+         for (int b = 0; b < nbatch; b++) {
+            p.ts[b]     = v[3 * b];
+            p.tsTrue[b] = v[3 * b + 1];
+            p.tsMax[b]  = v[3 * b + 2];
+         }
+      }
+   }
+   for (auto &L : layers) {
+      if (L.needActive) {
+         check(pvhip_active_compact(&L.loc, L.A, L.entries, L.numActive, L.posStart, L.compactScratch, D.stream),
+               "pvhip_active_compact");
+      }
+   }
+   for (auto &p : probes) { // the energies of the restored state, as the next AdaptTimestep reads them
+      if (p.keyword == "ColumnEnergyProbe") {
+         p.prev = probeEnergy(p);
+      }
+   }
+   check(pvhip_stream_sync(D.stream), "checkpoint read");
+   return timeinfo.simTime;
+}
+
 int Net::finish() {
    for (auto &L : layers) {
       if (!L.outFrames.empty()) {
          writeActivity(outputPath + "/" + L.name + ".pvp", L.loc.nx, L.loc.ny, L.loc.nf, L.outFrames, L.outTimes);
+      }
+   }
+   for (auto &C : conns) {
+      if (!C.outFrames.empty()) {
+         int nkx, nky;
+         pvhost_num_kernels(&layers[C.pre].loc, &layers[C.post].loc, &nkx, &nky);
+         writeWeights(outputPath + "/" + C.name + ".pvp", nkx, nky, layers[C.pre].loc.nf, C.nxp, C.nyp, C.nfp,
+               C.outFrames, C.outTimes);
       }
    }
    int status = 0;
@@ -1321,13 +2149,27 @@ int main(int argc, char **argv) {
       if (tune) {
          net.autotune();
       }
-      long n = steps >= 0 ? steps : (long)std::lround(net.stopTime / net.dtCol);
-      for (long s = 1; s <= n; s++) {
+      long n     = steps >= 0 ? steps : (long)std::lround(net.stopTime / net.dtCol);
+      long first = 1;
+      if (!net.initFromCheckpointDir.empty()) {
+         first = std::lround(net.readCheckpoint(net.initFromCheckpointDir) / net.dtCol) + 1;
+      }
+      // HyPerCol::advanceTimeLoop: checkpointWrite before each step (checkpoint k
+      // holds the state after k steps), then a final checkpoint
+      for (long s = first; s <= n; s++) {
+         bool restartStep = s == first && first > 1; // the checkpoint just read
+         if (net.checkpointWrite && (s - 1) % net.checkpointInterval == 0 && !restartStep) {
+            net.writeCheckpoint(net.checkpointDir(s - 1, n), (double)(s - 1) * net.dtCol, s - 1);
+         }
          net.step((double)s * net.dtCol);
+      }
+      if (net.checkpointWrite) {
+         check(pvhip_stream_sync(net.D.stream), "sync before the final checkpoint");
+         net.writeCheckpoint(net.checkpointDir(n, n), (double)n * net.dtCol, n);
       }
       check(pvhip_stream_sync(net.D.stream), "final sync");
       rc = net.finish();
-      std::printf("pvrun: %ld steps done%s\n", n, rc ? ", a probe failed" : "");
+      std::printf("pvrun: %ld steps done%s\n", n - first + 1, rc ? ", a probe failed" : "");
    } catch (const std::exception &e) {
       std::fprintf(stderr, "pvrun: %s\n", e.what());
       rc = 1;

@@ -191,7 +191,8 @@ HyPerConn "W" = {
     ('pvpatchAccumulateType = "sum";', "is unrecognized"),
     ('numAxonalArbors = 2;', "one arbor"),
     ('sharedWeights = false;', "shared weights"),
-    ('delay = [1.0];', "delays are not supported"),
+    ('delay = [1.0, 2.0];', "one delay per connection"),
+    ('delay = [-1.0];', "negative delay"),
 ])
 def test_pvrun_selection_errors_follow_the_delivery_creator(tmp_path, extra, msg):
     """HyPerDeliveryCreator's fatal cases (and what pvrun leaves out) are reported
