@@ -107,6 +107,50 @@ def time_form(arith, images, W, steps, warmup, torch, tuner=None):
     return col, s, ms / steps, wall, shapes
 
 
+ACCURACY_STEPS = 500   # the workload's run length
+
+
+def accuracy(forms, images, W, torch, steps=ACCURACY_STEPS):
+    """Each form against EXACT over the workload's own run (500 steps from the
+    same start, one hipGraph replay per step): the relative error of the total
+    energy (worst image) at the last step, its largest value over all steps
+    and over steps 50..500, and the relative difference of the active counts
+    at the last step (worst image).  Independent of --steps / --warmup."""
+    from openpv_amd import abi
+    from openpv_amd.lca import LCAColumn
+    arith = {"exact": abi.ARITH_EXACT, "fp32": abi.ARITH_FP32, "bf16": abi.ARITH_BF16,
+             "fp8": abi.ARITH_FP8}
+    traj = {}
+    for f in ["exact"] + [x for x in forms if x != "exact"]:
+        col = LCAColumn(images, W, NXV, NYV, NFV, NXP, NXP, VThresh=VTHRESH, timeConstantTau=TAU,
+                        dt=DT, arith=arith[f], fused=True, device="cuda")
+        s = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        col.capture(s)
+        energy = []
+        for _ in range(steps):
+            col.replay(s)
+            s.synchronize()
+            energy.append(col.probe_energy(stream=s).cpu().numpy())
+        traj[f] = (np.array(energy), col.num_active().astype(np.float64))
+        col.release_graph()
+        del col
+        torch.cuda.empty_cache()
+    e_ref, n_ref = traj["exact"]
+    out = {"steps": steps}
+    for f in forms:
+        if f == "exact":
+            continue
+        e, n = traj[f]
+        rel = np.abs(e - e_ref).max(1) / np.abs(e_ref).max(1)
+        out[f] = {"energy_rel_err_final": float(rel[-1]), "energy_rel_err_max": float(rel.max()),
+                  "energy_rel_err_max_step": int(rel.argmax()) + 1,
+                  "energy_rel_err_max_after_50": float(rel[49:].max()),
+                  "active_rel_diff_final": float(np.max(np.abs(n - n_ref) / np.maximum(n_ref, 1)))}
+    out["active_mean_final"] = float(n_ref.mean())
+    return out
+
+
 def kernel_timing(col, s, n, torch):
     """Average duration of the fused gather + LCA-update kernel
     (recv_post_mfma_bf16_kernel<4, 1, GatherBF16> for the BF16 form), launched
@@ -144,12 +188,38 @@ def profiled_traffic():
     return None
 
 
+def host_cores():
+    """Threads for the host baseline and what the machine reports: the cores
+    this process may run on, capped by OMP_NUM_THREADS where the job sets it
+    (the job's CPU share), plus nproc, the cgroup CPU quota and the CPU model."""
+    affinity = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(affinity, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else affinity
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            quota = None if q == "max" else round(int(q) / int(period), 2)
+    except (OSError, ValueError):
+        pass
+    return threads, {"nproc": os.cpu_count(), "affinity_cores": affinity, "omp_num_threads": omp,
+                     "cgroup_cpu_quota": quota, "cpu_model": model}
+
+
 def cpu_baseline(images, W, budget_s=12.0):
     """The oracle (C restatement of the reference loops, OpenMP over the host
     cores) on a bounded sample of the workload: a few images, a few steps."""
     import oracle as O
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
+    cores, machine = host_cores()
     nb = 2
     cfg = O.LCAConfig()
     mres = O.lib().or_requiredConvolveMargin(NX, NXV, O.lib().or_calcPostPatchSize(NXP, NXV, NX))
@@ -176,10 +246,12 @@ def cpu_baseline(images, W, budget_s=12.0):
             break
         steps = max(steps + 1, int(steps * min(4.0, budget_s / max(elapsed, 1e-3))))
     per_image_step = elapsed / (steps * nb)
-    return {"value": 1.0 / (per_image_step * NB), "unit": "LCA steps/s", "cores": cores,
-            "kind": "port",
-            "sample": "oracle LCA loop, %d images x %d steps of the 64-image workload (%.1f s), "
-                      "scaled to the 64-image batch" % (nb, steps, elapsed)}
+    out = {"value": 1.0 / (per_image_step * NB), "unit": "LCA steps/s", "cores": cores,
+           "kind": "port",
+           "sample": "oracle LCA loop, %d images x %d steps of the 64-image workload (%.1f s), "
+                     "scaled to the 64-image batch" % (nb, steps, elapsed)}
+    out.update(machine)
+    return out
 
 
 def _time_events(fn, s, reps, torch):
@@ -298,6 +370,65 @@ def c4_hebbian(torch, reps=5, tuner=None):
                                                 / (dws["fp32"] ** 2).mean()))
     # activity read once per update in FP32: the HBM floor of this kernel
     out["pre_activity_bytes"] = int(4 * pre.nbatch * pre.num_ext)
+    return out
+
+
+def c4_learning(torch, steps=300, updates=3):
+    """Large batched LCA + dictionary learning: 8 images 512x512x1 (1/f^2),
+    1024 features 16x16 stride 4 (V1 128x128x1024), VThresh 0.15, BF16 then
+    FP8 deliveries with FP32 state; every 300 steps one weight update (BF16
+    MFMA Hebbian dW at dWMax 1e-3 with normalizeDw, HebbianUpdater's default,
+    FP64 renormalisation per feature, the deliveries' weights re-derived), the
+    step and the update each
+    replayed from a hipGraph.  Timed: `updates` periods of `steps` steps plus
+    their update, with HIP events on the column's stream."""
+    from openpv_amd import abi
+    from openpv_amd.lca import LCAColumn
+    images = one_over_f_images(8, 512, 512, seed=7)
+    W = dictionary(1024, 16, 1, seed=8)
+    out = {"workload": "8 x 512x512x1 -> 128x128x1024, 16x16 stride 4, %d steps per weight update" % steps}
+    for name, ar in (("bf16", abi.ARITH_BF16), ("fp8", abi.ARITH_FP8)):
+        col = LCAColumn(images, W, 128, 128, 1024, 16, 16, VThresh=VTHRESH, timeConstantTau=TAU, dt=DT,
+                        arith=ar, fused=True, device="cuda")
+        col.enable_learning(1e-3, normalize_dw=True, renormalize="fp64")
+        s = torch.cuda.Stream()
+        col.update_weights(s)          # warm-up: the updater's scratch, outside any capture
+        s.synchronize()
+        torch.cuda.synchronize()
+        col.capture(s)
+        col.capture_learning(s)
+        for _ in range(5):
+            col.replay(s)
+        col.replay_learning(s)
+        s.synchronize()
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record(s)
+        for k in range(updates):
+            for _ in range(steps):
+                col.replay(s)
+            col.replay_learning(s)
+        e1.record(s)
+        s.synchronize()
+        total = e0.elapsed_time(e1)
+        # the update alone
+        e0.record(s)
+        for _ in range(5):
+            col.replay_learning(s)
+        e2.record(s)
+        s.synchronize()
+        upd = e0.elapsed_time(e2) / 5
+        energy = col.probe_energy(stream=s)
+        s.synchronize()
+        energy = energy.cpu().numpy()
+        norms = col.weights.double().view(1024, -1).pow(2).sum(1).sqrt()
+        out[name] = {"ms_per_step_with_learning": round(total / (updates * steps), 5),
+                     "steps_per_s_with_learning": round(updates * steps * 1000.0 / total, 2),
+                     "weight_update_ms": round(upd, 4),
+                     "energy_after": float(energy.sum()),
+                     "max_norm_error_fp64": float((norms - 1).abs().max().item())}
+        col.release_graph()
+        del col
+        torch.cuda.empty_cache()
     return out
 
 
@@ -422,21 +553,21 @@ def main():
     gather_macs = NB * NXV * NYV * NFV * NXP * NXP * NF
     smacs = scatter_macs(col)
     ms = res[head]["ms_per_step"]
-    # energy and sparsity of the BF16 form against the EXACT form (same steps)
+    # energy and sparsity of each form against EXACT over the workload's 500 steps
     extra = {}
     if "exact" in cols and head != "exact":
-        e_ref = cols["exact"][0].probe_energy().cpu().numpy()
-        n_ref = cols["exact"][0].num_active().astype(np.float64)
+        acc = accuracy([f for f in forms if f != "exact"], images, W, torch)
         for f in res:
-            if f == "exact":
-                continue
-            e_lp = cols[f][0].probe_energy().cpu().numpy()
-            n_lp = cols[f][0].num_active().astype(np.float64)
-            res[f]["energy_rel_err_vs_exact"] = float(np.max(np.abs(e_lp - e_ref) / np.abs(e_ref)))
-            res[f]["active_rel_diff_vs_exact"] = float(np.max(np.abs(n_lp - n_ref) / np.maximum(n_ref, 1)))
-        extra["energy_rel_err_vs_exact"] = res[head]["energy_rel_err_vs_exact"]
-        extra["active_rel_diff_vs_exact"] = res[head]["active_rel_diff_vs_exact"]
-        extra["active_mean"] = float(col.num_active().astype(np.float64).mean())
+            if f in acc:
+                res[f]["energy_rel_err_vs_exact"] = acc[f]["energy_rel_err_final"]
+                res[f]["energy_rel_err_vs_exact_max_over_steps"] = acc[f]["energy_rel_err_max"]
+                res[f]["active_rel_diff_vs_exact"] = acc[f]["active_rel_diff_final"]
+        extra["accuracy"] = acc
+        extra["energy_rel_err_vs_exact"] = acc[head]["energy_rel_err_final"]
+        extra["energy_rel_err_vs_exact_max_over_steps"] = acc[head]["energy_rel_err_max"]
+        extra["active_rel_diff_vs_exact"] = acc[head]["active_rel_diff_final"]
+        extra["accuracy_steps"] = acc["steps"]
+        extra["active_mean"] = acc["active_mean_final"]
     for f in res:
         res[f]["steps_per_s"] = round(res[f]["steps_per_s"], 2)
         res[f]["ms_per_step"] = round(res[f]["ms_per_step"], 5)
@@ -481,6 +612,7 @@ def main():
     if not args.no_extras:
         line["delivery_c1"] = c1_delivery(torch)
         line["hebbian_c4"] = c4_hebbian(torch, tuner=tuner)
+        line["learning_c4"] = c4_learning(torch)
         line["momentum_c2"] = c2_momentum(torch)
         line["adjoint_c3"] = c3_adjoint(torch)
     if not args.no_cpu_baseline:

@@ -329,7 +329,12 @@ class LCAColumn:
         else:
             cost_sum_f64(self.v1_loc, self.v1_activity(stream), True, 1, out=self.energy[1],
                          stream=stream)
-        return 0.5 * self.energy[0] + self.VThresh * self.energy[1]
+        if stream is None:
+            return 0.5 * self.energy[0] + self.VThresh * self.energy[1]
+        with torch.cuda.stream(stream):     # ordered after the sums on the same stream
+            e = 0.5 * self.energy[0] + self.VThresh * self.energy[1]
+        stream.synchronize()                # a reader on another stream sees the result
+        return e
 
     def set_input(self, images):
         """New input frame(s) [nb][ny][nx][nf] (an input layer's retrieveInput)."""
@@ -394,8 +399,11 @@ class LCAColumn:
             self._active_stale = False
         return self.active
 
-    def num_active(self):
-        return self.active_list().num_active.cpu().numpy()
+    def num_active(self, stream=None):
+        n = self.active_list(stream).num_active
+        if stream is not None:
+            stream.synchronize()
+        return n.cpu().numpy()
 
 
 def run_column(col, steps, frames=None, display_period=0, adaptive=None, energy_cost="l1",

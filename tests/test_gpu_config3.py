@@ -1,0 +1,178 @@
+"""The adjoint of the unrolled LCA loop at its full workload size.
+
+Workload: 16 images 64x64x3 ~N(0,1) (zero mean per image), 256 features
+8x8x3 at stride 2 (V1 32x32x256), VThresh 0.1 (soft threshold), tau 100,
+100 unrolled steps with a checkpoint every 10;
+E = 0.5*||x - W^T a||^2 + 0.1*||a||_1.
+
+Checks:
+  * dE/dW and dE/dx of the FP32 adjoint against the derivative of the same
+    loop in float64 on the GPU (torch autograd through conv2d and its
+    transpose), and the central finite differences (eps = 1e-4) of that loop
+    against the same derivative, along two random directions each;
+  * with the activity held fixed, the weight gradient is minus
+    HebbianUpdater's dW (dWMax 1, normalizeDw off), bit for bit against the
+    oracle's Hebbian restatement in the EXACT forms.
+"""
+import functools
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from openpv_amd import abi
+
+pytestmark = pytest.mark.gpu
+NB, N, NF, NV, NFV, P, STRIDE = 16, 64, 3, 32, 256, 8, 2
+T, CHECKPOINT, VTHRESH, EPS = 100, 10, 0.1, 1e-4
+
+
+@functools.lru_cache(maxsize=None)
+def _problem():
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((NB, N, N, NF)).astype(np.float32)
+    x -= x.mean(axis=(1, 2, 3), keepdims=True)
+    W = rng.standard_normal((NFV, P, P, NF))
+    W = (W / np.sqrt((W.reshape(NFV, -1) ** 2).sum(1)).reshape(-1, 1, 1, 1)).astype(np.float32)
+    return x, W
+
+
+def _adjoint(arith):
+    from openpv_amd.adjoint import LCAAdjoint
+    return LCAAdjoint(NB, N, N, NF, NV, NV, NFV, P, P, VThresh=VTHRESH, timeConstantTau=100.0,
+                      arith=arith, checkpoint=CHECKPOINT, device="cuda")
+
+
+class _Float64Loop:
+    """The same unrolled loop in float64 on the GPU, differentiable by torch
+    autograd: the gather is conv2d over the residual's extended grid from the
+    first window start (the reference's PatchGeometry start of the first V1
+    neuron), the scatter its transpose (conv_transpose2d placed at that start
+    and cropped to the restricted grid), the soft threshold and the LCA update
+    as in the reference kernels."""
+
+    def __init__(self, adj):
+        S, Tl = adj.res_loc, adj.v1_loc
+        oS = O.Loc.make(S.nbatch, S.nx, S.ny, S.nf, S.halo_tuple())
+        oT = O.Loc.make(Tl.nbatch, Tl.nx, Tl.ny, Tl.nf, Tl.halo_tuple())
+        ux = O.patch_axis(oT, oS, P, P, 0)["unshrunken"][Tl.halo.lt:Tl.halo.lt + Tl.nx]
+        uy = O.patch_axis(oT, oS, P, P, 1)["unshrunken"][Tl.halo.up:Tl.halo.up + Tl.ny]
+        assert int(ux[1] - ux[0]) == STRIDE and int(uy[1] - uy[0]) == STRIDE
+        self.S, self.Tl, self.x0, self.y0 = S, Tl, int(ux[0]), int(uy[0])
+        self.e = float(adj.lca.exp_tau[0].item())
+        self.dev = torch.device("cuda")
+
+    def gather(self, r, w):
+        S, Tl = self.S, self.Tl
+        ext = torch.zeros((NB, NF, S.ny_ext, S.nx_ext), dtype=torch.float64, device=self.dev)
+        ext[:, :, S.halo.up:S.halo.up + S.ny, S.halo.lt:S.halo.lt + S.nx] = r
+        out = torch.nn.functional.conv2d(ext[:, :, self.y0:, self.x0:], w, stride=STRIDE)
+        return out[:, :, :Tl.ny, :Tl.nx]
+
+    def scatter(self, a, w):
+        S = self.S
+        y = torch.nn.functional.conv_transpose2d(a, w, stride=STRIDE)
+        h = min(y.shape[2], S.ny_ext - self.y0)
+        wd = min(y.shape[3], S.nx_ext - self.x0)
+        full = torch.zeros((NB, NF, S.ny_ext, S.nx_ext), dtype=torch.float64, device=self.dev)
+        full[:, :, self.y0:self.y0 + h, self.x0:self.x0 + wd] = y[:, :, :h, :wd]
+        return full[:, :, S.halo.up:S.halo.up + S.ny, S.halo.lt:S.halo.lt + S.nx]
+
+    def energy(self, x, W, grad=False):
+        """E (and with grad, dE/dW [NFV][P][P][NF] and dE/dx [NB][N][N][NF] by
+        autograd through the 100 steps)."""
+        d = self.dev
+        w = torch.as_tensor(W, dtype=torch.float64, device=d).permute(0, 3, 1, 2).contiguous()
+        xx = torch.as_tensor(x, dtype=torch.float64, device=d).permute(0, 3, 1, 2).contiguous()
+        w.requires_grad_(grad)
+        xx.requires_grad_(grad)
+        with torch.set_grad_enabled(grad):
+            V = torch.zeros((NB, NFV, NV, NV), dtype=torch.float64, device=d)
+            A = torch.zeros_like(V)
+            rec = torch.zeros_like(xx)
+            for _ in range(T):
+                G = self.gather(xx - rec, w)
+                V = self.e * V + (1 - self.e) * (G + A)
+                A = torch.where(V >= VTHRESH, V - VTHRESH, torch.zeros_like(V))
+                rec = self.scatter(A, w)
+            E = 0.5 * ((xx - rec) ** 2).sum() + float(np.float32(VTHRESH)) * A.abs().sum()
+        if not grad:
+            return float(E)
+        gw, gx = torch.autograd.grad(E, (w, xx))
+        return (float(E), gw.permute(0, 2, 3, 1).contiguous().cpu().numpy(),
+                gx.permute(0, 2, 3, 1).contiguous().cpu().numpy())
+
+
+def test_float64_scatter_is_the_gathers_transpose():
+    """The checker's own consistency: <gather(r), a> = <r, scatter(a)>."""
+    loop = _Float64Loop(_adjoint(abi.ARITH_FP32))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    d = dict(dtype=torch.float64, device="cuda")
+    w = torch.randn(NFV, NF, P, P, generator=g, **d)
+    r = torch.randn(NB, NF, N, N, generator=g, **d)
+    a = torch.randn(NB, NFV, NV, NV, generator=g, **d)
+    lhs = float((loop.gather(r, w) * a).sum())
+    rhs = float((r * loop.scatter(a, w)).sum())
+    assert abs(lhs - rhs) <= 1e-9 * abs(lhs)
+
+
+def test_adjoint_full_size_against_fp64_gradients():
+    """dE/dW and dE/dx of the FP32 hand-written adjoint (checkpoint every 10)
+    against the derivative of the float64 loop (torch autograd), along two
+    random directions each, within 2e-3; and the central finite differences
+    of the float64 loop (eps = 1e-4) against that derivative within 2e-2 (the
+    soft threshold's kinks, crossed inside +-eps by some of the 4M V1
+    neurons over 100 steps, are the finite difference's own error)."""
+    x, W = _problem()
+    adj = _adjoint(abi.ARITH_FP32)
+    print("C3 adjoint: forward + backward, %d steps, checkpoint %d" % (T, CHECKPOINT), flush=True)
+    E, ck = adj.forward(x, W, T)
+    gW, gx = adj.backward(x, T, ck)
+    torch.cuda.synchronize()
+    gW = gW.cpu().numpy().astype(np.float64)
+    gx = gx.cpu().numpy().astype(np.float64)
+    loop = _Float64Loop(adj)
+    print("float64 loop with autograd", flush=True)
+    e64, gW64, gx64 = loop.energy(x, W, grad=True)
+    e32 = float(E.sum().item())
+    print("energy fp32 %.9e fp64 %.9e" % (e32, e64), flush=True)
+    assert abs(e32 - e64) <= 1e-4 * abs(e64)
+    rng = np.random.default_rng(41)
+    for i in range(2):
+        for name, g32, g64, base, perturb in (
+                ("dW", gW, gW64, W, lambda d: (x, W + EPS * d, x, W - EPS * d)),
+                ("dx", gx, gx64, x, lambda d: (x + EPS * d, W, x - EPS * d, W))):
+            d = rng.standard_normal(base.shape)
+            ad = float((g32 * d.ravel()).sum())
+            ex = float((g64.ravel() * d.ravel()).sum())
+            xp, wp, xm, wm = perturb(d)
+            fd = (loop.energy(xp, wp) - loop.energy(xm, wm)) / (2 * EPS)
+            print("%s direction %d: adjoint %.9e float64 autograd %.9e (rel %.2e); fd %.9e (rel %.2e)"
+                  % (name, i, ad, ex, abs(ad - ex) / abs(ex), fd, abs(fd - ex) / abs(ex)), flush=True)
+            assert abs(ad - ex) <= 2e-3 * abs(ex)
+            assert abs(fd - ex) <= 2e-2 * abs(ex)
+
+
+def test_adjoint_full_size_fixed_activity_equals_hebbian_dw():
+    """With the final activity held fixed, dE/dW = corr(A_T, -(x - Rec_T)),
+    minus the Hebbian dW with dWMax 1 and normalizeDw off; EXACT forms against
+    the oracle's or_hebbian_update_dw, bit for bit, at the full workload size
+    after the workload's 100 steps (one step from V = 0 leaves no neuron above
+    VThresh at tau 100)."""
+    x, W = _problem()
+    adj = _adjoint(abi.ARITH_EXACT)
+    E, ck = adj.forward(x, W, T)
+    gW, _ = adj.backward(x, T, ck, fixed_activity=True)
+    torch.cuda.synchronize()
+    S, Tl = adj.res_loc, adj.v1_loc
+    oS = O.Loc.make(S.nbatch, S.nx, S.ny, S.nf, S.halo_tuple())
+    oT = O.Loc.make(Tl.nbatch, Tl.nx, Tl.ny, Tl.nf, Tl.halo_tuple())
+    A1 = adj.A.cpu().numpy()
+    negR = -adj.R_ext.cpu().numpy()
+    assert np.count_nonzero(A1) > 0
+    print("oracle Hebbian dW at %d active of %d" % (np.count_nonzero(A1), A1.size), flush=True)
+    dw, _ = O.hebbian_dw(oT, oS, P, P, NF, 1.0, A1, negR, counts=False)
+    got = gW.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), dw.view(np.uint32))

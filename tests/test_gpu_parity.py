@@ -446,7 +446,9 @@ def test_lca_column_batched_graph_replay():
 
 
 def test_lca_column_bf16_energy_and_sparsity():
-    """BF16 MFMA delivery with FP32 state vs the EXACT column on config[0] shapes."""
+    """BF16 MFMA delivery with FP32 state vs the EXACT column (2 x 32x32x3 ->
+    16x16x128, 8x8 patches at stride 2), 200 steps: energy within 1e-3
+    relative and active counts within 0.5 %."""
     img, W = c0_problem(nb=2, seed=5)
     ex = LCAColumn(img, W, 16, 16, 128, 8, 8, VThresh=0.1, arith=abi.ARITH_EXACT, fused=True,
                    device=DEV)
@@ -460,8 +462,8 @@ def test_lca_column_bf16_energy_and_sparsity():
     n_ex, n_bf = ex.num_active(), bf.num_active()
     rel = np.abs(e_bf - e_ex) / np.abs(e_ex)
     print("bf16 energy rel err", rel, "active", n_ex, n_bf, flush=True)
-    assert rel.max() <= 1e-2
-    assert (np.abs(n_bf - n_ex) <= 0.05 * n_ex + 2).all()
+    assert rel.max() <= 1e-3
+    assert (np.abs(n_bf - n_ex) <= 0.005 * n_ex).all()
 
 
 @pytest.mark.parametrize("case", [
@@ -537,9 +539,9 @@ def test_recv_post_fp8_error_bound(case):
 
 
 def test_lca_column_fp8_energy_and_sparsity():
-    """The config-0 column with the FP8 gather (the scatter runs its BF16 route),
-    200 steps against the EXACT column: energy within 2 % and active counts
-    within 10 % (measured values printed)."""
+    """The same column with the FP8 gather (the scatter runs its BF16 route),
+    200 steps against the EXACT column: energy within 1e-3 relative and active
+    counts within 0.5 % (measured values printed)."""
     img, W = c0_problem(nb=2, seed=5)
     cols = {}
     for name, ar in (("exact", abi.ARITH_EXACT), ("fp8", abi.ARITH_FP8)):
@@ -551,8 +553,8 @@ def test_lca_column_fp8_energy_and_sparsity():
     n_ex, n_lp = cols["exact"].num_active(), cols["fp8"].num_active()
     rel = np.abs(e_lp - e_ex) / np.abs(e_ex)
     print("fp8 column energy rel err", rel, "active", n_ex, n_lp, flush=True)
-    assert rel.max() <= 2e-2
-    assert (np.abs(n_lp - n_ex) <= 0.1 * n_ex + 2).all()
+    assert rel.max() <= 1e-3
+    assert (np.abs(n_lp - n_ex) <= 0.005 * n_ex).all()
 
 
 @pytest.mark.parametrize("arith", [abi.ARITH_FP32, abi.ARITH_BF16])
@@ -955,7 +957,7 @@ def test_gpulca_inputs_adaptive_timescales_exact():
         assert same(got[t], ref[t]), "step %d differs (max %g)" % (t + 1, np.abs(got[t] - ref[t]).max())
 
 
-@pytest.mark.parametrize("arith,tol", [(abi.ARITH_FP32, 1e-4), (abi.ARITH_BF16, 3e-2)])
+@pytest.mark.parametrize("arith,tol", [(abi.ARITH_FP32, 1e-5), (abi.ARITH_BF16, 5e-3)])
 def test_gpulca_inputs_adaptive_timescales_fast_forms(arith, tol):
     steps = 25
     ref = _gpulca_oracle(steps)
@@ -985,7 +987,7 @@ def test_pvrun_two_columns_from_params(tmp_path):
     receiveGpu = false and a device column with receiveGpu = true on the gather)
     and runs 25 steps: the host column's reconstructions equal the oracle's bit
     for bit and its energy file matches the oracle's energies; the device column
-    (FP32) is within 1e-4 of the oracle's scale; the Difference probe's verdict
+    (FP32) is within 1e-5 of the oracle's scale; the Difference probe's verdict
     agrees with the two reconstructions."""
     from openpv_amd import pvp
     steps = 25
@@ -1001,7 +1003,7 @@ def test_pvrun_two_columns_from_params(tmp_path):
     err = np.abs(dev - ref).max()
     diff = np.abs(dev - host).max()
     print("device column: max |dev - oracle| %g (scale %g); max |dev - host| %g" % (err, np.abs(ref).max(), diff))
-    assert err <= 1e-4 * np.abs(ref).max()
+    assert err <= 1e-5 * np.abs(ref).max()
     # RequireAllZeroActivityProbe at nnzThreshold 2e-7 on Difference = ReconDevice - ReconHost
     assert (r.returncode == 0) == (diff <= np.float32(2e-7)), r.stderr[-500:]
 
@@ -1027,7 +1029,7 @@ def test_pvrun_momentum_lca_column(tmp_path):
     assert not same(host, _gpulca_oracle(steps))
 
 
-@pytest.mark.parametrize("form,tol", [("bf16", 3e-2), ("fp8", 6e-2)])
+@pytest.mark.parametrize("form,tol", [("bf16", 5e-3), ("fp8", 5e-2)])
 def test_pvrun_low_precision_device_column(tmp_path, form, tol):
     """gpuArithmetic = "bf16" / "fp8": the device column's gather runs on the
     BF16 or FP8 MFMA path (FP32 state; the FP8 source scale comes from the
