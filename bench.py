@@ -215,12 +215,13 @@ def host_cores():
                      "cgroup_cpu_quota": quota, "cpu_model": model}
 
 
-def cpu_baseline(images, W, budget_s=12.0):
+def cpu_baseline(images, W, budget_s=15.0):
     """The oracle (C restatement of the reference loops, OpenMP over the host
-    cores) on a bounded sample of the workload: a few images, a few steps."""
+    cores) on a bounded sample of the workload: 4 of its images for as many
+    steps as fit in about budget_s seconds."""
     import oracle as O
     cores, machine = host_cores()
-    nb = 2
+    nb = 4
     cfg = O.LCAConfig()
     mres = O.lib().or_requiredConvolveMargin(NX, NXV, O.lib().or_calcPostPatchSize(NXP, NXV, NX))
     mv1 = O.lib().or_requiredConvolveMargin(NXV, NX, NXP)
@@ -242,7 +243,7 @@ def cpu_baseline(images, W, budget_s=12.0):
         t0 = time.perf_counter()
         O.lca_run(cfg, W, img, None, steps)
         elapsed = time.perf_counter() - t0
-        if elapsed >= budget_s or steps >= 64:
+        if elapsed >= budget_s or steps >= 1000:
             break
         steps = max(steps + 1, int(steps * min(4.0, budget_s / max(elapsed, 1e-3))))
     per_image_step = elapsed / (steps * nb)

@@ -392,6 +392,49 @@ typedef struct {
 
 int pvhip_recv_pre(const pvhip_recv_pre_args *args, pvhip_stream stream);
 
+/* The BF16 form of the same delivery for a dense source whose neurons each
+ * send one shared patch (one data patch per source feature) into a target 4
+ * times finer along x and y (the reconstruction of an LCA column): for every
+ * source position p and patch offset o, P[p][o] = sum_c a[p][c]*w[c][o] on
+ * the matrix cores (BF16 operands, FP32 sums), added to the target neuron at
+ * the patch start of p plus o, in an order fixed by the launch (the result
+ * does not depend on scheduling).  Replaces PresynapticPerspectiveGPUDelivery::
+ * deliver (PresynapticPerspectiveGPUDelivery.cpp:198-270) for this geometry;
+ * the additions are those of PresynapticPerspectiveConvolveDelivery.cpp:61-192
+ * in another order.  Requires: target nx, ny = 4 x source; (nxp/4)*(nyp/4)*nfp
+ * == 16; pre.nf 128, 256 or 512. */
+typedef struct {
+   pvhip_loc pre;
+   pvhip_loc post;
+   int nxp, nyp, nfp; /* pre-perspective patch; nfp == post.nf */
+   /* Restricted target x (y) of the unshrunken patch start of source extended
+    * column (row) 0; column xe starts at 4*xe + start_x0
+    * (pvhost_recv_pre_dense_start). */
+   int start_x0, start_y0;
+   /* 1: the caller guarantees the source halo holds only zeros; only the
+    * restricted positions are multiplied.  0: every extended position. */
+   int halo_is_zero;
+   const void *pre_data_bf16c; /* the source activity in the pvhip_activity_to_bf16c layout */
+   const void *weights_bf16;   /* pvhip_recv_pre_dense_weights of the weights */
+   float dt_factor;
+   float *post_gsyn;
+   int accumulate;
+   /* Optional, with accumulate == 0: the target layer's update applied to the
+    * delivered values (as pvhip_recv_post_args.fused_layer); post_gsyn is
+    * then not written. */
+   const struct pvhip_layer_update_args *fused_layer;
+} pvhip_recv_pre_dense_args;
+
+int pvhip_recv_pre_dense(const pvhip_recv_pre_dense_args *args, pvhip_stream stream);
+/* BF16 weights of pvhip_recv_pre_dense: pre-perspective weights
+ * [num_features][patch_size] (patch_size = nyp*nxp*nfp) transposed to
+ * [patch_size][num_features], rounded to nearest even; call after every
+ * weight change (the role of the reference's GPU weight copy,
+ * Weights.cpp:333-341). */
+int pvhip_recv_pre_dense_weights(
+      const float *weights, void *weights_bf16, int num_features, int patch_size, pvhip_stream stream);
+size_t pvhip_recv_pre_dense_weights_bytes(int num_features, int patch_size);
+
 /* ------------------------------------------------------------------------ */
 /* Layer updates                                                             */
 /* ------------------------------------------------------------------------ */

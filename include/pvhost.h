@@ -67,6 +67,15 @@ int pvhost_recv_pre_tables(const pvhip_loc *pre, const pvhip_loc *post, int nxp,
       int32_t *col_ptr, int32_t *col_xe, int32_t *col_wx, int32_t *col_kx, int *ncol,
       int32_t *row_ptr, int32_t *row_ye, int32_t *row_wy, int32_t *row_ky, int *nrow);
 
+/* Patch starts of pvhip_recv_pre_dense_args from PatchGeometry::calcPatchData
+ * (src/components/PatchGeometry.cpp:391-453): 0 and the restricted target
+ * x (y) of the unshrunken patch start of source extended column (row) 0 when
+ * every source extended column xe (row ye) starts at stride*xe + *start_x0
+ * with stride = post.nx / pre.nx (resp. y) and one data patch along the axis
+ * (target no coarser than the source); -1 otherwise.  stride may be NULL. */
+int pvhost_recv_pre_dense_start(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int nyp,
+      int *stride, int *start_x0, int *start_y0);
+
 /* ANNActivityBuffer::setVertices + setSlopes (src/components/ANNActivityBuffer.cpp:227-397)
  * filled into a transfer descriptor; mode selects VTHRESH or VERTICES. */
 int pvhost_make_transfer(float VThresh, float AMin, float AMax, float AShift, float VWidth,

@@ -114,6 +114,13 @@ class RecvPreArgs(C.Structure):
                 ("weights", vp), ("post_gsyn", vp), ("accumulate", i32), ("arith", i32)]
 
 
+class RecvPreDenseArgs(C.Structure):
+    _fields_ = [("pre", Loc), ("post", Loc), ("nxp", i32), ("nyp", i32), ("nfp", i32),
+                ("start_x0", i32), ("start_y0", i32), ("halo_is_zero", i32),
+                ("pre_data_bf16c", vp), ("weights_bf16", vp), ("dt_factor", f32),
+                ("post_gsyn", vp), ("accumulate", i32), ("fused_layer", C.POINTER(LayerUpdateArgs))]
+
+
 class ErrorUpdateArgs(C.Structure):
     _fields_ = [("loc", Loc), ("in_loc", Loc), ("in_", vp), ("rec_loc", Loc), ("rec", vp),
                 ("channel_coeffs", f32 * 2), ("scale", f32), ("apply_scale", i32),
@@ -173,6 +180,9 @@ PVHIP_SYMBOLS = {
     "pvhip_recv_post_bf16_weights": (i32, [vp, vp, i32, i32, vp]),
     "pvhip_recv_post_bf16_weights_bytes": (C.c_size_t, [i32, i32]),
     "pvhip_recv_pre": (i32, [C.POINTER(RecvPreArgs), vp]),
+    "pvhip_recv_pre_dense": (i32, [C.POINTER(RecvPreDenseArgs), vp]),
+    "pvhip_recv_pre_dense_weights": (i32, [vp, vp, i32, i32, vp]),
+    "pvhip_recv_pre_dense_weights_bytes": (C.c_size_t, [i32, i32]),
     "pvhip_lca_update": (i32, [C.POINTER(LCAUpdateArgs), vp]),
     "pvhip_momentum_lca_update": (i32, [C.POINTER(LCAUpdateArgs), C.c_float, vp, vp]),
     "pvhip_ista_update": (i32, [C.POINTER(LCAUpdateArgs), C.c_float, vp]),
@@ -216,6 +226,8 @@ PVHOST_SYMBOLS = {
                                        C.POINTER(i32)]),
     "pvhost_recv_pre_tables": (i32, [C.POINTER(Loc), C.POINTER(Loc), i32, i32, vp, vp, vp, vp,
                                      C.POINTER(i32), vp, vp, vp, vp, C.POINTER(i32)]),
+    "pvhost_recv_pre_dense_start": (i32, [C.POINTER(Loc), C.POINTER(Loc), i32, i32, C.POINTER(i32),
+                                          C.POINTER(i32), C.POINTER(i32)]),
     "pvhost_make_transfer": (i32, [f32, f32, f32, f32, f32, i32, C.POINTER(Transfer)]),
     "pvhost_transpose_weights": (i32, [C.POINTER(Loc), C.POINTER(Loc), i32, i32, i32, vp, vp]),
     "pvhost_transpose_weights_index": (i32, [C.POINTER(Loc), C.POINTER(Loc), i32, i32, i32, vp,

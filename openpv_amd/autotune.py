@@ -132,20 +132,30 @@ class LaunchTuner:
         return best
 
     def tune_recv_pre(self, delivery, gsyn, active, pre_activity, stream=None):
-        """Dense MFMA route or ordered sparse scatter for a
-        PresynapticPerspectiveGPUDelivery (FP32/BF16 forms with a route).
-        Returns "route", "ordered", or None when there is no choice."""
-        if delivery.transposed is None or delivery.arith == abi.ARITH_EXACT or active is None:
+        """Mode of a PresynapticPerspectiveGPUDelivery (FP32/BF16 forms): the
+        dense GEMM ("dense", when its geometry and the source's BF16 copy are
+        there), the transposed route's channel GEMM ("route") or the ordered
+        sparse scatter ("ordered").  Returns the mode, or None when there is no
+        choice."""
+        if delivery.arith == abi.ARITH_EXACT or active is None:
             return None
+        cands = []
+        if delivery.dense is not None and delivery._pre_bf16c is not None:
+            cands.append(("dense",))
+        if delivery.transposed is not None:
+            cands.append(("route",))
+            self.tune_recv_post(delivery.transposed, pre_activity, gsyn, stream=stream)
+        if not cands:
+            return None
+        cands.append(("ordered",))
         stream = stream or torch.cuda.current_stream()
-        self.tune_recv_post(delivery.transposed, pre_activity, gsyn, stream=stream)
-        key = "recv_pre|%s|abi%d|arith%d|%s|%s|%dx%d" % (
+        key = "recv_pre|%s|abi%d|arith%d|%s|%s|%dx%d|%s" % (
             self._device(), abi.hip().pvhip_abi_version(), delivery.arith, _loc_key(delivery.pre),
-            _loc_key(delivery.post), delivery.nxp, delivery.nyp)
+            _loc_key(delivery.post), delivery.nxp, delivery.nyp, ",".join(c[0] for c in cands))
 
         def apply(mode):
-            delivery.use_route = mode == "route"
-        best = self._pick(key, [("route",), ("ordered",)], apply,
+            delivery.mode = mode
+        best = self._pick(key, cands, apply,
                           lambda: delivery.deliver(gsyn, active=active, pre_activity=pre_activity,
                                                    stream=stream), stream)
         return best[0]

@@ -15,7 +15,8 @@ LIBDIR = os.path.join(HERE, "lib")
 CSRC = os.path.join(HERE, "csrc")
 HOSTSRC = os.path.join(HERE, "host")
 
-HIP_SOURCES = ["device.hip", "layer.hip", "recv_post.hip", "recv_pre.hip", "hebbian.hip", "adjoint.hip"]
+HIP_SOURCES = ["device.hip", "layer.hip", "recv_post.hip", "recv_pre.hip", "recv_pre_dense.hip", "hebbian.hip",
+               "adjoint.hip"]
 HOST_SOURCES = ["geometry.cpp", "params.cpp"]
 
 # -ffp-contract=off keeps mul and add separately rounded in the EXACT kernels

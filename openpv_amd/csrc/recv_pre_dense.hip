@@ -1,0 +1,488 @@
+// Pre-perspective delivery of a dense source as a GEMM on the matrix cores
+// (PresynapticPerspectiveGPUDelivery / CudaRecvPre replacement for the
+// BF16 form; the reference order is PresynapticPerspectiveConvolveDelivery.cpp:
+// 61-192, for each source neuron: GSyn[patch] += a * w[patch]).
+//
+// For a source whose neurons each send one shared patch (one data patch per
+// source feature) into a target finer by a factor of 4 along x and y, every
+// source position p (all its C channels) and patch offset o give one product
+//     P[p][o] = sum_c a[p][c] * w[c][o],
+// added to the target neuron at start(p) + o.  That is a GEMM with M = source
+// positions, N = patch offsets, K = channels (v_mfma_f32_16x16x32_bf16 on the
+// BF16 copy of the activity and a BF16 copy of the weights), followed by the
+// overlap-add of the products into the target.
+//
+// Work split.  The target neurons whose row is q modulo 4 receive only the
+// offsets whose row is q - start_y modulo 4: one workgroup per (batch
+// element, q) owns those target rows outright, keeps them in LDS, and needs
+// only that quarter of the weights (staged in LDS once).  Within a workgroup
+// the 4 waves take the source positions in groups of DMG blocks of 16 and
+// compute all four column classes (target column modulo 4) for them, so each
+// source fragment feeds four MFMAs and each weight fragment DMG.  The
+// overlap-add runs in four rounds separated by barriers: in round rr wave w
+// adds its products of column class (w + rr) mod 4, so no two waves touch a
+// target neuron at the same time and every neuron receives its products in
+// a fixed order: the result does not depend on scheduling.
+//
+// Numerics: BF16 products, FP32 sums (MFMA order over the channels, then the
+// fixed overlap-add order), the form's tolerance, not the EXACT order.
+#include "pvhip_internal.hpp"
+
+#include <algorithm>
+
+namespace pvhip {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int DST = 4; // target cells per source cell along x and y
+constexpr int DMG = 4; // position blocks (16 source positions) per wave and group
+
+// Division by a divisor fixed per launch: n / d = (umulhi(n, mul) + n) >> shift
+// for 0 <= n < 2^31.
+struct FastDiv {
+   uint32_t mul;
+   int shift;
+};
+
+inline FastDiv make_fastdiv(int d) {
+   int l = 0;
+   while ((1u << l) < (uint32_t)d) {
+      l++;
+   }
+   FastDiv f;
+   f.shift = l;
+   f.mul   = (uint32_t)(((uint64_t)1 << 32) * (((uint64_t)1 << l) - (uint64_t)d) / (uint64_t)d + 1);
+   return f;
+}
+
+struct DenseParams {
+   Geo pre, post;
+   int nxp, nyp, nfp;
+   int x0, y0;             // target x (y) of the unshrunken patch start of source extended column (row) 0
+   int px0, py0, pnx, pny; // source positions multiplied (extended coordinates)
+   int C;                  // source features (channels)
+   const __bf16 *A;        // chunked BF16 activity [nbatch][C/32][nyExt][nxExt][32]
+   const __bf16 *Wt;       // [nyp*nxp*nfp][C]
+   float dtf;
+   float *gsyn;
+   int accumulate;
+   int layer; // the target layer's update applied instead of writing GSyn
+   Geo layerLoc;
+   float layerCoeff, layerScale;
+   int layerApplyScale;
+   float *layerV, *layerA;
+   Transfer layerT;
+   // LDS tile of a workgroup: target columns Xlo .. Xlo + pitch/nfp - 1 (the
+   // target plus the reach of the patches of the multiplied positions past
+   // its edges), rows for target rows Ymin .. Ymax (likewise), then 5 rows
+   // that absorb the products of the padding positions of the last group.
+   int pitch;
+   int Xlo, Ymin, Ymax;
+   int rows; // rows of every workgroup's tile before the absorbing rows
+   FastDiv fdx, fdxf; // division by pnx and by post.nx*nfp
+};
+
+__device__ __forceinline__ int pmod4(int a) { return a & 3; } // two's complement: a mod 4 in [0, 4)
+
+// n / d by a multiply for 0 <= n < 2^31 (Granlund-Montgomery, l = ceil(log2 d)).
+__device__ __forceinline__ int fdiv(int n, FastDiv f) {
+   uint32_t const t = __umulhi((uint32_t)n, f.mul);
+   return (int)((t + (uint32_t)n) >> f.shift);
+}
+
+// KS = C / 32 k-steps.  The source fragments of one k-step are one register
+// stage; NB stages are in flight (the loads of k-step s + NB - 1 are issued
+// before the MFMAs of k-step s, running on into the next group's first
+// k-steps).  NW = 8 waves per workgroup: two per SIMD hide each other's
+// latencies.  Wave w keeps the products of class (w + k) mod 4 in its
+// accumulator slot k, so that the overlap-add reads static registers.
+// ATOMIC: LDS atomic adds in the overlap-add, for patches wider than 16 or
+// position grids narrower than 12 + nxp/4 (the four positions of one add, 4
+// apart in row-major order, may then share a target neuron); otherwise the
+// adds of one instruction never share one and a plain read-add-write is
+// exact.
+template <int KS, int NW, int NB, bool ATOMIC>
+__global__ void __launch_bounds__(64 * NW) recv_pre_dense_kernel(DenseParams p) {
+   constexpr int NT = 64 * NW;
+   static_assert(KS % NB == 0 && NB % 2 == 0, "k-step s of every group lives in buffer s % NB");
+   static_assert(NW % 4 == 0, "whole rounds of four class owners");
+   extern __shared__ float smem[];
+   __bf16 *wl = reinterpret_cast<__bf16 *>(smem); // [4 classes][KS][16 columns][32 channels]
+   float *ol  = smem + 4 * KS * 16 * 32 / 2;       // tile [rows + 5][pitch]
+
+   // the four workgroups of one batch element on one XCD (workgroup L runs on
+   // XCD L % 8), so that they share the element's activity in that XCD's L2
+   int const L = blockIdx.x;
+   int const t = L / 8;
+   int const q = t % 4;
+   int const b = (t / 4) * 8 + L % 8;
+   if (b >= p.post.nbatch) {
+      return; // whole workgroup
+   }
+   constexpr int C = KS * 32;
+   int const tid   = threadIdx.x;
+   int const lane  = tid % 64;
+   int const wave  = __builtin_amdgcn_readfirstlane(tid / 64);
+   int const r     = lane % 16;
+   int const g     = lane / 16;
+   int const NXQ   = p.nxp / DST;
+   int const oyq   = pmod4(q - p.y0); // offset rows of this workgroup: oyq + 4*jy
+   int const jlo   = (p.Ymin - q) >> 2; // tile row 0 = target row q + 4*jlo (floor division)
+
+   // Weights of this workgroup's offsets, in BF16: per class c (offset column
+   // (c - x0) mod 4 + 4*ix) and k-step, 16 columns (iy, ix, f) x 32 channels.
+   // Item it = tid + u*NT: its column and 16-byte piece depend on tid only.
+   {
+      constexpr int WIT = 4 * KS * 64 / NT;
+      int const gg = tid % 4;
+      int const n  = (tid / 4) % 16;
+      int const f  = n % p.nfp;
+      int const ix = (n / p.nfp) % NXQ;
+      int const oy = oyq + DST * (n / (p.nfp * NXQ));
+      int const jw = tid / 64;
+      // element offset of (column, piece) in the [offset][C] weights, per class
+      int wcol[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+         wcol[c] = ((oy * p.nxp + pmod4(c - p.x0) + DST * ix) * p.nfp + f) * C + 8 * gg;
+      }
+#pragma unroll
+      for (int u = 0; u < WIT; u++) {
+         int const j = jw + u * NW; // class * KS + k-step
+         int const c = j / KS;
+         int wo      = wcol[0];
+#pragma unroll
+         for (int cc = 1; cc < 4; cc++) {
+            wo = c == cc ? wcol[cc] : wo;
+         }
+         uint4 const v = *reinterpret_cast<const uint4 *>(p.Wt + wo + (j % KS) * 32);
+         *reinterpret_cast<uint4 *>(wl + (j * 16 + n) * 32 + 8 * gg) = v;
+      }
+   }
+   for (int i = tid; i < (p.rows + 5) * p.pitch; i += NT) {
+      ol[i] = 0.0f;
+   }
+
+   int const nxExt      = p.pre.nxExt();
+   size_t const plane32 = (size_t)p.pre.nyExt() * nxExt * 32;
+   const __bf16 *Ab     = p.A + (size_t)b * (C / 32) * plane32;
+   int const nPos       = p.pnx * p.pny;
+   int const nMB        = (nPos + 15) / 16;
+   int const rounds     = (nMB + NW * DMG - 1) / (NW * DMG);
+
+   // source element offsets (chunk plane 0) of this lane's A-fragment rows.
+   // A row past the last position reads position 0 instead: its products
+   // land in output rows that the overlap-add skips (a row of the product
+   // depends on its own A row only), so the loads need no mask.
+   int aoff[DMG];
+   auto set_rows = [&](int mb0) {
+#pragma unroll
+      for (int m = 0; m < DMG; m++) {
+         int const pos = (mb0 + m) * 16 + r;
+         int const pc  = pos < nPos ? pos : 0;
+         int const yr  = fdiv(pc, p.fdx);
+         aoff[m]       = ((p.py0 + yr) * nxExt + p.px0 + pc - yr * p.pnx) * 32 + 8 * g;
+      }
+   };
+   bf16x8 av[NB][DMG];
+   auto fetch = [&](int buf, int ks) {
+#pragma unroll
+      for (int m = 0; m < DMG; m++) {
+         av[buf][m] = *reinterpret_cast<const bf16x8 *>(Ab + (size_t)ks * plane32 + aoff[m]);
+      }
+   };
+   // weight fragments of k-step ks for the wave's four slots
+   int wofs[4];
+#pragma unroll
+   for (int k = 0; k < 4; k++) {
+      wofs[k] = (((wave + k) & 3) * KS * 16 + r) * 32 + 8 * g;
+   }
+   bf16x8 bw[2][4];
+   auto loadB = [&](int buf, int ks) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+         bw[buf][k] = *reinterpret_cast<const bf16x8 *>(wl + wofs[k] + ks * 16 * 32);
+      }
+   };
+   // tile offset of this lane's column (jy, jx, f)
+   int const fo      = r % p.nfp;
+   int const jx      = (r / p.nfp) % NXQ;
+   int const jy      = r / (p.nfp * NXQ);
+   int const laneOff = jy * p.pitch + DST * jx * p.nfp + fo;
+   int const rowOff  = ((p.y0 + oyq - q) >> 2) - jlo; // (y0 + oyq - q) is a multiple of 4
+
+   set_rows(wave * DMG);
+#pragma unroll
+   for (int st = 0; st < NB - 1; st++) {
+      fetch(st, st); // in flight during the barrier below
+   }
+   __syncthreads(); // the staged weights and the cleared tile
+   for (int rd = 0; rd < rounds; rd++) {
+      int const mb0 = (rd * NW + wave) * DMG;
+      f32x4 acc[DMG][4];
+#pragma unroll
+      for (int m = 0; m < DMG; m++) {
+#pragma unroll
+         for (int k = 0; k < 4; k++) {
+            acc[m][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+         }
+      }
+      loadB(0, 0);
+#pragma unroll 1
+      for (int sg = 0; sg < KS / NB; sg++) {
+#pragma unroll
+         for (int bb = 0; bb < NB; bb++) {
+            int const ks = sg * NB + bb;
+            int const sf = ks + NB - 1; // the k-step whose loads are issued now
+            if (sf < KS) {
+               fetch((bb + NB - 1) % NB, sf);
+            }
+            else if (rd + 1 < rounds) {
+               if (sf == KS) {
+                  set_rows(((rd + 1) * NW + wave) * DMG); // the next group's first k-steps, in
+               }                                          // flight during this group's last ones
+               fetch((bb + NB - 1) % NB, sf - KS);        // and the overlap-add
+            }
+            if (ks + 1 < KS) {
+               loadB((bb + 1) & 1, ks + 1); // in flight during this k-step's MFMAs
+            }
+            // keep the loads ahead of this k-step's MFMAs (the scheduler would
+            // otherwise sink them next to their use)
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+#pragma unroll
+               for (int m = 0; m < DMG; m++) {
+                  acc[m][k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[bb][m], bw[bb & 1][k], acc[m][k], 0, 0, 0);
+               }
+            }
+         }
+      }
+      // Overlap-add: lane (r, g), register i of block m, slot k holds the
+      // product of source position (mb0 + m)*16 + 4g + i and column r of class
+      // c = (w + k) mod 4, i.e. of target row 4*ye + y0 + oyq + 4*jy and column
+      // 4*xe + x0 + (c - x0) mod 4 + 4*jx.  Tile address = position part +
+      // lane part + class part; a padding position adds into the absorbing
+      // rows.
+      int pb[DMG][4];
+#pragma unroll
+      for (int m = 0; m < DMG; m++) {
+#pragma unroll
+         for (int i = 0; i < 4; i++) {
+            int const pos = (mb0 + m) * 16 + 4 * g + i;
+            int const yr  = fdiv(pos, p.fdx);
+            int const xe  = p.px0 + pos - yr * p.pnx;
+            pb[m][i]      = pos < nPos ? (p.py0 + yr + rowOff) * p.pitch + (DST * xe + p.x0 - p.Xlo) * p.nfp
+                                       : p.rows * p.pitch;
+         }
+      }
+      // NW rounds: in round rr the four waves 4*(rr/4) .. +3 add, each its
+      // slot rr mod 4 (class (w + rr) mod 4), so no wave touches a tile
+      // element another wave touches in the same round; LDS keeps a wave's
+      // accesses in order, so when the 64 lanes of one add have distinct
+      // elements a plain read-add-write is exact (and far cheaper than LDS
+      // atomics).
+#pragma unroll
+      for (int rr = 0; rr < NW; rr++) {
+         __syncthreads();
+         if (wave / 4 == rr / 4) { // the same for the whole wave
+            int const k   = rr & 3;
+            int const off = laneOff + pmod4(((wave + k) & 3) - p.x0) * p.nfp;
+#pragma unroll
+            for (int m = 0; m < DMG; m++) {
+#pragma unroll
+               for (int i = 0; i < 4; i++) {
+                  if constexpr (ATOMIC) {
+                     atomicAdd(&ol[pb[m][i] + off], acc[m][rr & 3][i]);
+                  }
+                  else {
+                     ol[pb[m][i] + off] += acc[m][rr & 3][i];
+                  }
+               }
+            }
+         }
+      }
+   }
+   __syncthreads();
+
+   // Epilogue: the rows q + 4j of this batch element, written once.
+   int const nxf = p.post.nx * p.nfp;
+   int const nJ  = (p.post.ny - q + DST - 1) / DST; // target rows q + 4j < ny
+   for (int i = tid; i < nJ * nxf; i += NT) {
+      int const j     = fdiv(i, p.fdxf);
+      int const xf    = i - j * nxf;
+      int const Y     = q + DST * j;
+      float const s   = ol[(j - jlo) * p.pitch + xf - p.Xlo * p.nfp];
+      size_t const gi = (size_t)b * p.post.numRes() + (size_t)Y * nxf + xf;
+      if (p.layer) {
+         Geo const &Ll   = p.layerLoc;
+         int const X     = xf / p.nfp;
+         size_t const ai = (size_t)b * Ll.numExt() + ((size_t)(Y + Ll.up) * Ll.nxExt() + X + Ll.lt) * p.nfp + xf % p.nfp;
+         float const G   = 0.0f + p.dtf * s;
+         float V         = 0.0f;
+         float prod      = p.layerCoeff * G;
+         V               = V + prod;
+         if (p.layerApplyScale) {
+            V = V * p.layerScale;
+         }
+         p.layerV[gi] = V;
+         if (p.layerA) {
+            p.layerA[ai] = apply_transfer(p.layerT, V);
+         }
+      }
+      else {
+         float const base = p.accumulate ? p.gsyn[gi] : 0.0f;
+         p.gsyn[gi]       = base + p.dtf * s;
+      }
+   }
+}
+
+// weights [C][nyp][nxp][nfp] (pre-perspective, one data patch per source
+// feature) -> BF16 [nyp*nxp*nfp][C], rounded to nearest even
+__global__ void __launch_bounds__(256) recv_pre_dense_weights_kernel(
+      const float *w, __bf16 *out, int C, int patch) {
+   size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= (size_t)C * patch) {
+      return;
+   }
+   int const o = (int)(i / C);
+   int const c = (int)(i % C);
+   out[i]      = (__bf16)w[(size_t)c * patch + o];
+}
+
+size_t dense_lds_bytes(int KS, int nJ, int pitch) { return (size_t)4 * KS * 16 * 32 * 2 + (size_t)nJ * pitch * 4; }
+
+} // namespace
+} // namespace pvhip
+
+using namespace pvhip;
+
+extern "C" {
+
+int pvhip_recv_pre_dense(const pvhip_recv_pre_dense_args *a, pvhip_stream stream) {
+   PVHIP_REQUIRE(a != nullptr, "recv_pre_dense: NULL args");
+   PVHIP_REQUIRE(loc_valid(a->pre) && loc_valid(a->post), "recv_pre_dense: invalid loc");
+   PVHIP_REQUIRE(a->pre.nbatch == a->post.nbatch, "recv_pre_dense: batch sizes differ");
+   PVHIP_REQUIRE(a->post.nx == DST * a->pre.nx && a->post.ny == DST * a->pre.ny,
+         "recv_pre_dense: the target must be 4 times the source along x and y (%dx%d -> %dx%d)", a->pre.nx,
+         a->pre.ny, a->post.nx, a->post.ny);
+   PVHIP_REQUIRE(a->nfp == a->post.nf && a->nxp % DST == 0 && a->nyp % DST == 0
+                       && (a->nxp / DST) * (a->nyp / DST) * a->nfp == 16,
+         "recv_pre_dense: patch %dx%dx%d is not 16 offsets per class of 4x4", a->nxp, a->nyp, a->nfp);
+   int const C = a->pre.nf;
+   PVHIP_REQUIRE(C == 128 || C == 256 || C == 512, "recv_pre_dense: %d source features (128, 256 or 512)", C);
+   PVHIP_REQUIRE(a->pre_data_bf16c && a->weights_bf16, "recv_pre_dense: NULL activity or weights");
+   DenseParams p{};
+   p.pre   = geo(a->pre);
+   p.post  = geo(a->post);
+   p.nxp   = a->nxp;
+   p.nyp   = a->nyp;
+   p.nfp   = a->nfp;
+   p.x0    = a->start_x0;
+   p.y0    = a->start_y0;
+   if (a->halo_is_zero) {
+      p.px0 = a->pre.halo.lt;
+      p.py0 = a->pre.halo.up;
+      p.pnx = a->pre.nx;
+      p.pny = a->pre.ny;
+   }
+   else {
+      p.px0 = 0;
+      p.py0 = 0;
+      p.pnx = p.pre.nxExt();
+      p.pny = p.pre.nyExt();
+   }
+   p.C     = C;
+   p.A     = static_cast<const __bf16 *>(a->pre_data_bf16c);
+   p.Wt    = static_cast<const __bf16 *>(a->weights_bf16);
+   p.dtf   = a->dt_factor;
+   p.gsyn  = a->post_gsyn;
+   p.accumulate = a->accumulate;
+   p.layer = a->fused_layer != nullptr;
+   if (p.layer) {
+      const pvhip_layer_update_args &l = *a->fused_layer;
+      PVHIP_REQUIRE(!a->accumulate, "recv_pre_dense: fused_layer needs accumulate == 0");
+      PVHIP_REQUIRE(loc_valid(l.loc) && l.loc.nbatch == a->post.nbatch && l.loc.nx == a->post.nx
+                          && l.loc.ny == a->post.ny && l.loc.nf == a->post.nf,
+            "recv_pre_dense: fused_layer loc differs from post loc");
+      PVHIP_REQUIRE(l.V != nullptr, "recv_pre_dense: fused_layer needs V");
+      p.layerLoc        = geo(l.loc);
+      p.layerCoeff      = l.channel_coeffs[0];
+      p.layerScale      = l.scale;
+      p.layerApplyScale = l.apply_scale;
+      p.layerV          = l.V;
+      p.layerA          = l.A;
+      int st            = make_transfer(l.transfer, &p.layerT);
+      if (st) {
+         return st;
+      }
+   }
+   else {
+      PVHIP_REQUIRE(a->post_gsyn != nullptr, "recv_pre_dense: NULL post_gsyn");
+   }
+   // tile extent: the target, widened to every column / row a multiplied
+   // position's patch reaches
+   int const Xmin = std::min(0, DST * p.px0 + p.x0);
+   int const Xmax = std::max(a->post.nx - 1, DST * (p.px0 + p.pnx - 1) + p.x0 + a->nxp - 1);
+   p.Ymin         = std::min(0, DST * p.py0 + p.y0);
+   p.Ymax         = std::max(a->post.ny - 1, DST * (p.py0 + p.pny - 1) + p.y0 + a->nyp - 1);
+   p.Xlo          = Xmin;
+   p.pitch        = (Xmax - Xmin + 1) * a->nfp;
+   p.pitch += (p.pitch % 2 == 0); // odd: consecutive rows start in different banks
+   // rows (Y - q)/4 - floor((Ymin - q)/4) for Y <= Ymax, over every q
+   p.rows        = (p.Ymax - p.Ymin) / DST + 2;
+   p.fdx         = make_fastdiv(p.pnx);
+   p.fdxf        = make_fastdiv(a->post.nx * a->nfp);
+   int const KS  = C / 32;
+   size_t const lds = dense_lds_bytes(KS, p.rows + 5, p.pitch);
+   PVHIP_REQUIRE(lds <= 160 * 1024, "recv_pre_dense: %zu bytes of LDS (tile rows of %d values)", lds, p.pitch);
+   unsigned const groups = (unsigned)((a->post.nbatch + 7) / 8);
+   dim3 grid(groups * 8 * 4);
+   hipStream_t s = (hipStream_t)stream;
+#ifndef RPD_NB
+#define RPD_NB 4
+#endif
+   constexpr int NW = 8, NB = RPD_NB;
+   // the four positions of one add lie 4 apart in row-major order: in one row
+   // their patches (nxp/4 source columns wide) are disjoint when nxp <= 16,
+   // across a row wrap when the grid is at least 12 + nxp/4 wide
+   bool const atomic = a->nxp > 16 || p.pnx < 12 + a->nxp / DST;
+#define RPD_LAUNCH(KSV, NBV)                                                                      \
+   if (atomic) {                                                                                 \
+      hipLaunchKernelGGL((recv_pre_dense_kernel<KSV, NW, NBV, true>), grid, dim3(64 * NW), lds, s, p); \
+   }                                                                                             \
+   else {                                                                                        \
+      hipLaunchKernelGGL((recv_pre_dense_kernel<KSV, NW, NBV, false>), grid, dim3(64 * NW), lds, s, p); \
+   }
+   if (KS == 4) {
+      RPD_LAUNCH(4, (NB < 4 ? NB : 4))
+   }
+   else if (KS == 8) {
+      RPD_LAUNCH(8, NB)
+   }
+   else {
+      RPD_LAUNCH(16, NB)
+   }
+#undef RPD_LAUNCH
+   PVHIP_LAUNCHED("recv_pre_dense");
+   return PVHIP_OK;
+}
+
+int pvhip_recv_pre_dense_weights(
+      const float *weights, void *weights_bf16, int num_features, int patch_size, pvhip_stream stream) {
+   PVHIP_REQUIRE(weights && weights_bf16 && num_features > 0 && patch_size > 0,
+         "recv_pre_dense_weights: bad argument");
+   size_t const n = (size_t)num_features * patch_size;
+   hipLaunchKernelGGL(recv_pre_dense_weights_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+         (hipStream_t)stream, weights, static_cast<__bf16 *>(weights_bf16), num_features, patch_size);
+   PVHIP_LAUNCHED("recv_pre_dense_weights");
+   return PVHIP_OK;
+}
+
+size_t pvhip_recv_pre_dense_weights_bytes(int num_features, int patch_size) {
+   return (size_t)num_features * patch_size * 2;
+}
+
+} // extern "C"

@@ -391,6 +391,56 @@ int pvhost_recv_pre_tables(const pvhip_loc *pre, const pvhip_loc *post, int nxp,
    return 0;
 }
 
+int pvhost_recv_pre_dense_start(const pvhip_loc *pre, const pvhip_loc *post, int nxp, int nyp,
+      int *stride, int *start_x0, int *start_y0) {
+   if (!locOk(pre) || !locOk(post) || nxp <= 0 || nyp <= 0 || !start_x0 || !start_y0) {
+      return -1;
+   }
+   if (post->nx % pre->nx || post->ny % pre->ny || post->nx / pre->nx != post->ny / pre->ny) {
+      return -1; // a coarser target, or different strides along x and y
+   }
+   int const s = post->nx / pre->nx;
+   int starts[2];
+   for (int axis = 0; axis < 2; axis++) {
+      Axis a    = axisOf(*pre, *post, nxp, nyp, axis);
+      int nExt  = axis == 0 ? pre->nx + pre->halo.lt + pre->halo.rt : pre->ny + pre->halo.dn + pre->halo.up;
+      int patch = axis == 0 ? nxp : nyp;
+      int nPost = axis == 0 ? post->nx : post->ny;
+      bool have = false;
+      int s0    = 0;
+      for (int e = 0; e < nExt; e++) {
+         AxisEntry en = axisEntry(a, e);
+         if (en.dim <= 0) {
+            continue;
+         }
+         int start = en.postStartRes - en.patchStart; // restricted target index of patch element 0
+         if (!have) {
+            s0   = start - s * e;
+            have = true;
+         }
+         else if (start != s * e + s0) {
+            return -1;
+         }
+      }
+      if (!have) {
+         return -1;
+      }
+      // columns whose kept patch is empty must not reach the target either
+      for (int e = 0; e < nExt; e++) {
+         if (axisEntry(a, e).dim <= 0 && s * e + s0 + patch > 0 && s * e + s0 < nPost) {
+            return -1;
+         }
+      }
+      starts[axis] = s0;
+   }
+   if (stride) {
+      *stride = s;
+   }
+   *start_x0 = starts[0];
+   *start_y0 = starts[1];
+   return 0;
+}
+
 int pvhost_make_transfer(float VThresh, float AMin, float AMax, float AShift, float VWidth,
       int mode, pvhip_transfer *out) {
    if (!out || (mode != PVHIP_TRANSFER_VTHRESH && mode != PVHIP_TRANSFER_VERTICES)) {

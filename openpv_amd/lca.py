@@ -114,9 +114,11 @@ class LCAColumn:
         self.lca = HyPerLCAUpdate(self.v1_loc, self.v1_transfer, timeConstantTau, dt,
                                   self_interact, activity_from_v=True, device=d)
         self.energy = torch.zeros(2, nb, dtype=torch.float64, device=d)
-        # BF16 forms whose scatter route is a channel GEMM: V1's update writes
-        # its activity as chunked BF16, which the GEMM reads instead of
-        # converting the FP32 activity (the same BF16 values, half the bytes).
+        # BF16 forms whose scatter reads the dense activity (the dense GEMM or
+        # the transposed route's channel GEMM): V1's update writes its activity
+        # as chunked BF16, which the scatter reads instead of converting the
+        # FP32 activity (the same BF16 values, half the bytes); V1's halo is
+        # never written, so the dense GEMM multiplies restricted positions only.
         # Such a step writes no FP32 activity: nothing in it reads one (the next
         # update recomputes A(t-1) = T(V) from V), so v1_A is derived from V, bit
         # for bit, when a reader asks for it (v1_activity()).
@@ -124,7 +126,7 @@ class LCAColumn:
         self._a_stale = False
         if bf16_copy and arith in (abi.ARITH_BF16, abi.ARITH_FP8) and nf_v1 % 32 == 0:
             buf = bf16c_buffer(self.v1_loc, d)
-            if self.scatter.bind_pre_bf16c(buf):
+            if self.scatter.bind_pre_bf16c(buf, halo_is_zero=True):
                 self.v1_A16 = buf
         # Recon = ANNLayer with the scatter as its only channel: the dense
         # route's channel GEMM applies this update in its epilogue
@@ -389,7 +391,7 @@ class LCAColumn:
     def dense_scatter(self):
         """True when the Recon scatter reads V1's dense activity (the MFMA route)
         rather than its active list."""
-        return self.scatter.transposed is not None and getattr(self.scatter, "use_route", True)
+        return self.scatter.use_route
 
     def active_list(self, stream=None):
         """V1's active list after the last step (rebuilt here when the step did

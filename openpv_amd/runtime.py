@@ -266,8 +266,16 @@ class ActiveList:
 
 
 class PresynapticPerspectiveGPUDelivery:
-    """Scatter delivery (PresynapticPerspectiveGPUDelivery + CudaRecvPre), evaluated
-    as an ordered gather over the active list."""
+    """Scatter delivery (PresynapticPerspectiveGPUDelivery + CudaRecvPre).  Three
+    ways to run it, chosen by `mode`:
+      "ordered" -- an ordered gather over the active list (EXACT, or FP32 with FMA);
+      "route"   -- FP32/BF16: a post-perspective gather with the transposed
+                   weights (a channel GEMM over the dense activity);
+      "dense"   -- BF16, one data patch per source feature into a target 4x
+                   finer (pvhip_recv_pre_dense): a GEMM of the source positions
+                   by the patch offsets over the channels, then the overlap-add;
+                   needs the source's chunked BF16 copy (bind_pre_bf16c).
+    The default is the first of dense, route, ordered that the geometry allows."""
 
     def __init__(self, pre, post, nxp, nyp, weights, arith=abi.ARITH_EXACT, dt_factor=1.0):
         self.pre, self.post = pre, post
@@ -300,8 +308,70 @@ class PresynapticPerspectiveGPUDelivery:
         self.nkx, self.nky = nkx.value, nky.value
         self.weights = weights
         self.transposed = None
+        self.dense = None
+        self._pre_bf16c = None
+        self._dense_halo_zero = False
+        self._mode = None
         if arith in (abi.ARITH_BF16, abi.ARITH_FP32):
             self.transposed = self._transposed_route()
+        if arith == abi.ARITH_BF16:
+            self.dense = self._dense_route()
+
+    def _dense_route(self):
+        """The geometry of pvhip_recv_pre_dense (see the class docstring), with
+        its BF16 weights; None when the geometry does not fit."""
+        pre, post = self.pre, self.post
+        if (post.nx != 4 * pre.nx or post.ny != 4 * pre.ny or self.nxp % 4 or self.nyp % 4
+                or (self.nxp // 4) * (self.nyp // 4) * self.nfp != 16 or pre.nf not in (128, 256, 512)
+                or self.nkx != 1 or self.nky != 1):
+            return None
+        x0, y0 = C.c_int(), C.c_int()
+        if abi.host().pvhost_recv_pre_dense_start(C.byref(pre), C.byref(post), self.nxp, self.nyp, None,
+                                                  C.byref(x0), C.byref(y0)) != 0:
+            return None
+        patch = self.nxp * self.nyp * self.nfp
+        wb = torch.zeros(patch * pre.nf, dtype=torch.bfloat16, device=self.weights.device)
+        self.dense = {"start": (x0.value, y0.value), "weights": wb, "patch": patch}
+        self._refresh_dense(None)
+        return self.dense
+
+    def _refresh_dense(self, stream):
+        d = self.dense
+        abi.check(abi.hip().pvhip_recv_pre_dense_weights(_ptr(self.weights), _ptr(d["weights"]), self.pre.nf,
+                                                         d["patch"], _stream(stream)),
+                  "pvhip_recv_pre_dense_weights")
+
+    @property
+    def mode(self):
+        if self._mode is not None:
+            return self._mode
+        if self.dense is not None and self._pre_bf16c is not None:
+            return "dense"
+        return "route" if self.transposed is not None else "ordered"
+
+    @mode.setter
+    def mode(self, m):
+        if m not in (None, "dense", "route", "ordered"):
+            raise abi.PVHIPError("unknown scatter mode %r" % (m,))
+        if m == "dense" and (self.dense is None or self._pre_bf16c is None):
+            raise abi.PVHIPError("the dense scatter needs its geometry and the source's chunked BF16 copy")
+        if m == "route" and self.transposed is None:
+            raise abi.PVHIPError("this scatter has no transposed route")
+        self._mode = m
+
+    @property
+    def use_route(self):
+        """True when the delivery reads the dense activity (dense or route mode)."""
+        return self.mode != "ordered"
+
+    @use_route.setter
+    def use_route(self, v):
+        if not v:
+            self._mode = "ordered"
+        else:
+            self._mode = None
+            if self.mode == "ordered":
+                raise abi.PVHIPError("this scatter has no dense route")
 
     def _transposed_route(self):
         """BF16 / FP32 forms: the scatter as a post-perspective gather with the
@@ -346,7 +416,10 @@ class PresynapticPerspectiveGPUDelivery:
     def refresh_weights(self, stream=None):
         """Re-derive the transposed weights (and their low-precision shadow)
         after a weight change, on the device: WeightsPair::finalizeUpdate's
-        transposition as a permutation table (graph-capturable, no host copy)."""
+        transposition as a permutation table (graph-capturable, no host copy),
+        and the dense mode's BF16 weights."""
+        if getattr(self, "dense", None) is not None:
+            self._refresh_dense(stream)
         if self.transposed is None and getattr(self, "_route", None) is None:
             return
         route = self._route
@@ -364,17 +437,42 @@ class PresynapticPerspectiveGPUDelivery:
                                                   _stream(stream)), "pvhip_permute_weights")
         route.refresh_weights(stream)
 
-    def bind_pre_bf16c(self, buf):
+    def bind_pre_bf16c(self, buf, halo_is_zero=False):
         """Chunked BF16 copy of the pre layer's activity (pvhip_activity_to_bf16c
         layout), kept current by the pre layer's update: the BF16 channel GEMM of
-        the dense route then reads it instead of converting the FP32 activity
-        (same results, half the bytes).  None unbinds.  Returns True when the
-        route can use it."""
+        the transposed route then reads it instead of converting the FP32
+        activity (same results, half the bytes), and the dense mode becomes
+        available.  halo_is_zero: the caller guarantees the activity's halo
+        holds only zeros (the dense mode then multiplies restricted positions
+        only).  None unbinds.  Returns True when a route can use it."""
+        ok = False
         t = self.transposed
-        if t is None or t.inverse is None or t.arith != abi.ARITH_BF16 or self.pre.nf % 32:
-            return False
-        t.pre_bf16c = buf
-        return True
+        if t is not None and t.inverse is not None and t.arith == abi.ARITH_BF16 and self.pre.nf % 32 == 0:
+            t.pre_bf16c = buf
+            ok = True
+        if self.dense is not None:
+            self._pre_bf16c = buf
+            self._dense_halo_zero = bool(halo_is_zero)
+            ok = True
+        if buf is None and self._mode == "dense":
+            self._mode = None
+        return ok
+
+    def _deliver_dense(self, gsyn, accumulate, stream, fused_layer):
+        d = self.dense
+        a = abi.RecvPreDenseArgs()
+        a.pre, a.post = self.pre, self.post
+        a.nxp, a.nyp, a.nfp = self.nxp, self.nyp, self.nfp
+        a.start_x0, a.start_y0 = d["start"]
+        a.halo_is_zero = int(self._dense_halo_zero)
+        a.pre_data_bf16c = _ptr(self._pre_bf16c)
+        a.weights_bf16 = _ptr(d["weights"])
+        a.dt_factor = self.dt_factor
+        a.post_gsyn = _ptr(gsyn)
+        a.accumulate = int(accumulate)
+        if fused_layer is not None:
+            a.fused_layer = C.pointer(fused_layer)
+        abi.check(abi.hip().pvhip_recv_pre_dense(C.byref(a), _stream(stream)), "pvhip_recv_pre_dense")
 
     def deliver(self, gsyn, active=None, pre_activity=None, accumulate=False, stream=None,
                 fused_layer=None):
@@ -382,7 +480,14 @@ class PresynapticPerspectiveGPUDelivery:
         (layer_update_args, reading `gsyn` as its only channel) follows the
         delivery; the dense route's channel GEMM applies it in its own epilogue
         (no GSyn round trip), the other paths write GSyn and then update."""
-        if self.transposed is not None and (getattr(self, "use_route", True) or active is None):
+        mode = self.mode
+        if mode == "dense":
+            fl = fused_layer if not accumulate else None
+            self._deliver_dense(gsyn, accumulate, stream, fl)
+            if fused_layer is not None and fl is None:
+                launch_layer_update(fused_layer, stream)
+            return
+        if self.transposed is not None and (mode == "route" or active is None):
             if pre_activity is None:
                 raise abi.PVHIPError("the %s scatter reads the dense pre activity"
                                      % ("BF16" if self.arith == abi.ARITH_BF16 else "FP32"))

@@ -345,3 +345,39 @@ def test_momentum_factors_of_the_deprecated_momentum_tau():
         R.momentum_factors("simple", 1.5, 1.5, 1.0, deprecated_tau=True)
     with pytest.raises(abi.PVHIPError):     # the same value as timeConstantTau is rejected
         R.momentum_factors("simple", 1.5, 0.0, 1.0)
+
+
+@pytest.mark.parametrize("case", [((2, 32, 32, 512, 2), (2, 128, 128, 1, 0), 16),
+                                  ((9, 6, 5, 128, 3), (9, 24, 20, 1, 0), 16),
+                                  ((2, 16, 16, 256, (2, 1, 3, 2)), (2, 64, 64, 4, 0), 8),
+                                  ((1, 10, 10, 32, 1), (1, 20, 20, 1, 0), 6)])
+def test_recv_pre_dense_start_matches_patch_geometry(case):
+    """pvhost_recv_pre_dense_start: every source neuron with a kept patch has
+    its unshrunken patch start (the oracle's PatchGeometry: kept start minus
+    kept offset) at stride*extended index + start, along x and y."""
+    (hp, op), (hq, oq), p = both(case[0]), both(case[1]), case[2]
+    s, x0, y0 = C.c_int(), C.c_int(), C.c_int()
+    assert abi.host().pvhost_recv_pre_dense_start(C.byref(hp), C.byref(hq), p, p, C.byref(s), C.byref(x0),
+                                                  C.byref(y0)) == 0
+    assert s.value == hq.nx // hp.nx
+    pa, gs, _, _ = O.patch_geometry(op, oq, p, p, oq.nf)
+    seen = 0
+    for ye in range(op.ny_ext):
+        for xe in range(op.nx_ext):
+            kex = (ye * op.nx_ext + xe) * op.nf
+            pt = pa[kex]
+            if pt["nx"] == 0 or pt["ny"] == 0:
+                continue
+            Y0, X0 = divmod(int(gs[kex]) // oq.nf, oq.nx)
+            oy, ox = divmod(int(pt["offset"]) // oq.nf, p)
+            assert X0 - ox == s.value * xe + x0.value and Y0 - oy == s.value * ye + y0.value
+            seen += 1
+    assert seen > 0
+
+
+def test_recv_pre_dense_start_refuses_coarser_targets():
+    s, x0, y0 = C.c_int(), C.c_int(), C.c_int()
+    for pre, post in (((1, 64, 64, 32, 2), (1, 32, 32, 1, 0)), ((1, 16, 16, 32, 2), (1, 64, 32, 1, 0))):
+        (hp, _), (hq, _) = both(pre), both(post)
+        assert abi.host().pvhost_recv_pre_dense_start(C.byref(hp), C.byref(hq), 8, 8, C.byref(s), C.byref(x0),
+                                                      C.byref(y0)) == -1
