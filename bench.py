@@ -171,7 +171,25 @@ def kernel_timing(col, s, n, torch):
     return float(np.mean(d)), float(np.min(d))
 
 
-TRAFFIC_SUMMARY = os.path.join("profiles", "recv_post_mfma_bf16_lca64_traffic.md")
+def scatter_timing(col, s, n, torch):
+    """Average duration of the Recon scatter as the step runs it (with
+    Recon's update fused), launched eagerly between HIP events on the
+    column's stream."""
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(n)]
+    with torch.cuda.stream(s):
+        for a, b in evs:
+            a.record(s)
+            col.scatter.deliver(col.rec_gsyn, active=col.active, pre_activity=col.v1_A, accumulate=False,
+                                stream=s, fused_layer=col._rec_update)
+            b.record(s)
+    s.synchronize()
+    torch.cuda.synchronize()
+    d = [a.elapsed_time(b) for a, b in evs]
+    return float(np.mean(d)), float(np.min(d))
+
+
+TRAFFIC_SUMMARY = os.path.join("profiles", "lca64_bf16_traffic.md")
 
 
 def profiled_traffic():
@@ -253,6 +271,28 @@ def cpu_baseline(images, W, budget_s=15.0):
                      "scaled to the 64-image batch" % (nb, steps, elapsed)}
     out.update(machine)
     return out
+
+
+def copy_bandwidth(nbytes, torch, reps=20):
+    """GB/s of a plain device-to-device copy (torch's copy kernel) reading
+    nbytes/2 and writing nbytes/2: what a streaming kernel of that size reaches
+    on this device, beside the published peak."""
+    n = int(nbytes) // 8
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 8.0 * n / (ms * 1e-3) / 1e9
 
 
 def _time_events(fn, s, reps, torch):
@@ -582,8 +622,11 @@ def main():
     # V1's chunked BF16 copy, only that copy (2 B); source; weights
     alg_bytes = n_v1 * (10 if col.v1_A16 is not None else 12) + src_bytes + w_bytes
     achieved = alg_bytes / (kavg * 1e-3) / 1e9
+    copy_gbs = copy_bandwidth(alg_bytes, torch)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
+            # a plain device copy moving the same bytes (half read, half written)
+            "copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
             "traffic": profiled_traffic() if head == "bf16" else None,
             "kernel": ("recv_post_mfma_bf16_kernel<4, 1, GatherBF16> (gather + fused LCA update)"
                        if head == "bf16" else "recv_post (gather + fused LCA update)"),
@@ -610,6 +653,16 @@ def main():
         line["bf16_vs_exact"] = round(res["exact"]["ms_per_step"] / res["bf16"]["ms_per_step"], 3)
     line.update(extra)
     line["roofline"] = roof
+    if col.scatter.mode == "dense":
+        # the dense scatter: a GEMM of V1's 64 x 32x32 positions by the 16x16
+        # patch offsets over the 512 features, on the BF16 matrix cores
+        sms, smin = scatter_timing(col, s, 50, torch)
+        sflop = 2.0 * NB * NXV * NYV * NXP * NXP * NFV
+        line["scatter_kernel"] = {
+            "kernel": "recv_pre_dense_kernel<16, 8, 4, false> (BF16 GEMM, overlap-add, Recon update)",
+            "kernel_ms": round(sms, 5), "kernel_ms_min": round(smin, 5), "bound": "mfma",
+            "achieved": round(sflop / (sms * 1e-3) / 1e12, 1), "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": round(sflop / (sms * 1e-3) / 1e12 / 2500.0, 4)}
     if not args.no_extras:
         line["delivery_c1"] = c1_delivery(torch)
         line["hebbian_c4"] = c4_hebbian(torch, tuner=tuner)

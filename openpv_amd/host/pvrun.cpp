@@ -282,6 +282,9 @@ struct Layer {
    // when a connection reads this layer with a delay
    int levels = 1, head = 0;
    std::vector<float *> ring;
+   // chunked BF16 copy of A ([nbatch][nf/32][nyExt][nxExt][32]), kept by the
+   // layer's update when a dense BF16 scatter reads this layer
+   void *Ab = nullptr;
    // sparse publish
    bool sparse = false, needActive = false;
    pvhip_sparse_entry *entries = nullptr;
@@ -305,6 +308,9 @@ struct Conn {
    ConnMode mode = C_NONE;
    int arith     = PVHIP_ARITH_EXACT;
    bool routed   = false;    // a GPU scatter run as a gather with transposed weights
+   bool dense    = false;    // a BF16 GPU scatter run as pvhip_recv_pre_dense
+   void *Wdense  = nullptr;  // its BF16 weights [nyp*nxp*nfp][pre.nf]
+   int dx0 = 0, dy0 = 0;     // its patch start of source extended column / row 0
    std::vector<float> hostW; // pre-perspective weights (originals only)
    float *W = nullptr;       // device weights the delivery reads
    void *Wlp = nullptr;
@@ -415,6 +421,8 @@ struct Net {
 
    void build();
    bool transposedRoute(Conn &c);
+   bool denseRoute(Conn &c);
+   void syncBf16(Layer &L);
    void buildConn(Conn &c);
    void step(double t);
    pvhip_recv_post_args gatherArgs(Conn &c);
@@ -786,7 +794,16 @@ void Net::build() {
       }
    }
    for (auto &C : conns) {
-      if (C.mode == C_SCATTER && C.arith != PVHIP_ARITH_EXACT && transposedRoute(C)) {
+      if (C.mode == C_SCATTER && (C.arith == PVHIP_ARITH_BF16 || C.arith == PVHIP_ARITH_FP8) && denseRoute(C)) {
+         C.dense  = true; // the FP8 form's scatter runs in BF16, as its transposed route does
+         std::printf("pvrun: %s: dense BF16 scatter (pvhip_recv_pre_dense)\n", C.name.c_str());
+         Layer &L = layers[C.pre];
+         if (!L.Ab) {
+            L.Ab = D.alloc((size_t)nbatch * (L.loc.nx + 2 * L.halo) * (L.loc.ny + 2 * L.halo) * L.loc.nf * 2);
+            syncBf16(L);
+         }
+      }
+      else if (C.mode == C_SCATTER && C.arith != PVHIP_ARITH_EXACT && transposedRoute(C)) {
          C.mode   = C_GATHER;
          C.routed = true;
       }
@@ -980,6 +997,31 @@ bool Net::transposedRoute(Conn &C) {
           == 0;
 }
 
+// The dense BF16 scatter (pvhip_recv_pre_dense): one data patch per source
+// feature into a target 4x finer, 16 offsets per class of 4x4, source
+// features 128/256/512, no delay, and a source halo that holds only zeros
+// (no mirror boundary).
+bool Net::denseRoute(Conn &C) {
+   Layer &pre = layers[C.pre], &post = layers[C.post];
+   int nkx, nky;
+   pvhost_num_kernels(&pre.loc, &post.loc, &nkx, &nky);
+   int const f = pre.loc.nf;
+   if (C.delay != 0 || pre.mirrorBC || nkx != 1 || nky != 1 || post.loc.nx != 4 * pre.loc.nx
+       || post.loc.ny != 4 * pre.loc.ny || C.nxp % 4 || C.nyp % 4 || (C.nxp / 4) * (C.nyp / 4) * C.nfp != 16
+       || (f != 128 && f != 256 && f != 512)) {
+      return false;
+   }
+   return pvhost_recv_pre_dense_start(&pre.loc, &post.loc, C.nxp, C.nyp, nullptr, &C.dx0, &C.dy0) == 0;
+}
+
+// A layer's chunked BF16 activity from its FP32 activity (after A was set
+// other than by its update: initial state, checkpoint restore).
+void Net::syncBf16(Layer &L) {
+   if (L.Ab) {
+      check(pvhip_activity_to_bf16c(&L.loc, L.A, L.Ab, D.stream), "pvhip_activity_to_bf16c");
+   }
+}
+
 void Net::buildConn(Conn &C) {
    Layer &pre = layers[C.pre], &post = layers[C.post];
    Conn &O    = C.keyword == "HyPerConn" ? C : conns[connIndex(C.original)];
@@ -1060,6 +1102,12 @@ void Net::buildConn(Conn &C) {
       }
       else {
          C.W = O.W;
+      }
+      if (C.dense) {
+         int const patch = C.nxp * C.nyp * C.nfp;
+         C.Wdense        = D.alloc(pvhip_recv_pre_dense_weights_bytes(pre.loc.nf, patch));
+         check(pvhip_recv_pre_dense_weights(C.W, C.Wdense, pre.loc.nf, patch, D.stream),
+               "pvhip_recv_pre_dense_weights");
       }
       int ncol = 0, nrow = 0;
       if (pvhost_recv_pre_tables(&pre.loc, &post.loc, C.nxp, C.nyp, nullptr, nullptr, nullptr, nullptr, &ncol,
@@ -1192,6 +1240,23 @@ void Net::deliver(Conn &C) {
       pvhip_recv_post_args a = gatherArgs(C);
       check(pvhip_recv_post(&a, D.stream), "pvhip_recv_post");
    }
+   else if (C.mode == C_SCATTER && C.dense) {
+      pvhip_recv_pre_dense_args a{};
+      a.pre            = pre.loc;
+      a.post           = post.loc;
+      a.nxp            = C.nxp;
+      a.nyp            = C.nyp;
+      a.nfp            = C.nfp;
+      a.start_x0       = C.dx0;
+      a.start_y0       = C.dy0;
+      a.halo_is_zero   = 1; // denseRoute: no mirror boundary writes the halo
+      a.pre_data_bf16c = pre.Ab;
+      a.weights_bf16   = C.Wdense;
+      a.dt_factor      = 1.0f;
+      a.post_gsyn      = g;
+      a.accumulate     = 1;
+      check(pvhip_recv_pre_dense(&a, D.stream), "pvhip_recv_pre_dense");
+   }
    else if (C.mode == C_SCATTER) {
       pvhip_recv_pre_args a{};
       a.pre = pre.loc;
@@ -1291,6 +1356,7 @@ void Net::updateLayer(Layer &L, double t) {
          double n = (t - L.lastUpdate) / L.displayPeriod;
          if (std::fabs(n - std::nearbyint(n)) * L.displayPeriod < 0.5 * dtCol) {
             loadInputFrames(L);
+            syncBf16(L);
             L.frameIndex = (L.frameIndex + nbatch) % (int)L.input.frames.size();
             L.lastUpdate = t;
          }
@@ -1318,6 +1384,7 @@ void Net::updateLayer(Layer &L, double t) {
       u.self_interact = L.selfInteract;
       u.transfer = L.transfer;
       u.activity_from_v = 1;
+      u.A_bf16c = L.Ab;
       if (L.ista) {
          check(pvhip_ista_update(&u, (float)L.prm.num("VThresh", -FLT_MAX), D.stream), "pvhip_ista_update");
       }
@@ -1342,6 +1409,7 @@ void Net::updateLayer(Layer &L, double t) {
       u.apply_scale = L.kind == L_ERROR ? 1 : 0;
       u.transfer = L.transfer;
       check(pvhip_layer_update(&u, D.stream), "pvhip_layer_update");
+      syncBf16(L);
    }
    if (L.needActive) {
       check(pvhip_active_compact(&L.loc, L.A, L.entries, L.numActive, L.posStart, L.compactScratch, D.stream),
@@ -1620,6 +1688,10 @@ void Net::refreshDerived(Conn &O) {
       }
       if (X.perm) {
          check(pvhip_permute_weights(O.W, X.perm, X.W, X.permN, D.stream), "pvhip_permute_weights");
+      }
+      if (X.dense) {
+         check(pvhip_recv_pre_dense_weights(X.W, X.Wdense, layers[X.pre].loc.nf, X.nxp * X.nyp * X.nfp, D.stream),
+               "pvhip_recv_pre_dense_weights");
       }
       if (X.Wlp && X.arith == PVHIP_ARITH_BF16) {
          check(pvhip_recv_post_bf16_weights(X.W, X.Wlp, X.lpKernels, X.lpPatch, D.stream), "bf16 weights");
@@ -2015,6 +2087,7 @@ double Net::readCheckpoint(const std::string &dir) {
          setRestrictedFrames(L, L.V, false, get("V"), 0);
       }
       setRestrictedFrames(L, L.A, true, get("A"), 0);
+      syncBf16(L);
       auto levels = get("Delays");
       for (int lv = 1; lv < L.levels; lv++) {
          setRestrictedFrames(L, levelData(L, lv), true, levels, (size_t)lv * nbatch);

@@ -1,0 +1,136 @@
+"""The dense BF16 scatter through the .params drop-in: pvrun builds the LCA
+column of the batched workload's geometry (128x128x1 images, V1 32x32x128,
+16x16 patches at stride 4, VThresh 0.15, tau 100, fixed dt) with
+gpuArithmetic "bf16" or "fp8"; the CloneConn V1 -> Reconstruction then runs
+pvhip_recv_pre_dense (V1's update keeps its chunked BF16 activity), and the
+reconstructions stay within the BF16 / FP8 bounds of the oracle column's
+(EXACT) over 30 steps.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_column as OC
+from openpv_amd import abi, pvp
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = """
+HyPerCol "column" = {
+    nx = 128;
+    ny = 128;
+    nbatch = 2;
+    dt = 1;
+    stopTime = %(steps)d;
+    outputPath = "output/";
+    gpuArithmetic = "%(form)s";
+};
+
+PvpLayer "Input" = {
+    nxScale = 1;   nyScale = 1;   nf = 1;
+    phase = 0;
+    inputPath = "images.pvp";
+    displayPeriod = 0;
+    writeStep = -1;
+};
+
+ANNErrorLayer "Residual" = {
+    nxScale = 1;   nyScale = 1;   nf = 1;
+    phase = 1;
+    errScale = 1;
+    VThresh = -infinity;
+    writeStep = -1;
+};
+
+HyPerLCALayer "V1" = {
+    nxScale = 0.25; nyScale = 0.25; nf = 128;
+    phase = 2;
+    sparseLayer = true;
+    VThresh = 0.15;  AMin = 0;  AMax = infinity;  AShift = 0.15;  VWidth = 0;
+    timeConstantTau = 100;
+    selfInteract = true;
+    writeStep = -1;
+};
+
+ANNLayer "Reconstruction" = {
+    nxScale = 1;   nyScale = 1;   nf = 1;
+    phase = 3;
+    VThresh = -infinity;  AMin = -infinity;  AMax = infinity;  AShift = 0;  VWidth = 0;
+    writeStep = 1;
+    initialWriteTime = 1;
+};
+
+IdentConn "InputToResidual" = {
+    preLayerName = "Input";   postLayerName = "Residual";
+    channelCode = 0;
+};
+
+HyPerConn "V1ToResidual" = {
+    preLayerName = "V1";   postLayerName = "Residual";
+    channelCode = -1;
+    plasticityFlag = false;
+    sharedWeights = true;
+    weightInitType = "FileWeight";
+    initWeightsFile = "weights.pvp";
+    nxp = 16;  nyp = 16;  nfp = 1;
+    receiveGpu = true;
+    updateGSynFromPostPerspective = false;
+    pvpatchAccumulateType = "convolve";
+};
+
+TransposeConn "ResidualToV1" = {
+    preLayerName = "Residual";   postLayerName = "V1";
+    originalConnName = "V1ToResidual";
+    channelCode = 0;
+    receiveGpu = true;
+    updateGSynFromPostPerspective = true;
+    pvpatchAccumulateType = "convolve";
+};
+
+CloneConn "V1ToReconstruction" = {
+    preLayerName = "V1";   postLayerName = "Reconstruction";
+    originalConnName = "V1ToResidual";
+    channelCode = 0;
+    receiveGpu = true;
+    updateGSynFromPostPerspective = false;
+    pvpatchAccumulateType = "convolve";
+};
+
+IdentConn "ReconstructionToResidual" = {
+    preLayerName = "Reconstruction";   postLayerName = "Residual";
+    channelCode = 1;
+};
+"""
+
+
+@pytest.mark.parametrize("form,tol", [("bf16", 5e-3), ("fp8", 5e-2)])
+def test_pvrun_dense_scatter_column(tmp_path, form, tol):
+    import bench
+    steps = 30
+    images = bench.one_over_f_images(2, 128, 128, seed=31)
+    W = bench.dictionary(128, 16, 1, seed=32)
+    pvp.write_activity(str(tmp_path / "images.pvp"), images, [0.0, 0.0])
+    pvp.write_weights(str(tmp_path / "weights.pvp"), W[None], [0.0])
+    (tmp_path / "column.params").write_text(PARAMS % {"steps": steps, "form": form})
+    print("pvrun", form, "dense column", flush=True)
+    r = subprocess.run([abi.pvrun_path(), "-p", "column.params", "-o", str(tmp_path / "out"), "--steps",
+                        str(steps)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0, r.stderr[-800:]
+    assert "V1ToReconstruction: dense BF16 scatter" in r.stdout
+    _, times, rec = pvp.read_activity(str(tmp_path / "out" / "Reconstruction.pvp"))
+    assert list(times) == [float(t) for t in range(1, steps + 1) for _ in range(2)]   # two batch elements
+    rec = rec.reshape(steps, 2, 128, 128, 1)
+    col = OC.OracleColumn(images, W, 32, 32, 128, 16, 0.15)
+    refs = []
+    for t in range(steps):
+        col.step()
+        refs.append(col.rc.reshape(rec[t].shape).copy())
+    refs = np.stack(refs)
+    scale = np.abs(refs).max()   # the reconstruction grows from 0: one scale for all steps
+    err = np.abs(rec - refs).reshape(steps, -1).max(1)
+    print("%s: max |pvrun - oracle| per step %s (scale %g)" % (form, np.array2string(err, precision=2), scale),
+          flush=True)
+    assert (err <= tol * scale).all()
