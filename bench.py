@@ -153,7 +153,7 @@ def accuracy(forms, images, W, torch, steps=ACCURACY_STEPS):
 
 def kernel_timing(col, s, n, torch):
     """Average duration of the fused gather + LCA-update kernel
-    (recv_post_mfma_bf16_kernel<4, 1, GatherBF16> for the BF16 form), launched
+    (recv_post_mfma_persist_kernel<4, GatherBF16, 8> for the BF16 form), launched
     eagerly with the argument block the step uses when its scatter is the dense
     route (no active-list emission; V1's chunked BF16 activity written when the
     column keeps one), bracketed by HIP events recorded on the column's stream."""
@@ -628,7 +628,7 @@ def main():
             # a plain device copy moving the same bytes (half read, half written)
             "copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
             "traffic": profiled_traffic() if head == "bf16" else None,
-            "kernel": ("recv_post_mfma_bf16_kernel<4, 1, GatherBF16> (gather + fused LCA update)"
+            "kernel": ("recv_post_mfma_persist_kernel<4, GatherBF16, 8> (gather + fused LCA update)"
                        if head == "bf16" else "recv_post (gather + fused LCA update)"),
             "kernel_ms": round(kavg, 5), "kernel_ms_min": round(kmin, 5),
             "algorithmic_bytes": int(alg_bytes),

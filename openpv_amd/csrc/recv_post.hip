@@ -950,9 +950,10 @@ __global__ void __launch_bounds__(256, E::WAVES) recv_post_mfma_bf16_kernel(
 // 16x8x64 tiles, VTHRESH transfer, patches of KS*32 values read with vector
 // LDS loads): a workgroup keeps its 64 features' weight fragments in
 // registers and walks `ipw` tiles (over positions and batch), fetching the
-// next tile's old V and source window into registers while the current tile
-// runs its MFMAs and epilogue, so the loads of one tile overlap the stores of
-// the one before.  Launched for the FP32 form, where it measured faster.  Per neuron the arithmetic is that of
+// next tile's old V and source window into registers before the current tile
+// runs its MFMAs and epilogue, so the loads of one tile overlap the work and
+// the stores of the one before.  Launched for the FP32 and BF16 forms whose
+// weight rows are unpadded (K a multiple of 32).  Per neuron the arithmetic is that of
 // recv_post_mfma_bf16_kernel<ALIGN, 1, E> (same MFMA order, same epilogue):
 // the same bits.
 constexpr int PSREG = 16; // source window values per thread held in registers (window <= 4096 values)
@@ -1111,6 +1112,15 @@ __global__ void __launch_bounds__(256) recv_post_mfma_persist_kernel(
             acc[fb][pb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
          }
       }
+      // the next tile's loads, in flight during this tile's MFMAs and epilogue
+      bool const more = item + 1 < last; // the same for the whole workgroup
+      Tile nxt        = cur;
+      if (more) {
+         nxt = tile_of(item + 1);
+         fetch_v(nxt, vpreN);
+         fetch_src(nxt, sv);
+         bases(nxt, pbaseN);
+      }
 #pragma unroll
       for (int st = 0; st < KS; st++) {
 #pragma unroll
@@ -1127,15 +1137,6 @@ __global__ void __launch_bounds__(256) recv_post_mfma_persist_kernel(
                acc[fb][pb] = E::mfma(aAll[st][fb], bfrag, acc[fb][pb]);
             }
          }
-      }
-      // the next tile's loads, in flight during this tile's epilogue
-      bool const more = item + 1 < last; // the same for the whole workgroup
-      Tile nxt        = cur;
-      if (more) {
-         nxt = tile_of(item + 1);
-         fetch_v(nxt, vpreN);
-         fetch_src(nxt, sv);
-         bases(nxt, pbaseN);
       }
       if constexpr (E::SCALED) {
 #pragma unroll
@@ -2137,12 +2138,13 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
                static_cast<char *>(aout->scratch) + segment_stage_bytes(args->fused_update->loc));
          epi      = 3;
       }
-      // the persistent kernel for full tiles of the fused VTHRESH update in
-      // FP32 (measured: faster for FP32 operands, whose weight fragments it
-      // keeps in registers across tiles; no faster for BF16/FP8, which keep the
-      // one-tile kernel)
+      // the persistent kernel for full tiles of the fused VTHRESH update
+      // (measured faster than one tile per workgroup: FP32 by keeping its
+      // weight fragments in registers across tiles, BF16 by overlapping the
+      // next tile's loads with the current tile's MFMAs and stores; FP8
+      // measured slower and keeps the one-tile kernel)
       int const KS       = K / 32;
-      bool const persist = mfma32 && epi == 1 && align >= 2 && K % 32 == 0 && (KS == 6 || KS == 8)
+      bool const persist = !fp8 && epi == 1 && align >= 2 && K % 32 == 0 && K == Kpad && (KS == 6 || KS == 8)
                            && p.post.nx % MTX == 0 && p.post.ny % MTY == 0 && p.post.nf % MF == 0
                            && (long long)srcRowsMax * srcColsMax * p.nfp <= 256LL * PSREG;
       if (persist) {
@@ -2154,8 +2156,20 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
          size_t const plds = std::max(((size_t)srcRowsMax * pitch * esz + 3) / 4 * 4,
                                       (size_t)MTX * MTY * MOP * sizeof(float));
 #define PVHIP_PERSIST(AL, KSV)                                                                   \
-   hipLaunchKernelGGL((recv_post_mfma_persist_kernel<AL, GatherF32, KSV>), pgrid, dim3(256), plds, s, \
-         p, args->weights, srcRowsMax, srcColsMax, items, ipw)
+   do {                                                                                          \
+      if (mfma32) {                                                                              \
+         hipLaunchKernelGGL((recv_post_mfma_persist_kernel<AL, GatherF32, KSV>), pgrid, dim3(256), plds, s, \
+               p, args->weights, srcRowsMax, srcColsMax, items, ipw);                            \
+      }                                                                                          \
+      else if (fp8) {                                                                            \
+         hipLaunchKernelGGL((recv_post_mfma_persist_kernel<AL, GatherFP8, KSV>), pgrid, dim3(256), plds, s, \
+               p, static_cast<const uint8_t *>(args->weights_lp), srcRowsMax, srcColsMax, items, ipw); \
+      }                                                                                          \
+      else {                                                                                     \
+         hipLaunchKernelGGL((recv_post_mfma_persist_kernel<AL, GatherBF16, KSV>), pgrid, dim3(256), plds, s, \
+               p, static_cast<const __bf16 *>(args->weights_lp), srcRowsMax, srcColsMax, items, ipw); \
+      }                                                                                          \
+   } while (0)
 #define PVHIP_PERSIST_KS(AL)                                                                     \
    do {                                                                                          \
       if (KS == 8) PVHIP_PERSIST(AL, 8);                                                         \
