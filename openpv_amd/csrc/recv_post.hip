@@ -481,7 +481,7 @@ __device__ __forceinline__ void post_epilogue4(
       vn[i]    = t1 + t4;
       an[i]    = transfer_as<EPI == 1 ? 1 : 0>(q.transfer, vn[i]);
    }
-   __builtin_nontemporal_store(vn, reinterpret_cast<f32x4 *>(q.V + gi));
+   *reinterpret_cast<f32x4 *>(q.V + gi) = vn;   // read again by the next step
    if (q.A) {
       __builtin_nontemporal_store(an, reinterpret_cast<f32x4 *>(q.A + ai));
    }
@@ -593,13 +593,15 @@ __device__ __forceinline__ void lca_epilogue_vthresh(
             an[i]    = vthresh_select(vn[i], thr, ramp, clamp, t.VThresh, t.AMin, t.AShift, t.AMax, t.VWidth);
          }
       }
-      __builtin_nontemporal_store(vn, reinterpret_cast<f32x4 *>(Vp + yl * rowV + xo));
+      // V and the BF16 activity go through the caches (the next step reads
+      // both; a plain store measured faster than a streaming one here).
+      *reinterpret_cast<f32x4 *>(Vp + yl * rowV + xo) = vn;
       if (Ap) {
          __builtin_nontemporal_store(an, reinterpret_cast<f32x4 *>(Ap + yl * rowA + xo));
       }
       if (Abp) {
-         __builtin_nontemporal_store(bf16x4{(__bf16)an[0], (__bf16)an[1], (__bf16)an[2], (__bf16)an[3]},
-               reinterpret_cast<bf16x4 *>(Abp + (yl * q.loc.nxExt() + it * MEPP % MTX) * 32));
+         *reinterpret_cast<bf16x4 *>(Abp + (yl * q.loc.nxExt() + it * MEPP % MTX) * 32) =
+               bf16x4{(__bf16)an[0], (__bf16)an[1], (__bf16)an[2], (__bf16)an[3]};
       }
       if (EMIT) {
          // Rank of this thread's actives within its position's segment: the

@@ -98,16 +98,23 @@ __device__ __forceinline__ int fdiv(int n, FastDiv f) {
 // k-steps).  NW = 8 waves per workgroup: two per SIMD hide each other's
 // latencies.  Wave w keeps the products of class (w + k) mod 4 in its
 // accumulator slot k, so that the overlap-add reads static registers.
-// ATOMIC: LDS atomic adds in the overlap-add, for patches wider than 16 or
-// position grids narrower than 12 + nxp/4 (the four positions of one add, 4
-// apart in row-major order, may then share a target neuron); otherwise the
-// adds of one instruction never share one and a plain read-add-write is
-// exact.
-template <int KS, int NW, int NB, bool ATOMIC>
+// ADD selects the overlap-add:
+//   ADD_ATOMIC: LDS atomic adds, for patches wider than 16 or position grids
+//     narrower than 12 + nxp/4 (the four positions of one add, 4 apart in
+//     row-major order, may then share a target neuron); otherwise the adds of
+//     one instruction never share one and a plain read-add-write is exact;
+//   ADD_QUAD: plain adds, four waves at a time (one per class);
+//   ADD_ALL: plain adds, all eight waves at a time.  Waves w and w + 4 hold
+//     the same class for source positions 4*16*DMG apart, at least
+//     48*DMG + 1 apart position by position; when that spans nyp/4 source
+//     rows their target rows differ.
+constexpr int ADD_QUAD = 0, ADD_ATOMIC = 1, ADD_ALL = 2;
+template <int KS, int NW, int NB, int ADD>
 __global__ void __launch_bounds__(64 * NW) recv_pre_dense_kernel(DenseParams p) {
    constexpr int NT = 64 * NW;
    static_assert(KS % NB == 0 && NB % 2 == 0, "k-step s of every group lives in buffer s % NB");
    static_assert(NW % 4 == 0, "whole rounds of four class owners");
+   static_assert(ADD != ADD_ALL || NW == 8, "two waves per class");
    extern __shared__ float smem[];
    __bf16 *wl = reinterpret_cast<__bf16 *>(smem); // [4 classes][KS][16 columns][32 channels]
    float *ol  = smem + 4 * KS * 16 * 32 / 2;       // tile [rows + 5][pitch]
@@ -213,15 +220,37 @@ __global__ void __launch_bounds__(64 * NW) recv_pre_dense_kernel(DenseParams p) 
    int const laneOff = jy * p.pitch + DST * jx * p.nfp + fo;
    int const rowOff  = ((p.y0 + oyq - q) >> 2) - jlo; // (y0 + oyq - q) is a multiple of 4
 
+   // One k-step: issue the source loads of k-step fks (buffer (bb + NB - 1)
+   // mod NB) and the weight fragments of k-step bks, then the MFMAs of the
+   // k-step in buffer bb.  Every k-step issues the same loads, so the wait
+   // before each MFMA is for the oldest stage only (a k-step that issued
+   // none would make the compiler's wait count cover the youngest stages
+   // too, draining the pipeline).
+   f32x4 acc[DMG][4];
+   auto kstep = [&](int bb, int fks, int bks) {
+      fetch((bb + NB - 1) % NB, fks);
+      loadB((bb + 1) & 1, bks);
+      // keep the loads ahead of this k-step's MFMAs (the scheduler would
+      // otherwise sink them next to their use)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+#pragma unroll
+         for (int m = 0; m < DMG; m++) {
+            acc[m][k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[bb][m], bw[bb & 1][k], acc[m][k], 0, 0, 0);
+         }
+      }
+   };
+
    set_rows(wave * DMG);
 #pragma unroll
    for (int st = 0; st < NB - 1; st++) {
       fetch(st, st); // in flight during the barrier below
    }
    __syncthreads(); // the staged weights and the cleared tile
+   loadB(0, 0);
    for (int rd = 0; rd < rounds; rd++) {
       int const mb0 = (rd * NW + wave) * DMG;
-      f32x4 acc[DMG][4];
 #pragma unroll
       for (int m = 0; m < DMG; m++) {
 #pragma unroll
@@ -229,36 +258,24 @@ __global__ void __launch_bounds__(64 * NW) recv_pre_dense_kernel(DenseParams p) 
             acc[m][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
          }
       }
-      loadB(0, 0);
 #pragma unroll 1
-      for (int sg = 0; sg < KS / NB; sg++) {
+      for (int sg = 0; sg < KS / NB - 1; sg++) {
 #pragma unroll
          for (int bb = 0; bb < NB; bb++) {
             int const ks = sg * NB + bb;
-            int const sf = ks + NB - 1; // the k-step whose loads are issued now
-            if (sf < KS) {
-               fetch((bb + NB - 1) % NB, sf);
-            }
-            else if (rd + 1 < rounds) {
-               if (sf == KS) {
-                  set_rows(((rd + 1) * NW + wave) * DMG); // the next group's first k-steps, in
-               }                                          // flight during this group's last ones
-               fetch((bb + NB - 1) % NB, sf - KS);        // and the overlap-add
-            }
-            if (ks + 1 < KS) {
-               loadB((bb + 1) & 1, ks + 1); // in flight during this k-step's MFMAs
-            }
-            // keep the loads ahead of this k-step's MFMAs (the scheduler would
-            // otherwise sink them next to their use)
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-#pragma unroll
-               for (int m = 0; m < DMG; m++) {
-                  acc[m][k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[bb][m], bw[bb & 1][k], acc[m][k], 0, 0, 0);
-               }
-            }
+            kstep(bb, ks + NB - 1, ks + 1);
          }
+      }
+      // The group's last NB k-steps issue the loads of the next group's first
+      // NB - 1 k-steps (in flight during the overlap-add) and its first weight
+      // fragments.  After the last group the next group's positions are all
+      // past the end, so those loads read position 0: in bounds, unused.
+#pragma unroll
+      for (int bb = 0; bb < NB; bb++) {
+         if (bb == 1) {
+            set_rows(((rd + 1) * NW + wave) * DMG);
+         }
+         kstep(bb, bb == 0 ? KS - 1 : bb - 1, bb + 1 < NB ? KS - NB + bb + 1 : 0);
       }
       // Overlap-add: lane (r, g), register i of block m, slot k holds the
       // product of source position (mb0 + m)*16 + 4g + i and column r of class
@@ -278,23 +295,24 @@ __global__ void __launch_bounds__(64 * NW) recv_pre_dense_kernel(DenseParams p) 
                                        : p.rows * p.pitch;
          }
       }
-      // NW rounds: in round rr the four waves 4*(rr/4) .. +3 add, each its
-      // slot rr mod 4 (class (w + rr) mod 4), so no wave touches a tile
-      // element another wave touches in the same round; LDS keeps a wave's
-      // accesses in order, so when the 64 lanes of one add have distinct
-      // elements a plain read-add-write is exact (and far cheaper than LDS
-      // atomics).
+      // Rounds: in round rr the four waves 4*(rr/4) .. +3 (ADD_ALL: every
+      // wave) add, each its slot rr mod 4 (class (w + rr) mod 4), so no wave
+      // touches a tile element another wave touches in the same round; LDS
+      // keeps a wave's accesses in order, so when the 64 lanes of one add
+      // have distinct elements a plain read-add-write is exact (and far
+      // cheaper than LDS atomics).
+      constexpr int ROUNDS = ADD == ADD_ALL ? 4 : NW;
 #pragma unroll
-      for (int rr = 0; rr < NW; rr++) {
+      for (int rr = 0; rr < ROUNDS; rr++) {
          __syncthreads();
-         if (wave / 4 == rr / 4) { // the same for the whole wave
+         if (ADD == ADD_ALL || wave / 4 == rr / 4) { // the same for the whole wave
             int const k   = rr & 3;
             int const off = laneOff + pmod4(((wave + k) & 3) - p.x0) * p.nfp;
 #pragma unroll
             for (int m = 0; m < DMG; m++) {
 #pragma unroll
                for (int i = 0; i < 4; i++) {
-                  if constexpr (ATOMIC) {
+                  if constexpr (ADD == ADD_ATOMIC) {
                      atomicAdd(&ol[pb[m][i] + off], acc[m][rr & 3][i]);
                   }
                   else {
@@ -449,12 +467,18 @@ int pvhip_recv_pre_dense(const pvhip_recv_pre_dense_args *a, pvhip_stream stream
    // their patches (nxp/4 source columns wide) are disjoint when nxp <= 16,
    // across a row wrap when the grid is at least 12 + nxp/4 wide
    bool const atomic = a->nxp > 16 || p.pnx < 12 + a->nxp / DST;
-#define RPD_LAUNCH(KSV, NBV)                                                                      \
-   if (atomic) {                                                                                 \
-      hipLaunchKernelGGL((recv_pre_dense_kernel<KSV, NW, NBV, true>), grid, dim3(64 * NW), lds, s, p); \
-   }                                                                                             \
-   else {                                                                                        \
-      hipLaunchKernelGGL((recv_pre_dense_kernel<KSV, NW, NBV, false>), grid, dim3(64 * NW), lds, s, p); \
+   // waves w and w + 4 add together when positions 48*DMG + 1 apart lie
+   // nyp/4 source rows apart
+   bool const all = !atomic && (a->nyp / DST) * p.pnx <= 48 * DMG + 1;
+#define RPD_LAUNCH(KSV, NBV)                                                                          \
+   if (atomic) {                                                                                     \
+      hipLaunchKernelGGL((recv_pre_dense_kernel<KSV, NW, NBV, ADD_ATOMIC>), grid, dim3(64 * NW), lds, s, p); \
+   }                                                                                                 \
+   else if (all) {                                                                                   \
+      hipLaunchKernelGGL((recv_pre_dense_kernel<KSV, NW, NBV, ADD_ALL>), grid, dim3(64 * NW), lds, s, p); \
+   }                                                                                                 \
+   else {                                                                                            \
+      hipLaunchKernelGGL((recv_pre_dense_kernel<KSV, NW, NBV, ADD_QUAD>), grid, dim3(64 * NW), lds, s, p); \
    }
    if (KS == 4) {
       RPD_LAUNCH(4, (NB < 4 ? NB : 4))
