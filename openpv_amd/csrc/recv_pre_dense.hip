@@ -16,13 +16,14 @@
 // offsets whose row is q - start_y modulo 4: one workgroup per (batch
 // element, q) owns those target rows outright, keeps them in LDS, and needs
 // only that quarter of the weights (staged in LDS once).  Within a workgroup
-// the 4 waves take the source positions in groups of DMG blocks of 16 and
+// the 8 waves take the source positions in groups of DMG blocks of 16 and
 // compute all four column classes (target column modulo 4) for them, so each
 // source fragment feeds four MFMAs and each weight fragment DMG.  The
-// overlap-add runs in four rounds separated by barriers: in round rr wave w
-// adds its products of column class (w + rr) mod 4, so no two waves touch a
-// target neuron at the same time and every neuron receives its products in
-// a fixed order: the result does not depend on scheduling.
+// overlap-add runs in rounds separated by barriers: in round rr wave w adds
+// its products of column class (w + rr) mod 4 (four waves per round, or all
+// eight when the two waves of a class are far enough apart), so no two waves
+// touch a target neuron at the same time and every neuron receives its
+// products in a fixed order: the result does not depend on scheduling.
 //
 // Numerics: BF16 products, FP32 sums (MFMA order over the channels, then the
 // fixed overlap-add order), the form's tolerance, not the EXACT order.
