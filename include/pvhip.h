@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PVHIP_ABI_VERSION 7
+#define PVHIP_ABI_VERSION 8
 
 enum {
    PVHIP_OK              = 0,
@@ -392,11 +392,11 @@ typedef struct {
 
 int pvhip_recv_pre(const pvhip_recv_pre_args *args, pvhip_stream stream);
 
-/* The BF16 form of the same delivery for a dense source whose neurons each
+/* The BF16 (or FP32) form of the same delivery for a dense source whose neurons each
  * send one shared patch (one data patch per source feature) into a target 4
  * times finer along x and y (the reconstruction of an LCA column): for every
  * source position p and patch offset o, P[p][o] = sum_c a[p][c]*w[c][o] on
- * the matrix cores (BF16 operands, FP32 sums), added to the target neuron at
+ * the matrix cores (BF16 or FP32 operands, FP32 sums), added to the target neuron at
  * the patch start of p plus o, in an order fixed by the launch (the result
  * does not depend on scheduling).  Replaces PresynapticPerspectiveGPUDelivery::
  * deliver (PresynapticPerspectiveGPUDelivery.cpp:198-270) for this geometry;
@@ -423,6 +423,13 @@ typedef struct {
     * delivered values (as pvhip_recv_post_args.fused_layer); post_gsyn is
     * then not written. */
    const struct pvhip_layer_update_args *fused_layer;
+   /* 1: the FP32 form: FP32 products and sums on the matrix cores
+    * (v_mfma_f32_16x16x4f32), reading pre_data (the FP32 activity,
+    * [nbatch][numPreExt]) and weights_f32 (pvhip_recv_pre_dense_weights_f32)
+    * instead of the two BF16 buffers. */
+   int fp32;
+   const float *pre_data;
+   const float *weights_f32;
 } pvhip_recv_pre_dense_args;
 
 int pvhip_recv_pre_dense(const pvhip_recv_pre_dense_args *args, pvhip_stream stream);
@@ -434,6 +441,10 @@ int pvhip_recv_pre_dense(const pvhip_recv_pre_dense_args *args, pvhip_stream str
 int pvhip_recv_pre_dense_weights(
       const float *weights, void *weights_bf16, int num_features, int patch_size, pvhip_stream stream);
 size_t pvhip_recv_pre_dense_weights_bytes(int num_features, int patch_size);
+/* FP32 weights of the FP32 form: the same transposition, [patch_size][num_features]
+ * floats (num_features * patch_size * 4 bytes). */
+int pvhip_recv_pre_dense_weights_f32(
+      const float *weights, float *weights_f32, int num_features, int patch_size, pvhip_stream stream);
 
 /* ------------------------------------------------------------------------ */
 /* Layer updates                                                             */

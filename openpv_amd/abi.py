@@ -118,7 +118,8 @@ class RecvPreDenseArgs(C.Structure):
     _fields_ = [("pre", Loc), ("post", Loc), ("nxp", i32), ("nyp", i32), ("nfp", i32),
                 ("start_x0", i32), ("start_y0", i32), ("halo_is_zero", i32),
                 ("pre_data_bf16c", vp), ("weights_bf16", vp), ("dt_factor", f32),
-                ("post_gsyn", vp), ("accumulate", i32), ("fused_layer", C.POINTER(LayerUpdateArgs))]
+                ("post_gsyn", vp), ("accumulate", i32), ("fused_layer", C.POINTER(LayerUpdateArgs)),
+                ("fp32", i32), ("pre_data", vp), ("weights_f32", vp)]
 
 
 class ErrorUpdateArgs(C.Structure):
@@ -183,6 +184,7 @@ PVHIP_SYMBOLS = {
     "pvhip_recv_pre_dense": (i32, [C.POINTER(RecvPreDenseArgs), vp]),
     "pvhip_recv_pre_dense_weights": (i32, [vp, vp, i32, i32, vp]),
     "pvhip_recv_pre_dense_weights_bytes": (C.c_size_t, [i32, i32]),
+    "pvhip_recv_pre_dense_weights_f32": (i32, [vp, vp, i32, i32, vp]),
     "pvhip_lca_update": (i32, [C.POINTER(LCAUpdateArgs), vp]),
     "pvhip_momentum_lca_update": (i32, [C.POINTER(LCAUpdateArgs), C.c_float, vp, vp]),
     "pvhip_ista_update": (i32, [C.POINTER(LCAUpdateArgs), C.c_float, vp]),

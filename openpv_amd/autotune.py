@@ -133,14 +133,14 @@ class LaunchTuner:
 
     def tune_recv_pre(self, delivery, gsyn, active, pre_activity, stream=None):
         """Mode of a PresynapticPerspectiveGPUDelivery (FP32/BF16 forms): the
-        dense GEMM ("dense", when its geometry and the source's BF16 copy are
-        there), the transposed route's channel GEMM ("route") or the ordered
+        dense GEMM ("dense", when its geometry and, in BF16, the source's BF16
+        copy are there), the transposed route's channel GEMM ("route") or the ordered
         sparse scatter ("ordered").  Returns the mode, or None when there is no
         choice."""
         if delivery.arith == abi.ARITH_EXACT or active is None:
             return None
         cands = []
-        if delivery.dense is not None and delivery._pre_bf16c is not None:
+        if delivery._dense_ready():
             cands.append(("dense",))
         if delivery.transposed is not None:
             cands.append(("route",))

@@ -9,8 +9,9 @@
 //     P[p][o] = sum_c a[p][c] * w[c][o],
 // added to the target neuron at start(p) + o.  That is a GEMM with M = source
 // positions, N = patch offsets, K = channels (v_mfma_f32_16x16x32_bf16 on the
-// BF16 copy of the activity and a BF16 copy of the weights), followed by the
-// overlap-add of the products into the target.
+// BF16 copy of the activity and a BF16 copy of the weights; in the FP32 form
+// v_mfma_f32_16x16x4f32 on the FP32 activity and an FP32 copy of the weights),
+// followed by the overlap-add of the products into the target.
 //
 // Work split.  The target neurons whose row is q modulo 4 receive only the
 // offsets whose row is q - start_y modulo 4: one workgroup per (batch
@@ -25,11 +26,13 @@
 // touch a target neuron at the same time and every neuron receives its
 // products in a fixed order: the result does not depend on scheduling.
 //
-// Numerics: BF16 products, FP32 sums (MFMA order over the channels, then the
-// fixed overlap-add order), the form's tolerance, not the EXACT order.
+// Numerics: BF16 (FP32 form: FP32) products, FP32 sums (MFMA order over the
+// channels, then the fixed overlap-add order), the form's tolerance, not the
+// EXACT order.
 #include "pvhip_internal.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace pvhip {
 namespace {
@@ -64,8 +67,10 @@ struct DenseParams {
    int x0, y0;             // target x (y) of the unshrunken patch start of source extended column (row) 0
    int px0, py0, pnx, pny; // source positions multiplied (extended coordinates)
    int C;                  // source features (channels)
-   const __bf16 *A;        // chunked BF16 activity [nbatch][C/32][nyExt][nxExt][32]
-   const __bf16 *Wt;       // [nyp*nxp*nfp][C]
+   const __bf16 *A;        // BF16: chunked activity [nbatch][C/32][nyExt][nxExt][32]
+   const __bf16 *Wt;       // BF16: weights [nyp*nxp*nfp][C]
+   const float *Af;        // FP32: activity [nbatch][nyExt][nxExt][C]
+   const float *Wf;        // FP32: weights [nyp*nxp*nfp][C]
    float dtf;
    float *gsyn;
    int accumulate;
@@ -93,7 +98,7 @@ __device__ __forceinline__ int fdiv(int n, FastDiv f) {
    return (int)((t + (uint32_t)n) >> f.shift);
 }
 
-// KS = C / 32 k-steps.  The source fragments of one k-step are one register
+// KS = C / KC k-steps.  The source fragments of one k-step are one register
 // stage; NB stages are in flight (the loads of k-step s + NB - 1 are issued
 // before the MFMAs of k-step s, running on into the next group's first
 // k-steps).  NW = 8 waves per workgroup: two per SIMD hide each other's
@@ -110,15 +115,25 @@ __device__ __forceinline__ int fdiv(int n, FastDiv f) {
 //     48*DMG + 1 apart position by position; when that spans nyp/4 source
 //     rows their target rows differ.
 constexpr int ADD_QUAD = 0, ADD_ATOMIC = 1, ADD_ALL = 2;
-template <int KS, int NW, int NB, int ADD>
+// F32: FP32 operands on v_mfma_f32_16x16x4f32, KC = 16 channels per k-step;
+// otherwise BF16 operands on v_mfma_f32_16x16x32_bf16, KC = 32.  Either way a
+// lane's fragment of one k-step is 16 bytes: BF16 channels 8g..8g+7 of its
+// row, FP32 channels 4g..4g+3, which the FP32 form spreads over four MFMAs
+// (MFMA j takes channel 4g + j from lane group g, for A and B alike; the sum
+// over the channels is the same, in another order).
+template <bool F32, int KS, int NW, int NB, int ADD>
 __global__ void __launch_bounds__(64 * NW) recv_pre_dense_kernel(DenseParams p) {
-   constexpr int NT = 64 * NW;
+   constexpr int NT  = 64 * NW;
+   constexpr int KC  = F32 ? 16 : 32;        // channels per k-step
+   constexpr int EPP = F32 ? 4 : 8;          // channels per 16-byte piece
+   using Elt         = typename std::conditional<F32, float, __bf16>::type;
+   using Frag        = typename std::conditional<F32, f32x4, bf16x8>::type;
    static_assert(KS % NB == 0 && NB % 2 == 0, "k-step s of every group lives in buffer s % NB");
    static_assert(NW % 4 == 0, "whole rounds of four class owners");
    static_assert(ADD != ADD_ALL || NW == 8, "two waves per class");
    extern __shared__ float smem[];
-   __bf16 *wl = reinterpret_cast<__bf16 *>(smem); // [4 classes][KS][16 columns][32 channels]
-   float *ol  = smem + 4 * KS * 16 * 32 / 2;       // tile [rows + 5][pitch]
+   Elt *wl   = reinterpret_cast<Elt *>(smem);    // [4 classes][KS][16 columns][KC channels]
+   float *ol = smem + 4 * KS * 16 * KC * sizeof(Elt) / 4; // tile [rows + 5][pitch]
 
    // the four workgroups of one batch element on one XCD (workgroup L runs on
    // XCD L % 8), so that they share the element's activity in that XCD's L2
@@ -129,7 +144,7 @@ __global__ void __launch_bounds__(64 * NW) recv_pre_dense_kernel(DenseParams p) 
    if (b >= p.post.nbatch) {
       return; // whole workgroup
    }
-   constexpr int C = KS * 32;
+   constexpr int C = KS * KC;
    int const tid   = threadIdx.x;
    int const lane  = tid % 64;
    int const wave  = __builtin_amdgcn_readfirstlane(tid / 64);
@@ -139,9 +154,10 @@ __global__ void __launch_bounds__(64 * NW) recv_pre_dense_kernel(DenseParams p) 
    int const oyq   = pmod4(q - p.y0); // offset rows of this workgroup: oyq + 4*jy
    int const jlo   = (p.Ymin - q) >> 2; // tile row 0 = target row q + 4*jlo (floor division)
 
-   // Weights of this workgroup's offsets, in BF16: per class c (offset column
-   // (c - x0) mod 4 + 4*ix) and k-step, 16 columns (iy, ix, f) x 32 channels.
-   // Item it = tid + u*NT: its column and 16-byte piece depend on tid only.
+   // Weights of this workgroup's offsets: per class c (offset column
+   // (c - x0) mod 4 + 4*ix) and k-step, 16 columns (iy, ix, f) x KC channels,
+   // four 16-byte pieces per column.  Item it = tid + u*NT: its column and
+   // piece depend on tid only.
    {
       constexpr int WIT = 4 * KS * 64 / NT;
       int const gg = tid % 4;
@@ -154,7 +170,7 @@ __global__ void __launch_bounds__(64 * NW) recv_pre_dense_kernel(DenseParams p) 
       int wcol[4];
 #pragma unroll
       for (int c = 0; c < 4; c++) {
-         wcol[c] = ((oy * p.nxp + pmod4(c - p.x0) + DST * ix) * p.nfp + f) * C + 8 * gg;
+         wcol[c] = ((oy * p.nxp + pmod4(c - p.x0) + DST * ix) * p.nfp + f) * C + EPP * gg;
       }
 #pragma unroll
       for (int u = 0; u < WIT; u++) {
@@ -165,8 +181,15 @@ __global__ void __launch_bounds__(64 * NW) recv_pre_dense_kernel(DenseParams p) 
          for (int cc = 1; cc < 4; cc++) {
             wo = c == cc ? wcol[cc] : wo;
          }
-         uint4 const v = *reinterpret_cast<const uint4 *>(p.Wt + wo + (j % KS) * 32);
-         *reinterpret_cast<uint4 *>(wl + (j * 16 + n) * 32 + 8 * gg) = v;
+         const Elt *wsrc;
+         if constexpr (F32) {
+            wsrc = p.Wf;
+         }
+         else {
+            wsrc = p.Wt;
+         }
+         uint4 const v = *reinterpret_cast<const uint4 *>(wsrc + wo + (j % KS) * KC);
+         *reinterpret_cast<uint4 *>(wl + (j * 16 + n) * KC + EPP * gg) = v;
       }
    }
    for (int i = tid; i < (p.rows + 5) * p.pitch; i += NT) {
@@ -175,12 +198,19 @@ __global__ void __launch_bounds__(64 * NW) recv_pre_dense_kernel(DenseParams p) 
 
    int const nxExt      = p.pre.nxExt();
    size_t const plane32 = (size_t)p.pre.nyExt() * nxExt * 32;
-   const __bf16 *Ab     = p.A + (size_t)b * (C / 32) * plane32;
+   const __bf16 *Ab     = nullptr; // BF16 form: batch element b's chunked activity
+   const float *Afb     = nullptr; // FP32 form: its activity
+   if constexpr (F32) {
+      Afb = p.Af + (size_t)b * p.pre.numExt();
+   }
+   else {
+      Ab = p.A + (size_t)b * (C / 32) * plane32;
+   }
    int const nPos       = p.pnx * p.pny;
    int const nMB        = (nPos + 15) / 16;
    int const rounds     = (nMB + NW * DMG - 1) / (NW * DMG);
 
-   // source element offsets (chunk plane 0) of this lane's A-fragment rows.
+   // source element offsets (BF16: in chunk plane 0) of this lane's A-fragment rows.
    // A row past the last position reads position 0 instead: its products
    // land in output rows that the overlap-add skips (a row of the product
    // depends on its own A row only), so the loads need no mask.
@@ -191,27 +221,32 @@ __global__ void __launch_bounds__(64 * NW) recv_pre_dense_kernel(DenseParams p) 
          int const pos = (mb0 + m) * 16 + r;
          int const pc  = pos < nPos ? pos : 0;
          int const yr  = fdiv(pc, p.fdx);
-         aoff[m]       = ((p.py0 + yr) * nxExt + p.px0 + pc - yr * p.pnx) * 32 + 8 * g;
+         aoff[m]       = ((p.py0 + yr) * nxExt + p.px0 + pc - yr * p.pnx) * (F32 ? C : 32) + EPP * g;
       }
    };
-   bf16x8 av[NB][DMG];
+   Frag av[NB][DMG];
    auto fetch = [&](int buf, int ks) {
 #pragma unroll
       for (int m = 0; m < DMG; m++) {
-         av[buf][m] = *reinterpret_cast<const bf16x8 *>(Ab + (size_t)ks * plane32 + aoff[m]);
+         if constexpr (F32) {
+            av[buf][m] = *reinterpret_cast<const f32x4 *>(Afb + aoff[m] + ks * KC);
+         }
+         else {
+            av[buf][m] = *reinterpret_cast<const bf16x8 *>(Ab + (size_t)ks * plane32 + aoff[m]);
+         }
       }
    };
    // weight fragments of k-step ks for the wave's four slots
    int wofs[4];
 #pragma unroll
    for (int k = 0; k < 4; k++) {
-      wofs[k] = (((wave + k) & 3) * KS * 16 + r) * 32 + 8 * g;
+      wofs[k] = (((wave + k) & 3) * KS * 16 + r) * KC + EPP * g;
    }
-   bf16x8 bw[2][4];
+   Frag bw[2][4];
    auto loadB = [&](int buf, int ks) {
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-         bw[buf][k] = *reinterpret_cast<const bf16x8 *>(wl + wofs[k] + ks * 16 * 32);
+         bw[buf][k] = *reinterpret_cast<const Frag *>(wl + wofs[k] + ks * 16 * KC);
       }
    };
    // tile offset of this lane's column (jy, jx, f)
@@ -234,11 +269,27 @@ __global__ void __launch_bounds__(64 * NW) recv_pre_dense_kernel(DenseParams p) 
       // keep the loads ahead of this k-step's MFMAs (the scheduler would
       // otherwise sink them next to their use)
       __builtin_amdgcn_sched_barrier(0);
+      if constexpr (F32) {
+         // channel 4g + j of the k-step: sixteen independent MFMAs between
+         // two uses of one accumulator
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
+         for (int j = 0; j < 4; j++) {
 #pragma unroll
-         for (int m = 0; m < DMG; m++) {
-            acc[m][k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[bb][m], bw[bb & 1][k], acc[m][k], 0, 0, 0);
+            for (int k = 0; k < 4; k++) {
+#pragma unroll
+               for (int m = 0; m < DMG; m++) {
+                  acc[m][k] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[bb][m][j], bw[bb & 1][k][j], acc[m][k], 0, 0, 0);
+               }
+            }
+         }
+      }
+      else {
+#pragma unroll
+         for (int k = 0; k < 4; k++) {
+#pragma unroll
+            for (int m = 0; m < DMG; m++) {
+               acc[m][k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[bb][m], bw[bb & 1][k], acc[m][k], 0, 0, 0);
+            }
          }
       }
    };
@@ -371,7 +422,22 @@ __global__ void __launch_bounds__(256) recv_pre_dense_weights_kernel(
    out[i]      = (__bf16)w[(size_t)c * patch + o];
 }
 
-size_t dense_lds_bytes(int KS, int nJ, int pitch) { return (size_t)4 * KS * 16 * 32 * 2 + (size_t)nJ * pitch * 4; }
+// FP32 weights [C][patch] -> [patch][C]
+__global__ void __launch_bounds__(256) recv_pre_dense_weights_f32_kernel(
+      const float *w, float *out, int C, int patch) {
+   size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= (size_t)C * patch) {
+      return;
+   }
+   int const o = (int)(i / C);
+   int const c = (int)(i % C);
+   out[i]      = w[(size_t)c * patch + o];
+}
+
+// weights of four classes x C channels x 16 columns, then the tile
+size_t dense_lds_bytes(bool f32, int C, int nJ, int pitch) {
+   return (size_t)4 * C * 16 * (f32 ? 4 : 2) + (size_t)nJ * pitch * 4;
+}
 
 } // namespace
 } // namespace pvhip
@@ -392,7 +458,13 @@ int pvhip_recv_pre_dense(const pvhip_recv_pre_dense_args *a, pvhip_stream stream
          "recv_pre_dense: patch %dx%dx%d is not 16 offsets per class of 4x4", a->nxp, a->nyp, a->nfp);
    int const C = a->pre.nf;
    PVHIP_REQUIRE(C == 128 || C == 256 || C == 512, "recv_pre_dense: %d source features (128, 256 or 512)", C);
-   PVHIP_REQUIRE(a->pre_data_bf16c && a->weights_bf16, "recv_pre_dense: NULL activity or weights");
+   bool const f32 = a->fp32 != 0;
+   if (f32) {
+      PVHIP_REQUIRE(a->pre_data && a->weights_f32, "recv_pre_dense: FP32 form needs pre_data and weights_f32");
+   }
+   else {
+      PVHIP_REQUIRE(a->pre_data_bf16c && a->weights_bf16, "recv_pre_dense: NULL activity or weights");
+   }
    DenseParams p{};
    p.pre   = geo(a->pre);
    p.post  = geo(a->post);
@@ -414,8 +486,10 @@ int pvhip_recv_pre_dense(const pvhip_recv_pre_dense_args *a, pvhip_stream stream
       p.pny = p.pre.nyExt();
    }
    p.C     = C;
-   p.A     = static_cast<const __bf16 *>(a->pre_data_bf16c);
-   p.Wt    = static_cast<const __bf16 *>(a->weights_bf16);
+   p.A     = f32 ? nullptr : static_cast<const __bf16 *>(a->pre_data_bf16c);
+   p.Wt    = f32 ? nullptr : static_cast<const __bf16 *>(a->weights_bf16);
+   p.Af    = f32 ? a->pre_data : nullptr;
+   p.Wf    = f32 ? a->weights_f32 : nullptr;
    p.dtf   = a->dt_factor;
    p.gsyn  = a->post_gsyn;
    p.accumulate = a->accumulate;
@@ -454,16 +528,12 @@ int pvhip_recv_pre_dense(const pvhip_recv_pre_dense_args *a, pvhip_stream stream
    p.rows        = (p.Ymax - p.Ymin) / DST + 2;
    p.fdx         = make_fastdiv(p.pnx);
    p.fdxf        = make_fastdiv(a->post.nx * a->nfp);
-   int const KS  = C / 32;
-   size_t const lds = dense_lds_bytes(KS, p.rows + 5, p.pitch);
+   size_t const lds = dense_lds_bytes(f32, C, p.rows + 5, p.pitch);
    PVHIP_REQUIRE(lds <= 160 * 1024, "recv_pre_dense: %zu bytes of LDS (tile rows of %d values)", lds, p.pitch);
    unsigned const groups = (unsigned)((a->post.nbatch + 7) / 8);
    dim3 grid(groups * 8 * 4);
    hipStream_t s = (hipStream_t)stream;
-#ifndef RPD_NB
-#define RPD_NB 4
-#endif
-   constexpr int NW = 8, NB = RPD_NB;
+   constexpr int NW = 8, NB = 4;
    // the four positions of one add lie 4 apart in row-major order: in one row
    // their patches (nxp/4 source columns wide) are disjoint when nxp <= 16,
    // across a row wrap when the grid is at least 12 + nxp/4 wide
@@ -471,24 +541,34 @@ int pvhip_recv_pre_dense(const pvhip_recv_pre_dense_args *a, pvhip_stream stream
    // waves w and w + 4 add together when positions 48*DMG + 1 apart lie
    // nyp/4 source rows apart
    bool const all = !atomic && (a->nyp / DST) * p.pnx <= 48 * DMG + 1;
-#define RPD_LAUNCH(KSV, NBV)                                                                          \
-   if (atomic) {                                                                                     \
-      hipLaunchKernelGGL((recv_pre_dense_kernel<KSV, NW, NBV, ADD_ATOMIC>), grid, dim3(64 * NW), lds, s, p); \
-   }                                                                                                 \
-   else if (all) {                                                                                   \
-      hipLaunchKernelGGL((recv_pre_dense_kernel<KSV, NW, NBV, ADD_ALL>), grid, dim3(64 * NW), lds, s, p); \
-   }                                                                                                 \
-   else {                                                                                            \
-      hipLaunchKernelGGL((recv_pre_dense_kernel<KSV, NW, NBV, ADD_QUAD>), grid, dim3(64 * NW), lds, s, p); \
+#define RPD_LAUNCH(F, KSV)                                                                                \
+   if (atomic) {                                                                                         \
+      hipLaunchKernelGGL((recv_pre_dense_kernel<F, KSV, NW, NB, ADD_ATOMIC>), grid, dim3(64 * NW), lds, s, p); \
+   }                                                                                                     \
+   else if (all) {                                                                                       \
+      hipLaunchKernelGGL((recv_pre_dense_kernel<F, KSV, NW, NB, ADD_ALL>), grid, dim3(64 * NW), lds, s, p); \
+   }                                                                                                     \
+   else {                                                                                                \
+      hipLaunchKernelGGL((recv_pre_dense_kernel<F, KSV, NW, NB, ADD_QUAD>), grid, dim3(64 * NW), lds, s, p); \
    }
-   if (KS == 4) {
-      RPD_LAUNCH(4, (NB < 4 ? NB : 4))
+   // k-steps of 32 channels (BF16) or 16 (FP32)
+   if (!f32 && C == 128) {
+      RPD_LAUNCH(false, 4)
    }
-   else if (KS == 8) {
-      RPD_LAUNCH(8, NB)
+   else if (!f32 && C == 256) {
+      RPD_LAUNCH(false, 8)
+   }
+   else if (!f32) {
+      RPD_LAUNCH(false, 16)
+   }
+   else if (C == 128) {
+      RPD_LAUNCH(true, 8)
+   }
+   else if (C == 256) {
+      RPD_LAUNCH(true, 16)
    }
    else {
-      RPD_LAUNCH(16, NB)
+      RPD_LAUNCH(true, 32)
    }
 #undef RPD_LAUNCH
    PVHIP_LAUNCHED("recv_pre_dense");
@@ -503,6 +583,17 @@ int pvhip_recv_pre_dense_weights(
    hipLaunchKernelGGL(recv_pre_dense_weights_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
          (hipStream_t)stream, weights, static_cast<__bf16 *>(weights_bf16), num_features, patch_size);
    PVHIP_LAUNCHED("recv_pre_dense_weights");
+   return PVHIP_OK;
+}
+
+int pvhip_recv_pre_dense_weights_f32(
+      const float *weights, float *weights_f32, int num_features, int patch_size, pvhip_stream stream) {
+   PVHIP_REQUIRE(weights && weights_f32 && num_features > 0 && patch_size > 0,
+         "recv_pre_dense_weights_f32: bad argument");
+   size_t const n = (size_t)num_features * patch_size;
+   hipLaunchKernelGGL(recv_pre_dense_weights_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+         (hipStream_t)stream, weights, weights_f32, num_features, patch_size);
+   PVHIP_LAUNCHED("recv_pre_dense_weights_f32");
    return PVHIP_OK;
 }
 

@@ -128,6 +128,8 @@ class LCAColumn:
             buf = bf16c_buffer(self.v1_loc, d)
             if self.scatter.bind_pre_bf16c(buf, halo_is_zero=True):
                 self.v1_A16 = buf
+        # the FP32 dense scatter reads v1_A itself: restricted positions only
+        self.scatter.pre_halo_is_zero = True
         # Recon = ANNLayer with the scatter as its only channel: the dense
         # route's channel GEMM applies this update in its epilogue
         self._rec_update = layer_update_args(self.rec_loc, self.rec_gsyn, self.rec_V, self.rec_A,

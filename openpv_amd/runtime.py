@@ -271,11 +271,14 @@ class PresynapticPerspectiveGPUDelivery:
       "ordered" -- an ordered gather over the active list (EXACT, or FP32 with FMA);
       "route"   -- FP32/BF16: a post-perspective gather with the transposed
                    weights (a channel GEMM over the dense activity);
-      "dense"   -- BF16, one data patch per source feature into a target 4x
-                   finer (pvhip_recv_pre_dense): a GEMM of the source positions
-                   by the patch offsets over the channels, then the overlap-add;
-                   needs the source's chunked BF16 copy (bind_pre_bf16c).
-    The default is the first of dense, route, ordered that the geometry allows."""
+      "dense"   -- BF16/FP32, one data patch per source feature into a target
+                   4x finer (pvhip_recv_pre_dense): a GEMM of the source
+                   positions by the patch offsets over the channels, then the
+                   overlap-add; BF16 reads the source's chunked BF16 copy
+                   (bind_pre_bf16c), FP32 the dense activity.
+    The default is the first of dense, route, ordered that the geometry allows.
+    pre_halo_is_zero: the caller guarantees the source activity's halo holds
+    only zeros (the dense mode then multiplies restricted positions only)."""
 
     def __init__(self, pre, post, nxp, nyp, weights, arith=abi.ARITH_EXACT, dt_factor=1.0):
         self.pre, self.post = pre, post
@@ -310,16 +313,15 @@ class PresynapticPerspectiveGPUDelivery:
         self.transposed = None
         self.dense = None
         self._pre_bf16c = None
-        self._dense_halo_zero = False
+        self.pre_halo_is_zero = False
         self._mode = None
         if arith in (abi.ARITH_BF16, abi.ARITH_FP32):
             self.transposed = self._transposed_route()
-        if arith == abi.ARITH_BF16:
             self.dense = self._dense_route()
 
     def _dense_route(self):
         """The geometry of pvhip_recv_pre_dense (see the class docstring), with
-        its BF16 weights; None when the geometry does not fit."""
+        its BF16 (FP32 form: FP32) weights; None when the geometry does not fit."""
         pre, post = self.pre, self.post
         if (post.nx != 4 * pre.nx or post.ny != 4 * pre.ny or self.nxp % 4 or self.nyp % 4
                 or (self.nxp // 4) * (self.nyp // 4) * self.nfp != 16 or pre.nf not in (128, 256, 512)
@@ -330,22 +332,34 @@ class PresynapticPerspectiveGPUDelivery:
                                                   C.byref(x0), C.byref(y0)) != 0:
             return None
         patch = self.nxp * self.nyp * self.nfp
-        wb = torch.zeros(patch * pre.nf, dtype=torch.bfloat16, device=self.weights.device)
-        self.dense = {"start": (x0.value, y0.value), "weights": wb, "patch": patch}
+        f32 = self.arith == abi.ARITH_FP32
+        wb = torch.zeros(patch * pre.nf, dtype=torch.float32 if f32 else torch.bfloat16,
+                         device=self.weights.device)
+        self.dense = {"start": (x0.value, y0.value), "weights": wb, "patch": patch, "fp32": f32}
         self._refresh_dense(None)
         return self.dense
 
     def _refresh_dense(self, stream):
         d = self.dense
+        if d["fp32"]:
+            abi.check(abi.hip().pvhip_recv_pre_dense_weights_f32(
+                _ptr(self.weights), _ptr(d["weights"]), self.pre.nf, d["patch"], _stream(stream)),
+                "pvhip_recv_pre_dense_weights_f32")
+            return
         abi.check(abi.hip().pvhip_recv_pre_dense_weights(_ptr(self.weights), _ptr(d["weights"]), self.pre.nf,
                                                          d["patch"], _stream(stream)),
                   "pvhip_recv_pre_dense_weights")
+
+    def _dense_ready(self):
+        """The dense mode's source is there: FP32 reads the activity passed to
+        deliver, BF16 the bound chunked copy."""
+        return self.dense is not None and (self.dense["fp32"] or self._pre_bf16c is not None)
 
     @property
     def mode(self):
         if self._mode is not None:
             return self._mode
-        if self.dense is not None and self._pre_bf16c is not None:
+        if self._dense_ready():
             return "dense"
         return "route" if self.transposed is not None else "ordered"
 
@@ -353,8 +367,9 @@ class PresynapticPerspectiveGPUDelivery:
     def mode(self, m):
         if m not in (None, "dense", "route", "ordered"):
             raise abi.PVHIPError("unknown scatter mode %r" % (m,))
-        if m == "dense" and (self.dense is None or self._pre_bf16c is None):
-            raise abi.PVHIPError("the dense scatter needs its geometry and the source's chunked BF16 copy")
+        if m == "dense" and not self._dense_ready():
+            raise abi.PVHIPError("the dense scatter needs its geometry (and, in BF16, the source's chunked "
+                                 "BF16 copy)")
         if m == "route" and self.transposed is None:
             raise abi.PVHIPError("this scatter has no transposed route")
         self._mode = m
@@ -450,23 +465,30 @@ class PresynapticPerspectiveGPUDelivery:
         if t is not None and t.inverse is not None and t.arith == abi.ARITH_BF16 and self.pre.nf % 32 == 0:
             t.pre_bf16c = buf
             ok = True
-        if self.dense is not None:
+        if self.dense is not None and not self.dense["fp32"]:
             self._pre_bf16c = buf
-            self._dense_halo_zero = bool(halo_is_zero)
+            self.pre_halo_is_zero = bool(halo_is_zero)
             ok = True
         if buf is None and self._mode == "dense":
             self._mode = None
         return ok
 
-    def _deliver_dense(self, gsyn, accumulate, stream, fused_layer):
+    def _deliver_dense(self, gsyn, accumulate, stream, fused_layer, pre_activity=None):
         d = self.dense
         a = abi.RecvPreDenseArgs()
         a.pre, a.post = self.pre, self.post
         a.nxp, a.nyp, a.nfp = self.nxp, self.nyp, self.nfp
         a.start_x0, a.start_y0 = d["start"]
-        a.halo_is_zero = int(self._dense_halo_zero)
-        a.pre_data_bf16c = _ptr(self._pre_bf16c)
-        a.weights_bf16 = _ptr(d["weights"])
+        a.halo_is_zero = int(self.pre_halo_is_zero)
+        if d["fp32"]:
+            if pre_activity is None:
+                raise abi.PVHIPError("the FP32 dense scatter reads the dense pre activity")
+            a.fp32 = 1
+            a.pre_data = _ptr(pre_activity)
+            a.weights_f32 = _ptr(d["weights"])
+        else:
+            a.pre_data_bf16c = _ptr(self._pre_bf16c)
+            a.weights_bf16 = _ptr(d["weights"])
         a.dt_factor = self.dt_factor
         a.post_gsyn = _ptr(gsyn)
         a.accumulate = int(accumulate)
@@ -483,7 +505,7 @@ class PresynapticPerspectiveGPUDelivery:
         mode = self.mode
         if mode == "dense":
             fl = fused_layer if not accumulate else None
-            self._deliver_dense(gsyn, accumulate, stream, fl)
+            self._deliver_dense(gsyn, accumulate, stream, fl, pre_activity)
             if fused_layer is not None and fl is None:
                 launch_layer_update(fused_layer, stream)
             return

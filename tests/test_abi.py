@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol(header, loader, table):
 
 
 def test_abi_version_and_error_string():
-    assert abi.hip().pvhip_abi_version() == 7
+    assert abi.hip().pvhip_abi_version() == 8
     assert isinstance(abi.hip().pvhip_last_error(), bytes)
 
 

@@ -1,13 +1,15 @@
-"""The dense BF16 scatter (pvhip_recv_pre_dense): the pre-perspective delivery
-of a dense source, one data patch per source feature, into a target 4x finer,
-as a GEMM of source positions by patch offsets on the matrix cores followed by
-the overlap-add.
+"""The dense scatter (pvhip_recv_pre_dense, BF16 and FP32 forms): the
+pre-perspective delivery of a dense source, one data patch per source feature,
+into a target 4x finer, as a GEMM of source positions by patch offsets on the
+matrix cores followed by the overlap-add.
 
 Checks, against the oracle's scatter (the reference CPU loop order):
   * on operands already rounded to BF16 the products are exact in FP32, so
     the kernel differs from the oracle only by the order of FP32 sums:
     within 1e-5 of the largest output;
   * on FP32 operands, within the BF16 bound of the other BF16 forms;
+  * the FP32 form within 1e-5 of the largest output (FP32 products; only the
+    order of the sums differs from the oracle's);
   * halo activity (every extended position) and the restricted-only mode,
     accumulate, a batch that is not a multiple of 8, several target
     features, the fused target-layer update (the same bits as the delivery
@@ -72,9 +74,13 @@ def _setup(case, seed):
     return rng, hp, hq, op, oq, p, dens, W
 
 
-def _delivery(hp, hq, p, W, A, halo_is_zero):
-    d = R.PresynapticPerspectiveGPUDelivery(hp, hq, p, p, dev(W), arith=abi.ARITH_BF16)
+def _delivery(hp, hq, p, W, A, halo_is_zero, arith=abi.ARITH_BF16):
+    d = R.PresynapticPerspectiveGPUDelivery(hp, hq, p, p, dev(W), arith=arith)
     assert d.dense is not None
+    if arith == abi.ARITH_FP32:
+        d.pre_halo_is_zero = halo_is_zero
+        assert d.mode == "dense"
+        return d, None
     A16 = R.bf16c_buffer(hp, DEV)
     R.activity_to_bf16c(hp, dev(A), A16)
     assert d.bind_pre_bf16c(A16, halo_is_zero=halo_is_zero)
@@ -116,14 +122,35 @@ def test_dense_scatter_fp32_operands_and_accumulate(case):
     assert np.sqrt((err ** 2).mean()) <= 1e-2 * np.sqrt((ref ** 2).mean())
 
 
-def test_dense_scatter_fused_layer_deterministic_and_graph():
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("halo_too", [False, True])
+def test_dense_scatter_fp32_form(case, halo_too):
+    """The FP32 form (FP32 operands on v_mfma_f32_16x16x4f32): the oracle's
+    FP32 scatter within 1e-5 of the largest output (only the order of the
+    FP32 sums differs); accumulate adds to GSyn."""
+    rng, hp, hq, op, oq, p, dens, W = _setup(case, 75)
+    A = activity(hp, rng, dens, halo_too)
+    print("FP32 dense scatter", case, "halo activity" if halo_too else "zero halo", flush=True)
+    d, _ = _delivery(hp, hq, p, W, A, halo_is_zero=not halo_too, arith=abi.ARITH_FP32)
+    ref = O.recv_pre(op, oq, p, p, W, pre_activity=A, num_threads=1).astype(np.float64)
+    g = torch.full((hq.nbatch * hq.num_res,), 7.0, device=DEV)
+    d.deliver(g, pre_activity=dev(A), accumulate=False)
+    assert np.abs(host(g) - ref).max() <= 1e-5 * np.abs(ref).max()
+    g0 = rng.standard_normal(hq.nbatch * hq.num_res).astype(np.float32)
+    g = dev(g0)
+    d.deliver(g, pre_activity=dev(A), accumulate=True)
+    assert np.abs(host(g) - g0 - ref).max() <= 1e-5 * (np.abs(ref).max() + np.abs(g0).max())
+
+
+@pytest.mark.parametrize("arith", [abi.ARITH_BF16, abi.ARITH_FP32])
+def test_dense_scatter_fused_layer_deterministic_and_graph(arith):
     """The fused target-layer update equals the delivery followed by
     pvhip_layer_update bit for bit; repeated calls and hipGraph replays give
     the same bits."""
     rng, hp, hq, op, oq, p, dens, W = _setup(CASES[0], 73)
     hq = abi.Loc.make(2, 128, 128, 1, halo=1)      # the target layer's own halo for A
     A = activity(hp, rng, dens, False)
-    d, _ = _delivery(hp, hq, p, W, A, halo_is_zero=True)
+    d, _ = _delivery(hp, hq, p, W, A, halo_is_zero=True, arith=arith)
     tr = R.make_transfer(0.05, 0.0, FMAX, 0.02, 0.0)
     n, ne = hq.nbatch * hq.num_res, hq.nbatch * hq.num_ext
     Ad = dev(A)
@@ -157,12 +184,13 @@ def test_dense_scatter_fused_layer_deterministic_and_graph():
         assert np.array_equal(host(g).view(np.uint32), first.view(np.uint32))
 
 
-def test_dense_weights_follow_refresh():
-    """refresh_weights re-derives the dense mode's BF16 weights after a weight
-    change (the delivery then matches the oracle on the new weights)."""
+@pytest.mark.parametrize("arith", [abi.ARITH_BF16, abi.ARITH_FP32])
+def test_dense_weights_follow_refresh(arith):
+    """refresh_weights re-derives the dense mode's BF16 (FP32) weights after a
+    weight change (the delivery then matches the oracle on the new weights)."""
     rng, hp, hq, op, oq, p, dens, W = _setup(CASES[1], 74)
     A = bf16_round(activity(hp, rng, dens, False))
-    d, _ = _delivery(hp, hq, p, bf16_round(W), A, halo_is_zero=True)
+    d, _ = _delivery(hp, hq, p, bf16_round(W), A, halo_is_zero=True, arith=arith)
     W2 = bf16_round(W[::-1].copy() * 0.5)
     d.weights.copy_(dev(W2).reshape(d.weights.shape))
     d.refresh_weights()
