@@ -308,8 +308,9 @@ struct Conn {
    ConnMode mode = C_NONE;
    int arith     = PVHIP_ARITH_EXACT;
    bool routed   = false;    // a GPU scatter run as a gather with transposed weights
-   bool dense    = false;    // a BF16 GPU scatter run as pvhip_recv_pre_dense
-   void *Wdense  = nullptr;  // its BF16 weights [nyp*nxp*nfp][pre.nf]
+   bool dense    = false;    // a BF16 or FP32 GPU scatter run as pvhip_recv_pre_dense
+   bool denseF32 = false;    // its FP32 form (FP32 activity and weights)
+   void *Wdense  = nullptr;  // its BF16 (FP32) weights [nyp*nxp*nfp][pre.nf]
    int dx0 = 0, dy0 = 0;     // its patch start of source extended column / row 0
    std::vector<float> hostW; // pre-perspective weights (originals only)
    float *W = nullptr;       // device weights the delivery reads
@@ -423,6 +424,7 @@ struct Net {
    bool transposedRoute(Conn &c);
    bool denseRoute(Conn &c);
    void syncBf16(Layer &L);
+   void deriveDense(Conn &c);
    void buildConn(Conn &c);
    void step(double t);
    pvhip_recv_post_args gatherArgs(Conn &c);
@@ -794,7 +796,13 @@ void Net::build() {
       }
    }
    for (auto &C : conns) {
-      if (C.mode == C_SCATTER && (C.arith == PVHIP_ARITH_BF16 || C.arith == PVHIP_ARITH_FP8) && denseRoute(C)) {
+      if (C.mode == C_SCATTER && C.arith == PVHIP_ARITH_FP32 && denseRoute(C)) {
+         C.dense    = true; // reads the source's FP32 activity
+         C.denseF32 = true;
+         std::printf("pvrun: %s: dense FP32 scatter (pvhip_recv_pre_dense)\n", C.name.c_str());
+      }
+      else if (C.mode == C_SCATTER && (C.arith == PVHIP_ARITH_BF16 || C.arith == PVHIP_ARITH_FP8)
+               && denseRoute(C)) {
          C.dense  = true; // the FP8 form's scatter runs in BF16, as its transposed route does
          std::printf("pvrun: %s: dense BF16 scatter (pvhip_recv_pre_dense)\n", C.name.c_str());
          Layer &L = layers[C.pre];
@@ -997,7 +1005,7 @@ bool Net::transposedRoute(Conn &C) {
           == 0;
 }
 
-// The dense BF16 scatter (pvhip_recv_pre_dense): one data patch per source
+// The dense scatter (pvhip_recv_pre_dense, BF16 or FP32): one data patch per source
 // feature into a target 4x finer, 16 offsets per class of 4x4, source
 // features 128/256/512, no delay, and a source halo that holds only zeros
 // (no mirror boundary).
@@ -1012,6 +1020,20 @@ bool Net::denseRoute(Conn &C) {
       return false;
    }
    return pvhost_recv_pre_dense_start(&pre.loc, &post.loc, C.nxp, C.nyp, nullptr, &C.dx0, &C.dy0) == 0;
+}
+
+// The dense scatter's transposed weights ([nyp*nxp*nfp][pre.nf], BF16 or FP32)
+// from the connection's weights.
+void Net::deriveDense(Conn &C) {
+   int const nf    = layers[C.pre].loc.nf;
+   int const patch = C.nxp * C.nyp * C.nfp;
+   if (C.denseF32) {
+      check(pvhip_recv_pre_dense_weights_f32(C.W, static_cast<float *>(C.Wdense), nf, patch, D.stream),
+            "pvhip_recv_pre_dense_weights_f32");
+   }
+   else {
+      check(pvhip_recv_pre_dense_weights(C.W, C.Wdense, nf, patch, D.stream), "pvhip_recv_pre_dense_weights");
+   }
 }
 
 // A layer's chunked BF16 activity from its FP32 activity (after A was set
@@ -1105,9 +1127,9 @@ void Net::buildConn(Conn &C) {
       }
       if (C.dense) {
          int const patch = C.nxp * C.nyp * C.nfp;
-         C.Wdense        = D.alloc(pvhip_recv_pre_dense_weights_bytes(pre.loc.nf, patch));
-         check(pvhip_recv_pre_dense_weights(C.W, C.Wdense, pre.loc.nf, patch, D.stream),
-               "pvhip_recv_pre_dense_weights");
+         C.Wdense        = D.alloc(C.denseF32 ? (size_t)pre.loc.nf * patch * 4
+                                              : pvhip_recv_pre_dense_weights_bytes(pre.loc.nf, patch));
+         deriveDense(C);
       }
       int ncol = 0, nrow = 0;
       if (pvhost_recv_pre_tables(&pre.loc, &post.loc, C.nxp, C.nyp, nullptr, nullptr, nullptr, nullptr, &ncol,
@@ -1250,8 +1272,15 @@ void Net::deliver(Conn &C) {
       a.start_x0       = C.dx0;
       a.start_y0       = C.dy0;
       a.halo_is_zero   = 1; // denseRoute: no mirror boundary writes the halo
-      a.pre_data_bf16c = pre.Ab;
-      a.weights_bf16   = C.Wdense;
+      if (C.denseF32) {
+         a.fp32        = 1;
+         a.pre_data    = pre.A;
+         a.weights_f32 = static_cast<const float *>(C.Wdense);
+      }
+      else {
+         a.pre_data_bf16c = pre.Ab;
+         a.weights_bf16   = C.Wdense;
+      }
       a.dt_factor      = 1.0f;
       a.post_gsyn      = g;
       a.accumulate     = 1;
@@ -1690,8 +1719,7 @@ void Net::refreshDerived(Conn &O) {
          check(pvhip_permute_weights(O.W, X.perm, X.W, X.permN, D.stream), "pvhip_permute_weights");
       }
       if (X.dense) {
-         check(pvhip_recv_pre_dense_weights(X.W, X.Wdense, layers[X.pre].loc.nf, X.nxp * X.nyp * X.nfp, D.stream),
-               "pvhip_recv_pre_dense_weights");
+         deriveDense(X);
       }
       if (X.Wlp && X.arith == PVHIP_ARITH_BF16) {
          check(pvhip_recv_post_bf16_weights(X.W, X.Wlp, X.lpKernels, X.lpPatch, D.stream), "bf16 weights");

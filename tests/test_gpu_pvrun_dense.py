@@ -1,10 +1,10 @@
-"""The dense BF16 scatter through the .params drop-in: pvrun builds the LCA
+"""The dense scatter through the .params drop-in: pvrun builds the LCA
 column of the batched workload's geometry (128x128x1 images, V1 32x32x128,
 16x16 patches at stride 4, VThresh 0.15, tau 100, fixed dt) with
-gpuArithmetic "bf16" or "fp8"; the CloneConn V1 -> Reconstruction then runs
-pvhip_recv_pre_dense (V1's update keeps its chunked BF16 activity), and the
-reconstructions stay within the BF16 / FP8 bounds of the oracle column's
-(EXACT) over 30 steps.
+gpuArithmetic "bf16", "fp8" or "fp32"; the CloneConn V1 -> Reconstruction then
+runs pvhip_recv_pre_dense (BF16: V1's update keeps its chunked BF16 activity;
+FP32: it reads V1's FP32 activity), and the reconstructions stay within the
+BF16 / FP8 / FP32 bounds of the oracle column's (EXACT) over 30 steps.
 """
 import os
 import subprocess
@@ -105,7 +105,7 @@ IdentConn "ReconstructionToResidual" = {
 """
 
 
-@pytest.mark.parametrize("form,tol", [("bf16", 5e-3), ("fp8", 5e-2)])
+@pytest.mark.parametrize("form,tol", [("bf16", 5e-3), ("fp8", 5e-2), ("fp32", 1e-4)])
 def test_pvrun_dense_scatter_column(tmp_path, form, tol):
     import bench
     steps = 30
@@ -119,7 +119,7 @@ def test_pvrun_dense_scatter_column(tmp_path, form, tol):
                         str(steps)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
     assert r.returncode == 0, r.stderr[-800:]
-    assert "V1ToReconstruction: dense BF16 scatter" in r.stdout
+    assert "V1ToReconstruction: dense %s scatter" % ("FP32" if form == "fp32" else "BF16") in r.stdout
     _, times, rec = pvp.read_activity(str(tmp_path / "out" / "Reconstruction.pvp"))
     assert list(times) == [float(t) for t in range(1, steps + 1) for _ in range(2)]   # two batch elements
     rec = rec.reshape(steps, 2, 128, 128, 1)
