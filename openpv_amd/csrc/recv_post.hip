@@ -2142,11 +2142,11 @@ int recv_post_launch(const pvhip_recv_post_args *args, PostParams &p, hipStream_
       }
       // the persistent kernel for full tiles of the fused VTHRESH update
       // (measured faster than one tile per workgroup: FP32 by keeping its
-      // weight fragments in registers across tiles, BF16 by overlapping the
-      // next tile's loads with the current tile's MFMAs and stores; FP8
-      // measured slower and keeps the one-tile kernel)
+      // weight fragments in registers across tiles, BF16 and FP8 by
+      // overlapping the next tile's loads with the current tile's MFMAs and
+      // stores)
       int const KS       = K / 32;
-      bool const persist = !fp8 && epi == 1 && align >= 2 && K % 32 == 0 && K == Kpad && (KS == 6 || KS == 8)
+      bool const persist = epi == 1 && align >= 2 && K % 32 == 0 && K == Kpad && (KS == 6 || KS == 8)
                            && p.post.nx % MTX == 0 && p.post.ny % MTY == 0 && p.post.nf % MF == 0
                            && (long long)srcRowsMax * srcColsMax * p.nfp <= 256LL * PSREG;
       if (persist) {

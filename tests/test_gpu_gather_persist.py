@@ -1,7 +1,7 @@
-"""The gather + fused LCA update over a large batch (the FP32 and BF16 forms
-run the persistent kernel there, whose workgroups walk several tiles and
-prefetch the next tile while computing the current one; FP8 runs one tile
-per workgroup): every batch element equals, bit for bit, the same delivery
+"""The gather + fused LCA update over a large batch (the FP32, BF16 and FP8
+forms run the persistent kernel there, whose workgroups walk several tiles
+and prefetch the next tile while computing the current one): every batch
+element equals, bit for bit, the same delivery
 run on that element alone.  Per neuron the arithmetic does not depend on how tiles
 are assigned to workgroups, so any difference is a pipelining or indexing
 mistake."""
