@@ -525,7 +525,18 @@ __device__ __forceinline__ float vthresh_select(
    return a;
 }
 
-template <bool EMIT>
+// The plain threshold (no ramp, no AMax), dtFactor 1, A(t-1) from V, only the
+// chunked BF16 copy of A written: the batched LCA's BF16 / FP8 step.
+__device__ __forceinline__ bool lca_plain_bf16(const PostParams &p) {
+   const LCAParams &q = p.lca;
+   Transfer const &t  = q.transfer;
+   return t.VThresh > -FLT_MAX && !(t.VThresh + t.VWidth > t.VThresh) && !(t.AMax < FLT_MAX) && q.aFromV
+          && p.dtf == 1.0f && q.A == nullptr && q.Ab != nullptr;
+}
+
+// PLAIN: the caller has checked lca_plain_bf16(p) (the plain threshold, no
+// FP32 activity, the chunked BF16 copy written): straight-line code for it.
+template <bool EMIT, bool PLAIN = false>
 __device__ __forceinline__ void lca_epilogue_vthresh(
       const PostParams &p, int b, int x0, int y0, int epr, int eq, int ef, const float *out,
       const f32x4 *vpre) {
@@ -537,7 +548,7 @@ __device__ __forceinline__ void lca_epilogue_vthresh(
    bool const thr     = t.VThresh > -FLT_MAX;
    bool const ramp    = t.VThresh + t.VWidth > t.VThresh;
    bool const clamp   = t.AMax < FLT_MAX;
-   bool const simple  = thr && !ramp && !clamp && q.aFromV && p.dtf == 1.0f; // uniform
+   bool const simple  = PLAIN || (thr && !ramp && !clamp && q.aFromV && p.dtf == 1.0f); // uniform
    int const nf       = p.post.nf;
    int const rowV     = p.post.nx * nf;
    int const rowA     = q.loc.nxExt() * nf;
@@ -596,10 +607,10 @@ __device__ __forceinline__ void lca_epilogue_vthresh(
       // V and the BF16 activity go through the caches (the next step reads
       // both; a plain store measured faster than a streaming one here).
       *reinterpret_cast<f32x4 *>(Vp + yl * rowV + xo) = vn;
-      if (Ap) {
+      if (!PLAIN && Ap) {
          __builtin_nontemporal_store(an, reinterpret_cast<f32x4 *>(Ap + yl * rowA + xo));
       }
-      if (Abp) {
+      if (PLAIN || Abp) {
          *reinterpret_cast<bf16x4 *>(Abp + (yl * q.loc.nxExt() + it * MEPP % MTX) * 32) =
                bf16x4{(__bf16)an[0], (__bf16)an[1], (__bf16)an[2], (__bf16)an[3]};
       }
@@ -1096,6 +1107,7 @@ __global__ void __launch_bounds__(256) recv_post_mfma_persist_kernel(
       }
    };
 
+   bool const plain = lca_plain_bf16(p); // the same for every tile
    Tile cur = tile_of(first);
    f32x4 vpre[MEP], vpreN[MEP];
    float sv[PSREG];
@@ -1163,7 +1175,12 @@ __global__ void __launch_bounds__(256) recv_post_mfma_persist_kernel(
          }
       }
       __syncthreads();
-      lca_epilogue_vthresh<false>(p, cur.b, cur.x0, cur.y0, epr, eq, ef, outl + epr * MOP + 4 * eq, vpre);
+      if (plain) {
+         lca_epilogue_vthresh<false, true>(p, cur.b, cur.x0, cur.y0, epr, eq, ef, outl + epr * MOP + 4 * eq, vpre);
+      }
+      else {
+         lca_epilogue_vthresh<false>(p, cur.b, cur.x0, cur.y0, epr, eq, ef, outl + epr * MOP + 4 * eq, vpre);
+      }
       if (more) {
          __syncthreads(); // every wave is done reading the output tile
          store_src(nxt, sv);
