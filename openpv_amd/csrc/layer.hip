@@ -253,15 +253,26 @@ __global__ void __launch_bounds__(kRowThreads) error_update_kernel(
    int const numRes = loc.numRes();
    int const rowLen = loc.nx * loc.nf;
    int const b      = blockIdx.y;
+   // every element's two reads first (clamped in bounds), so they are in
+   // flight together rather than one round trip per element
+   float in0[kRowIter], rec0[kRowIter];
+#pragma unroll
+   for (int u = 0; u < kRowIter; u++) {
+      int const k = min(blockIdx.x * kRowSpan + u * kRowThreads + threadIdx.x, numRes - 1);
+      int const y = k / rowLen;
+      int const j = k - y * rowLen;
+      in0[u]      = in[ext_row(inLoc, b, y) + j];
+      rec0[u]     = rec[ext_row(recLoc, b, y) + j];
+   }
 #pragma unroll
    for (int u = 0; u < kRowIter; u++) {
       int const k = blockIdx.x * kRowSpan + u * kRowThreads + threadIdx.x;
       if (k < numRes) {
          int const y    = k / rowLen;
          int const j    = k - y * rowLen;
-         float const g0 = 0.0f + in[ext_row(inLoc, b, y) + j]; // IdentDelivery, reset GSyn
-         float const g1 = 0.0f + rec[ext_row(recLoc, b, y) + j];
-         float value    = 0.0f;                                 // GSynAccumulator
+         float const g0 = 0.0f + in0[u]; // IdentDelivery, reset GSyn
+         float const g1 = 0.0f + rec0[u];
+         float value    = 0.0f;          // GSynAccumulator
          float prod     = c0 * g0;
          value          = value + prod;
          prod           = c1 * g1;
