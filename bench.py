@@ -663,7 +663,7 @@ def main():
         # the flops (9.5 us against 6.9 us), so HBM bounds the kernel
         sbytes = 2 * NB * NXV * NYV * NFV + 2 * NXP * NXP * NFV + 8 * NB * (4 * NXV) * (4 * NYV)
         line["scatter_kernel"] = {
-            "kernel": "recv_pre_dense_kernel<16, 8, 4, 2> (BF16 GEMM, overlap-add, Recon update)",
+            "kernel": "recv_pre_dense_kernel<false, 16, 8, 4, 2> (BF16 GEMM, overlap-add, Recon update)",
             "kernel_ms": round(sms, 5), "kernel_ms_min": round(smin, 5), "bound": "hbm",
             "achieved": round(sbytes / (sms * 1e-3) / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(sbytes / (sms * 1e-3) / 1e9 / 8000.0, 4), "algorithmic_bytes": sbytes,
