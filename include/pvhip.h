@@ -433,6 +433,11 @@ typedef struct {
 } pvhip_recv_pre_dense_args;
 
 int pvhip_recv_pre_dense(const pvhip_recv_pre_dense_args *args, pvhip_stream stream);
+/* 1 when pvhip_recv_pre_dense can run these arguments' geometry and form
+ * (pre, post, nxp, nyp, nfp, start_x0/y0, halo_is_zero, fp32 are read; the
+ * workgroup's weights and target tile fit its LDS), else 0.  Host only, no
+ * device call; the choice of the dense form before any buffer exists. */
+int pvhip_recv_pre_dense_fits(const pvhip_recv_pre_dense_args *args);
 /* BF16 weights of pvhip_recv_pre_dense: pre-perspective weights
  * [num_features][patch_size] (patch_size = nyp*nxp*nfp) transposed to
  * [patch_size][num_features], rounded to nearest even; call after every

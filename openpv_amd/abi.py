@@ -185,6 +185,7 @@ PVHIP_SYMBOLS = {
     "pvhip_recv_pre_dense_weights": (i32, [vp, vp, i32, i32, vp]),
     "pvhip_recv_pre_dense_weights_bytes": (C.c_size_t, [i32, i32]),
     "pvhip_recv_pre_dense_weights_f32": (i32, [vp, vp, i32, i32, vp]),
+    "pvhip_recv_pre_dense_fits": (i32, [C.POINTER(RecvPreDenseArgs)]),
     "pvhip_lca_update": (i32, [C.POINTER(LCAUpdateArgs), vp]),
     "pvhip_momentum_lca_update": (i32, [C.POINTER(LCAUpdateArgs), C.c_float, vp, vp]),
     "pvhip_ista_update": (i32, [C.POINTER(LCAUpdateArgs), C.c_float, vp]),

@@ -439,6 +439,38 @@ size_t dense_lds_bytes(bool f32, int C, int nJ, int pitch) {
    return (size_t)4 * C * 16 * (f32 ? 4 : 2) + (size_t)nJ * pitch * 4;
 }
 
+constexpr size_t kDenseLdsMax = 160 * 1024;
+
+// The positions multiplied (all extended ones, or the restricted ones when
+// the halo holds only zeros) and the LDS tile of a workgroup: the target,
+// widened to every column / row a multiplied position's patch reaches, then 5
+// absorbing rows.  Returns the workgroup's LDS bytes.  Needs p.x0, p.y0 and
+// p.pre set.
+size_t dense_tile(const pvhip_recv_pre_dense_args &a, DenseParams &p) {
+   if (a.halo_is_zero) {
+      p.px0 = a.pre.halo.lt;
+      p.py0 = a.pre.halo.up;
+      p.pnx = a.pre.nx;
+      p.pny = a.pre.ny;
+   }
+   else {
+      p.px0 = 0;
+      p.py0 = 0;
+      p.pnx = p.pre.nxExt();
+      p.pny = p.pre.nyExt();
+   }
+   int const Xmin = std::min(0, DST * p.px0 + p.x0);
+   int const Xmax = std::max(a.post.nx - 1, DST * (p.px0 + p.pnx - 1) + p.x0 + a.nxp - 1);
+   p.Ymin         = std::min(0, DST * p.py0 + p.y0);
+   p.Ymax         = std::max(a.post.ny - 1, DST * (p.py0 + p.pny - 1) + p.y0 + a.nyp - 1);
+   p.Xlo          = Xmin;
+   p.pitch        = (Xmax - Xmin + 1) * a.nfp;
+   p.pitch += (p.pitch % 2 == 0); // odd: consecutive rows start in different banks
+   // rows (Y - q)/4 - floor((Ymin - q)/4) for Y <= Ymax, over every q
+   p.rows = (p.Ymax - p.Ymin) / DST + 2;
+   return dense_lds_bytes(a.fp32 != 0, a.pre.nf, p.rows + 5, p.pitch);
+}
+
 } // namespace
 } // namespace pvhip
 
@@ -473,18 +505,6 @@ int pvhip_recv_pre_dense(const pvhip_recv_pre_dense_args *a, pvhip_stream stream
    p.nfp   = a->nfp;
    p.x0    = a->start_x0;
    p.y0    = a->start_y0;
-   if (a->halo_is_zero) {
-      p.px0 = a->pre.halo.lt;
-      p.py0 = a->pre.halo.up;
-      p.pnx = a->pre.nx;
-      p.pny = a->pre.ny;
-   }
-   else {
-      p.px0 = 0;
-      p.py0 = 0;
-      p.pnx = p.pre.nxExt();
-      p.pny = p.pre.nyExt();
-   }
    p.C     = C;
    p.A     = f32 ? nullptr : static_cast<const __bf16 *>(a->pre_data_bf16c);
    p.Wt    = f32 ? nullptr : static_cast<const __bf16 *>(a->weights_bf16);
@@ -515,21 +535,10 @@ int pvhip_recv_pre_dense(const pvhip_recv_pre_dense_args *a, pvhip_stream stream
    else {
       PVHIP_REQUIRE(a->post_gsyn != nullptr, "recv_pre_dense: NULL post_gsyn");
    }
-   // tile extent: the target, widened to every column / row a multiplied
-   // position's patch reaches
-   int const Xmin = std::min(0, DST * p.px0 + p.x0);
-   int const Xmax = std::max(a->post.nx - 1, DST * (p.px0 + p.pnx - 1) + p.x0 + a->nxp - 1);
-   p.Ymin         = std::min(0, DST * p.py0 + p.y0);
-   p.Ymax         = std::max(a->post.ny - 1, DST * (p.py0 + p.pny - 1) + p.y0 + a->nyp - 1);
-   p.Xlo          = Xmin;
-   p.pitch        = (Xmax - Xmin + 1) * a->nfp;
-   p.pitch += (p.pitch % 2 == 0); // odd: consecutive rows start in different banks
-   // rows (Y - q)/4 - floor((Ymin - q)/4) for Y <= Ymax, over every q
-   p.rows        = (p.Ymax - p.Ymin) / DST + 2;
+   size_t const lds = dense_tile(*a, p);
    p.fdx         = make_fastdiv(p.pnx);
    p.fdxf        = make_fastdiv(a->post.nx * a->nfp);
-   size_t const lds = dense_lds_bytes(f32, C, p.rows + 5, p.pitch);
-   PVHIP_REQUIRE(lds <= 160 * 1024, "recv_pre_dense: %zu bytes of LDS (tile rows of %d values)", lds, p.pitch);
+   PVHIP_REQUIRE(lds <= kDenseLdsMax, "recv_pre_dense: %zu bytes of LDS (tile rows of %d values)", lds, p.pitch);
    unsigned const groups = (unsigned)((a->post.nbatch + 7) / 8);
    dim3 grid(groups * 8 * 4);
    hipStream_t s = (hipStream_t)stream;
@@ -573,6 +582,20 @@ int pvhip_recv_pre_dense(const pvhip_recv_pre_dense_args *a, pvhip_stream stream
 #undef RPD_LAUNCH
    PVHIP_LAUNCHED("recv_pre_dense");
    return PVHIP_OK;
+}
+
+int pvhip_recv_pre_dense_fits(const pvhip_recv_pre_dense_args *a) {
+   if (a == nullptr || !loc_valid(a->pre) || !loc_valid(a->post) || a->post.nx != DST * a->pre.nx
+       || a->post.ny != DST * a->pre.ny || a->nfp != a->post.nf || a->nxp % DST || a->nyp % DST
+       || (a->nxp / DST) * (a->nyp / DST) * a->nfp != 16
+       || (a->pre.nf != 128 && a->pre.nf != 256 && a->pre.nf != 512)) {
+      return 0;
+   }
+   DenseParams p{};
+   p.pre = geo(a->pre);
+   p.x0  = a->start_x0;
+   p.y0  = a->start_y0;
+   return dense_tile(*a, p) <= kDenseLdsMax;
 }
 
 int pvhip_recv_pre_dense_weights(

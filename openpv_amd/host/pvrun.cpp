@@ -1019,7 +1019,21 @@ bool Net::denseRoute(Conn &C) {
        || (f != 128 && f != 256 && f != 512)) {
       return false;
    }
-   return pvhost_recv_pre_dense_start(&pre.loc, &post.loc, C.nxp, C.nyp, nullptr, &C.dx0, &C.dy0) == 0;
+   if (pvhost_recv_pre_dense_start(&pre.loc, &post.loc, C.nxp, C.nyp, nullptr, &C.dx0, &C.dy0) != 0) {
+      return false;
+   }
+   // the workgroup's weights and target tile fit its LDS
+   pvhip_recv_pre_dense_args a{};
+   a.pre          = pre.loc;
+   a.post         = post.loc;
+   a.nxp          = C.nxp;
+   a.nyp          = C.nyp;
+   a.nfp          = C.nfp;
+   a.start_x0     = C.dx0;
+   a.start_y0     = C.dy0;
+   a.halo_is_zero = 1;
+   a.fp32         = C.arith == PVHIP_ARITH_FP32;
+   return pvhip_recv_pre_dense_fits(&a) != 0;
 }
 
 // The dense scatter's transposed weights ([nyp*nxp*nfp][pre.nf], BF16 or FP32)

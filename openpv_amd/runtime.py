@@ -333,6 +333,16 @@ class PresynapticPerspectiveGPUDelivery:
             return None
         patch = self.nxp * self.nyp * self.nfp
         f32 = self.arith == abi.ARITH_FP32
+        # the workgroup's weights and target tile must fit its LDS, checked for
+        # every extended position (the restricted-only mode needs less)
+        q = abi.RecvPreDenseArgs()
+        q.pre, q.post = pre, post
+        q.nxp, q.nyp, q.nfp = self.nxp, self.nyp, self.nfp
+        q.start_x0, q.start_y0 = x0.value, y0.value
+        q.halo_is_zero = 0
+        q.fp32 = int(f32)
+        if not abi.hip().pvhip_recv_pre_dense_fits(C.byref(q)):
+            return None
         wb = torch.zeros(patch * pre.nf, dtype=torch.float32 if f32 else torch.bfloat16,
                          device=self.weights.device)
         self.dense = {"start": (x0.value, y0.value), "weights": wb, "patch": patch, "fp32": f32}

@@ -381,3 +381,33 @@ def test_recv_pre_dense_start_refuses_coarser_targets():
         (hp, _), (hq, _) = both(pre), both(post)
         assert abi.host().pvhost_recv_pre_dense_start(C.byref(hp), C.byref(hq), 8, 8, C.byref(s), C.byref(x0),
                                                       C.byref(y0)) == -1
+
+
+def _dense_fits(pre, post, nxp, halo_is_zero, fp32):
+    x0, y0 = C.c_int(), C.c_int()
+    assert abi.host().pvhost_recv_pre_dense_start(C.byref(pre), C.byref(post), nxp, nxp, None, C.byref(x0),
+                                                  C.byref(y0)) == 0
+    q = abi.RecvPreDenseArgs()
+    q.pre, q.post = pre, post
+    q.nxp = q.nyp = nxp
+    q.nfp = post.nf
+    q.start_x0, q.start_y0 = x0.value, y0.value
+    q.halo_is_zero, q.fp32 = halo_is_zero, fp32
+    return abi.hip().pvhip_recv_pre_dense_fits(C.byref(q))
+
+
+def test_recv_pre_dense_fits():
+    """pvhip_recv_pre_dense_fits (host only): the batched LCA's reconstruction
+    fits both forms with or without its halo; a 512x512 target does not (its
+    tile rows of 528 values exceed the LDS beside the weights); nor does a
+    source whose feature count is not 128, 256 or 512."""
+    pre, post = abi.Loc.make(64, 32, 32, 512, halo=2), abi.Loc.make(64, 128, 128, 1, halo=0)
+    for halo_is_zero in (0, 1):
+        for fp32 in (0, 1):
+            assert _dense_fits(pre, post, 16, halo_is_zero, fp32) == 1
+    pre, post = abi.Loc.make(8, 128, 128, 512, halo=2), abi.Loc.make(8, 512, 512, 1, halo=0)
+    assert _dense_fits(pre, post, 16, 1, 0) == 0
+    assert _dense_fits(pre, post, 16, 1, 1) == 0
+    pre, post = abi.Loc.make(2, 16, 16, 64, halo=2), abi.Loc.make(2, 64, 64, 1, halo=0)
+    assert _dense_fits(pre, post, 16, 1, 0) == 0
+    assert abi.hip().pvhip_recv_pre_dense_fits(None) == 0
