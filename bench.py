@@ -151,42 +151,24 @@ def accuracy(forms, images, W, torch, steps=ACCURACY_STEPS):
     return out
 
 
-def kernel_timing(col, s, n, torch):
-    """Average duration of the fused gather + LCA-update kernel
-    (recv_post_mfma_persist_kernel<4, GatherBF16, 8> for the BF16 form), launched
-    eagerly with the argument block the step uses when its scatter is the dense
-    route (no active-list emission; V1's chunked BF16 activity written when the
-    column keeps one), bracketed by HIP events recorded on the column's stream."""
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(n)]
-    args = col._gather_args_plain
+def step_kernel_timing(col, s, n, torch):
+    """Average and minimum durations of the fused gather + LCA-update kernel
+    (recv_post_mfma_persist_kernel<4, GatherBF16, 8> for the BF16 form) and of
+    the Recon scatter (recv_pre_dense_kernel for the BF16 form) as the step
+    runs them: n steps enqueued eagerly on the column's stream, each kernel
+    bracketed by HIP events recorded on that stream, so every kernel runs
+    between the same neighbours as in the replayed graph (the Residual update
+    before the gather, the scatter after it).  Returns ((avg, min) gather,
+    (avg, min) scatter) in ms."""
+    ev = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(4)) for _ in range(n)]
     with torch.cuda.stream(s):
-        for a, b in evs:
-            a.record(s)
-            col.gather.launch(args, stream=s)
-            b.record(s)
+        for g0, g1, c0, c1 in ev:
+            col.step(s, gather_events=(g0, g1), scatter_events=(c0, c1))
     s.synchronize()
     torch.cuda.synchronize()
-    d = [a.elapsed_time(b) for a, b in evs]
-    return float(np.mean(d)), float(np.min(d))
-
-
-def scatter_timing(col, s, n, torch):
-    """Average duration of the Recon scatter as the step runs it (with
-    Recon's update fused), launched eagerly between HIP events on the
-    column's stream."""
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(n)]
-    with torch.cuda.stream(s):
-        for a, b in evs:
-            a.record(s)
-            col.scatter.deliver(col.rec_gsyn, active=col.active, pre_activity=col.v1_A, accumulate=False,
-                                stream=s, fused_layer=col._rec_update)
-            b.record(s)
-    s.synchronize()
-    torch.cuda.synchronize()
-    d = [a.elapsed_time(b) for a, b in evs]
-    return float(np.mean(d)), float(np.min(d))
+    dg = [g0.elapsed_time(g1) for g0, g1, _, _ in ev]
+    dc = [c0.elapsed_time(c1) for _, _, c0, c1 in ev]
+    return (float(np.mean(dg)), float(np.min(dg))), (float(np.mean(dc)), float(np.min(dc)))
 
 
 TRAFFIC_SUMMARY = os.path.join("profiles", "lca64_bf16_traffic.md")
@@ -614,7 +596,7 @@ def main():
         res[f]["ms_per_step"] = round(res[f]["ms_per_step"], 5)
         res[f]["wall_s"] = round(res[f]["wall_s"], 3)
     # dominant kernel: fused gather + LCA update of the headline form
-    kavg, kmin = kernel_timing(col, s, 50, torch)
+    (kavg, kmin), (sms, smin) = step_kernel_timing(col, s, 50, torch)
     n_v1 = NB * NXV * NYV * NFV
     src_bytes = NB * col.res_loc.num_ext * 4
     w_bytes = NFV * NXP * NXP * NF * (2 if head == "bf16" else 4)
@@ -656,7 +638,6 @@ def main():
     if col.scatter.mode == "dense":
         # the dense scatter: a GEMM of V1's 64 x 32x32 positions by the 16x16
         # patch offsets over the 512 features, on the BF16 matrix cores
-        sms, smin = scatter_timing(col, s, 50, torch)
         sflop = 2.0 * NB * NXV * NYV * NXP * NXP * NFV
         # algorithmic bytes: V1's BF16 activity read once, the BF16 weights,
         # Recon's V and A written; at the peaks the bytes take longer than

@@ -203,9 +203,10 @@ class LCAColumn:
         return out
 
     # one timestep ---------------------------------------------------------
-    def step(self, stream=None, gather_events=None):
+    def step(self, stream=None, gather_events=None, scatter_events=None):
         """One timestep.  gather_events=(start, stop) torch events are recorded on
-        `stream` around the V1 gather (+ fused update) launch."""
+        `stream` around the V1 gather (+ fused update) launch, scatter_events
+        around the Recon scatter (+ Recon's update)."""
         s = stream
         nb = self.in_loc.nbatch
         # phase 1: Residual = err_scale * (Input - Recon(t-1)): both identity
@@ -233,8 +234,12 @@ class LCAColumn:
         self._active_stale = not needs_list
         self._a_stale = self.fused and not needs_list and self.v1_A16 is not None
         # phase 3: Recon = scatter of the sparse V1 activity
+        if scatter_events is not None:
+            scatter_events[0].record(s)
         self.scatter.deliver(self.rec_gsyn, active=self.active, pre_activity=self.v1_A,
                              accumulate=False, stream=s, fused_layer=self._rec_update)
+        if scatter_events is not None:
+            scatter_events[1].record(s)
 
     # dictionary learning -----------------------------------------------------
     def enable_learning(self, dw_max, normalize_dw=False, arith=None, renormalize="fp64",
