@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -285,6 +286,9 @@ struct Layer {
    // chunked BF16 copy of A ([nbatch][nf/32][nyExt][nxExt][32]), kept by the
    // layer's update when a dense BF16 scatter reads this layer
    void *Ab = nullptr;
+   // HyPerLCA layer fed by one gather only: that connection's index, the
+   // gather then applies the update in its epilogue (GSyn not materialised)
+   int fusedConn = -1;
    // sparse publish
    bool sparse = false, needActive = false;
    pvhip_sparse_entry *entries = nullptr;
@@ -814,6 +818,35 @@ void Net::build() {
       else if (C.mode == C_SCATTER && C.arith != PVHIP_ARITH_EXACT && transposedRoute(C)) {
          C.mode   = C_GATHER;
          C.routed = true;
+      }
+   }
+   // A HyPerLCALayer whose one channel is fed by one gather alone (one data
+   // patch per feature, no delay, its source updated in an earlier phase) has
+   // its update applied in the gather's epilogue: the same arithmetic as
+   // receive + pvhip_lca_update (LayerInputBuffer::recvAllSynapticInput, then
+   // HyPerLCAInternalStateBuffer::updateBuffer), without GSyn's reset, write
+   // and read.
+   for (size_t li = 0; li < layers.size(); li++) {
+      Layer &L = layers[li];
+      if (L.kind != L_LCA || L.ista || L.momentum || L.numChannels != 1) {
+         continue;
+      }
+      int found = -1, n = 0;
+      for (size_t ci = 0; ci < conns.size(); ci++) {
+         Conn &C = conns[ci];
+         if (C.post != (int)li || C.mode == C_NONE) {
+            continue;
+         }
+         n++;
+         if (C.mode == C_GATHER && C.channel == 0 && C.delay == 0 && !C.inv[0]
+             && layers[C.pre].phase < L.phase) {
+            found = (int)ci;
+         }
+      }
+      if (n == 1 && found >= 0) {
+         L.fusedConn = found;
+         std::printf("pvrun: %s: gather %s with the LCA update fused\n", L.name.c_str(),
+               conns[found].name.c_str());
       }
    }
    for (auto &C : conns) {
@@ -1380,8 +1413,8 @@ void Net::advanceDataStore(Layer &L) {
 
 // resetGSynBuffers, then every connection into L adds its channel
 void Net::receive(Layer &L) {
-   if (L.kind == L_INPUT) {
-      return;
+   if (L.kind == L_INPUT || L.fusedConn >= 0) {
+      return; // a fused layer's gather runs in its update
    }
    size_t nres = (size_t)nbatch * L.loc.nx * L.loc.ny * L.loc.nf;
    check(pvhip_fill_zero(L.gsyn, (size_t)L.numChannels * nres * 4, D.stream), "reset GSyn");
@@ -1428,7 +1461,17 @@ void Net::updateLayer(Layer &L, double t) {
       u.transfer = L.transfer;
       u.activity_from_v = 1;
       u.A_bf16c = L.Ab;
-      if (L.ista) {
+      if (L.fusedConn >= 0) {
+         // the gather with the update in its epilogue (one channel, coefficient 1)
+         u.num_channels           = 1;
+         u.gsyn                   = nullptr;
+         pvhip_recv_post_args a   = gatherArgs(conns[L.fusedConn]);
+         a.post_gsyn              = nullptr;
+         a.accumulate             = 0;
+         a.fused_update           = &u;
+         check(pvhip_recv_post(&a, D.stream), "pvhip_recv_post (fused LCA update)");
+      }
+      else if (L.ista) {
          check(pvhip_ista_update(&u, (float)L.prm.num("VThresh", -FLT_MAX), D.stream), "pvhip_ista_update");
       }
       else if (L.momentum) {
@@ -2271,6 +2314,8 @@ int main(int argc, char **argv) {
       }
       // HyPerCol::advanceTimeLoop: checkpointWrite before each step (checkpoint k
       // holds the state after k steps), then a final checkpoint
+      check(pvhip_stream_sync(net.D.stream), "sync before the run");
+      auto const t0 = std::chrono::steady_clock::now();
       for (long s = first; s <= n; s++) {
          bool restartStep = s == first && first > 1; // the checkpoint just read
          if (net.checkpointWrite && (s - 1) % net.checkpointInterval == 0 && !restartStep) {
@@ -2283,8 +2328,10 @@ int main(int argc, char **argv) {
          net.writeCheckpoint(net.checkpointDir(n, n), (double)n * net.dtCol, n);
       }
       check(pvhip_stream_sync(net.D.stream), "final sync");
+      double const secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       rc = net.finish();
-      std::printf("pvrun: %ld steps done%s\n", n - first + 1, rc ? ", a probe failed" : "");
+      std::printf("pvrun: %ld steps done%s (%.3f s, %.4f ms per step with the host's own work)\n", n - first + 1,
+            rc ? ", a probe failed" : "", secs, n >= first ? 1e3 * secs / (double)(n - first + 1) : 0.0);
    } catch (const std::exception &e) {
       std::fprintf(stderr, "pvrun: %s\n", e.what());
       rc = 1;
