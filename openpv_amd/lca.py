@@ -353,18 +353,25 @@ class LCAColumn:
         """Per-batch dt of the V1 update for the coming steps (adaptive time scales)."""
         self.lca.set_dt(dt)
 
-    def capture(self, stream):
-        """Capture one step as a hipGraph on `stream` (a non-default torch stream)."""
+    def capture(self, stream, steps=1):
+        """Capture `steps` consecutive steps as one hipGraph on `stream` (a
+        non-default torch stream); each replay() advances the column by that
+        many steps.  Several steps per graph spare the gap between two graph
+        launches, which costs about as much as a short kernel."""
+        if steps < 1:
+            raise ValueError("capture: steps must be at least 1")
         H = abi.hip()
         with torch.cuda.stream(stream):
             abi.check(H.pvhip_graph_begin(_stream(stream)), "pvhip_graph_begin")
             try:
-                self.step(stream)
+                for _ in range(steps):
+                    self.step(stream)
             finally:
                 g = C.c_void_p()
                 st = H.pvhip_graph_end(_stream(stream), C.byref(g))
             abi.check(st, "pvhip_graph_end")
         self._graph = g
+        self.graph_steps = steps
         self._graph_leaves_stale = self._active_stale   # host flags set while capturing
         self._graph_leaves_a_stale = self._a_stale
         return g

@@ -445,6 +445,32 @@ def test_lca_column_batched_graph_replay():
     graph.release_graph()
 
 
+@pytest.mark.parametrize("arith", [abi.ARITH_EXACT, abi.ARITH_BF16])
+def test_lca_column_multi_step_graph_replay(arith):
+    """A hipGraph of four captured steps: each replay advances the column by
+    four steps, to the same bits as four eager steps."""
+    img, W = c0_problem(nb=4, seed=4)
+    eager = LCAColumn(img, W, 16, 16, 128, 8, 8, VThresh=0.1, arith=arith, fused=True, device=DEV)
+    graph = LCAColumn(img, W, 16, 16, 128, 8, 8, VThresh=0.1, arith=arith, fused=True, device=DEV)
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    graph.capture(s, steps=4)
+    assert graph.graph_steps == 4
+    for i in range(3):
+        for _ in range(4):
+            eager.step()
+        graph.replay(s)
+        s.synchronize()
+        torch.cuda.synchronize()
+        for x, y in ((eager.v1_V, graph.v1_V), (eager.v1_activity(), graph.v1_activity()),
+                     (eager.rec_A, graph.rec_A), (eager.res_A, graph.res_A)):
+            assert np.array_equal(host(x).view(np.uint32), host(y).view(np.uint32)), "replay %d" % i
+        assert (eager.num_active() == graph.num_active()).all()
+    graph.release_graph()
+    with pytest.raises(ValueError):
+        graph.capture(s, steps=0)
+
+
 def test_lca_column_bf16_energy_and_sparsity():
     """BF16 MFMA delivery with FP32 state vs the EXACT column (2 x 32x32x3 ->
     16x16x128, 8x8 patches at stride 2), 200 steps: energy within 1e-3
