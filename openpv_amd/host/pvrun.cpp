@@ -272,6 +272,7 @@ struct Layer {
    float *prevDrive = nullptr;     // restricted, zero at start
    std::string adaptiveProbe;
    float *expTau = nullptr;
+   std::vector<float> expTauHost; // the values expTau holds (uploaded when they change)
    std::vector<double> dt;
    // input
    PvpFrames input;
@@ -289,6 +290,15 @@ struct Layer {
    // HyPerLCA layer fed by one gather only: that connection's index, the
    // gather then applies the update in its epilogue (GSyn not materialised)
    int fusedConn = -1;
+   // ANNErrorLayer fed by two IdentConn only (channels 0 and 1): their
+   // indices, the update then runs pvhip_error_update (GSyn not materialised)
+   int errIn = -1, errRec = -1;
+   // layer fed by one dense scatter only: that connection's index, the
+   // scatter then applies the update in its epilogue (GSyn not materialised)
+   int denseFusedConn = -1;
+   // LCA layer whose update writes only the chunked BF16 copy Ab (nothing in
+   // the step reads its FP32 activity): A is derived from V on request (act)
+   bool aLazy = false, aStale = false;
    // sparse publish
    bool sparse = false, needActive = false;
    pvhip_sparse_entry *entries = nullptr;
@@ -432,6 +442,7 @@ struct Net {
    void buildConn(Conn &c);
    void step(double t);
    pvhip_recv_post_args gatherArgs(Conn &c);
+   pvhip_recv_pre_dense_args denseArgs(Conn &c);
    double sourceBound(const Layer &pre);
    void autotune();
    void deliver(Conn &c);
@@ -440,6 +451,7 @@ struct Net {
    void checkAllZero(Probe &p, double t);
    void loadInputFrames(Layer &L);
    float *levelData(Layer &L, int delay);
+   float *act(Layer &L);
    void advanceDataStore(Layer &L);
    void readPlasticity(Conn &C);
    void momentumFactors(Conn &C);
@@ -849,8 +861,57 @@ void Net::build() {
                conns[found].name.c_str());
       }
    }
+   // The same for two other layers of the column, where the receive and the
+   // update see the same source activity (every source in another phase, so
+   // none updates between the two): an ANNErrorLayer fed by two IdentConn
+   // alone (channels 0 and 1) runs pvhip_error_update (both deliveries and
+   // ErrScaleInternalStateBuffer's update in one pass), and a layer fed by
+   // one dense scatter alone has its update applied in that scatter's
+   // epilogue.  Both give the bits of the separate launches.
+   for (size_t li = 0; li < layers.size(); li++) {
+      Layer &L = layers[li];
+      if (L.kind == L_INPUT || L.kind == L_LCA) {
+         continue;
+      }
+      std::vector<int> in;
+      for (size_t ci = 0; ci < conns.size(); ci++) {
+         if (conns[ci].post == (int)li && conns[ci].mode != C_NONE) {
+            in.push_back((int)ci);
+         }
+      }
+      auto otherPhase = [&](int ci) { return layers[conns[ci].pre].phase != L.phase; };
+      auto sameShape  = [&](int ci) {
+         const pvhip_loc &s = layers[conns[ci].pre].loc;
+         return s.nbatch == L.loc.nbatch && s.nx == L.loc.nx && s.ny == L.loc.ny && s.nf == L.loc.nf;
+      };
+      if (L.kind == L_ERROR && L.numChannels == 2 && in.size() == 2) {
+         int c0 = conns[in[0]].channel == 0 ? in[0] : in[1];
+         int c1 = c0 == in[0] ? in[1] : in[0];
+         bool ok = conns[c0].channel == 0 && conns[c1].channel == 1;
+         for (int ci : {c0, c1}) {
+            ok = ok && conns[ci].mode == C_IDENT && conns[ci].delay == 0 && otherPhase(ci) && sameShape(ci);
+         }
+         if (ok) {
+            L.errIn  = c0;
+            L.errRec = c1;
+            std::printf("pvrun: %s: deliveries %s, %s and the update in one pass\n", L.name.c_str(),
+                  conns[c0].name.c_str(), conns[c1].name.c_str());
+         }
+      }
+      else if (L.numChannels == 1 && in.size() == 1) {
+         Conn &C = conns[in[0]];
+         if (C.mode == C_SCATTER && C.dense && C.channel == 0 && C.delay == 0 && otherPhase(in[0])) {
+            L.denseFusedConn = in[0];
+            std::printf("pvrun: %s: dense scatter %s with the layer update fused\n", L.name.c_str(),
+                  C.name.c_str());
+         }
+      }
+   }
    for (auto &C : conns) {
       Layer &L = layers[C.pre];
+      if (C.dense) {
+         continue; // reads the dense activity, no active list
+      }
       if (C.mode == C_SCATTER && L.sparse && C.delay == 0) {
          L.needActive = true;
       }
@@ -874,6 +935,13 @@ void Net::build() {
          check(pvhip_active_compact(&L.loc, L.A, L.entries, L.numActive, L.posStart, L.compactScratch, D.stream),
                "pvhip_active_compact");
       }
+   }
+   // An LCA layer that keeps the chunked BF16 copy for its dense scatter and
+   // publishes no active list: its update writes only the copy, and the
+   // readers of its FP32 activity (other connections, probes, outputs,
+   // checkpoints) get A = T(V) from act(), the bits the update would write.
+   for (auto &L : layers) {
+      L.aLazy = L.kind == L_LCA && L.Ab && !L.needActive && !L.ista && !L.momentum;
    }
    for (auto &C : conns) {
       if (C.mode == C_GATHER || C.mode == C_SCATTER) {
@@ -1087,7 +1155,7 @@ void Net::deriveDense(Conn &C) {
 // other than by its update: initial state, checkpoint restore).
 void Net::syncBf16(Layer &L) {
    if (L.Ab) {
-      check(pvhip_activity_to_bf16c(&L.loc, L.A, L.Ab, D.stream), "pvhip_activity_to_bf16c");
+      check(pvhip_activity_to_bf16c(&L.loc, act(L), L.Ab, D.stream), "pvhip_activity_to_bf16c");
    }
 }
 
@@ -1298,6 +1366,31 @@ void Net::autotune() {
    pvhip_event_destroy(e1);
 }
 
+// The dense scatter's arguments for C (no output set).
+pvhip_recv_pre_dense_args Net::denseArgs(Conn &C) {
+   Layer &pre = layers[C.pre], &post = layers[C.post];
+   pvhip_recv_pre_dense_args a{};
+   a.pre          = pre.loc;
+   a.post         = post.loc;
+   a.nxp          = C.nxp;
+   a.nyp          = C.nyp;
+   a.nfp          = C.nfp;
+   a.start_x0     = C.dx0;
+   a.start_y0     = C.dy0;
+   a.halo_is_zero = 1; // denseRoute: no mirror boundary writes the halo
+   if (C.denseF32) {
+      a.fp32        = 1;
+      a.pre_data    = act(pre);
+      a.weights_f32 = static_cast<const float *>(C.Wdense);
+   }
+   else {
+      a.pre_data_bf16c = pre.Ab;
+      a.weights_bf16   = C.Wdense;
+   }
+   a.dt_factor = 1.0f;
+   return a;
+}
+
 void Net::deliver(Conn &C) {
    Layer &pre = layers[C.pre], &post = layers[C.post];
    size_t nres = (size_t)nbatch * post.loc.nx * post.loc.ny * post.loc.nf;
@@ -1310,27 +1403,9 @@ void Net::deliver(Conn &C) {
       check(pvhip_recv_post(&a, D.stream), "pvhip_recv_post");
    }
    else if (C.mode == C_SCATTER && C.dense) {
-      pvhip_recv_pre_dense_args a{};
-      a.pre            = pre.loc;
-      a.post           = post.loc;
-      a.nxp            = C.nxp;
-      a.nyp            = C.nyp;
-      a.nfp            = C.nfp;
-      a.start_x0       = C.dx0;
-      a.start_y0       = C.dy0;
-      a.halo_is_zero   = 1; // denseRoute: no mirror boundary writes the halo
-      if (C.denseF32) {
-         a.fp32        = 1;
-         a.pre_data    = pre.A;
-         a.weights_f32 = static_cast<const float *>(C.Wdense);
-      }
-      else {
-         a.pre_data_bf16c = pre.Ab;
-         a.weights_bf16   = C.Wdense;
-      }
-      a.dt_factor      = 1.0f;
-      a.post_gsyn      = g;
-      a.accumulate     = 1;
+      pvhip_recv_pre_dense_args a = denseArgs(C);
+      a.post_gsyn                 = g;
+      a.accumulate                = 1;
       check(pvhip_recv_pre_dense(&a, D.stream), "pvhip_recv_pre_dense");
    }
    else if (C.mode == C_SCATTER) {
@@ -1388,11 +1463,22 @@ void Net::loadInputFrames(Layer &L) {
    }
 }
 
+// L's FP32 activity.  A layer whose update wrote only its BF16 copy (aLazy)
+// has it derived here, A = T(V) (ANNActivityBuffer's transfer of the V the
+// update wrote: the same bits), once per update, when a reader asks.
+float *Net::act(Layer &L) {
+   if (L.aStale) {
+      check(pvhip_ann_activity(&L.loc, &L.transfer, L.V, L.A, D.stream), "pvhip_ann_activity");
+      L.aStale = false;
+   }
+   return L.A;
+}
+
 // DataStore level `delay` of L: the live activity for 0, else the ring slot
 // holding the activity published `delay` steps before it.
 float *Net::levelData(Layer &L, int delay) {
    if (delay == 0) {
-      return L.A;
+      return act(L);
    }
    return L.ring[(L.head + delay - 1) % (int)L.ring.size()];
 }
@@ -1408,13 +1494,13 @@ void Net::advanceDataStore(Layer &L) {
    int nlev = (int)L.ring.size();
    L.head   = (L.head + nlev - 1) % nlev;
    size_t next = (size_t)nbatch * (L.loc.nx + 2 * L.halo) * (L.loc.ny + 2 * L.halo) * L.loc.nf;
-   check(pvhip_copy_device(L.ring[L.head], L.A, next * 4, D.stream), "DataStore level copy");
+   check(pvhip_copy_device(L.ring[L.head], act(L), next * 4, D.stream), "DataStore level copy");
 }
 
 // resetGSynBuffers, then every connection into L adds its channel
 void Net::receive(Layer &L) {
-   if (L.kind == L_INPUT || L.fusedConn >= 0) {
-      return; // a fused layer's gather runs in its update
+   if (L.kind == L_INPUT || L.fusedConn >= 0 || L.errIn >= 0 || L.denseFusedConn >= 0) {
+      return; // a fused layer's deliveries run in its update
    }
    size_t nres = (size_t)nbatch * L.loc.nx * L.loc.ny * L.loc.nf;
    check(pvhip_fill_zero(L.gsyn, (size_t)L.numChannels * nres * 4, D.stream), "reset GSyn");
@@ -1447,7 +1533,10 @@ void Net::updateLayer(Layer &L, double t) {
       else {
          pvhip_lca_exp_tau(L.dt.data(), nbatch, L.tau, e.data());
       }
-      D.h2d(L.expTau, e.data(), nbatch * 4);
+      if (e != L.expTauHost) { // a synchronous copy: only when dt changed
+         D.h2d(L.expTau, e.data(), nbatch * 4);
+         L.expTauHost = e;
+      }
       pvhip_lca_update_args u{};
       u.loc = L.loc;
       u.num_channels = L.numChannels;
@@ -1461,6 +1550,10 @@ void Net::updateLayer(Layer &L, double t) {
       u.transfer = L.transfer;
       u.activity_from_v = 1;
       u.A_bf16c = L.Ab;
+      if (L.aLazy) {
+         u.A      = nullptr; // only the BF16 copy; act() derives A when read
+         L.aStale = true;
+      }
       if (L.fusedConn >= 0) {
          // the gather with the update in its epilogue (one channel, coefficient 1)
          u.num_channels           = 1;
@@ -1494,7 +1587,37 @@ void Net::updateLayer(Layer &L, double t) {
       u.scale = L.errScale;
       u.apply_scale = L.kind == L_ERROR ? 1 : 0;
       u.transfer = L.transfer;
-      check(pvhip_layer_update(&u, D.stream), "pvhip_layer_update");
+      if (L.errIn >= 0) {
+         // both IdentConn deliveries and the update in one pass
+         Layer &in = layers[conns[L.errIn].pre], &rec = layers[conns[L.errRec].pre];
+         pvhip_error_update_args x{};
+         x.loc               = L.loc;
+         x.in_loc            = in.loc;
+         x.in                = act(in);
+         x.rec_loc           = rec.loc;
+         x.rec               = act(rec);
+         x.channel_coeffs[0] = 1.0f;
+         x.channel_coeffs[1] = -1.0f;
+         x.scale             = u.scale;
+         x.apply_scale       = u.apply_scale;
+         x.transfer          = u.transfer;
+         x.V                 = L.V;
+         x.A                 = L.A;
+         check(pvhip_error_update(&x, D.stream), "pvhip_error_update");
+      }
+      else if (L.denseFusedConn >= 0) {
+         // the dense scatter with this update in its epilogue (one channel)
+         u.num_channels               = 1;
+         u.gsyn                       = nullptr;
+         pvhip_recv_pre_dense_args a  = denseArgs(conns[L.denseFusedConn]);
+         a.post_gsyn                  = nullptr;
+         a.accumulate                 = 0;
+         a.fused_layer                = &u;
+         check(pvhip_recv_pre_dense(&a, D.stream), "pvhip_recv_pre_dense (fused layer update)");
+      }
+      else {
+         check(pvhip_layer_update(&u, D.stream), "pvhip_layer_update");
+      }
       syncBf16(L);
    }
    if (L.needActive) {
@@ -1530,7 +1653,7 @@ std::vector<double> Net::probeEnergy(Probe &p) {
    else {
       kind = 1;
    }
-   check(pvhip_cost_sum_f64(&L.loc, L.A, 1, kind, p0, p1, energyDev, D.stream), "pvhip_cost_sum_f64");
+   check(pvhip_cost_sum_f64(&L.loc, act(L), 1, kind, p0, p1, energyDev, D.stream), "pvhip_cost_sum_f64");
    std::vector<double> v(nbatch);
    D.d2h(v.data(), energyDev, nbatch * 8);
    if (p.keyword == "L2NormProbe") {
@@ -1702,7 +1825,7 @@ pvhip_hebbian_args Net::hebbianArgs(Conn &C) {
    a.apost_y         = C.hebbTab[5];
    a.dw_max          = C.dWMax;
    a.pre_data        = levelData(pre, C.delay); // HebbianUpdater::update_dW: getLayerData(delay)
-   a.post_data       = post.A;
+   a.post_data       = act(post);
    a.dw              = C.dW;
    a.counts          = C.counts;
    a.arith           = C.dwArith;
@@ -1837,7 +1960,7 @@ void Net::updateConnections(double t) {
 // at the end, where exitOnFailure makes the run fail.
 void Net::checkAllZero(Probe &p, double t) {
    Layer &L = layers[p.target];
-   check(pvhip_cost_sum_f64(&L.loc, L.A, 1, 3, p.nnzThreshold, 0.0, energyDev, D.stream), "pvhip_cost_sum_f64");
+   check(pvhip_cost_sum_f64(&L.loc, act(L), 1, 3, p.nnzThreshold, 0.0, energyDev, D.stream), "pvhip_cost_sum_f64");
    std::vector<double> cnt(nbatch);
    D.d2h(cnt.data(), energyDev, nbatch * 8);
    for (int b = 0; b < nbatch; b++) {
@@ -1968,7 +2091,7 @@ void Net::output(double t) {
       }
       size_t nxe = L.loc.nx + 2 * L.halo, nye = L.loc.ny + 2 * L.halo;
       std::vector<float> ext((size_t)nbatch * nxe * nye * L.loc.nf);
-      D.d2h(ext.data(), L.A, ext.size() * 4);
+      D.d2h(ext.data(), act(L), ext.size() * 4);
       for (int b = 0; b < nbatch; b++) {
          std::vector<float> fr((size_t)L.loc.nx * L.loc.ny * L.loc.nf);
          for (int y = 0; y < L.loc.ny; y++) {
@@ -2080,7 +2203,7 @@ void Net::writeCheckpoint(const std::string &dir, double simTime, long step) {
       if (L.kind != L_INPUT) {
          put("V", restrictedFrames(L, L.V, false));
       }
-      put("A", restrictedFrames(L, L.A, true));
+      put("A", restrictedFrames(L, act(L), true));
       std::vector<std::vector<float>> levels;
       for (int lv = 0; lv < L.levels; lv++) {
          auto f = restrictedFrames(L, levelData(L, lv), true);
@@ -2171,6 +2294,7 @@ double Net::readCheckpoint(const std::string &dir) {
       if (L.kind != L_INPUT) {
          setRestrictedFrames(L, L.V, false, get("V"), 0);
       }
+      L.aStale = false; // A is read from the checkpoint
       setRestrictedFrames(L, L.A, true, get("A"), 0);
       syncBf16(L);
       auto levels = get("Delays");

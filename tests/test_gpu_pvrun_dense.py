@@ -134,3 +134,57 @@ def test_pvrun_dense_scatter_column(tmp_path, form, tol):
     print("%s: max |pvrun - oracle| per step %s (scale %g)" % (form, np.array2string(err, precision=2), scale),
           flush=True)
     assert (err <= tol * scale).all()
+
+
+def test_pvrun_fused_column_with_activity_readers(tmp_path):
+    """In the BF16 column pvrun fuses every layer's deliveries into its update
+    (Residual: both IdentConn in one pass; V1: the gather; Reconstruction: the
+    dense scatter), and V1's update writes only its BF16 copy, its FP32
+    activity derived from V when something reads it.  With V1's activity
+    written every step (a reader) the reconstructions keep the bits of the run
+    without it, and V1's written activity is the oracle column's within the
+    BF16 bound."""
+    import bench
+    steps, nb = 12, 2
+    images = bench.one_over_f_images(nb, 128, 128, seed=33)
+    W = bench.dictionary(128, 16, 1, seed=34)
+    pvp.write_activity(str(tmp_path / "images.pvp"), images, [0.0, 0.0])
+    pvp.write_weights(str(tmp_path / "weights.pvp"), W[None], [0.0])
+    base = PARAMS % {"steps": steps, "form": "bf16"}
+    v1_block = base.split('HyPerLCALayer "V1"')[1].split("};")[0]
+    reader = base.replace(v1_block, v1_block.replace("writeStep = -1;", "writeStep = 1;\n    initialWriteTime = 1;"))
+    assert reader != base
+    recs, outs = [], []
+    for name, params in (("base", base), ("reader", reader)):
+        d = tmp_path / name
+        d.mkdir()
+        for f in ("images.pvp", "weights.pvp"):
+            (d / f).write_bytes((tmp_path / f).read_bytes())
+        (d / "column.params").write_text(params)
+        print("pvrun bf16 fused column", name, flush=True)
+        r = subprocess.run([abi.pvrun_path(), "-p", "column.params", "-o", str(d / "out"), "--steps", str(steps)],
+                           cwd=d, capture_output=True, text=True, timeout=300)
+        print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+        assert r.returncode == 0, r.stderr[-800:]
+        for msg in ("Residual: deliveries InputToResidual, ReconstructionToResidual and the update in one pass",
+                    "V1: gather ResidualToV1 with the LCA update fused",
+                    "Reconstruction: dense scatter V1ToReconstruction with the layer update fused"):
+            assert msg in r.stdout
+        recs.append(pvp.read_activity(str(d / "out" / "Reconstruction.pvp"))[2])
+        outs.append(d / "out")
+    assert np.array_equal(recs[0].view(np.uint32), recs[1].view(np.uint32))
+    _, times, a1 = pvp.read_activity(str(outs[1] / "V1.pvp"))
+    assert list(times) == [float(t) for t in range(1, steps + 1) for _ in range(nb)]
+    a1 = a1.reshape(steps, nb, 32, 32, 128)
+    col = OC.OracleColumn(images, W, 32, 32, 128, 16, 0.15)
+    m = (int(round(np.sqrt(col.v1.num_ext / 128))) - 32) // 2
+    refs = []
+    for _ in range(steps):
+        col.step()
+        refs.append(col.A.reshape(nb, 32 + 2 * m, 32 + 2 * m, 128)[:, m:m + 32, m:m + 32, :].copy())
+    refs = np.stack(refs)
+    scale = np.abs(refs).max()
+    err = np.abs(a1 - refs).reshape(steps, -1).max(1)
+    print("V1 activity: max |pvrun - oracle| per step %s (scale %g)"
+          % (" ".join("%.2e" % e for e in err), scale), flush=True)
+    assert scale > 0 and (err <= 5e-3 * scale).all()
