@@ -188,3 +188,74 @@ def test_pvrun_fused_column_with_activity_readers(tmp_path):
     print("V1 activity: max |pvrun - oracle| per step %s (scale %g)"
           % (" ".join("%.2e" % e for e in err), scale), flush=True)
     assert scale > 0 and (err <= 5e-3 * scale).all()
+
+
+def _run(d, params, steps):
+    (d / "column.params").write_text(params)
+    r = subprocess.run([abi.pvrun_path(), "-p", "column.params", "-o", str(d / "out"), "--steps", str(steps)],
+                       cwd=d, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1200:], r.stderr[-1200:], flush=True)
+    assert r.returncode == 0, r.stderr[-800:]
+    return r
+
+
+def test_pvrun_fused_column_probes_and_restart(tmp_path):
+    """The BF16 fused column (V1's FP32 activity derived from V on request)
+    with an energy probe (0.5*|Residual|^2 + VThresh*|V1|_1, read every step)
+    and checkpoints every 6 steps: the energies follow the oracle column's
+    within the BF16 bound, the probes leave the reconstructions' bits
+    unchanged, and a run restarted from Checkpoint6 reproduces steps 7..12 bit
+    for bit."""
+    import bench
+    steps, nb = 12, 2
+    images = bench.one_over_f_images(nb, 128, 128, seed=35)
+    W = bench.dictionary(128, 16, 1, seed=36)
+    base = PARAMS % {"steps": steps, "form": "bf16"}
+    ck = tmp_path / "ck"
+    probes = base.replace('    outputPath = "output/";',
+                          '    outputPath = "output/";\n    checkpointWrite = true;\n'
+                          '    checkpointWriteDir = "%s";\n    checkpointWriteTriggerMode = "step";\n'
+                          '    checkpointWriteStepInterval = 6;' % ck)
+    probes += """
+ColumnEnergyProbe "Energy" = {
+    probeOutputFile = "Energy.txt";
+};
+L2NormProbe "ResidualNorm" = {
+    targetLayer = "Residual";
+    energyProbe = "Energy";
+    coefficient = 0.5;
+    exponent = 2;
+};
+L1NormLCAProbe "Sparsity" = {
+    targetLayer = "V1";
+    energyProbe = "Energy";
+};
+"""
+    restart = probes.replace("    checkpointWrite = true;",
+                             '    checkpointWrite = false;\n    initializeFromCheckpointDir = "%s";'
+                             % (ck / "Checkpoint06"))
+    recs = {}
+    for name, params in (("base", base), ("probes", probes), ("restart", restart)):
+        d = tmp_path / name
+        d.mkdir()
+        pvp.write_activity(str(d / "images.pvp"), images, [0.0, 0.0])
+        pvp.write_weights(str(d / "weights.pvp"), W[None], [0.0])
+        print("pvrun bf16 fused column", name, flush=True)
+        _run(d, params, steps)
+        recs[name] = pvp.read_activity(str(d / "out" / "Reconstruction.pvp"))[1:]
+    assert np.array_equal(recs["base"][1].view(np.uint32), recs["probes"][1].view(np.uint32))
+    tb, rb = recs["restart"]
+    assert list(tb) == [float(t) for t in range(7, steps + 1) for _ in range(nb)]
+    assert np.array_equal(recs["base"][1][6 * nb:].view(np.uint32), rb.view(np.uint32))
+    rows = [l.split(",") for l in (tmp_path / "probes" / "out" / "Energy.txt").read_text().splitlines()
+            if l.strip()]
+    e = {(round(float(t)), int(b)): float(v) for t, b, v in rows}
+    col = OC.OracleColumn(images, W, 32, 32, 128, 16, 0.15)
+    worst = 0.0
+    for t in range(1, steps + 1):
+        col.step()
+        ref = col.energy()
+        for b in range(nb):
+            worst = max(worst, abs(e[(t, b)] - ref[b]) / abs(ref[b]))
+    print("energy: largest relative difference to the oracle %.2e" % worst, flush=True)
+    assert worst <= 1e-3
