@@ -286,6 +286,55 @@ __global__ void __launch_bounds__(kRowThreads) error_update_kernel(
    }
 }
 
+// The same update with 16-byte accesses: each thread takes four consecutive
+// elements of one row (rows of a multiple of 4 values), reads them with one
+// load per input, writes V with one store and A with AW-wide stores (A's rows
+// start on multiples of AW values).  Per element the operations are those of
+// error_update_kernel.
+template <int AW>
+__global__ void __launch_bounds__(kRowThreads) error_update_vec_kernel(
+      Geo loc, Geo inLoc, const float *in, Geo recLoc, const float *rec, float c0, float c1,
+      int applyScale, float scale, Transfer t, float *V, float *A) {
+   int const numRes = loc.numRes();
+   int const rowLen = loc.nx * loc.nf;
+   int const b      = blockIdx.y;
+   int const k      = (blockIdx.x * kRowThreads + threadIdx.x) * 4;
+   if (k >= numRes) {
+      return;
+   }
+   int const y = k / rowLen;
+   int const j = k - y * rowLen;
+   float4 const i4 = *reinterpret_cast<const float4 *>(in + ext_row(inLoc, b, y) + j);
+   float4 const r4 = *reinterpret_cast<const float4 *>(rec + ext_row(recLoc, b, y) + j);
+   float const iv[4] = {i4.x, i4.y, i4.z, i4.w};
+   float const rv[4] = {r4.x, r4.y, r4.z, r4.w};
+   float v[4], a[4];
+#pragma unroll
+   for (int u = 0; u < 4; u++) {
+      float const g0 = 0.0f + iv[u]; // IdentDelivery, reset GSyn
+      float const g1 = 0.0f + rv[u];
+      float value    = 0.0f;         // GSynAccumulator
+      float prod     = c0 * g0;
+      value          = value + prod;
+      prod           = c1 * g1;
+      value          = value + prod;
+      if (applyScale) {
+         value = value * scale;
+      }
+      v[u] = value;
+      a[u] = apply_transfer(t, value);
+   }
+   *reinterpret_cast<float4 *>(V + (size_t)b * numRes + k) = make_float4(v[0], v[1], v[2], v[3]);
+   float *Ar = A + ext_row(loc, b, y) + j;
+   if (AW == 4) {
+      *reinterpret_cast<float4 *>(Ar) = make_float4(a[0], a[1], a[2], a[3]);
+   }
+   else {
+      *reinterpret_cast<float2 *>(Ar)     = make_float2(a[0], a[1]);
+      *reinterpret_cast<float2 *>(Ar + 2) = make_float2(a[2], a[3]);
+   }
+}
+
 // ---------------------------------------------------------------- compaction
 //
 // Ordered stream compaction of the extended buffer of each batch element
@@ -936,9 +985,33 @@ int pvhip_error_update(const pvhip_error_update_args *a, pvhip_stream stream) {
    }
    Geo g = geo(a->loc);
    PVHIP_REQUIRE(row_grid_ok(g), "error_update: more than 65535 rows or batch elements");
-   hipLaunchKernelGGL(error_update_kernel, row_grid(g), dim3(kRowThreads), 0, (hipStream_t)stream, g,
-         geo(a->in_loc), a->in, geo(a->rec_loc), a->rec, a->channel_coeffs[0], a->channel_coeffs[1],
-         a->apply_scale, a->scale, t, a->V, a->A);
+   // 16-byte form when every row segment of four values is aligned for it:
+   // rows of a multiple of 4 values, the inputs' and V's rows on 16-byte
+   // boundaries, A's rows on multiples of 4 (else 2) values
+   auto rows_aligned = [](const Geo &x, const void *p, int m) {
+      return ((uintptr_t)p % (4 * m)) == 0 && (x.numExt() % m) == 0 && ((x.nxExt() * x.nf) % m) == 0
+             && (((x.up * x.nxExt() + x.lt) * x.nf) % m) == 0;
+   };
+   Geo const gi = geo(a->in_loc), gr = geo(a->rec_loc);
+   bool const vec = (g.nx * g.nf) % 4 == 0 && rows_aligned(gi, a->in, 4) && rows_aligned(gr, a->rec, 4)
+                    && (uintptr_t)a->V % 16 == 0 && rows_aligned(g, a->A, 2);
+   if (vec) {
+      dim3 const grid((unsigned)((g.numRes() / 4 + kRowThreads - 1) / kRowThreads), (unsigned)g.nbatch);
+      if (rows_aligned(g, a->A, 4)) {
+         hipLaunchKernelGGL(error_update_vec_kernel<4>, grid, dim3(kRowThreads), 0, (hipStream_t)stream, g, gi,
+               a->in, gr, a->rec, a->channel_coeffs[0], a->channel_coeffs[1], a->apply_scale, a->scale, t,
+               a->V, a->A);
+      }
+      else {
+         hipLaunchKernelGGL(error_update_vec_kernel<2>, grid, dim3(kRowThreads), 0, (hipStream_t)stream, g, gi,
+               a->in, gr, a->rec, a->channel_coeffs[0], a->channel_coeffs[1], a->apply_scale, a->scale, t,
+               a->V, a->A);
+      }
+      PVHIP_LAUNCHED("error_update");
+      return PVHIP_OK;
+   }
+   hipLaunchKernelGGL(error_update_kernel, row_grid(g), dim3(kRowThreads), 0, (hipStream_t)stream, g, gi, a->in,
+         gr, a->rec, a->channel_coeffs[0], a->channel_coeffs[1], a->apply_scale, a->scale, t, a->V, a->A);
    PVHIP_LAUNCHED("error_update");
    return PVHIP_OK;
 }

@@ -113,14 +113,20 @@ def test_lca_update_bit_exact(vertices, params, from_v):
     assert same(host(Vd), Vr) and same(host(Ad), Ar)
 
 
-def test_error_update_equals_ident_ident_layer_update():
+@pytest.mark.parametrize("nx,ny,nf,halo", [
+    (9, 7, 3, 3),     # rows of 27 values: the one-value-per-thread form
+    (16, 6, 1, 6),    # rows of 16, A's rows start on 2 mod 4: 16-byte reads, 8-byte A stores
+    (16, 5, 4, 2),    # A's rows on multiples of 4 values: 16-byte accesses throughout
+])
+def test_error_update_equals_ident_ident_layer_update(nx, ny, nf, halo):
     """pvhip_error_update (the residual layer in one pass) against the separate
     pvhip_ident_deliver x2 + pvhip_layer_update, bit for bit, signed zeros
-    included, with and without errScale and with a non-identity transfer."""
+    included, with and without errScale and with a non-identity transfer, in
+    each of its access forms."""
     rng = np.random.default_rng(24)
-    hl = abi.Loc.make(2, 9, 7, 3, 3)
-    hi = abi.Loc.make(2, 9, 7, 3, 0)
-    hr = abi.Loc.make(2, 9, 7, 3, (1, 2, 0, 1))
+    hl = abi.Loc.make(2, nx, ny, nf, halo)
+    hi = abi.Loc.make(2, nx, ny, nf, 0)
+    hr = abi.Loc.make(2, nx, ny, nf, (1, 2, 0, 1) if nf != 1 else (4, 0, 0, 2))
     n = hl.nbatch * hl.num_res
     xin = (rng.standard_normal(hi.nbatch * hi.num_ext) * 0.5).astype(np.float32)
     xrec = (rng.standard_normal(hr.nbatch * hr.num_ext) * 0.5).astype(np.float32)
