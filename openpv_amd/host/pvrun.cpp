@@ -936,13 +936,6 @@ void Net::build() {
                "pvhip_active_compact");
       }
    }
-   // An LCA layer that keeps the chunked BF16 copy for its dense scatter and
-   // publishes no active list: its update writes only the copy, and the
-   // readers of its FP32 activity (other connections, probes, outputs,
-   // checkpoints) get A = T(V) from act(), the bits the update would write.
-   for (auto &L : layers) {
-      L.aLazy = L.kind == L_LCA && L.Ab && !L.needActive && !L.ista && !L.momentum;
-   }
    for (auto &C : conns) {
       if (C.mode == C_GATHER || C.mode == C_SCATTER) {
          buildConn(C);
@@ -981,6 +974,21 @@ void Net::build() {
          std::string path = outputPath + "/" + p.prm.str("probeOutputFile");
          p.out            = std::fopen(path.c_str(), "w");
       }
+   }
+   // An LCA layer that keeps the chunked BF16 copy for its dense scatter,
+   // publishes no active list, writes no output and is no probe's target:
+   // its update writes only the copy, and the occasional readers of its FP32
+   // activity (other connections, checkpoints) get A = T(V) from act(), the
+   // bits the update would write.  A layer read every step keeps writing A
+   // in its update, which costs less than deriving it afterwards.
+   for (size_t li = 0; li < layers.size(); li++) {
+      Layer &L   = layers[li];
+      bool probe = false;
+      for (auto &p : probes) {
+         probe = probe || p.target == (int)li;
+      }
+      L.aLazy = L.kind == L_LCA && L.Ab && !L.needActive && !L.ista && !L.momentum && !probe
+                && L.writeStep <= 0;
    }
    energyDev = static_cast<double *>(D.alloc(nbatch * 8));
    check(pvhip_stream_sync(D.stream), "initialisation");

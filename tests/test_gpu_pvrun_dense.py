@@ -139,11 +139,10 @@ def test_pvrun_dense_scatter_column(tmp_path, form, tol):
 def test_pvrun_fused_column_with_activity_readers(tmp_path):
     """In the BF16 column pvrun fuses every layer's deliveries into its update
     (Residual: both IdentConn in one pass; V1: the gather; Reconstruction: the
-    dense scatter), and V1's update writes only its BF16 copy, its FP32
-    activity derived from V when something reads it.  With V1's activity
-    written every step (a reader) the reconstructions keep the bits of the run
-    without it, and V1's written activity is the oracle column's within the
-    BF16 bound."""
+    dense scatter).  Without readers V1's update writes only its BF16 copy;
+    with V1's activity written every step the update writes the FP32 activity
+    too.  Both runs give the same reconstructions bit for bit, and V1's
+    written activity is the oracle column's within the BF16 bound."""
     import bench
     steps, nb = 12, 2
     images = bench.one_over_f_images(nb, 128, 128, seed=33)
@@ -200,22 +199,32 @@ def _run(d, params, steps):
 
 
 def test_pvrun_fused_column_probes_and_restart(tmp_path):
-    """The BF16 fused column (V1's FP32 activity derived from V on request)
-    with an energy probe (0.5*|Residual|^2 + VThresh*|V1|_1, read every step)
-    and checkpoints every 6 steps: the energies follow the oracle column's
-    within the BF16 bound, the probes leave the reconstructions' bits
-    unchanged, and a run restarted from Checkpoint6 reproduces steps 7..12 bit
-    for bit."""
+    """The BF16 fused column with checkpoints every 6 steps, without readers of
+    V1's FP32 activity (derived from V on request: the checkpoint asks for it)
+    and with an energy probe (0.5*|Residual|^2 + VThresh*|V1|_1, read every
+    step; V1's update then writes it): the energies follow the oracle
+    column's within the BF16 bound, both runs give the same reconstructions
+    bit for bit, and each run restarted from its Checkpoint06 reproduces its
+    steps 7..12 bit for bit."""
     import bench
     steps, nb = 12, 2
     images = bench.one_over_f_images(nb, 128, 128, seed=35)
     W = bench.dictionary(128, 16, 1, seed=36)
-    base = PARAMS % {"steps": steps, "form": "bf16"}
-    ck = tmp_path / "ck"
-    probes = base.replace('    outputPath = "output/";',
-                          '    outputPath = "output/";\n    checkpointWrite = true;\n'
-                          '    checkpointWriteDir = "%s";\n    checkpointWriteTriggerMode = "step";\n'
-                          '    checkpointWriteStepInterval = 6;' % ck)
+    def checkpointed(text, ck):
+        return text.replace('    outputPath = "output/";',
+                            '    outputPath = "output/";\n    checkpointWrite = true;\n'
+                            '    checkpointWriteDir = "%s";\n    checkpointWriteTriggerMode = "step";\n'
+                            '    checkpointWriteStepInterval = 6;' % ck)
+
+    def restarted(text, ck):
+        return text.replace("    checkpointWrite = true;",
+                            '    checkpointWrite = false;\n    initializeFromCheckpointDir = "%s";'
+                            % (ck / "Checkpoint06"))
+
+    plain = PARAMS % {"steps": steps, "form": "bf16"}
+    ckb, ck = tmp_path / "ckb", tmp_path / "ck"
+    base = checkpointed(plain, ckb)
+    probes = checkpointed(plain, ck)
     probes += """
 ColumnEnergyProbe "Energy" = {
     probeOutputFile = "Energy.txt";
@@ -231,11 +240,9 @@ L1NormLCAProbe "Sparsity" = {
     energyProbe = "Energy";
 };
 """
-    restart = probes.replace("    checkpointWrite = true;",
-                             '    checkpointWrite = false;\n    initializeFromCheckpointDir = "%s";'
-                             % (ck / "Checkpoint06"))
     recs = {}
-    for name, params in (("base", base), ("probes", probes), ("restart", restart)):
+    for name, params in (("base", base), ("probes", probes), ("restart_base", restarted(base, ckb)),
+                         ("restart", restarted(probes, ck))):
         d = tmp_path / name
         d.mkdir()
         pvp.write_activity(str(d / "images.pvp"), images, [0.0, 0.0])
@@ -244,9 +251,10 @@ L1NormLCAProbe "Sparsity" = {
         _run(d, params, steps)
         recs[name] = pvp.read_activity(str(d / "out" / "Reconstruction.pvp"))[1:]
     assert np.array_equal(recs["base"][1].view(np.uint32), recs["probes"][1].view(np.uint32))
-    tb, rb = recs["restart"]
-    assert list(tb) == [float(t) for t in range(7, steps + 1) for _ in range(nb)]
-    assert np.array_equal(recs["base"][1][6 * nb:].view(np.uint32), rb.view(np.uint32))
+    for name in ("restart_base", "restart"):
+        tb, rb = recs[name]
+        assert list(tb) == [float(t) for t in range(7, steps + 1) for _ in range(nb)]
+        assert np.array_equal(recs["base"][1][6 * nb:].view(np.uint32), rb.view(np.uint32)), name
     rows = [l.split(",") for l in (tmp_path / "probes" / "out" / "Energy.txt").read_text().splitlines()
             if l.strip()]
     e = {(round(float(t)), int(b)): float(v) for t, b, v in rows}
