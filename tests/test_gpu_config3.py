@@ -100,7 +100,7 @@ class _Float64Loop:
         if not grad:
             return float(E)
         gw, gx = torch.autograd.grad(E, (w, xx))
-        return (float(E), gw.permute(0, 2, 3, 1).contiguous().cpu().numpy(),
+        return (float(E.detach()), gw.permute(0, 2, 3, 1).contiguous().cpu().numpy(),
                 gx.permute(0, 2, 3, 1).contiguous().cpu().numpy())
 
 
